@@ -1,0 +1,109 @@
+/*
+ * tkfmi -- MI355X (gfx950) pair-HMM forward/backward/expected-count library.
+ *
+ * C ABI (no C++ types, no torch types): plain pointers and sizes.  Every
+ * array argument documented as "device" must be device memory on the
+ * current HIP device; "host" arrays are read by the library on the CPU.
+ *
+ * The reference has no FFI: its operator surface is a set of Python
+ * functions over FP64 JAX arrays.  The entry points below replace, for a
+ * ragged batch of pairs in one launch, the reference calls named beside
+ * each one (paths relative to the reference's python/ directory).  The
+ * Python package tkfmi binds them with ctypes (tkfmi/_abi.py).
+ *
+ * Conventions
+ *   - FP64 throughout; log-space inputs exactly as the reference passes
+ *     them (log transition matrix, log emission tables).
+ *   - Return 0 on success, a negative TKFMI_E* code on failure; the message
+ *     is available from tkfmi_last_error() (thread-local).
+ *   - Calls are stream-ordered and asynchronous on `stream`; numerical
+ *     failure (impossible pair) is returned as a NEG_INF log-probability,
+ *     not raised, as the reference does.
+ */
+#ifndef TKFMI_H
+#define TKFMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TKFMI_ABI_VERSION 1
+
+/* error codes */
+#define TKFMI_OK 0
+#define TKFMI_EINVAL (-1)       /* invalid argument / unsupported model   */
+#define TKFMI_EWORKSPACE (-2)   /* workspace missing or too small         */
+#define TKFMI_EHIP (-3)         /* HIP runtime error                      */
+
+/* flags */
+#define TKFMI_FORWARD_ONLY 1u    /* logP only (dp/hmm.py:1915-1916)       */
+#define TKFMI_WANT_POSTERIORS 2u /* write the posterior lattice           */
+#define TKFMI_NO_COUNTS 4u       /* skip expected transition counts       */
+
+/* emission tables: per group g, 21x21 match block then 21 insert then 21
+ * delete entries (index 20 = wildcard), all natural-log values. */
+#define TKFMI_A_PAD 21
+#define TKFMI_TAB_SZ (TKFMI_A_PAD * TKFMI_A_PAD + 2 * TKFMI_A_PAD)
+
+typedef struct tkfmi_problem {
+  int32_t B;                   /* number of pairs                           */
+  int32_t S;                   /* number of states                          */
+  const int32_t* state_types;  /* host [S]: 0=S 1=M 2=I 3=D 4=E (core/params.py:22) */
+  const int32_t* state_group;  /* host [S]: emission group of each state    */
+  int32_t n_groups;            /* G                                         */
+  int32_t n_trans;             /* 1 (shared) or B (per pair)                */
+  const double* log_trans;     /* device [n_trans,S,S] log transitions      */
+  int32_t n_tab;               /* 1 (shared) or B (per pair)                */
+  const double* log_tab;       /* device [n_tab,G,TKFMI_TAB_SZ]             */
+  const uint8_t* x_cat;        /* device: concatenated ancestor codes 0..20 */
+  const uint8_t* y_cat;        /* device: concatenated descendant codes     */
+  const int64_t* h_x_off;      /* host [B] offsets into x_cat               */
+  const int64_t* h_y_off;      /* host [B] offsets into y_cat               */
+  const int32_t* h_x_len;      /* host [B] Lx                               */
+  const int32_t* h_y_len;      /* host [B] Ly                               */
+} tkfmi_problem;
+
+typedef struct tkfmi_outputs {
+  double* log_prob;   /* device [B]                                          */
+  double* n_trans;    /* device [B,S,S], zero-filled by the caller, or NULL  */
+  double* posteriors; /* device ragged: pair b at offset sum_{b'<b}
+                         (Lx+1)(Ly+1)S, layout [i][j][state], zero-filled by
+                         the caller; NULL unless TKFMI_WANT_POSTERIORS       */
+} tkfmi_outputs;
+
+/* Version of this header's ABI (TKFMI_ABI_VERSION). */
+int tkfmi_abi_version(void);
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* tkfmi_last_error(void);
+
+/* Bytes of device workspace tkfmi_fb2d_batched needs for `prob`/`flags`. */
+size_t tkfmi_fb2d_workspace_bytes(const tkfmi_problem* prob, uint32_t flags);
+
+/*
+ * Batched 2D pair-HMM forward/backward + expected transition counts.
+ * One launch over the whole ragged batch.  Replaces, per pair:
+ *   forward_backward_2d      dp/hmm.py:1834-1973 (logP, posteriors, n)
+ *   forward_2d (logP only)   dp/hmm.py:1775-1801 (TKFMI_FORWARD_ONLY)
+ * and the vmapped callers batched_fb_2d_per_chi (train_pfam.py:1674-1741)
+ * and estep_batch_tkf92 (train/tkf92_svi_bw.py:263-334), without
+ * geometric-bin padding (dp/hmm.py:66-91): every pair runs at its real
+ * (Lx, Ly).  Emissions are looked up from per-group tables exactly as the
+ * reference's builders define them (dp/hmm.py:628-851).
+ *   n_trans[b] follows dp/hmm.py:1930-1971 (S,S); log_prob[b] is
+ *   logsumexp_k F[Lx,Ly,k] + T[k,E] (dp/hmm.py:1911-1913).
+ * Requirements: state 0 has type S, some state has type E, at most 15
+ * emitting (M/I/D) states, 0 <= Lx, Ly, Lx + Ly <= 32000.
+ */
+int tkfmi_fb2d_batched(const tkfmi_problem* prob, const tkfmi_outputs* out,
+                       void* workspace, size_t workspace_bytes,
+                       void* stream /* hipStream_t */, uint32_t flags);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TKFMI_H */

@@ -1,0 +1,144 @@
+"""GPU parity: libtkfmi batched forward/backward vs the FP64 oracle.
+
+Tolerance: 1e-10 relative (logP, counts, posteriors); entries that are
+tiny compared with the largest of their kind get an absolute floor of
+1e-12 x that maximum.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def _close(a, b, rtol=RTOL, floor=1e-12):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = max(np.abs(b).max(), 1e-300)
+    err = np.abs(a - b)
+    ok = err <= rtol * np.abs(b) + floor * scale
+    return bool(ok.all()), float((err / (np.abs(b) + floor * scale)).max())
+
+
+def _oracle_pair(log_trans, types, emit):
+    from oracle import dp2d
+    return dp2d.forward_backward(log_trans, np.asarray(types), emit)
+
+
+def _run(w, forward_only=False, want_post=True):
+    from tkfmi import hmm
+    return hmm.fb2d_batch(w.log_trans, w.state_types, w.xs, w.ys, w.log_tab, w.state_group,
+                          forward_only=forward_only, want_posteriors=want_post and not forward_only)
+
+
+def _tkf_workload(lengths_x, lengths_y, seed=0, ext=None):
+    from tkfmi import synth
+    import torch
+    rng = np.random.default_rng(seed)
+    w = synth.make_tkf(seed=seed, B=1, ext=ext)
+    w.xs = [rng.integers(0, 20, size=n) for n in lengths_x]
+    w.ys = [rng.integers(0, 20, size=n) for n in lengths_y]
+    w.ts = np.full(len(w.xs), w.ts[0])
+    return w
+
+
+def _tkf_oracle_emit(w, x, y):
+    from oracle import emissions
+    return emissions.pair_hmm_emissions(np.asarray(w.state_types), x, y,
+                                        w.params["sub"], w.params["pi"])
+
+
+EDGE_X = [0, 0, 3, 1, 2, 37, 64, 65, 70, 5, 130]
+EDGE_Y = [0, 4, 0, 1, 2, 64, 37, 3, 1, 140, 96]
+
+
+@pytest.mark.parametrize("ext", [None, 0.6])
+def test_tkf_edge_lengths_match_oracle(ext):
+    w = _tkf_workload(EDGE_X, EDGE_Y, seed=3, ext=ext)
+    res = _run(w)
+    lt = w.log_trans.numpy()
+    for b, (x, y) in enumerate(zip(w.xs, w.ys)):
+        print(f"pair {b} Lx={len(x)} Ly={len(y)}", flush=True)
+        lp_o, post_o, n_o, _ = _oracle_pair(lt, w.state_types, _tkf_oracle_emit(w, x, y))
+        lp = float(res["log_p"][b])
+        assert abs(lp - lp_o) <= RTOL * abs(lp_o), (b, lp, lp_o)
+        ok, worst = _close(res["n"][b].cpu().numpy(), n_o)
+        assert ok, (b, worst)
+        ok, worst = _close(res["posteriors"][b].cpu().numpy(), post_o)
+        assert ok, (b, worst)
+
+
+def test_tkf91_forward_only_matches_oracle():
+    from oracle import dp2d
+    w = _tkf_workload([10, 50, 150, 0], [12, 47, 160, 9], seed=5)
+    res = _run(w, forward_only=True)
+    assert "n" not in res
+    lt = w.log_trans.numpy()
+    for b, (x, y) in enumerate(zip(w.xs, w.ys)):
+        lp_o, _ = dp2d.forward(lt, np.asarray(w.state_types), _tkf_oracle_emit(w, x, y))
+        assert abs(float(res["log_p"][b]) - lp_o) <= RTOL * abs(lp_o)
+
+
+def _mixdom_oracle_emit(w, b, x, y):
+    from oracle import emissions
+    p = w.params
+    return emissions.pair_hmm_emissions_per_class(np.asarray(w.state_types), x, y,
+                                                  p["class_subs"][b], p["class_pi"],
+                                                  p["class_dist"], p["K"], p["F"])
+
+
+@pytest.mark.parametrize("K,F,C", [(3, 1, 1), (3, 1, 2), (1, 3, 2), (2, 1, 3)])
+def test_mixdom2_matches_oracle(K, F, C):
+    from tkfmi import synth
+    lengths = [0, 1, 31, 32, 33, 45, 90, 120]
+    w = synth.make_mixdom2(seed=11 + K + F + C, B=len(lengths), K=K, F=F, C=C, lengths=lengths)
+    # include empty / very short descendants
+    w.ys[0] = w.ys[0][:0]
+    w.ys[1] = w.ys[1][:3]
+    res = _run(w)
+    for b, (x, y) in enumerate(zip(w.xs, w.ys)):
+        print(f"pair {b} Lx={len(x)} Ly={len(y)}", flush=True)
+        lt = w.log_trans[b].numpy()
+        lp_o, post_o, n_o, _ = _oracle_pair(lt, w.state_types, _mixdom_oracle_emit(w, b, x, y))
+        lp = float(res["log_p"][b])
+        assert abs(lp - lp_o) <= RTOL * abs(lp_o), (b, lp, lp_o)
+        ok, worst = _close(res["n"][b].cpu().numpy(), n_o)
+        assert ok, (b, worst)
+        ok, worst = _close(res["posteriors"][b].cpu().numpy(), post_o)
+        assert ok, (b, worst)
+
+
+def test_batched_equals_alone():
+    """A pair's results do not depend on the rest of the batch (bitwise)."""
+    from tkfmi import hmm, synth
+    w = synth.make_mixdom2(seed=4, B=12, lengths=[40, 200, 7, 90, 150, 33, 64, 65, 2, 180, 120, 99])
+    full = _run(w, want_post=False)
+    for b in (0, 3, 8):
+        alone = hmm.fb2d_batch(w.log_trans[b], w.state_types, [w.xs[b]], [w.ys[b]],
+                               w.log_tab[b], w.state_group)
+        assert float(alone["log_p"][0]) == float(full["log_p"][b])
+        assert np.array_equal(alone["n"][0].cpu().numpy(), full["n"][b].cpu().numpy())
+
+
+def test_count_identities_full_size():
+    """Size-independent identities at the benchmark's length range:
+    sum of counts into M/D states == Lx, into M/I states == Ly, start row sums
+    to 1, and flow conservation for body states."""
+    from tkfmi import synth
+    w = synth.make_mixdom2(seed=7, B=6, lengths=[800, 30, 400, 650, 200, 777])
+    res = _run(w, want_post=False)
+    types = np.asarray(w.state_types)
+    n = res["n"].cpu().numpy()
+    for b, (x, y) in enumerate(zip(w.xs, w.ys)):
+        nb = n[b]
+        into_x = nb[:, np.isin(types, (1, 3))].sum()
+        into_y = nb[:, np.isin(types, (1, 2))].sum()
+        assert abs(into_x - len(x)) <= 1e-9 * len(x) + 1e-9
+        assert abs(into_y - len(y)) <= 1e-9 * len(y) + 1e-9
+        assert abs(nb[0].sum() - 1.0) < 1e-12
+        body = np.isin(types, (1, 2, 3))
+        inflow = nb[:, body].sum(axis=0)
+        outflow = nb[body, :].sum(axis=1)
+        assert np.allclose(inflow, outflow, rtol=1e-10, atol=1e-10)
+        assert np.isfinite(float(res["log_p"][b]))

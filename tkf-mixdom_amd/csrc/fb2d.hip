@@ -1,0 +1,604 @@
+// tkfmi: 2D pair-HMM forward/backward + expected counts for gfx950 (MI355X).
+//
+// Reference algorithm: python/tkfmixdom/jax/dp/hmm.py (forward
+// hmm.py:1186-1292, backward hmm.py:1295-1437, counts hmm.py:1930-1971).
+// This file is an independent MI355X design, not a translation:
+//
+//  * One workgroup per pair (grid = B, pairs in decreasing-cost order so the
+//    hardware dispatcher performs longest-first scheduling).
+//  * A lattice cell is held by a group of GL lanes (GL = 16 for up to 15
+//    emitting states, GL = 4 for up to 3); lane k of the group holds state
+//    k, lane GL-1 carries the cell's binary exponent.  A wave therefore holds
+//    64/GL cells, a workgroup H = W*64/GL consecutive lattice rows (a band).
+//  * The band is swept by antidiagonals: at step t band-row R handles column
+//    t - R.  When a row reaches the end of its band it wraps to the same row
+//    of the next band, so the sweep never drains between bands; the band
+//    boundary row goes through a small row buffer in global memory.
+//  * Probability space with one exponent per cell: F(i,j,k) = m_k * 2^E(i,j).
+//    The transition contraction is sum_s m_s P[s,k]: m_s is broadcast within
+//    the lane group with DPP (row_newbcast / quad_perm), P[s,k] lives in the
+//    lane's registers.  No exp/log in the inner loop.
+//  * Forward pushes h = m . P to the type-shifted successor cells; the
+//    backward pulls g (emission x B of the successors) and accumulates the
+//    expected counts sum_c F[c,k] g_c[k'] in registers (15 per lane).
+//  * The forward lattice (mantissas + exponent, GL doubles = 128 B per cell
+//    for GL = 16) is stored to HBM row-major for the backward pass.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "tkfmi.h"
+
+namespace {
+
+constexpr int A_PAD = TKFMI_A_PAD;
+constexpr int TAB_SZ = TKFMI_TAB_SZ;
+constexpr int EXP_NONE = -(1 << 29);  // exponent of an all-zero cell
+constexpr int ST_S = 0, ST_M = 1, ST_I = 2, ST_D = 3, ST_E = 4;
+constexpr int MAX_SEQ_LDS = 32000;
+
+struct PairDesc {
+  int32_t pair;  // index in the caller's order
+  int32_t Lx, Ly, pad;
+  int64_t x_off, y_off;
+  int64_t lat_off;   // doubles, into the lattice region
+  int64_t row_off;   // doubles, into the row-buffer region
+  int64_t post_off;  // doubles, into the posterior output
+};
+
+struct DevParams {
+  int32_t S, nb, G, e_idx;
+  int32_t n_trans, n_tab, flags, smem_seq_off;
+  int32_t bidx[16];   // state index of lane k (-1: unused lane)
+  int32_t btype[16];  // type of lane k (0: unused)
+  int32_t bgrp[16];   // emission group of lane k
+  const double* log_trans;
+  const double* log_tab;
+  const uint8_t* x_cat;
+  const uint8_t* y_cat;
+  const PairDesc* desc;
+  double* lattice;
+  double* rowbuf;
+  double* out_logP;
+  double* out_n;
+  double* out_post;
+};
+
+// ---------------------------------------------------------------- DPP helpers
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = dpp_i32<CTRL>(__double2loint(v));
+  const int hi = dpp_i32<CTRL>(__double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+// Value of lane SRC of this lane's group.
+template <int GL, int SRC>
+__device__ __forceinline__ double gbcast(double v) {
+  if constexpr (GL == 16) {
+    return dpp_f64<0x150 + SRC>(v);  // row_newbcast:SRC
+  } else {
+    return dpp_f64<SRC | (SRC << 2) | (SRC << 4) | (SRC << 6)>(v);  // quad_perm
+  }
+}
+// Max over the lane group (every lane gets the result).
+template <int GL>
+__device__ __forceinline__ int gmax_i32(int v) {
+  v = max(v, dpp_i32<0xB1>(v));  // quad_perm [1,0,3,2]
+  v = max(v, dpp_i32<0x4E>(v));  // quad_perm [2,3,0,1]
+  if constexpr (GL == 16) {
+    v = max(v, dpp_i32<0x141>(v));  // row_half_mirror
+    v = max(v, dpp_i32<0x140>(v));  // row_mirror
+  }
+  return v;
+}
+// Sum over the lane group; the butterfly pairs are symmetric so every lane
+// obtains the bitwise-identical value.
+template <int GL>
+__device__ __forceinline__ double gsum_f64(double v) {
+  v = v + dpp_f64<0xB1>(v);
+  v = v + dpp_f64<0x4E>(v);
+  if constexpr (GL == 16) {
+    v = v + dpp_f64<0x141>(v);
+    v = v + dpp_f64<0x140>(v);
+  }
+  return v;
+}
+
+__device__ __forceinline__ int frexp_exp(double v) {
+  return __builtin_amdgcn_frexp_exp(v);
+}
+
+// h = sum_s bcast(m, s) * c[s], fixed order s = 0..NB-1.
+template <int GL, int NB, int... Is>
+__device__ __forceinline__ double push_dot(double m, const double (&c)[NB],
+                                           std::integer_sequence<int, Is...>) {
+  double acc = 0.0;
+  ((acc = fma(gbcast<GL, Is>(m), c[Is], acc)), ...);
+  return acc;
+}
+// Backward fused step: b = sum_s prow[s] * bcast(g, s), acc[s] += bcast(g,s)*fs.
+template <int GL, int NB, int... Is>
+__device__ __forceinline__ double pull_dot_count(double g, const double (&prow)[NB],
+                                                 double fs, double (&acc)[NB],
+                                                 std::integer_sequence<int, Is...>) {
+  double b = 0.0;
+  double tmp;
+  ((tmp = gbcast<GL, Is>(g), b = fma(prow[Is], tmp, b), acc[Is] = fma(tmp, fs, acc[Is])), ...);
+  return b;
+}
+
+// Normalise a per-lane value v * 2^e to the group: returns the mantissa and
+// sets E to the group exponent (max |m| in [0.5, 1)).
+template <int GL>
+__device__ __forceinline__ double group_normalise(double v, int e, int& E) {
+  const int le = (v != 0.0) ? e + frexp_exp(v) : EXP_NONE;
+  int En = gmax_i32<GL>(le);
+  if (En == EXP_NONE) En = 0;
+  E = En;
+  return (v != 0.0) ? ldexp(v, e - En) : 0.0;
+}
+
+// ------------------------------------------------------------------- kernel
+template <int GL, int NB, int W>
+__global__ __launch_bounds__(W * 64) void fb2d_kernel(DevParams P) {
+  static_assert(NB <= GL - 1, "exponent lane must be free");
+  constexpr int CPW = 64 / GL;
+  constexpr int H = CPW * W;
+  constexpr double LN2 = 0.69314718055994530942;
+  using Seq = std::make_integer_sequence<int, NB>;
+
+  extern __shared__ double smem[];
+  const int G = P.G;
+  double* s_tab = smem;                                   // G*TAB_SZ
+  double* s_hbuf = s_tab + G * TAB_SZ;                    // 2*H*GL (also reduction)
+  constexpr int HB = (2 * H * GL > W * GL * NB) ? 2 * H * GL : W * GL * NB;
+  double* s_misc = s_hbuf + HB;                           // 8
+  uint8_t* s_x = reinterpret_cast<uint8_t*>(smem) + P.smem_seq_off;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int R = wave * CPW + lane / GL;  // band row held by this lane group
+  const int k = lane % GL;               // state slot
+
+  const PairDesc pd = P.desc[blockIdx.x];
+  const int Lx = pd.Lx, Ly = pd.Ly, W1 = Ly + 1;
+  const int S = P.S;
+  uint8_t* s_y = s_x + ((Lx + 8) & ~7);
+
+  // ---- stage emission tables (probability space) and sequences in LDS
+  const double* ltab = P.log_tab + (P.n_tab > 1 ? (size_t)pd.pair * G * TAB_SZ : 0);
+  for (int q = tid; q < G * TAB_SZ; q += W * 64) s_tab[q] = exp(ltab[q]);
+  for (int q = tid; q < Lx; q += W * 64) s_x[q] = P.x_cat[pd.x_off + q];
+  for (int q = tid; q < Ly; q += W * 64) s_y[q] = P.y_cat[pd.y_off + q];
+  if (tid == 0) {  // dummy characters for empty sequences (never used)
+    if (Lx == 0) s_x[0] = 0;
+    if (Ly == 0) s_y[0] = 0;
+  }
+
+  // ---- this lane's state, its transitions and emission addressing
+  const double* lt = P.log_trans + (P.n_trans > 1 ? (size_t)pd.pair * S * S : 0);
+  const int my_state = (k < NB) ? P.bidx[k] : -1;
+  const int my_type = (my_state >= 0) ? P.btype[k] : 0;
+  const int e_idx = P.e_idx;
+  double pcol[NB];  // P[s -> me]
+#pragma unroll
+  for (int s = 0; s < NB; ++s)
+    pcol[s] = (my_state >= 0 && P.bidx[s] >= 0) ? exp(lt[P.bidx[s] * S + my_state]) : 0.0;
+  const double p_start = (my_state >= 0) ? exp(lt[my_state]) : 0.0;             // S -> me
+  const double p_end = (my_state >= 0) ? exp(lt[my_state * S + e_idx]) : 0.0;    // me -> E
+  const int ebase = P.bgrp[k < NB ? k : 0] * TAB_SZ +
+                    (my_type == ST_I ? A_PAD * A_PAD : (my_type == ST_D ? A_PAD * A_PAD + A_PAD : 0));
+  const int emx = (my_type == ST_M) ? A_PAD : (my_type == ST_D ? 1 : 0);
+  const int emy = (my_type == ST_M || my_type == ST_I) ? 1 : 0;
+  const bool want_lattice = !(P.flags & TKFMI_FORWARD_ONLY);
+  const bool want_counts = want_lattice && !(P.flags & TKFMI_NO_COUNTS) && P.out_n != nullptr;
+  const bool want_post = want_lattice && (P.flags & TKFMI_WANT_POSTERIORS) && P.out_post != nullptr;
+  double* out_n = want_counts ? P.out_n + (size_t)pd.pair * S * S : nullptr;
+
+  const int nbands = (Lx + H) / H;  // ceil((Lx+1)/H)
+  const int R_last = Lx - (nbands - 1) * H;
+  const int Pper = max(W1, H + 2);
+  const int T_steps = (nbands - 1) * Pper + Ly + R_last + 1;
+  double* lat = P.lattice + pd.lat_off;
+  double* rowb = P.rowbuf + pd.row_off;
+  __syncthreads();
+
+  // ======================================================== forward sweep
+  {
+    double hI = 0.0; int eI = EXP_NONE;  // own h of the previous step
+    double hM = 0.0; int eM = EXP_NONE;  // D-source of the previous step
+    for (int t = 0; t < T_steps; ++t) {
+      const int p = t - R;
+      const int m = (p >= 0) ? p / Pper : -1;
+      const int j = p - m * Pper;
+      const int i = m * H + R;
+      const bool valid = (p >= 0) && (m < nbands) && (j <= Ly) && (i <= Lx);
+      // D-source: h of (i-1, j)
+      double dV = 0.0; int dE = EXP_NONE;
+      if (valid && i >= 1) {
+        const double* src = (R > 0) ? s_hbuf + ((((t - 1) & 1) * H) + (R - 1)) * GL
+                                    : rowb + (size_t)j * GL;
+        dV = src[k];
+        dE = (int)src[GL - 1];
+      }
+      double sV; int sE; bool ok;
+      if (my_type == ST_M) { sV = hM; sE = eM; ok = valid && i >= 1 && j >= 1; }
+      else if (my_type == ST_I) { sV = hI; sE = eI; ok = valid && j >= 1; }
+      else { sV = dV; sE = dE; ok = valid && i >= 1 && my_type == ST_D; }
+      const int xi = valid ? s_x[i >= 1 ? i - 1 : 0] : 0;
+      const int yj = valid ? s_y[j >= 1 ? j - 1 : 0] : 0;
+      const double em = s_tab[ebase + xi * emx + yj * emy];
+      const double v = ok ? sV * em : 0.0;
+      int En;
+      double mv = group_normalise<GL>(v, sE, En);
+      const bool origin = valid && i == 0 && j == 0;
+      if (origin) { mv = 0.0; En = 0; }
+      if (want_lattice && valid)
+        lat[((size_t)i * W1 + j) * GL + k] = (k == GL - 1) ? (double)En : mv;
+      double h = push_dot<GL, NB>(mv, pcol, Seq{});
+      if (origin) h = p_start;
+      const double hout = (k == GL - 1) ? (double)En : h;
+      s_hbuf[((t & 1) * H + R) * GL + k] = hout;
+      if (valid && R == H - 1) rowb[(size_t)j * GL + k] = hout;
+      if (valid && i == Lx && j == Ly) {  // corner: log P and the E column
+        double tot = gsum_f64<GL>(mv * p_end);
+        double lp;
+        if (origin) {
+          tot = exp(lt[e_idx]);  // empty pair: S -> E
+          lp = lt[e_idx];
+        } else {
+          lp = log(tot) + (double)En * LN2;
+        }
+        const int ep = (tot > 0.0) ? En + frexp_exp(tot) : 0;
+        const double mp = (tot > 0.0) ? ldexp(tot, -frexp_exp(tot)) : 0.0;
+        if (k == 0) {
+          P.out_logP[pd.pair] = (tot > 0.0) ? lp : -1e30;
+          s_misc[0] = mp;
+          s_misc[1] = (double)ep;
+        }
+        if (want_counts && tot > 0.0) {
+          if (origin) {
+            if (k == 0) out_n[e_idx] = 1.0;
+          } else if (my_state >= 0) {
+            out_n[my_state * S + e_idx] = mv * p_end / tot;
+          }
+        }
+      }
+      hI = h; eI = En;
+      hM = dV; eM = dE;
+      __syncthreads();
+    }
+  }
+  if (!want_lattice) return;
+
+  // ======================================================= backward sweep
+  const double mP = s_misc[0];
+  const int EP = (int)s_misc[1];
+  if (!(mP > 0.0)) return;  // impossible pair: counts/posteriors stay zero
+  const double inv_mP = 1.0 / mP;
+  double prow[NB];  // P[me -> s]
+#pragma unroll
+  for (int s = 0; s < NB; ++s)
+    prow[s] = (my_state >= 0 && P.bidx[s] >= 0) ? exp(lt[my_state * S + P.bidx[s]]) : 0.0;
+  double cacc[NB];
+#pragma unroll
+  for (int s = 0; s < NB; ++s) cacc[s] = 0.0;
+  double* post = want_post ? P.out_post + pd.post_off : nullptr;
+  __syncthreads();
+  {
+    double qI = 0.0; int eI = EXP_NONE;  // own q of the previous step
+    double qM = 0.0; int eM = EXP_NONE;  // D-successor of the previous step
+    for (int t = 0; t < T_steps; ++t) {
+      const int p = t - R;
+      const int mb = (p >= 0) ? p / Pper : -1;
+      const int c = p - mb * Pper;
+      const int j = Ly - c;
+      const int i = Lx - mb * H - R;
+      const bool valid = (p >= 0) && (mb < nbands) && (c <= Ly) && (i >= 0);
+      // D-successor: q of (i+1, j)
+      double dV = 0.0; int dE = EXP_NONE;
+      if (valid && i < Lx) {
+        const double* src = (R > 0) ? s_hbuf + ((((t - 1) & 1) * H) + (R - 1)) * GL
+                                    : rowb + (size_t)j * GL;
+        dV = src[k];
+        dE = (int)src[GL - 1];
+      }
+      double sV; int sE; bool ok;
+      if (my_type == ST_M) { sV = qM; sE = eM; ok = valid && i < Lx && j < Ly; }
+      else if (my_type == ST_I) { sV = qI; sE = eI; ok = valid && j < Ly; }
+      else { sV = dV; sE = dE; ok = valid && i < Lx && my_type == ST_D; }
+      int Eg;
+      const double mg = group_normalise<GL>(ok ? sV : 0.0, sE, Eg);
+      // forward value of this cell
+      const double fv = valid ? lat[((size_t)i * W1 + j) * GL + k] : 0.0;
+      const int EF = (int)gbcast<GL, GL - 1>(fv);
+      const double fs = (my_state >= 0 && fv != 0.0) ? ldexp(fv, EF + Eg - EP) * inv_mP : 0.0;
+      double bsum = pull_dot_count<GL, NB>(mg, prow, fs, cacc, Seq{});
+      const bool corner = valid && i == Lx && j == Ly;
+      int EBraw = Eg;
+      if (corner) { bsum = p_end; EBraw = 0; }
+      int EB;
+      const double mB = group_normalise<GL>(bsum, EBraw, EB);
+      if (valid && i == 0 && j == 0) {  // start state: S -> body counts, post[0,0,S]
+        const double scale = ldexp(inv_mP, Eg - EP);
+        if (want_counts && my_state >= 0) out_n[my_state] = p_start * mg * scale;
+        const double b00 = gsum_f64<GL>(p_start * mg);
+        if (want_post && k == 0) post[0] = corner ? 1.0 : b00 * scale;
+      }
+      if (want_post && valid && my_state >= 0)
+        post[((size_t)i * W1 + j) * S + my_state] = fv * ldexp(mB * inv_mP, EF + EB - EP);
+      // q = e(i,j,k) * B(i,j,k) for the predecessors
+      const int xi = valid ? s_x[i >= 1 ? i - 1 : 0] : 0;
+      const int yj = valid ? s_y[j >= 1 ? j - 1 : 0] : 0;
+      const bool eok = (my_type == ST_M) ? (i >= 1 && j >= 1)
+                     : (my_type == ST_I) ? (j >= 1)
+                     : (my_type == ST_D) ? (i >= 1) : false;
+      const double q = eok ? s_tab[ebase + xi * emx + yj * emy] * mB : 0.0;
+      const double qout = (k == GL - 1) ? (double)EB : q;
+      s_hbuf[((t & 1) * H + R) * GL + k] = qout;
+      if (valid && R == H - 1) rowb[(size_t)j * GL + k] = qout;
+      qI = q; eI = EB;
+      qM = dV; eM = dE;
+      __syncthreads();
+    }
+  }
+  if (!want_counts) return;
+
+  // ---- reduce the count accumulators over the band rows (fixed order)
+#pragma unroll
+  for (int s = 0; s < NB; ++s) {
+    double v = cacc[s];
+#pragma unroll
+    for (int off = GL; off < 64; off <<= 1) v += __shfl_xor(v, off);
+    cacc[s] = v;
+  }
+  double* s_red = s_hbuf;  // [W][GL][NB]
+  if (lane < GL) {
+#pragma unroll
+    for (int s = 0; s < NB; ++s) s_red[(wave * GL + lane) * NB + s] = cacc[s];
+  }
+  __syncthreads();
+  for (int q = tid; q < NB * NB; q += W * 64) {
+    const int kk = q / NB, s = q % NB;
+    const int a = P.bidx[kk], b = P.bidx[s];
+    if (a < 0 || b < 0) continue;
+    double acc = 0.0;
+    for (int w = 0; w < W; ++w) acc += s_red[(w * GL + kk) * NB + s];
+    const double pr = exp(lt[a * S + b]);
+    out_n[a * S + b] = (pr > 0.0) ? pr * acc : 0.0;
+  }
+}
+
+// ------------------------------------------------------------- host side
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+struct Plan {
+  int GL = 0, NB = 0, W = 0, H = 0;
+  int S = 0, nb = 0, e_idx = -1;
+  int bidx[16], btype[16], bgrp[16];
+  size_t desc_bytes = 0, lat_doubles = 0, row_doubles = 0;
+  int max_lx = 0, max_ly = 0;
+};
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+int make_plan(const tkfmi_problem* p, uint32_t flags, Plan& pl) {
+  if (!p) return fail(TKFMI_EINVAL, "null problem");
+  if (p->B < 0) return fail(TKFMI_EINVAL, "B < 0");
+  if (p->S < 2 || p->S > 4096) return fail(TKFMI_EINVAL, "S out of range");
+  if (!p->state_types || !p->state_group) return fail(TKFMI_EINVAL, "state_types/state_group missing");
+  if (p->n_groups < 1 || p->n_groups > 32) return fail(TKFMI_EINVAL, "n_groups must be in 1..32");
+  if (p->state_types[0] != ST_S) return fail(TKFMI_EINVAL, "state 0 must have type S");
+  pl.S = p->S;
+  pl.nb = 0;
+  for (int s = 0; s < p->S; ++s) {
+    const int ty = p->state_types[s];
+    if (ty < ST_S || ty > ST_E) return fail(TKFMI_EINVAL, "bad state type");
+    if (ty == ST_E && pl.e_idx < 0) pl.e_idx = s;
+    if (ty == ST_M || ty == ST_I || ty == ST_D) {
+      if (pl.nb >= 15) return fail(TKFMI_EINVAL, "more than 15 emitting states is not supported by this kernel");
+      const int g = p->state_group[s];
+      if (g < 0 || g >= p->n_groups) return fail(TKFMI_EINVAL, "state_group out of range");
+      pl.bidx[pl.nb] = s;
+      pl.btype[pl.nb] = ty;
+      pl.bgrp[pl.nb] = g;
+      ++pl.nb;
+    }
+  }
+  if (pl.e_idx < 0) return fail(TKFMI_EINVAL, "no state of type E");
+  if (pl.nb <= 3) { pl.GL = 4; pl.NB = 3; pl.W = 4; }
+  else { pl.GL = 16; pl.NB = 15; pl.W = 8; }
+  pl.H = pl.W * 64 / pl.GL;
+  for (int s = pl.nb; s < 16; ++s) { pl.bidx[s] = -1; pl.btype[s] = 0; pl.bgrp[s] = 0; }
+  if (p->n_trans != 1 && p->n_trans != p->B) return fail(TKFMI_EINVAL, "n_trans must be 1 or B");
+  if (p->n_tab != 1 && p->n_tab != p->B) return fail(TKFMI_EINVAL, "n_tab must be 1 or B");
+  if (p->B > 0 && (!p->h_x_len || !p->h_y_len || !p->h_x_off || !p->h_y_off))
+    return fail(TKFMI_EINVAL, "host length/offset arrays missing");
+  pl.lat_doubles = 0;
+  pl.row_doubles = 0;
+  for (int b = 0; b < p->B; ++b) {
+    const int lx = p->h_x_len[b], ly = p->h_y_len[b];
+    if (lx < 0 || ly < 0) return fail(TKFMI_EINVAL, "negative length");
+    if (lx + ly > MAX_SEQ_LDS) return fail(TKFMI_EINVAL, "Lx + Ly exceeds 32000");
+    pl.max_lx = std::max(pl.max_lx, lx);
+    pl.max_ly = std::max(pl.max_ly, ly);
+    if (!(flags & TKFMI_FORWARD_ONLY))
+      pl.lat_doubles += (size_t)(lx + 1) * (ly + 1) * pl.GL;
+    pl.row_doubles += (size_t)(ly + 1) * pl.GL;
+  }
+  pl.desc_bytes = align_up(sizeof(PairDesc) * (size_t)std::max(p->B, 1), 256);
+  return TKFMI_OK;
+}
+
+size_t plan_ws_bytes(const Plan& pl) {
+  return pl.desc_bytes + align_up(pl.lat_doubles * 8, 256) + align_up(pl.row_doubles * 8, 256);
+}
+
+// Pinned staging buffer for the per-pair descriptors (reused across calls;
+// an event guards reuse until the previous upload has completed).
+struct Stage {
+  void* buf = nullptr;
+  size_t cap = 0;
+  hipEvent_t ev = nullptr;
+  bool pending = false;
+};
+thread_local Stage g_stage;
+
+template <int GL, int NB, int W>
+int launch(const Plan& pl, DevParams dp, int B, size_t smem, hipStream_t stream) {
+  auto kern = fb2d_kernel<GL, NB, W>;
+  static std::once_flag once;
+  static hipError_t attr_err = hipSuccess;
+  std::call_once(once, [&] {
+    attr_err = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   160 * 1024);
+  });
+  if (attr_err != hipSuccess) return fail(TKFMI_EHIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(attr_err));
+  hipLaunchKernelGGL(kern, dim3(B), dim3(W * 64), smem, stream, dp);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(TKFMI_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  (void)pl;
+  return TKFMI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tkfmi_abi_version(void) { return TKFMI_ABI_VERSION; }
+
+const char* tkfmi_last_error(void) { return g_err.c_str(); }
+
+size_t tkfmi_fb2d_workspace_bytes(const tkfmi_problem* prob, uint32_t flags) {
+  Plan pl;
+  if (make_plan(prob, flags, pl) != TKFMI_OK) return 0;
+  return plan_ws_bytes(pl);
+}
+
+int tkfmi_fb2d_batched(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws,
+                       size_t ws_bytes, void* stream_v, uint32_t flags) {
+  g_err.clear();
+  Plan pl;
+  int rc = make_plan(p, flags, pl);
+  if (rc != TKFMI_OK) return rc;
+  if (!o || !o->log_prob) return fail(TKFMI_EINVAL, "outputs.log_prob missing");
+  if ((flags & TKFMI_WANT_POSTERIORS) && !(flags & TKFMI_FORWARD_ONLY) && !o->posteriors)
+    return fail(TKFMI_EINVAL, "TKFMI_WANT_POSTERIORS without a posterior buffer");
+  if (p->B == 0) return TKFMI_OK;
+  if (!p->log_trans || !p->log_tab || !p->x_cat || !p->y_cat)
+    return fail(TKFMI_EINVAL, "device input pointer missing");
+  const size_t need = plan_ws_bytes(pl);
+  if (!ws || ws_bytes < need) return fail(TKFMI_EWORKSPACE, "workspace too small: need " + std::to_string(need));
+  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_v);
+
+  // descriptors in decreasing-cost order
+  const int B = p->B;
+  std::vector<int> order(B);
+  for (int b = 0; b < B; ++b) order[b] = b;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const int64_t ca = (int64_t)(p->h_x_len[a] + 1) * (p->h_y_len[a] + 1);
+    const int64_t cb = (int64_t)(p->h_x_len[b] + 1) * (p->h_y_len[b] + 1);
+    return ca > cb;
+  });
+  std::vector<int64_t> post_off(B), lat_off(B), row_off(B);
+  int64_t acc_post = 0, acc_lat = 0, acc_row = 0;
+  for (int b = 0; b < B; ++b) {  // caller-order offsets for outputs
+    post_off[b] = acc_post;
+    acc_post += (int64_t)(p->h_x_len[b] + 1) * (p->h_y_len[b] + 1) * pl.S;
+    lat_off[b] = acc_lat;
+    if (!(flags & TKFMI_FORWARD_ONLY)) acc_lat += (int64_t)(p->h_x_len[b] + 1) * (p->h_y_len[b] + 1) * pl.GL;
+    row_off[b] = acc_row;
+    acc_row += (int64_t)(p->h_y_len[b] + 1) * pl.GL;
+  }
+  const size_t dbytes = sizeof(PairDesc) * (size_t)B;
+  Stage& st = g_stage;
+  if (st.pending) {
+    hipError_t e = hipEventSynchronize(st.ev);
+    if (e != hipSuccess) return fail(TKFMI_EHIP, std::string("hipEventSynchronize: ") + hipGetErrorString(e));
+    st.pending = false;
+  }
+  if (st.cap < dbytes) {
+    if (st.buf) (void)hipHostFree(st.buf);
+    st.buf = nullptr;
+    st.cap = 0;
+    hipError_t e = hipHostMalloc(&st.buf, dbytes, hipHostMallocDefault);
+    if (e != hipSuccess) return fail(TKFMI_EHIP, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    st.cap = dbytes;
+  }
+  if (!st.ev) {
+    hipError_t e = hipEventCreateWithFlags(&st.ev, hipEventDisableTiming);
+    if (e != hipSuccess) return fail(TKFMI_EHIP, std::string("hipEventCreate: ") + hipGetErrorString(e));
+  }
+  PairDesc* hd = static_cast<PairDesc*>(st.buf);
+  for (int q = 0; q < B; ++q) {
+    const int b = order[q];
+    PairDesc d;
+    d.pair = b;
+    d.Lx = p->h_x_len[b];
+    d.Ly = p->h_y_len[b];
+    d.pad = 0;
+    d.x_off = p->h_x_off[b];
+    d.y_off = p->h_y_off[b];
+    d.lat_off = lat_off[b];
+    d.row_off = row_off[b];
+    d.post_off = post_off[b];
+    hd[q] = d;
+  }
+  char* wsb = static_cast<char*>(ws);
+  PairDesc* ddesc = reinterpret_cast<PairDesc*>(wsb);
+  double* lattice = reinterpret_cast<double*>(wsb + pl.desc_bytes);
+  double* rowbuf = reinterpret_cast<double*>(wsb + pl.desc_bytes + align_up(pl.lat_doubles * 8, 256));
+  hipError_t e = hipMemcpyAsync(ddesc, hd, dbytes, hipMemcpyHostToDevice, stream);
+  if (e != hipSuccess) return fail(TKFMI_EHIP, std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
+  e = hipEventRecord(st.ev, stream);
+  if (e != hipSuccess) return fail(TKFMI_EHIP, std::string("hipEventRecord: ") + hipGetErrorString(e));
+  st.pending = true;
+
+  DevParams dp;
+  memset(&dp, 0, sizeof(dp));
+  dp.S = pl.S;
+  dp.nb = pl.nb;
+  dp.G = p->n_groups;
+  dp.e_idx = pl.e_idx;
+  dp.n_trans = p->n_trans;
+  dp.n_tab = p->n_tab;
+  dp.flags = (int32_t)flags;
+  for (int s = 0; s < 16; ++s) { dp.bidx[s] = pl.bidx[s]; dp.btype[s] = pl.btype[s]; dp.bgrp[s] = pl.bgrp[s]; }
+  dp.log_trans = p->log_trans;
+  dp.log_tab = p->log_tab;
+  dp.x_cat = p->x_cat;
+  dp.y_cat = p->y_cat;
+  dp.desc = ddesc;
+  dp.lattice = lattice;
+  dp.rowbuf = rowbuf;
+  dp.out_logP = o->log_prob;
+  dp.out_n = (flags & TKFMI_NO_COUNTS) ? nullptr : o->n_trans;
+  dp.out_post = o->posteriors;
+  const int H = pl.H;
+  const int HB = std::max(2 * H * pl.GL, pl.W * pl.GL * pl.NB);
+  const size_t seq_off = (size_t)(p->n_groups * TAB_SZ + HB + 8) * 8;
+  dp.smem_seq_off = (int32_t)seq_off;
+  const size_t smem = seq_off + (size_t)((pl.max_lx + 8) & ~7) + (size_t)((pl.max_ly + 8) & ~7) + 16;
+  if (smem > 160 * 1024) return fail(TKFMI_EINVAL, "LDS budget exceeded (sequences too long for the emission groups)");
+  if (pl.GL == 4) return launch<4, 3, 4>(pl, dp, B, smem, stream);
+  return launch<16, 15, 8>(pl, dp, B, smem, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,201 @@
+"""ctypes binding of libtkfmi.so (the C ABI declared in include/tkfmi.h).
+
+The shared library is built in-tree (``tkfmi/_lib/libtkfmi.so``) by
+``tkfmi._build.build()``.  There is no fallback: if the library is missing,
+:func:`lib` raises, so a GPU run can never silently use another code path.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libtkfmi.so")
+
+ABI_VERSION = 1
+FORWARD_ONLY = 1
+WANT_POSTERIORS = 2
+NO_COUNTS = 4
+A_PAD = 21
+TAB_SZ = A_PAD * A_PAD + 2 * A_PAD
+
+_c_p = ctypes.c_void_p
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [
+        ("B", ctypes.c_int32),
+        ("S", ctypes.c_int32),
+        ("state_types", _c_p),
+        ("state_group", _c_p),
+        ("n_groups", ctypes.c_int32),
+        ("n_trans", ctypes.c_int32),
+        ("log_trans", _c_p),
+        ("n_tab", ctypes.c_int32),
+        ("log_tab", _c_p),
+        ("x_cat", _c_p),
+        ("y_cat", _c_p),
+        ("h_x_off", _c_p),
+        ("h_y_off", _c_p),
+        ("h_x_len", _c_p),
+        ("h_y_len", _c_p),
+    ]
+
+
+class Outputs(ctypes.Structure):
+    _fields_ = [
+        ("log_prob", _c_p),
+        ("n_trans", _c_p),
+        ("posteriors", _c_p),
+    ]
+
+
+_lib = None
+_lock = threading.Lock()
+
+# Every symbol include/tkfmi.h declares (checked by the CPU test-suite).
+EXPORTED = ("tkfmi_abi_version", "tkfmi_last_error",
+            "tkfmi_fb2d_workspace_bytes", "tkfmi_fb2d_batched")
+
+
+class TkfmiError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libtkfmi.so (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise TkfmiError(
+                    f"libtkfmi.so not found at {LIB_PATH}; build it with "
+                    "tkfmi._build.build() (hipcc, gfx950)")
+            L = ctypes.CDLL(LIB_PATH)
+            L.tkfmi_abi_version.restype = ctypes.c_int
+            L.tkfmi_last_error.restype = ctypes.c_char_p
+            L.tkfmi_fb2d_workspace_bytes.restype = ctypes.c_size_t
+            L.tkfmi_fb2d_workspace_bytes.argtypes = [ctypes.POINTER(Problem), ctypes.c_uint32]
+            L.tkfmi_fb2d_batched.restype = ctypes.c_int
+            L.tkfmi_fb2d_batched.argtypes = [ctypes.POINTER(Problem), ctypes.POINTER(Outputs),
+                                             _c_p, ctypes.c_size_t, _c_p, ctypes.c_uint32]
+            v = L.tkfmi_abi_version()
+            if v != ABI_VERSION:
+                raise TkfmiError(f"libtkfmi ABI {v} != expected {ABI_VERSION}")
+            _lib = L
+        return _lib
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class PackedPairs:
+    """A ragged batch of (x, y) code sequences resident on the device."""
+
+    def __init__(self, xs, ys, device):
+        if len(xs) != len(ys):
+            raise ValueError("xs and ys must have the same length")
+        self.B = len(xs)
+        xs = [normalise_codes(x) for x in xs]
+        ys = [normalise_codes(y) for y in ys]
+        self.x_len = np.array([len(x) for x in xs], dtype=np.int32)
+        self.y_len = np.array([len(y) for y in ys], dtype=np.int32)
+        self.x_off = np.zeros(self.B, dtype=np.int64)
+        self.y_off = np.zeros(self.B, dtype=np.int64)
+        if self.B:
+            self.x_off[1:] = np.cumsum(self.x_len[:-1], dtype=np.int64)
+            self.y_off[1:] = np.cumsum(self.y_len[:-1], dtype=np.int64)
+        xcat = np.concatenate(xs + [np.zeros(1, np.uint8)]).astype(np.uint8)
+        ycat = np.concatenate(ys + [np.zeros(1, np.uint8)]).astype(np.uint8)
+        self.x_cat = torch.from_numpy(xcat).to(device)
+        self.y_cat = torch.from_numpy(ycat).to(device)
+        self.cells = (self.x_len.astype(np.int64) + 1) * (self.y_len.astype(np.int64) + 1)
+
+
+def normalise_codes(seq):
+    """Residue codes as uint8 in 0..20.  The reference indexes its padded
+    (21,)-tables with jnp gathers, which clamp indices above the table; a
+    negative index wraps once (Python semantics)."""
+    a = np.asarray(seq.detach().cpu() if isinstance(seq, torch.Tensor) else seq).astype(np.int64).ravel()
+    a = np.where(a < 0, a + A_PAD, a)
+    return np.clip(a, 0, A_PAD - 1).astype(np.uint8)
+
+
+def fb2d_batched(log_trans, state_types, state_group, log_tab, pairs, *,
+                 forward_only=False, want_posteriors=False, want_counts=True):
+    """Run the batched forward/backward kernel.
+
+    log_trans: float64 device tensor (S,S) or (B,S,S) of log transitions.
+    state_types, state_group: int sequences of length S.
+    log_tab: float64 device tensor (G, TAB_SZ) or (B, G, TAB_SZ).
+    pairs: PackedPairs.
+    Returns (log_prob (B,), n (B,S,S) or None, posteriors (flat) or None).
+    """
+    L = lib()
+    dev = log_trans.device
+    if dev.type != "cuda":
+        raise TkfmiError("tkfmi kernels need a CUDA/HIP device tensor")
+    S = int(log_trans.shape[-1])
+    B = pairs.B
+    st = np.ascontiguousarray(np.asarray(state_types, dtype=np.int32))
+    sg = np.ascontiguousarray(np.asarray(state_group, dtype=np.int32))
+    if st.shape != (S,) or sg.shape != (S,):
+        raise ValueError("state_types/state_group must have length S")
+    log_trans = log_trans.to(torch.float64).contiguous()
+    log_tab = log_tab.to(torch.float64).contiguous()
+    n_trans = 1 if log_trans.dim() == 2 else int(log_trans.shape[0])
+    if log_tab.dim() == 2:
+        n_tab, G = 1, int(log_tab.shape[0])
+    else:
+        n_tab, G = int(log_tab.shape[0]), int(log_tab.shape[1])
+    if log_tab.shape[-1] != TAB_SZ:
+        raise ValueError(f"log_tab last dim must be {TAB_SZ}")
+    flags = 0
+    if forward_only:
+        flags |= FORWARD_ONLY
+    if want_posteriors and not forward_only:
+        flags |= WANT_POSTERIORS
+    if not want_counts:
+        flags |= NO_COUNTS
+    prob = Problem(B, S, _np_ptr(st), _np_ptr(sg), G, n_trans, _ptr(log_trans), n_tab,
+                   _ptr(log_tab), _ptr(pairs.x_cat), _ptr(pairs.y_cat),
+                   _np_ptr(pairs.x_off), _np_ptr(pairs.y_off),
+                   _np_ptr(pairs.x_len), _np_ptr(pairs.y_len))
+    ws_bytes = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(prob), flags)
+    if ws_bytes == 0 and B > 0:
+        # size query failed: run the call anyway to obtain the message
+        rc = L.tkfmi_fb2d_batched(ctypes.byref(prob), ctypes.byref(Outputs()), None, 0, None, flags)
+        raise TkfmiError(L.tkfmi_last_error().decode() or f"tkfmi error {rc}")
+    logp = torch.empty(B, dtype=torch.float64, device=dev)
+    n = None
+    if want_counts and not forward_only:
+        n = torch.zeros((B, S, S), dtype=torch.float64, device=dev)
+    post = None
+    if want_posteriors and not forward_only:
+        post = torch.zeros(int(pairs.cells.sum()) * S + 1, dtype=torch.float64, device=dev)
+    ws = torch.empty(max(int(ws_bytes), 1), dtype=torch.uint8, device=dev)
+    out = Outputs(_ptr(logp), _ptr(n), _ptr(post))
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    rc = L.tkfmi_fb2d_batched(ctypes.byref(prob), ctypes.byref(out), _ptr(ws),
+                              ctypes.c_size_t(int(ws_bytes)), ctypes.c_void_p(stream), flags)
+    if rc != 0:
+        raise TkfmiError(L.tkfmi_last_error().decode() or f"tkfmi error {rc}")
+    return logp, n, post
+
+
+def split_posteriors(post_flat, pairs, S):
+    """Views (Lx+1, Ly+1, S) of each pair in a flat posterior buffer."""
+    out = []
+    off = 0
+    for b in range(pairs.B):
+        c = int(pairs.cells[b]) * S
+        out.append(post_flat[off:off + c].view(int(pairs.x_len[b]) + 1, int(pairs.y_len[b]) + 1, S))
+        off += c
+    return out
