@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Benchmark: batched MixDom2 K=3 forward+backward+counts on one MI355X.
+
+One step = one launch of the batched kernel over the whole c1 workload
+(1024 MixDom2 K=3 F=1 C=1 pairs, 17 states, ancestor lengths
+round(LogNormal(ln 200, 0.5)) clipped to [30, 800], t ~ U(0.05, 2)) with all
+inputs already resident in HBM.  Prints one JSON line.
+
+  value       = sum_pairs (Lx+1)(Ly+1) S / step time         (cell-states/s)
+  roofline    = HBM-bound: 16 B of lattice traffic per cell-state
+                (write + read of the forward lattice), divided by the
+                measured kernel time, against 8 TB/s
+  cpu_baseline= the C restatement of the reference recursion (log space,
+                OpenMP over pairs) on a bounded sample of the same pairs
+  plain       = the per-pair drop-in (one launch per pair, as the reference's
+                per-pair forward_backward_2d call) on a sample of pairs
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "tkf-mixdom_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "fwd+bwd+counts DP cell-states/s, MixDom2 K=3 batched vs plain; p99 latency"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_STATE = 16.0
+
+
+def _log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, seconds=15.0, threads=None):
+    """Time the C oracle on a bounded sample of the workload's pairs."""
+    import numpy as np
+    from oracle import cfb2d
+    threads = threads or min(16, os.cpu_count() or 1)
+    rng = np.random.default_rng(123)
+    order = rng.permutation(w.B)
+    lt = w.log_trans.numpy()
+    tab = w.log_tab.numpy()
+    # calibrate on a few pairs, then size the sample to ~`seconds`
+    done_cells, elapsed, used = 0, 0.0, []
+    for chunk in np.array_split(order, max(1, w.B // threads)):
+        if elapsed > seconds:
+            break
+        idx = list(chunk)
+        t0 = time.perf_counter()
+        cfb2d.fb2d_batch(lt[idx], w.state_types, w.state_group, tab[idx],
+                         [w.xs[i] for i in idx], [w.ys[i] for i in idx], nthreads=threads)
+        elapsed += time.perf_counter() - t0
+        done_cells += int(sum((len(w.xs[i]) + 1) * (len(w.ys[i]) + 1) for i in idx)) * w.S
+        used += idx
+    return {"value": done_cells / elapsed, "unit": "cell-states/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{len(used)} of {w.B} c1 pairs ({done_cells:.3e} cell-states, "
+                      f"{elapsed:.1f} s, C log-space restatement, OpenMP over pairs)"}
+
+
+def plain_per_pair(w, n_pairs=48):
+    """Per-pair drop-in: one launch per pair (the reference calls
+    forward_backward_2d per pair), timed over a sample of pairs."""
+    import numpy as np
+    import torch
+    from tkfmi import _abi
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(7)
+    idx = rng.choice(w.B, size=min(n_pairs, w.B), replace=False)
+    runners = []
+    for i in idx:
+        pp = _abi.PackedPairs([w.xs[i]], [w.ys[i]], dev)
+        runners.append(_abi.BatchRunner(w.log_trans[i].to(dev), w.state_types, w.state_group,
+                                        w.log_tab[i].to(dev), pp))
+    for r in runners[:4]:
+        r.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in runners:
+        r.run()
+        torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    cells = int(sum((len(w.xs[i]) + 1) * (len(w.ys[i]) + 1) for i in idx)) * w.S
+    return {"value": cells / el, "unit": "cell-states/s", "pairs": len(idx),
+            "ms_per_pair": 1e3 * el / len(idx)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--pairs", type=int, default=1024)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(pairs shard across GPUs by cost; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from tkfmi import _abi, synth
+
+    _log("generating c1 workload")
+    w = synth.make_mixdom2(seed=1, B=args.pairs)
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    pairs = _abi.PackedPairs(w.xs, w.ys, dev)
+    runner = _abi.BatchRunner(w.log_trans.to(dev), w.state_types, w.state_group,
+                              w.log_tab.to(dev), pairs)
+    cell_states = w.cell_states()
+    _log(f"pairs={w.B} S={w.S} cell-states={cell_states:.4e} workspace={runner.ws_bytes/2**30:.2f} GiB")
+    for _ in range(args.warmup):
+        runner.run()
+    torch.cuda.synchronize()
+    # per-launch kernel time (HIP events on the launch stream)
+    stream = torch.cuda.current_stream(dev)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    for i in range(args.steps):
+        runner.n.zero_()
+        evs[i][0].record(stream)
+        runner.run(zero_outputs=False)
+        evs[i][1].record(stream)
+    torch.cuda.synchronize()
+    kern_ms = [a.elapsed_time(b) for a, b in evs]
+    # whole-step wall time
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        runner.run()
+    torch.cuda.synchronize()
+    step_ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    logp = runner.log_p.cpu().numpy()
+    n = runner.n.cpu().numpy()
+    assert np.all(np.isfinite(logp)), "non-finite log-likelihood"
+    value = cell_states / (step_ms / 1e3)
+    kavg = float(np.mean(kern_ms))
+    achieved = BYTES_PER_CELL_STATE * cell_states / (kavg / 1e3) / 1e9
+    _log(f"step {step_ms:.3f} ms  kernel avg {kavg:.3f} ms  min {min(kern_ms):.3f} ms")
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-states/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": step_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "c1: MixDom2 K=3 F=1 C=1 (17 states) forward+backward+expected "
+                               "counts, 1024 uneven pairs, LogNormal(ln200,0.5) lengths in [30,800], "
+                               "t~U(0.05,2), one launch",
+                   "pairs": w.B, "states": w.S, "cell_states": cell_states,
+                   "sum_logp": float(logp.sum()), "sum_counts": float(n.sum())},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "fb2d_kernel<16,15,8>", "kernel_ms": kavg},
+    }
+    if not args.no_plain:
+        _log("timing the per-pair drop-in")
+        pl = plain_per_pair(w)
+        out["plain_per_pair"] = pl
+        out["batched_over_plain"] = value / pl["value"]
+    if not args.no_cpu:
+        _log("timing the CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(w, seconds=args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

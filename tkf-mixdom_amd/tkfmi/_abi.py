@@ -199,3 +199,60 @@ def split_posteriors(post_flat, pairs, S):
         out.append(post_flat[off:off + c].view(int(pairs.x_len[b]) + 1, int(pairs.y_len[b]) + 1, S))
         off += c
     return out
+
+
+class BatchRunner:
+    """Reusable launcher: device inputs, workspace and outputs are allocated
+    once; :meth:`run` only enqueues the kernel on the current stream (the
+    per-call host work is the descriptor sort/upload inside the library)."""
+
+    def __init__(self, log_trans, state_types, state_group, log_tab, pairs, *,
+                 forward_only=False, want_posteriors=False, want_counts=True):
+        self.L = lib()
+        dev = log_trans.device
+        self.dev = dev
+        self.pairs = pairs
+        self.S = S = int(log_trans.shape[-1])
+        self.st = np.ascontiguousarray(np.asarray(state_types, dtype=np.int32))
+        self.sg = np.ascontiguousarray(np.asarray(state_group, dtype=np.int32))
+        self.log_trans = log_trans.to(torch.float64).contiguous()
+        self.log_tab = log_tab.to(torch.float64).contiguous()
+        n_trans = 1 if self.log_trans.dim() == 2 else int(self.log_trans.shape[0])
+        if self.log_tab.dim() == 2:
+            n_tab, G = 1, int(self.log_tab.shape[0])
+        else:
+            n_tab, G = int(self.log_tab.shape[0]), int(self.log_tab.shape[1])
+        self.flags = ((FORWARD_ONLY if forward_only else 0)
+                      | (WANT_POSTERIORS if (want_posteriors and not forward_only) else 0)
+                      | (0 if want_counts else NO_COUNTS))
+        B = pairs.B
+        self.prob = Problem(B, S, _np_ptr(self.st), _np_ptr(self.sg), G, n_trans,
+                            _ptr(self.log_trans), n_tab, _ptr(self.log_tab),
+                            _ptr(pairs.x_cat), _ptr(pairs.y_cat), _np_ptr(pairs.x_off),
+                            _np_ptr(pairs.y_off), _np_ptr(pairs.x_len), _np_ptr(pairs.y_len))
+        self.ws_bytes = int(self.L.tkfmi_fb2d_workspace_bytes(ctypes.byref(self.prob), self.flags))
+        if self.ws_bytes == 0 and B > 0:
+            rc = self.L.tkfmi_fb2d_batched(ctypes.byref(self.prob), ctypes.byref(Outputs()),
+                                           None, 0, None, self.flags)
+            raise TkfmiError(self.L.tkfmi_last_error().decode() or f"tkfmi error {rc}")
+        self.ws = torch.empty(max(self.ws_bytes, 1), dtype=torch.uint8, device=dev)
+        self.log_p = torch.empty(B, dtype=torch.float64, device=dev)
+        self.n = (torch.zeros((B, S, S), dtype=torch.float64, device=dev)
+                  if (want_counts and not forward_only) else None)
+        self.post = (torch.zeros(int(pairs.cells.sum()) * S + 1, dtype=torch.float64, device=dev)
+                     if (self.flags & WANT_POSTERIORS) else None)
+        self.out = Outputs(_ptr(self.log_p), _ptr(self.n), _ptr(self.post))
+
+    def run(self, zero_outputs=True):
+        if zero_outputs:
+            if self.n is not None:
+                self.n.zero_()
+            if self.post is not None:
+                self.post.zero_()
+        stream = torch.cuda.current_stream(self.dev).cuda_stream
+        rc = self.L.tkfmi_fb2d_batched(ctypes.byref(self.prob), ctypes.byref(self.out),
+                                       _ptr(self.ws), ctypes.c_size_t(self.ws_bytes),
+                                       ctypes.c_void_p(stream), self.flags)
+        if rc != 0:
+            raise TkfmiError(self.L.tkfmi_last_error().decode() or f"tkfmi error {rc}")
+        return self.log_p, self.n, self.post
