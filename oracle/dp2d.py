@@ -38,17 +38,20 @@ def _diag_cells(d, Lx, Ly):
     return i, d - i
 
 
-def forward(log_trans, state_types, emit):
-    """Returns (logP, F) with F of shape (Lx+1, Ly+1, S)."""
-    T = np.asarray(log_trans, dtype=np.float64)
+def forward(log_trans, state_types, emit, dtype=np.float64):
+    """Returns (logP, F) with F of shape (Lx+1, Ly+1, S).  ``dtype`` may be
+    np.longdouble (80-bit on x86) for long pairs, where the float64 log-space
+    recursion itself loses ~1e-16 * |log F| per step."""
+    T = np.asarray(log_trans, dtype=dtype)
+    emit = np.asarray(emit, dtype=dtype)
     st = np.asarray(state_types)
     Lx, Ly, ns = emit.shape[0] - 1, emit.shape[1] - 1, emit.shape[2]
     masks = [(M, st == M, 1, 1), (I, st == I, 0, 1), (D, st == D, 1, 0)]
-    F = np.full((Lx + 1, Ly + 1, ns), NEG_INF)
+    F = np.full((Lx + 1, Ly + 1, ns), NEG_INF, dtype=dtype)
     F[0, 0, 0] = 0.0
     for d in range(1, Lx + Ly + 1):
         i, j = _diag_cells(d, Lx, Ly)
-        val = np.full((len(i), ns), NEG_INF)
+        val = np.full((len(i), ns), NEG_INF, dtype=dtype)
         for _, kmask, di, dj in masks:
             ok = (i >= di) & (j >= dj)
             if not ok.any() or not kmask.any():
@@ -60,20 +63,21 @@ def forward(log_trans, state_types, emit):
             val[ok] = sub
         F[i, j] = val
     logP = _lse(F[Lx, Ly] + T[:, e_index(st)], axis=0)
-    return float(logP), F
+    return logP, F
 
 
-def backward(log_trans, state_types, emit):
-    T = np.asarray(log_trans, dtype=np.float64)
+def backward(log_trans, state_types, emit, dtype=np.float64):
+    T = np.asarray(log_trans, dtype=dtype)
+    emit = np.asarray(emit, dtype=dtype)
     st = np.asarray(state_types)
     Lx, Ly, ns = emit.shape[0] - 1, emit.shape[1] - 1, emit.shape[2]
     e = e_index(st)
-    B = np.full((Lx + 1, Ly + 1, ns), NEG_INF)
+    B = np.full((Lx + 1, Ly + 1, ns), NEG_INF, dtype=dtype)
     B[Lx, Ly] = T[:, e]
     masks = [(st == M, 1, 1), (st == I, 0, 1), (st == D, 1, 0)]
     for d in range(Lx + Ly - 1, -1, -1):
         i, j = _diag_cells(d, Lx, Ly)
-        acc = np.full((len(i), ns), NEG_INF)
+        acc = np.full((len(i), ns), NEG_INF, dtype=dtype)
         for kmask, di, dj in masks:
             ok = (i + di <= Lx) & (j + dj <= Ly)
             if not ok.any() or not kmask.any():
@@ -88,15 +92,17 @@ def backward(log_trans, state_types, emit):
     return B
 
 
-def forward_backward(log_trans, state_types, emit):
-    """Returns (logP, posteriors (Lx+1,Ly+1,S), n (S,S), logP_backward)."""
-    T = np.asarray(log_trans, dtype=np.float64)
+def forward_backward(log_trans, state_types, emit, dtype=np.float64):
+    """Returns (logP, posteriors (Lx+1,Ly+1,S), n (S,S), logP_backward) as
+    float64 arrays (computed in ``dtype``)."""
+    T = np.asarray(log_trans, dtype=dtype)
+    emit = np.asarray(emit, dtype=dtype)
     st = np.asarray(state_types)
-    logP, F = forward(T, st, emit)
-    B = backward(T, st, emit)
+    logP, F = forward(T, st, emit, dtype)
+    B = backward(T, st, emit, dtype)
     Lx, Ly, ns = emit.shape[0] - 1, emit.shape[1] - 1, emit.shape[2]
     post = np.exp(F + B - logP)
-    n = np.zeros((ns, ns))
+    n = np.zeros((ns, ns), dtype=dtype)
     for kmask, di, dj in [(st == M, 1, 1), (st == I, 0, 1), (st == D, 1, 0)]:
         if not kmask.any():
             continue
@@ -107,7 +113,7 @@ def forward_backward(log_trans, state_types, emit):
         n += np.exp(ln).sum(axis=(0, 1))
     e = e_index(st)
     n[:, e] += np.exp(F[Lx, Ly] + T[:, e] - logP)
-    return logP, post, n, float(B[0, 0, 0])
+    return (float(logP), post.astype(np.float64), n.astype(np.float64), float(B[0, 0, 0]))
 
 
 def forward_backward_batch(log_trans, state_types, emits):

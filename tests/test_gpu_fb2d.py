@@ -142,3 +142,29 @@ def test_count_identities_full_size():
         outflow = nb[body, :].sum(axis=1)
         assert np.allclose(inflow, outflow, rtol=1e-10, atol=1e-10)
         assert np.isfinite(float(res["log_p"][b]))
+
+
+def test_long_rows_global_row_buffer_matches_golden():
+    """Descendants too long for the LDS row buffer (1200 and 2000 residues)
+    use the global one, in one launch with a short pair that uses LDS.
+    Expected values: tests/golden/long_pairs.npz, the oracle in 80-bit long
+    double (the float64 log-space recursion itself drifts ~1e-10 here)."""
+    import os
+    from tkfmi import hmm
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "long_pairs.npz"))
+    B = 3
+    xs = [g[f"x{b}"] for b in range(B)]
+    ys = [g[f"y{b}"] for b in range(B)]
+    res = hmm.fb2d_batch(g["log_trans"], g["state_types"], xs, ys, g["log_tab"],
+                         list(g["state_group"]), want_posteriors=True)
+    stride = int(g["post_stride"])
+    for b in range(B):
+        print(f"pair {b} Lx={len(xs[b])} Ly={len(ys[b])}", flush=True)
+        lp_o = float(g[f"logp{b}"])
+        lp = float(res["log_p"][b])
+        assert abs(lp - lp_o) <= RTOL * abs(lp_o), (b, lp, lp_o)
+        ok, worst = _close(res["n"][b].cpu().numpy(), g[f"n{b}"])
+        assert ok, (b, worst)
+        post = res["posteriors"][b].cpu().numpy()[:, ::stride, :]
+        ok, worst = _close(post, g[f"post{b}"])
+        assert ok, (b, worst)

@@ -56,6 +56,9 @@ struct PairDesc {
 struct DevParams {
   int32_t S, nb, G, e_idx;
   int32_t n_trans, n_tab, flags, smem_seq_off;
+  int32_t smem_rb_off;  // doubles: start of the LDS row buffer
+  int32_t rb_cap;       // doubles available for the LDS row buffer
+
   int32_t bidx[16];   // state index of lane k (-1: unused lane)
   int32_t btype[16];  // type of lane k (0: unused)
   int32_t bgrp[16];   // emission group of lane k
@@ -119,23 +122,64 @@ __device__ __forceinline__ int frexp_exp(double v) {
   return __builtin_amdgcn_frexp_exp(v);
 }
 
-// h = sum_s bcast(m, s) * c[s], fixed order s = 0..NB-1.
+// Workgroup barrier that orders LDS only: it waits for LDS operations but not
+// for outstanding global loads/stores, so the lattice stores and the
+// prefetched lattice loads stay in flight across steps.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// acc += bcast(src, SRC) * c with the broadcast fused into the FP64 FMA
+// (v_fmac_f64_dpp row_newbcast, 16-lane groups only).  The asm statements
+// are volatile so they keep their order; NOP2 = true inserts the two wait
+// states a DPP read needs after a VALU write of its source register, and is
+// used on the first statement that reads a freshly computed source.
+template <int SRC, bool NOP2>
+__device__ __forceinline__ void fmac_bcast16(double& acc, double src, double c) {
+  if constexpr (NOP2) {
+    asm volatile("s_nop 1\n\tv_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+                 : "+v"(acc) : "v"(src), "v"(c), "n"(SRC));
+  } else {
+    asm volatile("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+                 : "+v"(acc) : "v"(src), "v"(c), "n"(SRC));
+  }
+}
+
+// h = sum_s bcast(m, s) * c[s]: two interleaved partial sums (even / odd s)
+// combined at the end -- a fixed order, independent of the batch.
 template <int GL, int NB, int... Is>
 __device__ __forceinline__ double push_dot(double m, const double (&c)[NB],
                                            std::integer_sequence<int, Is...>) {
-  double acc = 0.0;
-  ((acc = fma(gbcast<GL, Is>(m), c[Is], acc)), ...);
-  return acc;
+  double a0 = 0.0, a1 = 0.0;
+  if constexpr (GL == 16) {
+    ((Is % 2 == 0 ? fmac_bcast16<Is, Is == 0>(a0, m, c[Is])
+                  : fmac_bcast16<Is, false>(a1, m, c[Is])), ...);
+  } else {
+    ((Is % 2 == 0 ? (void)(a0 = fma(gbcast<GL, Is>(m), c[Is], a0))
+                  : (void)(a1 = fma(gbcast<GL, Is>(m), c[Is], a1))), ...);
+  }
+  return a0 + a1;
 }
-// Backward fused step: b = sum_s prow[s] * bcast(g, s), acc[s] += bcast(g,s)*fs.
+// Backward: b = sum_s prow[s] * bcast(g, s) (two partial sums) and
+// acc[s] += bcast(g, s) * fs.
 template <int GL, int NB, int... Is>
 __device__ __forceinline__ double pull_dot_count(double g, const double (&prow)[NB],
                                                  double fs, double (&acc)[NB],
                                                  std::integer_sequence<int, Is...>) {
-  double b = 0.0;
-  double tmp;
-  ((tmp = gbcast<GL, Is>(g), b = fma(prow[Is], tmp, b), acc[Is] = fma(tmp, fs, acc[Is])), ...);
-  return b;
+  double b0 = 0.0, b1 = 0.0;
+  if constexpr (GL == 16) {
+    ((Is % 2 == 0 ? fmac_bcast16<Is, Is == 0>(b0, g, prow[Is])
+                  : fmac_bcast16<Is, false>(b1, g, prow[Is]),
+      fmac_bcast16<Is, false>(acc[Is], g, fs)), ...);
+  } else {
+    double tmp;
+    ((tmp = gbcast<GL, Is>(g),
+      (Is % 2 == 0 ? (void)(b0 = fma(prow[Is], tmp, b0)) : (void)(b1 = fma(prow[Is], tmp, b1))),
+      acc[Is] = fma(tmp, fs, acc[Is])), ...);
+  }
+  return b0 + b1;
 }
 
 // Normalise a per-lane value v * 2^e to the group: returns the mantissa and
@@ -207,83 +251,126 @@ __global__ __launch_bounds__(W * 64) void fb2d_kernel(DevParams P) {
   const bool want_post = want_lattice && (P.flags & TKFMI_WANT_POSTERIORS) && P.out_post != nullptr;
   double* out_n = want_counts ? P.out_n + (size_t)pd.pair * S * S : nullptr;
 
+  // Band bookkeeping.  Band-row R handles, at step t, position t - R of its
+  // stream: band m = position / Pper, column index c = position % Pper.  The
+  // period Pper >= H + PF + 1 guarantees that the row-buffer entry a band-row
+  // 0 needs at step t was written (by row H-1 of the previous band) before
+  // step t - PF, when it is prefetched.
+  //
+  // The row buffer lives in LDS when the row fits (rb_lds); otherwise in
+  // global memory, where a full barrier (which also waits for global stores)
+  // every QF steps and the longer period make each write visible before the
+  // step that prefetches it.
+  constexpr int PF = 6;   // prefetch distance in steps
+  constexpr int QF = 32;  // full-barrier interval for a global row buffer
   const int nbands = (Lx + H) / H;  // ceil((Lx+1)/H)
   const int R_last = Lx - (nbands - 1) * H;
-  const int Pper = max(W1, H + 2);
+  const bool rb_lds = (W1 * GL) <= P.rb_cap;
+  const int Pper = max(W1, rb_lds ? H + PF + 1 : H + PF + QF + 1);
   const int T_steps = (nbands - 1) * Pper + Ly + R_last + 1;
   double* lat = P.lattice + pd.lat_off;
   double* rowb = P.rowbuf + pd.row_off;
+  double* s_rowb = smem + P.smem_rb_off;
   __syncthreads();
 
   // ======================================================== forward sweep
   {
+    // position of the step being prefetched (t + PF) and of the current step
+    int qm = 0, qc = -R;
+    int m = 0, c = -R;
+    double rb[PF];  // row-buffer values of band-row 0, PF steps ahead
+    auto rb_load = [&](int m_, int c_) -> double {
+      const bool need = (R == 0) && (m_ >= 1) && (m_ < nbands) && (c_ >= 0) && (c_ <= Ly) &&
+                        (m_ * H <= Lx);
+      if (!need) return 0.0;
+      return rb_lds ? s_rowb[c_ * GL + k] : rowb[(size_t)c_ * GL + k];
+    };
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      rb[u] = rb_load(qm, qc);
+      if (++qc == Pper) { qc = 0; ++qm; }
+    }
     double hI = 0.0; int eI = EXP_NONE;  // own h of the previous step
     double hM = 0.0; int eM = EXP_NONE;  // D-source of the previous step
-    for (int t = 0; t < T_steps; ++t) {
-      const int p = t - R;
-      const int m = (p >= 0) ? p / Pper : -1;
-      const int j = p - m * Pper;
-      const int i = m * H + R;
-      const bool valid = (p >= 0) && (m < nbands) && (j <= Ly) && (i <= Lx);
-      // D-source: h of (i-1, j)
-      double dV = 0.0; int dE = EXP_NONE;
-      if (valid && i >= 1) {
-        const double* src = (R > 0) ? s_hbuf + ((((t - 1) & 1) * H) + (R - 1)) * GL
-                                    : rowb + (size_t)j * GL;
-        dV = src[k];
-        dE = (int)src[GL - 1];
-      }
-      double sV; int sE; bool ok;
-      if (my_type == ST_M) { sV = hM; sE = eM; ok = valid && i >= 1 && j >= 1; }
-      else if (my_type == ST_I) { sV = hI; sE = eI; ok = valid && j >= 1; }
-      else { sV = dV; sE = dE; ok = valid && i >= 1 && my_type == ST_D; }
-      const int xi = valid ? s_x[i >= 1 ? i - 1 : 0] : 0;
-      const int yj = valid ? s_y[j >= 1 ? j - 1 : 0] : 0;
-      const double em = s_tab[ebase + xi * emx + yj * emy];
-      const double v = ok ? sV * em : 0.0;
-      int En;
-      double mv = group_normalise<GL>(v, sE, En);
-      const bool origin = valid && i == 0 && j == 0;
-      if (origin) { mv = 0.0; En = 0; }
-      if (want_lattice && valid)
-        lat[((size_t)i * W1 + j) * GL + k] = (k == GL - 1) ? (double)En : mv;
-      double h = push_dot<GL, NB>(mv, pcol, Seq{});
-      if (origin) h = p_start;
-      const double hout = (k == GL - 1) ? (double)En : h;
-      s_hbuf[((t & 1) * H + R) * GL + k] = hout;
-      if (valid && R == H - 1) rowb[(size_t)j * GL + k] = hout;
-      if (valid && i == Lx && j == Ly) {  // corner: log P and the E column
-        double tot = gsum_f64<GL>(mv * p_end);
-        double lp;
-        if (origin) {
-          tot = exp(lt[e_idx]);  // empty pair: S -> E
-          lp = lt[e_idx];
-        } else {
-          lp = log(tot) + (double)En * LN2;
-        }
-        const int ep = (tot > 0.0) ? En + frexp_exp(tot) : 0;
-        const double mp = (tot > 0.0) ? ldexp(tot, -frexp_exp(tot)) : 0.0;
-        if (k == 0) {
-          P.out_logP[pd.pair] = (tot > 0.0) ? lp : -1e30;
-          s_misc[0] = mp;
-          s_misc[1] = (double)ep;
-        }
-        if (want_counts && tot > 0.0) {
-          if (origin) {
-            if (k == 0) out_n[e_idx] = 1.0;
-          } else if (my_state >= 0) {
-            out_n[my_state * S + e_idx] = mv * p_end / tot;
+    for (int t0 = 0; t0 < T_steps; t0 += PF) {
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        const int t = t0 + u;
+        if (t < T_steps) {
+          const double rbv = rb[u];
+          rb[u] = rb_load(qm, qc);
+          if (++qc == Pper) { qc = 0; ++qm; }
+          const int j = c;
+          const int i = m * H + R;
+          const bool valid = (c >= 0) && (m < nbands) && (j <= Ly) && (i <= Lx);
+          // D-source: h of (i-1, j) -- previous step of band-row R-1, or the
+          // row buffer for band-row 0
+          const bool has_d = valid && i >= 1;
+          double src = 0.0;
+          if (has_d) src = (R > 0) ? s_hbuf[((((t - 1) & 1) * H) + (R - 1)) * GL + k] : rbv;
+          const int src_e = (int)gbcast<GL, GL - 1>(src);
+          const double dV = has_d ? src : 0.0;
+          const int dE = has_d ? src_e : EXP_NONE;
+          double sV; int sE; bool ok;
+          if (my_type == ST_M) { sV = hM; sE = eM; ok = valid && i >= 1 && j >= 1; }
+          else if (my_type == ST_I) { sV = hI; sE = eI; ok = valid && j >= 1; }
+          else { sV = dV; sE = dE; ok = has_d && my_type == ST_D; }
+          const int xi = valid ? s_x[i >= 1 ? i - 1 : 0] : 0;
+          const int yj = valid ? s_y[j >= 1 ? j - 1 : 0] : 0;
+          const double em = s_tab[ebase + xi * emx + yj * emy];
+          const double v = ok ? sV * em : 0.0;
+          int En;
+          double mv = group_normalise<GL>(v, sE, En);
+          const bool origin = valid && i == 0 && j == 0;
+          if (origin) { mv = 0.0; En = 0; }
+          if (want_lattice && valid)
+            lat[((size_t)i * W1 + j) * GL + k] = (k == GL - 1) ? (double)En : mv;
+          double h = push_dot<GL, NB>(mv, pcol, Seq{});
+          if (origin) h = p_start;
+          const double hout = (k == GL - 1) ? (double)En : h;
+          s_hbuf[((t & 1) * H + R) * GL + k] = hout;
+          if (valid && R == H - 1) {
+            if (rb_lds) s_rowb[j * GL + k] = hout;
+            else rowb[(size_t)j * GL + k] = hout;
           }
+          if (valid && i == Lx && j == Ly) {  // corner: log P and the E column
+            double tot = gsum_f64<GL>(mv * p_end);
+            double lp;
+            if (origin) {
+              tot = exp(lt[e_idx]);  // empty pair: S -> E
+              lp = lt[e_idx];
+            } else {
+              lp = log(tot) + (double)En * LN2;
+            }
+            const int ep = (tot > 0.0) ? En + frexp_exp(tot) : 0;
+            const double mp = (tot > 0.0) ? ldexp(tot, -frexp_exp(tot)) : 0.0;
+            if (k == 0) {
+              P.out_logP[pd.pair] = (tot > 0.0) ? lp : -1e30;
+              s_misc[0] = mp;
+              s_misc[1] = (double)ep;
+            }
+            if (want_counts && tot > 0.0) {
+              if (origin) {
+                if (k == 0) out_n[e_idx] = 1.0;
+              } else if (my_state >= 0) {
+                out_n[my_state * S + e_idx] = mv * p_end / tot;
+              }
+            }
+          }
+          hI = h; eI = En;
+          hM = dV; eM = dE;
+          if (++c == Pper) { c = 0; ++m; }
+          if (!rb_lds && (t & (QF - 1)) == QF - 1) __syncthreads();
+          else lds_barrier();
         }
       }
-      hI = h; eI = En;
-      hM = dV; eM = dE;
-      __syncthreads();
     }
   }
   if (!want_lattice) return;
 
   // ======================================================= backward sweep
+  // Band-row R of backward band mb holds lattice row i = Lx - mb*H - R and
+  // visits columns j = Ly - c, c = 0, 1, ...
   const double mP = s_misc[0];
   const int EP = (int)s_misc[1];
   if (!(mP > 0.0)) return;  // impossible pair: counts/posteriors stay zero
@@ -298,60 +385,93 @@ __global__ __launch_bounds__(W * 64) void fb2d_kernel(DevParams P) {
   double* post = want_post ? P.out_post + pd.post_off : nullptr;
   __syncthreads();
   {
+    int qm = 0, qc = -R;
+    int mb = 0, c = -R;
+    double fr[PF];  // forward-lattice values, PF steps ahead
+    double rb[PF];  // row-buffer values of band-row 0, PF steps ahead
+    auto f_load = [&](int mb_, int c_) -> double {
+      const int i_ = Lx - mb_ * H - R;
+      const bool v_ = (c_ >= 0) && (mb_ < nbands) && (c_ <= Ly) && (i_ >= 0);
+      return v_ ? lat[((size_t)i_ * W1 + (Ly - c_)) * GL + k] : 0.0;
+    };
+    auto rb_load = [&](int mb_, int c_) -> double {
+      const bool need = (R == 0) && (mb_ >= 1) && (mb_ < nbands) && (c_ >= 0) && (c_ <= Ly) &&
+                        (Lx - mb_ * H >= 0);
+      if (!need) return 0.0;
+      return rb_lds ? s_rowb[(Ly - c_) * GL + k] : rowb[(size_t)(Ly - c_) * GL + k];
+    };
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      fr[u] = f_load(qm, qc);
+      rb[u] = rb_load(qm, qc);
+      if (++qc == Pper) { qc = 0; ++qm; }
+    }
     double qI = 0.0; int eI = EXP_NONE;  // own q of the previous step
     double qM = 0.0; int eM = EXP_NONE;  // D-successor of the previous step
-    for (int t = 0; t < T_steps; ++t) {
-      const int p = t - R;
-      const int mb = (p >= 0) ? p / Pper : -1;
-      const int c = p - mb * Pper;
-      const int j = Ly - c;
-      const int i = Lx - mb * H - R;
-      const bool valid = (p >= 0) && (mb < nbands) && (c <= Ly) && (i >= 0);
-      // D-successor: q of (i+1, j)
-      double dV = 0.0; int dE = EXP_NONE;
-      if (valid && i < Lx) {
-        const double* src = (R > 0) ? s_hbuf + ((((t - 1) & 1) * H) + (R - 1)) * GL
-                                    : rowb + (size_t)j * GL;
-        dV = src[k];
-        dE = (int)src[GL - 1];
+    for (int t0 = 0; t0 < T_steps; t0 += PF) {
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        const int t = t0 + u;
+        if (t < T_steps) {
+          const double fv_in = fr[u];
+          const double rbv = rb[u];
+          fr[u] = f_load(qm, qc);
+          rb[u] = rb_load(qm, qc);
+          if (++qc == Pper) { qc = 0; ++qm; }
+          const int j = Ly - c;
+          const int i = Lx - mb * H - R;
+          const bool valid = (c >= 0) && (mb < nbands) && (c <= Ly) && (i >= 0);
+          // D-successor: q of (i+1, j)
+          const bool has_d = valid && i < Lx;
+          double src = 0.0;
+          if (has_d) src = (R > 0) ? s_hbuf[((((t - 1) & 1) * H) + (R - 1)) * GL + k] : rbv;
+          const int src_e = (int)gbcast<GL, GL - 1>(src);
+          const double dV = has_d ? src : 0.0;
+          const int dE = has_d ? src_e : EXP_NONE;
+          double sV; int sE; bool ok;
+          if (my_type == ST_M) { sV = qM; sE = eM; ok = valid && i < Lx && j < Ly; }
+          else if (my_type == ST_I) { sV = qI; sE = eI; ok = valid && j < Ly; }
+          else { sV = dV; sE = dE; ok = has_d && my_type == ST_D; }
+          int Eg;
+          const double mg = group_normalise<GL>(ok ? sV : 0.0, sE, Eg);
+          // forward value of this cell (prefetched)
+          const double fv = valid ? fv_in : 0.0;
+          const int EF = (int)gbcast<GL, GL - 1>(fv);
+          const double fs = (my_state >= 0 && fv != 0.0) ? ldexp(fv, EF + Eg - EP) * inv_mP : 0.0;
+          double bsum = pull_dot_count<GL, NB>(mg, prow, fs, cacc, Seq{});
+          const bool corner = valid && i == Lx && j == Ly;
+          int EBraw = Eg;
+          if (corner) { bsum = p_end; EBraw = 0; }
+          int EB;
+          const double mB = group_normalise<GL>(bsum, EBraw, EB);
+          if (valid && i == 0 && j == 0) {  // start state: S -> body counts, post[0,0,S]
+            const double scale = ldexp(inv_mP, Eg - EP);
+            if (want_counts && my_state >= 0) out_n[my_state] = p_start * mg * scale;
+            const double b00 = gsum_f64<GL>(p_start * mg);
+            if (want_post && k == 0) post[0] = corner ? 1.0 : b00 * scale;
+          }
+          if (want_post && valid && my_state >= 0)
+            post[((size_t)i * W1 + j) * S + my_state] = fv * ldexp(mB * inv_mP, EF + EB - EP);
+          // q = e(i,j,k) * B(i,j,k) for the predecessors
+          const int xi = valid ? s_x[i >= 1 ? i - 1 : 0] : 0;
+          const int yj = valid ? s_y[j >= 1 ? j - 1 : 0] : 0;
+          const bool eok = (my_type == ST_M) ? (i >= 1 && j >= 1)
+                         : (my_type == ST_I) ? (j >= 1)
+                         : (my_type == ST_D) ? (i >= 1) : false;
+          const double q = eok ? s_tab[ebase + xi * emx + yj * emy] * mB : 0.0;
+          const double qout = (k == GL - 1) ? (double)EB : q;
+          s_hbuf[((t & 1) * H + R) * GL + k] = qout;
+          if (valid && R == H - 1) {
+            if (rb_lds) s_rowb[j * GL + k] = qout;
+            else rowb[(size_t)j * GL + k] = qout;
+          }
+          qI = q; eI = EB;
+          qM = dV; eM = dE;
+          if (++c == Pper) { c = 0; ++mb; }
+          if (!rb_lds && (t & (QF - 1)) == QF - 1) __syncthreads();
+          else lds_barrier();
+        }
       }
-      double sV; int sE; bool ok;
-      if (my_type == ST_M) { sV = qM; sE = eM; ok = valid && i < Lx && j < Ly; }
-      else if (my_type == ST_I) { sV = qI; sE = eI; ok = valid && j < Ly; }
-      else { sV = dV; sE = dE; ok = valid && i < Lx && my_type == ST_D; }
-      int Eg;
-      const double mg = group_normalise<GL>(ok ? sV : 0.0, sE, Eg);
-      // forward value of this cell
-      const double fv = valid ? lat[((size_t)i * W1 + j) * GL + k] : 0.0;
-      const int EF = (int)gbcast<GL, GL - 1>(fv);
-      const double fs = (my_state >= 0 && fv != 0.0) ? ldexp(fv, EF + Eg - EP) * inv_mP : 0.0;
-      double bsum = pull_dot_count<GL, NB>(mg, prow, fs, cacc, Seq{});
-      const bool corner = valid && i == Lx && j == Ly;
-      int EBraw = Eg;
-      if (corner) { bsum = p_end; EBraw = 0; }
-      int EB;
-      const double mB = group_normalise<GL>(bsum, EBraw, EB);
-      if (valid && i == 0 && j == 0) {  // start state: S -> body counts, post[0,0,S]
-        const double scale = ldexp(inv_mP, Eg - EP);
-        if (want_counts && my_state >= 0) out_n[my_state] = p_start * mg * scale;
-        const double b00 = gsum_f64<GL>(p_start * mg);
-        if (want_post && k == 0) post[0] = corner ? 1.0 : b00 * scale;
-      }
-      if (want_post && valid && my_state >= 0)
-        post[((size_t)i * W1 + j) * S + my_state] = fv * ldexp(mB * inv_mP, EF + EB - EP);
-      // q = e(i,j,k) * B(i,j,k) for the predecessors
-      const int xi = valid ? s_x[i >= 1 ? i - 1 : 0] : 0;
-      const int yj = valid ? s_y[j >= 1 ? j - 1 : 0] : 0;
-      const bool eok = (my_type == ST_M) ? (i >= 1 && j >= 1)
-                     : (my_type == ST_I) ? (j >= 1)
-                     : (my_type == ST_D) ? (i >= 1) : false;
-      const double q = eok ? s_tab[ebase + xi * emx + yj * emy] * mB : 0.0;
-      const double qout = (k == GL - 1) ? (double)EB : q;
-      s_hbuf[((t & 1) * H + R) * GL + k] = qout;
-      if (valid && R == H - 1) rowb[(size_t)j * GL + k] = qout;
-      qI = q; eI = EB;
-      qM = dV; eM = dE;
-      __syncthreads();
     }
   }
   if (!want_counts) return;
@@ -593,10 +713,20 @@ int tkfmi_fb2d_batched(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws,
   dp.out_post = o->posteriors;
   const int H = pl.H;
   const int HB = std::max(2 * H * pl.GL, pl.W * pl.GL * pl.NB);
-  const size_t seq_off = (size_t)(p->n_groups * TAB_SZ + HB + 8) * 8;
+  // LDS layout (doubles): tables | exchange/reduction | misc | row buffer | sequences
+  const size_t kLdsBudget = 160 * 1024;
+  const size_t fixed_d = (size_t)p->n_groups * TAB_SZ + HB + 8;
+  const size_t seq_bytes = (size_t)((pl.max_lx + 8) & ~7) + (size_t)((pl.max_ly + 8) & ~7) + 16;
+  if (fixed_d * 8 + seq_bytes > kLdsBudget)
+    return fail(TKFMI_EINVAL, "LDS budget exceeded (sequences too long for the emission groups)");
+  const size_t rb_want = (size_t)(pl.max_ly + 1) * pl.GL;
+  const size_t rb_room = (kLdsBudget - fixed_d * 8 - seq_bytes) / 8;
+  const size_t rb_cap = std::min(rb_want, rb_room);
+  dp.smem_rb_off = (int32_t)fixed_d;
+  dp.rb_cap = (int32_t)rb_cap;
+  const size_t seq_off = (fixed_d + rb_cap) * 8;
   dp.smem_seq_off = (int32_t)seq_off;
-  const size_t smem = seq_off + (size_t)((pl.max_lx + 8) & ~7) + (size_t)((pl.max_ly + 8) & ~7) + 16;
-  if (smem > 160 * 1024) return fail(TKFMI_EINVAL, "LDS budget exceeded (sequences too long for the emission groups)");
+  const size_t smem = seq_off + seq_bytes;
   if (pl.GL == 4) return launch<4, 3, 4>(pl, dp, B, smem, stream);
   return launch<16, 15, 8>(pl, dp, B, smem, stream);
 }
