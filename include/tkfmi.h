@@ -42,6 +42,7 @@ extern "C" {
 #define TKFMI_FORWARD_ONLY 1u    /* logP only (dp/hmm.py:1915-1916)       */
 #define TKFMI_WANT_POSTERIORS 2u /* write the posterior lattice           */
 #define TKFMI_NO_COUNTS 4u       /* skip expected transition counts       */
+#define TKFMI_GENERIC_KERNEL 8u  /* force the generic (any-layout) kernel  */
 
 /* emission tables: per group g, 21x21 match block then 21 insert then 21
  * delete entries (index 20 = wildcard), all natural-log values. */
@@ -97,6 +98,10 @@ size_t tkfmi_fb2d_workspace_bytes(const tkfmi_problem* prob, uint32_t flags);
  *   logsumexp_k F[Lx,Ly,k] + T[k,E] (dp/hmm.py:1911-1913).
  * Requirements: state 0 has type S, some state has type E, at most 15
  * emitting (M/I/D) states, 0 <= Lx, Ly, Lx + Ly <= 32000.
+ * The TKF (S,M,I,D,E) layout and the MixDom layout (core/params.py:22,
+ * models/mixdom.py:69-77) with K*F <= 3 run on the cell-per-lane kernel;
+ * other layouts (or TKFMI_GENERIC_KERNEL) on the generic one.  Both compute
+ * the same quantities.
  */
 int tkfmi_fb2d_batched(const tkfmi_problem* prob, const tkfmi_outputs* out,
                        void* workspace, size_t workspace_bytes,

@@ -26,10 +26,14 @@ def _oracle_pair(log_trans, types, emit):
     return dp2d.forward_backward(log_trans, np.asarray(types), emit)
 
 
-def _run(w, forward_only=False, want_post=True):
+def _run(w, forward_only=False, want_post=True, generic=False):
     from tkfmi import hmm
     return hmm.fb2d_batch(w.log_trans, w.state_types, w.xs, w.ys, w.log_tab, w.state_group,
-                          forward_only=forward_only, want_posteriors=want_post and not forward_only)
+                          forward_only=forward_only, want_posteriors=want_post and not forward_only,
+                          generic=generic)
+
+
+KERNELS = [pytest.param(False, id="cell-per-lane"), pytest.param(True, id="generic")]
 
 
 def _tkf_workload(lengths_x, lengths_y, seed=0, ext=None):
@@ -53,10 +57,11 @@ EDGE_X = [0, 0, 3, 1, 2, 37, 64, 65, 70, 5, 130]
 EDGE_Y = [0, 4, 0, 1, 2, 64, 37, 3, 1, 140, 96]
 
 
+@pytest.mark.parametrize("generic", KERNELS)
 @pytest.mark.parametrize("ext", [None, 0.6])
-def test_tkf_edge_lengths_match_oracle(ext):
-    w = _tkf_workload(EDGE_X, EDGE_Y, seed=3, ext=ext)
-    res = _run(w)
+def test_tkf_edge_lengths_match_oracle(ext, generic):
+    w = _tkf_workload(EDGE_X + [300, 20], EDGE_Y + [520, 700], seed=3, ext=ext)
+    res = _run(w, generic=generic)
     lt = w.log_trans.numpy()
     for b, (x, y) in enumerate(zip(w.xs, w.ys)):
         print(f"pair {b} Lx={len(x)} Ly={len(y)}", flush=True)
@@ -69,10 +74,11 @@ def test_tkf_edge_lengths_match_oracle(ext):
         assert ok, (b, worst)
 
 
-def test_tkf91_forward_only_matches_oracle():
+@pytest.mark.parametrize("generic", KERNELS)
+def test_tkf91_forward_only_matches_oracle(generic):
     from oracle import dp2d
     w = _tkf_workload([10, 50, 150, 0], [12, 47, 160, 9], seed=5)
-    res = _run(w, forward_only=True)
+    res = _run(w, forward_only=True, generic=generic)
     assert "n" not in res
     lt = w.log_trans.numpy()
     for b, (x, y) in enumerate(zip(w.xs, w.ys)):
@@ -88,15 +94,19 @@ def _mixdom_oracle_emit(w, b, x, y):
                                                   p["class_dist"], p["K"], p["F"])
 
 
-@pytest.mark.parametrize("K,F,C", [(3, 1, 1), (3, 1, 2), (1, 3, 2), (2, 1, 3)])
-def test_mixdom2_matches_oracle(K, F, C):
+@pytest.mark.parametrize("generic", KERNELS)
+@pytest.mark.parametrize("K,F,C", [(3, 1, 1), (3, 1, 2), (1, 3, 2), (2, 1, 3), (1, 2, 1), (1, 1, 2)])
+def test_mixdom2_matches_oracle(K, F, C, generic):
     from tkfmi import synth
-    lengths = [0, 1, 31, 32, 33, 45, 90, 120]
+    lengths = [0, 1, 31, 63, 64, 65, 130, 200, 140]
     w = synth.make_mixdom2(seed=11 + K + F + C, B=len(lengths), K=K, F=F, C=C, lengths=lengths)
-    # include empty / very short descendants
+    # include empty / very short / long descendants (1..8 waves per pair)
     w.ys[0] = w.ys[0][:0]
     w.ys[1] = w.ys[1][:3]
-    res = _run(w)
+    rng = np.random.default_rng(K * 100 + F * 10 + C)
+    w.ys[7] = rng.integers(0, 20, size=560)
+    w.ys[8] = rng.integers(0, 20, size=300)
+    res = _run(w, generic=generic)
     for b, (x, y) in enumerate(zip(w.xs, w.ys)):
         print(f"pair {b} Lx={len(x)} Ly={len(y)}", flush=True)
         lt = w.log_trans[b].numpy()
@@ -109,25 +119,27 @@ def test_mixdom2_matches_oracle(K, F, C):
         assert ok, (b, worst)
 
 
-def test_batched_equals_alone():
+@pytest.mark.parametrize("generic", KERNELS)
+def test_batched_equals_alone(generic):
     """A pair's results do not depend on the rest of the batch (bitwise)."""
     from tkfmi import hmm, synth
     w = synth.make_mixdom2(seed=4, B=12, lengths=[40, 200, 7, 90, 150, 33, 64, 65, 2, 180, 120, 99])
-    full = _run(w, want_post=False)
+    full = _run(w, want_post=False, generic=generic)
     for b in (0, 3, 8):
         alone = hmm.fb2d_batch(w.log_trans[b], w.state_types, [w.xs[b]], [w.ys[b]],
-                               w.log_tab[b], w.state_group)
+                               w.log_tab[b], w.state_group, generic=generic)
         assert float(alone["log_p"][0]) == float(full["log_p"][b])
         assert np.array_equal(alone["n"][0].cpu().numpy(), full["n"][b].cpu().numpy())
 
 
-def test_count_identities_full_size():
+@pytest.mark.parametrize("generic", KERNELS)
+def test_count_identities_full_size(generic):
     """Size-independent identities at the benchmark's length range:
     sum of counts into M/D states == Lx, into M/I states == Ly, start row sums
     to 1, and flow conservation for body states."""
     from tkfmi import synth
     w = synth.make_mixdom2(seed=7, B=6, lengths=[800, 30, 400, 650, 200, 777])
-    res = _run(w, want_post=False)
+    res = _run(w, want_post=False, generic=generic)
     types = np.asarray(w.state_types)
     n = res["n"].cpu().numpy()
     for b, (x, y) in enumerate(zip(w.xs, w.ys)):
@@ -144,7 +156,8 @@ def test_count_identities_full_size():
         assert np.isfinite(float(res["log_p"][b]))
 
 
-def test_long_rows_global_row_buffer_matches_golden():
+@pytest.mark.parametrize("generic", KERNELS)
+def test_long_rows_global_row_buffer_matches_golden(generic):
     """Descendants too long for the LDS row buffer (1200 and 2000 residues)
     use the global one, in one launch with a short pair that uses LDS.
     Expected values: tests/golden/long_pairs.npz, the oracle in 80-bit long
@@ -156,7 +169,7 @@ def test_long_rows_global_row_buffer_matches_golden():
     xs = [g[f"x{b}"] for b in range(B)]
     ys = [g[f"y{b}"] for b in range(B)]
     res = hmm.fb2d_batch(g["log_trans"], g["state_types"], xs, ys, g["log_tab"],
-                         list(g["state_group"]), want_posteriors=True)
+                         list(g["state_group"]), want_posteriors=True, generic=generic)
     stride = int(g["post_stride"])
     for b in range(B):
         print(f"pair {b} Lx={len(xs[b])} Ly={len(ys[b])}", flush=True)
@@ -168,3 +181,41 @@ def test_long_rows_global_row_buffer_matches_golden():
         post = res["posteriors"][b].cpu().numpy()[:, ::stride, :]
         ok, worst = _close(post, g[f"post{b}"])
         assert ok, (b, worst)
+
+
+def test_cell_per_lane_matches_generic_kernel_long_rows():
+    """Pairs whose wrap-around ring does not fit in LDS (8 waves, Ly >= 1500)
+    use a global-memory ring; compare both kernels and the exact count
+    identities (sum into M/D == Lx, into M/I == Ly)."""
+    from tkfmi import synth
+    w = synth.make_mixdom2(seed=33, B=4, lengths=[600, 700, 130, 520])
+    rng = np.random.default_rng(5)
+    w.ys[0] = rng.integers(0, 20, size=2000)
+    w.ys[1] = rng.integers(0, 20, size=1500)
+    w.ys[2] = rng.integers(0, 20, size=900)
+    a = _run(w, want_post=True)
+    b = _run(w, want_post=True, generic=True)
+    types = np.asarray(w.state_types)
+    for p in range(w.B):
+        print(f"pair {p} Lx={len(w.xs[p])} Ly={len(w.ys[p])}", flush=True)
+        la, lb = float(a["log_p"][p]), float(b["log_p"][p])
+        assert abs(la - lb) <= 1e-12 * abs(lb), (p, la, lb)
+        na, nb_ = a["n"][p].cpu().numpy(), b["n"][p].cpu().numpy()
+        ok, worst = _close(na, nb_)
+        assert ok, (p, worst)
+        ok, worst = _close(a["posteriors"][p].cpu().numpy(), b["posteriors"][p].cpu().numpy())
+        assert ok, (p, worst)
+        assert abs(na[:, np.isin(types, (1, 3))].sum() - len(w.xs[p])) < 1e-9 * len(w.xs[p])
+        assert abs(na[:, np.isin(types, (1, 2))].sum() - len(w.ys[p])) < 1e-9 * len(w.ys[p])
+
+
+def test_c1_subset_both_kernels_agree():
+    """The benchmark workload's first 96 pairs: cell-per-lane == generic."""
+    from tkfmi import synth
+    w = synth.make_mixdom2(seed=1, B=96)
+    a = _run(w, want_post=False)
+    b = _run(w, want_post=False, generic=True)
+    la, lb = a["log_p"].cpu().numpy(), b["log_p"].cpu().numpy()
+    assert np.all(np.abs(la - lb) <= 1e-12 * np.abs(lb))
+    ok, worst = _close(a["n"].cpu().numpy(), b["n"].cpu().numpy())
+    assert ok, worst

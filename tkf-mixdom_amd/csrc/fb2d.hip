@@ -34,6 +34,7 @@
 #include <utility>
 #include <vector>
 
+#include "fb2d_internal.h"
 #include "tkfmi.h"
 
 namespace {
@@ -224,8 +225,8 @@ __global__ __launch_bounds__(W * 64) void fb2d_kernel(DevParams P) {
   // ---- stage emission tables (probability space) and sequences in LDS
   const double* ltab = P.log_tab + (P.n_tab > 1 ? (size_t)pd.pair * G * TAB_SZ : 0);
   for (int q = tid; q < G * TAB_SZ; q += W * 64) s_tab[q] = exp(ltab[q]);
-  for (int q = tid; q < Lx; q += W * 64) s_x[q] = P.x_cat[pd.x_off + q];
-  for (int q = tid; q < Ly; q += W * 64) s_y[q] = P.y_cat[pd.y_off + q];
+  for (int q = tid; q < Lx; q += W * 64) s_x[q] = min((int)P.x_cat[pd.x_off + q], A_PAD - 1);
+  for (int q = tid; q < Ly; q += W * 64) s_y[q] = min((int)P.y_cat[pd.y_off + q], A_PAD - 1);
   if (tid == 0) {  // dummy characters for empty sequences (never used)
     if (Lx == 0) s_x[0] = 0;
     if (Ly == 0) s_y[0] = 0;
@@ -600,6 +601,35 @@ int launch(const Plan& pl, DevParams dp, int B, size_t smem, hipStream_t stream)
 
 }  // namespace
 
+int tkfmi_stage_upload(void* dst, const void* src, size_t bytes, hipStream_t stream,
+                       std::string& err) {
+  Stage& st = g_stage;
+  if (st.pending) {
+    hipError_t e = hipEventSynchronize(st.ev);
+    if (e != hipSuccess) { err = std::string("hipEventSynchronize: ") + hipGetErrorString(e); return TKFMI_EHIP; }
+    st.pending = false;
+  }
+  if (st.cap < bytes) {
+    if (st.buf) (void)hipHostFree(st.buf);
+    st.buf = nullptr;
+    st.cap = 0;
+    hipError_t e = hipHostMalloc(&st.buf, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) { err = std::string("hipHostMalloc: ") + hipGetErrorString(e); return TKFMI_EHIP; }
+    st.cap = bytes;
+  }
+  if (!st.ev) {
+    hipError_t e = hipEventCreateWithFlags(&st.ev, hipEventDisableTiming);
+    if (e != hipSuccess) { err = std::string("hipEventCreate: ") + hipGetErrorString(e); return TKFMI_EHIP; }
+  }
+  memcpy(st.buf, src, bytes);
+  hipError_t e = hipMemcpyAsync(dst, st.buf, bytes, hipMemcpyHostToDevice, stream);
+  if (e != hipSuccess) { err = std::string("hipMemcpyAsync: ") + hipGetErrorString(e); return TKFMI_EHIP; }
+  e = hipEventRecord(st.ev, stream);
+  if (e != hipSuccess) { err = std::string("hipEventRecord: ") + hipGetErrorString(e); return TKFMI_EHIP; }
+  st.pending = true;
+  return TKFMI_OK;
+}
+
 extern "C" {
 
 int tkfmi_abi_version(void) { return TKFMI_ABI_VERSION; }
@@ -607,6 +637,9 @@ int tkfmi_abi_version(void) { return TKFMI_ABI_VERSION; }
 const char* tkfmi_last_error(void) { return g_err.c_str(); }
 
 size_t tkfmi_fb2d_workspace_bytes(const tkfmi_problem* prob, uint32_t flags) {
+  size_t v2b = 0;
+  if (!(flags & TKFMI_GENERIC_KERNEL) && tkfmi_v2_workspace(prob, flags, &v2b))
+    return v2b > 0 ? v2b : 1;
   Plan pl;
   if (make_plan(prob, flags, pl) != TKFMI_OK) return 0;
   return plan_ws_bytes(pl);
@@ -624,6 +657,12 @@ int tkfmi_fb2d_batched(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws,
   if (p->B == 0) return TKFMI_OK;
   if (!p->log_trans || !p->log_tab || !p->x_cat || !p->y_cat)
     return fail(TKFMI_EINVAL, "device input pointer missing");
+  size_t v2b = 0;
+  if (!(flags & TKFMI_GENERIC_KERNEL) && tkfmi_v2_workspace(p, flags, &v2b)) {
+    std::string err;
+    const int rc2 = tkfmi_v2_run(p, o, ws, ws_bytes, reinterpret_cast<hipStream_t>(stream_v), flags, err);
+    return rc2 == TKFMI_OK ? TKFMI_OK : fail(rc2, err);
+  }
   const size_t need = plan_ws_bytes(pl);
   if (!ws || ws_bytes < need) return fail(TKFMI_EWORKSPACE, "workspace too small: need " + std::to_string(need));
   hipStream_t stream = reinterpret_cast<hipStream_t>(stream_v);
@@ -648,25 +687,7 @@ int tkfmi_fb2d_batched(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws,
     acc_row += (int64_t)(p->h_y_len[b] + 1) * pl.GL;
   }
   const size_t dbytes = sizeof(PairDesc) * (size_t)B;
-  Stage& st = g_stage;
-  if (st.pending) {
-    hipError_t e = hipEventSynchronize(st.ev);
-    if (e != hipSuccess) return fail(TKFMI_EHIP, std::string("hipEventSynchronize: ") + hipGetErrorString(e));
-    st.pending = false;
-  }
-  if (st.cap < dbytes) {
-    if (st.buf) (void)hipHostFree(st.buf);
-    st.buf = nullptr;
-    st.cap = 0;
-    hipError_t e = hipHostMalloc(&st.buf, dbytes, hipHostMallocDefault);
-    if (e != hipSuccess) return fail(TKFMI_EHIP, std::string("hipHostMalloc: ") + hipGetErrorString(e));
-    st.cap = dbytes;
-  }
-  if (!st.ev) {
-    hipError_t e = hipEventCreateWithFlags(&st.ev, hipEventDisableTiming);
-    if (e != hipSuccess) return fail(TKFMI_EHIP, std::string("hipEventCreate: ") + hipGetErrorString(e));
-  }
-  PairDesc* hd = static_cast<PairDesc*>(st.buf);
+  std::vector<PairDesc> hd(B);
   for (int q = 0; q < B; ++q) {
     const int b = order[q];
     PairDesc d;
@@ -685,11 +706,11 @@ int tkfmi_fb2d_batched(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws,
   PairDesc* ddesc = reinterpret_cast<PairDesc*>(wsb);
   double* lattice = reinterpret_cast<double*>(wsb + pl.desc_bytes);
   double* rowbuf = reinterpret_cast<double*>(wsb + pl.desc_bytes + align_up(pl.lat_doubles * 8, 256));
-  hipError_t e = hipMemcpyAsync(ddesc, hd, dbytes, hipMemcpyHostToDevice, stream);
-  if (e != hipSuccess) return fail(TKFMI_EHIP, std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
-  e = hipEventRecord(st.ev, stream);
-  if (e != hipSuccess) return fail(TKFMI_EHIP, std::string("hipEventRecord: ") + hipGetErrorString(e));
-  st.pending = true;
+  {
+    std::string err;
+    const int rcu = tkfmi_stage_upload(ddesc, hd.data(), dbytes, stream, err);
+    if (rcu != TKFMI_OK) return fail(rcu, err);
+  }
 
   DevParams dp;
   memset(&dp, 0, sizeof(dp));

@@ -1,0 +1,25 @@
+// Internal interfaces between the generic (state-per-lane) and the
+// cell-per-lane forward/backward implementations of libtkfmi.
+#ifndef TKFMI_FB2D_INTERNAL_H
+#define TKFMI_FB2D_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "tkfmi.h"
+
+// Copies `bytes` host bytes to device `dst` on `stream` through a pinned
+// staging buffer owned by the library (reused once the previous upload is
+// complete).
+int tkfmi_stage_upload(void* dst, const void* src, size_t bytes, hipStream_t stream,
+                       std::string& err);
+
+// Cell-per-lane path (TKF91/TKF92 and MixDom K*F <= 3 layouts).
+bool tkfmi_v2_workspace(const tkfmi_problem* p, uint32_t flags, size_t* bytes);
+int tkfmi_v2_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, size_t ws_bytes,
+                 hipStream_t stream, uint32_t flags, std::string& err);
+
+#endif

@@ -18,6 +18,7 @@ ABI_VERSION = 1
 FORWARD_ONLY = 1
 WANT_POSTERIORS = 2
 NO_COUNTS = 4
+GENERIC_KERNEL = 8
 A_PAD = 21
 TAB_SZ = A_PAD * A_PAD + 2 * A_PAD
 
@@ -129,7 +130,7 @@ def normalise_codes(seq):
 
 
 def fb2d_batched(log_trans, state_types, state_group, log_tab, pairs, *,
-                 forward_only=False, want_posteriors=False, want_counts=True):
+                 forward_only=False, want_posteriors=False, want_counts=True, generic=False):
     """Run the batched forward/backward kernel.
 
     log_trans: float64 device tensor (S,S) or (B,S,S) of log transitions.
@@ -164,6 +165,8 @@ def fb2d_batched(log_trans, state_types, state_group, log_tab, pairs, *,
         flags |= WANT_POSTERIORS
     if not want_counts:
         flags |= NO_COUNTS
+    if generic:
+        flags |= GENERIC_KERNEL
     prob = Problem(B, S, _np_ptr(st), _np_ptr(sg), G, n_trans, _ptr(log_trans), n_tab,
                    _ptr(log_tab), _ptr(pairs.x_cat), _ptr(pairs.y_cat),
                    _np_ptr(pairs.x_off), _np_ptr(pairs.y_off),
@@ -207,7 +210,7 @@ class BatchRunner:
     per-call host work is the descriptor sort/upload inside the library)."""
 
     def __init__(self, log_trans, state_types, state_group, log_tab, pairs, *,
-                 forward_only=False, want_posteriors=False, want_counts=True):
+                 forward_only=False, want_posteriors=False, want_counts=True, generic=False):
         self.L = lib()
         dev = log_trans.device
         self.dev = dev
@@ -224,7 +227,8 @@ class BatchRunner:
             n_tab, G = int(self.log_tab.shape[0]), int(self.log_tab.shape[1])
         self.flags = ((FORWARD_ONLY if forward_only else 0)
                       | (WANT_POSTERIORS if (want_posteriors and not forward_only) else 0)
-                      | (0 if want_counts else NO_COUNTS))
+                      | (0 if want_counts else NO_COUNTS)
+                      | (GENERIC_KERNEL if generic else 0))
         B = pairs.B
         self.prob = Problem(B, S, _np_ptr(self.st), _np_ptr(self.sg), G, n_trans,
                             _ptr(self.log_trans), n_tab, _ptr(self.log_tab),

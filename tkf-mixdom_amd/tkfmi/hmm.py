@@ -83,7 +83,7 @@ def _types_list(state_types):
 
 
 def fb2d_batch(log_trans, state_types, xs, ys, log_tab, state_group=None, *,
-               forward_only=False, want_posteriors=False, want_counts=True):
+               forward_only=False, want_posteriors=False, want_counts=True, generic=False):
     """One launch over a ragged batch of pairs.
 
     log_trans: (S,S) or (B,S,S) log transitions; log_tab: (G,TAB) or
@@ -91,6 +91,8 @@ def fb2d_batch(log_trans, state_types, xs, ys, log_tab, state_group=None, *,
     of integer code arrays (real lengths, no padding).
     Returns dict with 'log_p' (B,), 'n' (B,S,S) (unless forward_only or
     want_counts=False) and 'posteriors' (list of (Lx+1,Ly+1,S) views).
+    ``generic=True`` forces the any-layout kernel (the TKF and MixDom K*F<=3
+    layouts otherwise run on the cell-per-lane kernel).
     """
     dev = device()
     lt = _dev64(log_trans, dev)
@@ -103,7 +105,7 @@ def fb2d_batch(log_trans, state_types, xs, ys, log_tab, state_group=None, *,
     logp, n, post = _abi.fb2d_batched(lt, st, state_group, tab, pairs,
                                       forward_only=forward_only,
                                       want_posteriors=want_posteriors,
-                                      want_counts=want_counts)
+                                      want_counts=want_counts, generic=generic)
     out = {"log_p": logp}
     if n is not None:
         out["n"] = n
