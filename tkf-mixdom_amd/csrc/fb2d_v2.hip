@@ -155,6 +155,16 @@ constexpr int DPP_WAVE_SHL1 = 0x130;  // lane l <- lane l+1
 
 __device__ __forceinline__ int frexp_exp(double v) { return __builtin_amdgcn_frexp_exp(v); }
 
+// Hides a (uniform) pointer's value from the optimiser so that loads through
+// it are not hoisted out of the step loop: the NB^2 transition values would
+// not fit in SGPRs and would be spilled.  Each row is instead re-read through
+// the scalar cache right before its FMAs.
+typedef const __attribute__((address_space(4))) double* const_dptr;
+__device__ __forceinline__ const_dptr launder(const double* p) {
+  asm volatile("" : "+s"(p));
+  return (const_dptr)p;  // constant address space: read with s_load
+}
+
 __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();
@@ -361,8 +371,9 @@ __global__ __launch_bounds__(W * 64) void fbv2_kernel(V2Params Pr, const double*
       for (int k = 0; k < NB; ++k) h[k] = 0.0;
 #pragma unroll
       for (int s = 0; s < NB; ++s) {
+        const_dptr row = launder(Pf + s * NB);
 #pragma unroll
-        for (int k = 0; k < NB; ++k) h[k] = fma(F[s], Pf[s * NB + k], h[k]);
+        for (int k = 0; k < NB; ++k) h[k] = fma(F[s], row[k], h[k]);
       }
       if (origin) {
 #pragma unroll
@@ -534,8 +545,9 @@ __global__ __launch_bounds__(W * 64) void fbv2_kernel(V2Params Pr, const double*
       for (int k = 0; k < NB; ++k) Bv[k] = 0.0;
 #pragma unroll
       for (int k2 = 0; k2 < NB; ++k2) {
+        const_dptr row = launder(Pb + k2 * NB);
 #pragma unroll
-        for (int k = 0; k < NB; ++k) Bv[k] = fma(Pb[k2 * NB + k], g[k2], Bv[k]);
+        for (int k = 0; k < NB; ++k) Bv[k] = fma(row[k], g[k2], Bv[k]);
       }
       const bool corner = valid && i == Lx && j == Ly;
       int EBraw = Eg;
