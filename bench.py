@@ -8,8 +8,10 @@ inputs already resident in HBM.  Prints one JSON line.
 
   value       = sum_pairs (Lx+1)(Ly+1) S / step time         (cell-states/s)
   roofline    = HBM-bound: 16 B of lattice traffic per cell-state
-                (write + read of the forward lattice), divided by the
-                measured kernel time, against 8 TB/s
+                (write + read of the forward lattice) of the pairs in the
+                dominant kernel launch (the longest of the per-wave-count
+                class launches), divided by that launch's HIP-event duration
+                on its own stream, against 8 TB/s
   cpu_baseline= the C restatement of the reference recursion (log space,
                 OpenMP over pairs) on a bounded sample of the same pairs
   plain       = the per-pair drop-in (one launch per pair, as the reference's
@@ -29,6 +31,10 @@ for _p in (ROOT, os.path.join(ROOT, "tkf-mixdom_amd")):
 METRIC = "fwd+bwd+counts DP cell-states/s, MixDom2 K=3 batched vs plain; p99 latency"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_STATE = 16.0
+# HBM bytes per launch of the dominant kernel from rocprofv3 --pmc FETCH_SIZE /
+# WRITE_SIZE (KiB per dispatch, scaled by the copy-kernel calibration); filled
+# in from the committed profile summaries.
+TRAFFIC = {}
 
 
 def _log(msg):
@@ -62,7 +68,7 @@ def cpu_baseline(w, seconds=15.0, threads=None):
                       f"{elapsed:.1f} s, C log-space restatement, OpenMP over pairs)"}
 
 
-def plain_per_pair(w, n_pairs=48):
+def plain_per_pair(w, n_pairs=48, emit=True):
     """Per-pair drop-in: one launch per pair (the reference calls
     forward_backward_2d per pair), timed over a sample of pairs."""
     import numpy as np
@@ -75,7 +81,7 @@ def plain_per_pair(w, n_pairs=48):
     for i in idx:
         pp = _abi.PackedPairs([w.xs[i]], [w.ys[i]], dev)
         runners.append(_abi.BatchRunner(w.log_trans[i].to(dev), w.state_types, w.state_group,
-                                        w.log_tab[i].to(dev), pp))
+                                        w.log_tab[i].to(dev), pp, want_emit_counts=emit))
     for r in runners[:4]:
         r.run()
     torch.cuda.synchronize()
@@ -98,6 +104,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-emit", action="store_true",
+                    help="transition counts only (no expected emission counts)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -114,25 +122,24 @@ def main():
     torch.cuda.set_device(dev)
     pairs = _abi.PackedPairs(w.xs, w.ys, dev)
     runner = _abi.BatchRunner(w.log_trans.to(dev), w.state_types, w.state_group,
-                              w.log_tab.to(dev), pairs)
+                              w.log_tab.to(dev), pairs, want_emit_counts=not args.no_emit)
     cell_states = w.cell_states()
     _log(f"pairs={w.B} S={w.S} cell-states={cell_states:.4e} workspace={runner.ws_bytes/2**30:.2f} GiB")
     for _ in range(args.warmup):
         runner.run()
     torch.cuda.synchronize()
-    # per-launch kernel time (HIP events on the launch stream)
-    stream = torch.cuda.current_stream(dev)
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(args.steps)]
+    # per-kernel durations: HIP events recorded by the library on the stream
+    # each kernel runs on (one kernel per wave-count class)
+    base_flags = runner.flags
+    runner.flags = base_flags | _abi.TIME_KERNELS
+    per_kernel = {}
     for i in range(args.steps):
-        runner.n.zero_()
-        evs[i][0].record(stream)
-        runner.run(zero_outputs=False)
-        evs[i][1].record(stream)
+        runner.run()
+        for k in _abi.last_kernel_times():
+            per_kernel.setdefault(k["name"], []).append(k)
+    runner.flags = base_flags
     torch.cuda.synchronize()
-    kern_ms = [a.elapsed_time(b) for a, b in evs]
-    # whole-step wall time
-    torch.cuda.synchronize()
+    # whole-step wall time (outputs zeroed inside the step, as a caller would)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         runner.run()
@@ -142,27 +149,37 @@ def main():
     n = runner.n.cpu().numpy()
     assert np.all(np.isfinite(logp)), "non-finite log-likelihood"
     value = cell_states / (step_ms / 1e3)
-    kavg = float(np.mean(kern_ms))
-    achieved = BYTES_PER_CELL_STATE * cell_states / (kavg / 1e3) / 1e9
-    _log(f"step {step_ms:.3f} ms  kernel avg {kavg:.3f} ms  min {min(kern_ms):.3f} ms")
+    kernels = []
+    for name, recs in per_kernel.items():
+        ms = float(np.mean([r["ms"] for r in recs]))
+        cs = recs[0]["cells"] * w.S
+        kernels.append({"name": name, "pairs": recs[0]["n_pairs"], "cell_states": cs, "ms": ms,
+                        "achieved_GBs": BYTES_PER_CELL_STATE * cs / (ms / 1e3) / 1e9})
+    kernels.sort(key=lambda k: -k["ms"])
+    dom = kernels[0]
+    _log(f"step {step_ms:.3f} ms; kernels: " +
+         ", ".join(f"{k['name']} {k['ms']:.3f} ms ({k['pairs']} pairs)" for k in kernels))
 
     out = {
         "metric": METRIC, "value": value, "unit": "cell-states/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": step_ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "c1: MixDom2 K=3 F=1 C=1 (17 states) forward+backward+expected "
-                               "counts, 1024 uneven pairs, LogNormal(ln200,0.5) lengths in [30,800], "
-                               "t~U(0.05,2), one launch",
+        "config": {"workload": "MixDom2 K=3 F=1 C=1 (17 states) forward+backward+expected "
+                               "transition and emission counts, 1024 uneven pairs, "
+                               "LogNormal(ln200,0.5) lengths in [30,800], t~U(0.05,2), one call",
                    "pairs": w.B, "states": w.S, "cell_states": cell_states,
+                   "emission_counts": not args.no_emit,
                    "sum_logp": float(logp.sum()), "sum_counts": float(n.sum())},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "fb2d_kernel<16,15,8>", "kernel_ms": kavg},
+        "roofline": {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": dom["achieved_GBs"] / HBM_PEAK_GBS,
+                     "traffic": TRAFFIC.get(dom["name"]),
+                     "kernel": dom["name"], "kernel_ms": dom["ms"],
+                     "bytes_per_cell_state": BYTES_PER_CELL_STATE, "kernels": kernels},
     }
     if not args.no_plain:
         _log("timing the per-pair drop-in")
-        pl = plain_per_pair(w)
+        pl = plain_per_pair(w, emit=not args.no_emit)
         out["plain_per_pair"] = pl
         out["batched_over_plain"] = value / pl["value"]
     if not args.no_cpu:

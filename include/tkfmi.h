@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define TKFMI_ABI_VERSION 1
+#define TKFMI_ABI_VERSION 2
 
 /* error codes */
 #define TKFMI_OK 0
@@ -43,6 +43,8 @@ extern "C" {
 #define TKFMI_WANT_POSTERIORS 2u /* write the posterior lattice           */
 #define TKFMI_NO_COUNTS 4u       /* skip expected transition counts       */
 #define TKFMI_GENERIC_KERNEL 8u  /* force the generic (any-layout) kernel  */
+#define TKFMI_WANT_EMIT_COUNTS 16u /* expected emission counts (see below)  */
+#define TKFMI_TIME_KERNELS 32u   /* record per-kernel HIP-event timings    */
 
 /* emission tables: per group g, 21x21 match block then 21 insert then 21
  * delete entries (index 20 = wildcard), all natural-log values. */
@@ -73,6 +75,11 @@ typedef struct tkfmi_outputs {
   double* posteriors; /* device ragged: pair b at offset sum_{b'<b}
                          (Lx+1)(Ly+1)S, layout [i][j][state], zero-filled by
                          the caller; NULL unless TKFMI_WANT_POSTERIORS       */
+  double* emit_counts; /* device [B, n_groups, TKFMI_TAB_SZ], zero-filled by
+                          the caller; NULL unless TKFMI_WANT_EMIT_COUNTS.
+                          Same layout as log_tab: entry (g, e) is the
+                          posterior-expected number of times table entry e of
+                          group g is used, i.e. d logP / d log_tab[g, e].    */
 } tkfmi_outputs;
 
 /* Version of this header's ABI (TKFMI_ABI_VERSION). */
@@ -96,6 +103,16 @@ size_t tkfmi_fb2d_workspace_bytes(const tkfmi_problem* prob, uint32_t flags);
  * reference's builders define them (dp/hmm.py:628-851).
  *   n_trans[b] follows dp/hmm.py:1930-1971 (S,S); log_prob[b] is
  *   logsumexp_k F[Lx,Ly,k] + T[k,E] (dp/hmm.py:1911-1913).
+ *   emit_counts[b] (TKFMI_WANT_EMIT_COUNTS) replaces the posterior
+ *   reductions over the lattice: dom_match / dom_insert / dom_delete /
+ *   agg_match (train_pfam.py:1705-1734; the reference's (K,A,A), (K,A)
+ *   blocks are the [:20,:20] / [:20] parts of each group's 21-wide block),
+ *   observation_counts_2d (train/likelihood.py:962-1028: I counts include
+ *   row i=0, D counts column j=0) and the emission gradients
+ *   (train/vjp.py:27-106); the MixDom2 class counts follow from them
+ *   (train/likelihood.py:296-319).  Accumulated in LDS with FP64 atomics,
+ *   so the summation order (not the value beyond rounding) may vary from
+ *   run to run.
  * Requirements: state 0 has type S, some state has type E, at most 15
  * emitting (M/I/D) states, 0 <= Lx, Ly, Lx + Ly <= 32000.
  * The TKF (S,M,I,D,E) layout and the MixDom layout (core/params.py:22,
@@ -106,6 +123,26 @@ size_t tkfmi_fb2d_workspace_bytes(const tkfmi_problem* prob, uint32_t flags);
 int tkfmi_fb2d_batched(const tkfmi_problem* prob, const tkfmi_outputs* out,
                        void* workspace, size_t workspace_bytes,
                        void* stream /* hipStream_t */, uint32_t flags);
+
+/* One kernel launch of a timed call. */
+typedef struct tkfmi_kernel_time {
+  char name[64];    /* kernel, e.g. "fbv2_kernel<LMix<3,1>,8>"            */
+  int32_t n_pairs;  /* pairs processed by the launch                       */
+  int32_t pad;
+  int64_t cells;    /* sum over those pairs of (Lx+1)(Ly+1)                */
+  double ms;        /* HIP-event duration on the stream it ran on          */
+} tkfmi_kernel_time;
+
+/*
+ * Measurement aid with no counterpart in the reference.  Durations of the
+ * kernels launched by the last tkfmi_fb2d_batched call on this thread that
+ * had TKFMI_TIME_KERNELS set, measured with HIP events recorded on the
+ * stream each kernel was launched on (the cell-per-lane path launches one
+ * kernel per wave-count class on library-owned streams).  Waits for those
+ * kernels.  Returns the number of entries written (at most `max`) or a
+ * negative TKFMI_E* code.  Not meant for concurrent timed calls.
+ */
+int tkfmi_last_kernel_times(tkfmi_kernel_time* out, int32_t max);
 
 #ifdef __cplusplus
 }

@@ -129,3 +129,53 @@ def pair_hmm_emissions_per_class(state_types, x, y, class_subs, class_pis,
         ins = np.logaddexp(ins, cpis[c][yp][:, None] + slw[None, :, c])
         dele = np.logaddexp(dele, cpis[c][xp][:, None] + slw[None, :, c])
     return _assemble(state_types, match, ins, dele)
+
+
+TAB_SZ = A_PAD * A_PAD + 2 * A_PAD
+
+
+def emit_from_tables(state_types, state_group, log_tab, x, y):
+    """Emission lattice (Lx+1, Ly+1, S) from per-group tables laid out as
+    [21x21 match | 21 insert | 21 delete] (the kernel's table format).  Grid
+    row/column 0 use the dummy character 0 (hmm.py:650-651); codes are
+    clipped to the wildcard index 20."""
+    st = np.asarray(state_types)
+    grp = np.asarray(state_group, dtype=np.int64)
+    tab = np.asarray(log_tab, dtype=np.float64)
+    xp, yp = _grid_chars(np.minimum(np.asarray(x, dtype=np.int64), A_PAD - 1),
+                         np.minimum(np.asarray(y, dtype=np.int64), A_PAD - 1))
+    mt = tab[:, :A_PAD * A_PAD].reshape(-1, A_PAD, A_PAD)
+    it = tab[:, A_PAD * A_PAD:A_PAD * A_PAD + A_PAD]
+    dt = tab[:, A_PAD * A_PAD + A_PAD:]
+    match = mt[grp][:, xp][:, :, yp].transpose(1, 2, 0)   # (Lx+1, Ly+1, S)
+    ins = it[grp][:, yp].T                                 # (Ly+1, S)
+    dele = dt[grp][:, xp].T                                # (Lx+1, S)
+    return _assemble(st, match, ins, dele)
+
+
+def emission_counts(post, state_types, state_group, x, y, n_groups):
+    """Expected emission counts (n_groups, TAB_SZ) in the table layout, from
+    a posterior lattice (Lx+1, Ly+1, S).  The per-type slices follow
+    observation_counts_2d (train/likelihood.py:962-1028): M over
+    i in [1,Lx], j in [1,Ly] onto (x[i-1], y[j-1]); I over i in [0,Lx],
+    j in [1,Ly] onto y[j-1]; D over i in [1,Lx], j in [0,Ly] onto x[i-1];
+    per-group (domain) masks as in train_pfam.py:1705-1734."""
+    st = np.asarray(state_types)
+    grp = np.asarray(state_group, dtype=np.int64)
+    post = np.asarray(post, dtype=np.float64)
+    x = np.minimum(np.asarray(x, dtype=np.int64), A_PAD - 1)
+    y = np.minimum(np.asarray(y, dtype=np.int64), A_PAD - 1)
+    Lx, Ly = len(x), len(y)
+    out = np.zeros((n_groups, TAB_SZ))
+    for s in range(len(st)):
+        g = grp[s]
+        if st[s] == M and Lx and Ly:
+            w = post[1:, 1:, s]                                         # (Lx, Ly)
+            blk = np.zeros((A_PAD, A_PAD))
+            np.add.at(blk, (x[:, None].repeat(Ly, 1), y[None, :].repeat(Lx, 0)), w)
+            out[g, :A_PAD * A_PAD] += blk.ravel()
+        elif st[s] == I and Ly:
+            np.add.at(out[g], A_PAD * A_PAD + y, post[:, 1:, s].sum(axis=0))
+        elif st[s] == D and Lx:
+            np.add.at(out[g], A_PAD * A_PAD + A_PAD + x, post[1:, :, s].sum(axis=1))
+    return out

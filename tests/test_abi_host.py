@@ -1,0 +1,120 @@
+"""Host-side checks of the C ABI (no GPU calls): libtkfmi.so loads, exports
+every entry point include/tkfmi.h declares, and its host-only paths (version,
+workspace query, argument validation) behave as documented.  Also checks the
+ragged packing and the geometric-bin helper of the Python mirror."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "tkfmi.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(tkfmi_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    decl = _declared()
+    assert "tkfmi_fb2d_batched" in decl and "tkfmi_fb2d_workspace_bytes" in decl
+    from tkfmi import _abi
+    assert sorted(_abi.EXPORTED) == decl
+
+
+def test_library_exports_every_declared_symbol():
+    from tkfmi import _abi
+    L = _abi.lib()
+    for name in _declared():
+        assert hasattr(L, name), name
+    assert L.tkfmi_abi_version() == _abi.ABI_VERSION
+    m = re.search(r"#define TKFMI_ABI_VERSION (\d+)", open(HEADER).read())
+    assert int(m.group(1)) == _abi.ABI_VERSION
+
+
+def test_struct_layout_matches_header():
+    """Field order of the ctypes mirrors == the C structs."""
+    from tkfmi import _abi
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    for cname, py in (("tkfmi_problem", _abi.Problem), ("tkfmi_outputs", _abi.Outputs),
+                      ("tkfmi_kernel_time", _abi.KernelTime)):
+        body = re.search(r"typedef struct %s \{(.*?)\}" % cname, text, flags=re.S).group(1)
+        fields = re.findall(r"\*?\s*(\w+)(?:\[\d+\])?;", body)
+        assert fields == [f[0] for f in py._fields_], cname
+    flags = dict(re.findall(r"#define TKFMI_(\w+) (\d+)u", text))
+    for k in ("FORWARD_ONLY", "WANT_POSTERIORS", "NO_COUNTS", "GENERIC_KERNEL", "WANT_EMIT_COUNTS",
+              "TIME_KERNELS"):
+        assert int(flags[k]) == getattr(_abi, k)
+
+
+def _problem(st, sg, xl, yl, G=1, n_trans=1):
+    from tkfmi import _abi
+    B = len(xl)
+    keep = dict(st=np.asarray(st, np.int32), sg=np.asarray(sg, np.int32),
+                xl=np.asarray(xl, np.int32), yl=np.asarray(yl, np.int32),
+                xo=np.zeros(B, np.int64), yo=np.zeros(B, np.int64))
+    p = _abi.Problem(B, len(st), _abi._np_ptr(keep["st"]), _abi._np_ptr(keep["sg"]), G, n_trans,
+                     None, 1, None, None, None, _abi._np_ptr(keep["xo"]), _abi._np_ptr(keep["yo"]),
+                     _abi._np_ptr(keep["xl"]), _abi._np_ptr(keep["yl"]))
+    return p, keep
+
+
+def test_workspace_query_is_host_only_and_monotone():
+    from tkfmi import _abi, mixdom
+    L = _abi.lib()
+    st = list(mixdom.state_types(3, 1))
+    sg = [0, 0] + [(s - 2) // 5 for s in range(2, 17)]
+    p_small, k1 = _problem(st, sg, [50, 80], [60, 70], G=3)
+    p_big, k2 = _problem(st, sg, [500, 800], [600, 700], G=3)
+    a = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p_small), 0)
+    b = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p_big), 0)
+    c = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p_big), _abi.FORWARD_ONLY)
+    g = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p_big), _abi.GENERIC_KERNEL)
+    assert 0 < a < b and 0 < c < b and g > 0
+
+
+def test_invalid_problems_rejected_with_message():
+    from tkfmi import _abi
+    L = _abi.lib()
+    # no E state
+    p, k = _problem([0, 1, 2, 3, 1], [0] * 5, [3], [4])
+    assert L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), 0) == 0
+    rc = L.tkfmi_fb2d_batched(ctypes.byref(p), ctypes.byref(_abi.Outputs()), None, 0, None, 0)
+    assert rc == -1 and L.tkfmi_last_error().decode()
+    # negative length
+    p, k = _problem([0, 1, 2, 3, 4], [0] * 5, [-1], [4])
+    rc = L.tkfmi_fb2d_batched(ctypes.byref(p), ctypes.byref(_abi.Outputs()), None, 0, None, 0)
+    assert rc == -1
+
+
+def test_packed_pairs_and_code_normalisation():
+    import torch
+    from tkfmi import _abi
+    pp = _abi.PackedPairs([np.array([1, 2, 25]), np.array([], int)],
+                          [np.array([-1, 0]), np.array([4])], torch.device("cpu"))
+    assert list(pp.x_len) == [3, 0] and list(pp.y_len) == [2, 1]
+    assert list(pp.x_off) == [0, 3] and list(pp.y_off) == [0, 2]
+    assert pp.x_cat.tolist()[:3] == [1, 2, 20]          # clamp to the wildcard
+    assert pp.y_cat.tolist()[:2] == [20, 0]             # -1 wraps to index 20
+    assert list(pp.cells) == [12, 2]
+
+
+def test_pad_to_bin_matches_reference_bins(monkeypatch):
+    """dp/hmm.py:46-91: [0,1,2,3] + strictly increasing round(4 * 1.2^k)
+    while below 4096 (the last one overshoots to 4899), then powers of 2;
+    TKFMIXDOM_MAX_PAD rounds up to multiples of 32, capped."""
+    from tkfmi import hmm
+    monkeypatch.delenv("TKFMIXDOM_MAX_PAD", raising=False)
+    bins = hmm._GEOM_BINS
+    assert bins[:8] == [0, 1, 2, 3, 4, 5, 6, 7] and bins[-1] == 4899 and len(bins) == 44
+    assert all(b2 > b1 for b1, b2 in zip(bins, bins[1:]))
+    assert [hmm._pad_to_bin(n) for n in (0, 1, 3, 8, 4096, 4899, 4900)] == [1, 1, 3, 8, 4899, 4899, 8192]
+    assert hmm._pad_to_bin(200) in bins and 200 <= hmm._pad_to_bin(200) < 240
+    monkeypatch.setenv("TKFMIXDOM_MAX_PAD", "100")
+    assert hmm._pad_to_bin(70) == 96 and hmm._pad_to_bin(97) == 100 and hmm._pad_to_bin(0) == 32
+    with pytest.raises(ValueError):
+        hmm._pad_to_bin(101)

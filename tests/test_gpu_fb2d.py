@@ -219,3 +219,59 @@ def test_c1_subset_both_kernels_agree():
     assert np.all(np.abs(la - lb) <= 1e-12 * np.abs(lb))
     ok, worst = _close(a["n"].cpu().numpy(), b["n"].cpu().numpy())
     assert ok, worst
+
+
+@pytest.mark.parametrize("generic", KERNELS)
+@pytest.mark.parametrize("model", ["tkf92", "mixdom2_k3c2", "mixdom2_k1f3"])
+def test_emit_counts_match_oracle(model, generic):
+    """Expected emission counts (d logP / d log_tab) vs the oracle's
+    posterior reduction, over edge and long lengths."""
+    from oracle import emissions
+    from tkfmi import synth
+    if model == "tkf92":
+        w = _tkf_workload([0, 3, 1, 64, 65, 130, 300], [4, 0, 1, 63, 140, 96, 310], seed=9, ext=0.5)
+        emit = lambda b, x, y: _tkf_oracle_emit(w, x, y)
+        lt_of = lambda b: w.log_trans.numpy()
+    else:
+        K, F, C = (3, 1, 2) if model == "mixdom2_k3c2" else (1, 3, 1)
+        w = synth.make_mixdom2(seed=21, B=7, K=K, F=F, C=C, lengths=[0, 1, 40, 64, 65, 200, 330])
+        emit = lambda b, x, y: _mixdom_oracle_emit(w, b, x, y)
+        lt_of = lambda b: w.log_trans[b].numpy()
+    res = _run_ec(w, generic)
+    G = res["emit_counts"].shape[1]
+    ec = res["emit_counts"].cpu().numpy()
+    for b, (x, y) in enumerate(zip(w.xs, w.ys)):
+        print(f"pair {b} Lx={len(x)} Ly={len(y)}", flush=True)
+        lp_o, post_o, n_o, _ = _oracle_pair(lt_of(b), w.state_types, emit(b, x, y))
+        want = emissions.emission_counts(post_o, w.state_types, w.state_group, x, y, G)
+        ok, worst = _close(ec[b], want)
+        assert ok, (b, worst)
+        assert abs(float(res["log_p"][b]) - lp_o) <= RTOL * abs(lp_o)
+
+
+def _run_ec(w, generic):
+    from tkfmi import hmm
+    return hmm.fb2d_batch(w.log_trans, w.state_types, w.xs, w.ys, w.log_tab, w.state_group,
+                          want_emit_counts=True, generic=generic)
+
+
+@pytest.mark.parametrize("generic", KERNELS)
+def test_emit_counts_full_size_identities(generic):
+    """At the benchmark's lengths: emission totals per type equal the
+    expected visits in n, and every residue of x (of y) is emitted once."""
+    from tkfmi import synth
+    w = synth.make_mixdom2(seed=8, B=5, lengths=[800, 650, 30, 420, 777])
+    res = _run_ec(w, generic)
+    types = np.asarray(w.state_types)
+    n = res["n"].cpu().numpy()
+    ec = res["emit_counts"].cpu().numpy()
+    nm = 21 * 21
+    for b, (x, y) in enumerate(zip(w.xs, w.ys)):
+        e = ec[b]
+        assert abs(e[:, :nm].sum() - n[b][:, types == 1].sum()) <= 1e-10 * len(x)
+        assert abs(e[:, nm:nm + 21].sum() - n[b][:, types == 2].sum()) <= 1e-10 * len(y) + 1e-12
+        m = e[:, :nm].reshape(-1, 21, 21)
+        np.testing.assert_allclose(m.sum(axis=(0, 2)) + e[:, nm + 21:].sum(axis=0),
+                                   np.bincount(np.minimum(x, 20), minlength=21), rtol=1e-10, atol=1e-9)
+        np.testing.assert_allclose(m.sum(axis=(0, 1)) + e[:, nm:nm + 21].sum(axis=0),
+                                   np.bincount(np.minimum(y, 20), minlength=21), rtol=1e-10, atol=1e-9)

@@ -59,6 +59,8 @@ struct DevParams {
   int32_t n_trans, n_tab, flags, smem_seq_off;
   int32_t smem_rb_off;  // doubles: start of the LDS row buffer
   int32_t rb_cap;       // doubles available for the LDS row buffer
+  int32_t smem_ecnt_off;  // doubles: emission-count tables (G*TAB_SZ)
+  int32_t pad2;
 
   int32_t bidx[16];   // state index of lane k (-1: unused lane)
   int32_t btype[16];  // type of lane k (0: unused)
@@ -73,6 +75,7 @@ struct DevParams {
   double* out_logP;
   double* out_n;
   double* out_post;
+  double* out_ecnt;
 };
 
 // ---------------------------------------------------------------- DPP helpers
@@ -231,6 +234,11 @@ __global__ __launch_bounds__(W * 64) void fb2d_kernel(DevParams P) {
     if (Lx == 0) s_x[0] = 0;
     if (Ly == 0) s_y[0] = 0;
   }
+  const bool want_ecnt = !(P.flags & TKFMI_FORWARD_ONLY) && (P.flags & TKFMI_WANT_EMIT_COUNTS) &&
+                         P.out_ecnt != nullptr;
+  double* s_ecnt = smem + P.smem_ecnt_off;  // [G][TAB_SZ] expected emission counts
+  if (want_ecnt)
+    for (int q = tid; q < G * TAB_SZ; q += W * 64) s_ecnt[q] = 0.0;
 
   // ---- this lane's state, its transitions and emission addressing
   const double* lt = P.log_trans + (P.n_trans > 1 ? (size_t)pd.pair * S * S : 0);
@@ -460,6 +468,8 @@ __global__ __launch_bounds__(W * 64) void fb2d_kernel(DevParams P) {
                          : (my_type == ST_I) ? (j >= 1)
                          : (my_type == ST_D) ? (i >= 1) : false;
           const double q = eok ? s_tab[ebase + xi * emx + yj * emy] * mB : 0.0;
+          if (want_ecnt && valid && eok && my_state >= 0)
+            atomicAdd(s_ecnt + ebase + xi * emx + yj * emy, fv * ldexp(mB * inv_mP, EF + EB - EP));
           const double qout = (k == GL - 1) ? (double)EB : q;
           s_hbuf[((t & 1) * H + R) * GL + k] = qout;
           if (valid && R == H - 1) {
@@ -474,6 +484,11 @@ __global__ __launch_bounds__(W * 64) void fb2d_kernel(DevParams P) {
         }
       }
     }
+  }
+  if (want_ecnt) {
+    __syncthreads();
+    double* oe = P.out_ecnt + (size_t)pd.pair * G * TAB_SZ;
+    for (int q = tid; q < G * TAB_SZ; q += W * 64) oe[q] = s_ecnt[q];
   }
   if (!want_counts) return;
 
@@ -648,12 +663,15 @@ size_t tkfmi_fb2d_workspace_bytes(const tkfmi_problem* prob, uint32_t flags) {
 int tkfmi_fb2d_batched(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws,
                        size_t ws_bytes, void* stream_v, uint32_t flags) {
   g_err.clear();
+  tkfmi_ktime_begin_call((flags & TKFMI_TIME_KERNELS) != 0);
   Plan pl;
   int rc = make_plan(p, flags, pl);
   if (rc != TKFMI_OK) return rc;
   if (!o || !o->log_prob) return fail(TKFMI_EINVAL, "outputs.log_prob missing");
   if ((flags & TKFMI_WANT_POSTERIORS) && !(flags & TKFMI_FORWARD_ONLY) && !o->posteriors)
     return fail(TKFMI_EINVAL, "TKFMI_WANT_POSTERIORS without a posterior buffer");
+  if ((flags & TKFMI_WANT_EMIT_COUNTS) && !(flags & TKFMI_FORWARD_ONLY) && !o->emit_counts)
+    return fail(TKFMI_EINVAL, "TKFMI_WANT_EMIT_COUNTS without an emission-count buffer");
   if (p->B == 0) return TKFMI_OK;
   if (!p->log_trans || !p->log_tab || !p->x_cat || !p->y_cat)
     return fail(TKFMI_EINVAL, "device input pointer missing");
@@ -732,11 +750,14 @@ int tkfmi_fb2d_batched(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws,
   dp.out_logP = o->log_prob;
   dp.out_n = (flags & TKFMI_NO_COUNTS) ? nullptr : o->n_trans;
   dp.out_post = o->posteriors;
+  dp.out_ecnt = (flags & TKFMI_WANT_EMIT_COUNTS) ? o->emit_counts : nullptr;
   const int H = pl.H;
   const int HB = std::max(2 * H * pl.GL, pl.W * pl.GL * pl.NB);
   // LDS layout (doubles): tables | exchange/reduction | misc | row buffer | sequences
   const size_t kLdsBudget = 160 * 1024;
-  const size_t fixed_d = (size_t)p->n_groups * TAB_SZ + HB + 8;
+  const bool ecnt = (flags & TKFMI_WANT_EMIT_COUNTS) && !(flags & TKFMI_FORWARD_ONLY);
+  const size_t fixed_d = (size_t)p->n_groups * TAB_SZ * (ecnt ? 2 : 1) + HB + 8;
+  dp.smem_ecnt_off = (int32_t)((size_t)p->n_groups * TAB_SZ + HB + 8);
   const size_t seq_bytes = (size_t)((pl.max_lx + 8) & ~7) + (size_t)((pl.max_ly + 8) & ~7) + 16;
   if (fixed_d * 8 + seq_bytes > kLdsBudget)
     return fail(TKFMI_EINVAL, "LDS budget exceeded (sequences too long for the emission groups)");
@@ -748,8 +769,16 @@ int tkfmi_fb2d_batched(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws,
   const size_t seq_off = (fixed_d + rb_cap) * 8;
   dp.smem_seq_off = (int32_t)seq_off;
   const size_t smem = seq_off + seq_bytes;
-  if (pl.GL == 4) return launch<4, 3, 4>(pl, dp, B, smem, stream);
-  return launch<16, 15, 8>(pl, dp, B, smem, stream);
+  int64_t cells = 0;
+  for (int b = 0; b < B; ++b) cells += (int64_t)(p->h_x_len[b] + 1) * (p->h_y_len[b] + 1);
+  std::string err;
+  const int slot = tkfmi_ktime_start(pl.GL == 4 ? "fb2d_kernel<4,3,4>" : "fb2d_kernel<16,15,8>", B,
+                                     cells, stream, err);
+  if (slot == -2) return fail(TKFMI_EHIP, err);
+  rc = (pl.GL == 4) ? launch<4, 3, 4>(pl, dp, B, smem, stream) : launch<16, 15, 8>(pl, dp, B, smem, stream);
+  if (rc != TKFMI_OK) return rc;
+  if (tkfmi_ktime_stop(slot, stream, err) != TKFMI_OK) return fail(TKFMI_EHIP, err);
+  return TKFMI_OK;
 }
 
 }  // extern "C"

@@ -22,4 +22,11 @@ bool tkfmi_v2_workspace(const tkfmi_problem* p, uint32_t flags, size_t* bytes);
 int tkfmi_v2_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, size_t ws_bytes,
                  hipStream_t stream, uint32_t flags, std::string& err);
 
+// Per-kernel timing (TKFMI_TIME_KERNELS): begin_call arms or disarms the
+// recorder for this thread's call; start/stop bracket one kernel launch on
+// stream `s` (start returns a slot, -1 when disarmed, -2 on error).
+void tkfmi_ktime_begin_call(bool enabled);
+int tkfmi_ktime_start(const char* name, int n_pairs, int64_t cells, hipStream_t s, std::string& err);
+int tkfmi_ktime_stop(int slot, hipStream_t s, std::string& err);
+
 #endif

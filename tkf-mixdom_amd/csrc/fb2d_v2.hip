@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -94,7 +95,7 @@ struct V2Desc {
 struct V2Params {
   int32_t S, flags, n_tab, G_tab;
   int32_t gmap[16];  // layout group -> caller table group
-  int32_t smem_ring_off, smem_stage_off, smem_seq_off, pad;
+  int32_t smem_ring_off, smem_stage_off, smem_seq_off, smem_ecnt_off;
   const double* log_tab;
   const uint8_t* x_cat;
   const uint8_t* y_cat;
@@ -105,6 +106,7 @@ struct V2Params {
   double* out_logP;
   double* out_n;
   double* out_post;
+  double* out_ecnt;
 };
 
 // Per pair transition block (probability space), doubles:
@@ -228,6 +230,7 @@ __global__ __launch_bounds__(W * 64) void fbv2_kernel(V2Params Pr, const double*
   double* s_misc = s_tab + NG * TAB_SZ;               // 8
   double* s_ring = smem + Pr.smem_ring_off;           // (W-1)*2*NX short + RING*NX long
   double* s_stage = smem + Pr.smem_stage_off;         // W * 1024 doubles
+  double* s_ecnt = smem + Pr.smem_ecnt_off;           // NG * TAB_SZ emission counts
   uint8_t* s_x = reinterpret_cast<uint8_t*>(smem) + Pr.smem_seq_off;
   uint8_t* s_y = s_x + ((Lx + 8) & ~7);
 
@@ -240,6 +243,10 @@ __global__ __launch_bounds__(W * 64) void fbv2_kernel(V2Params Pr, const double*
   for (int q = threadIdx.x; q < Lx; q += W * 64) s_x[q] = min((int)Pr.x_cat[pd.x_off + q], A_PAD - 1);
   for (int q = threadIdx.x; q < Ly; q += W * 64) s_y[q] = min((int)Pr.y_cat[pd.y_off + q], A_PAD - 1);
   if (threadIdx.x == 0) { s_x[Lx] = 0; s_y[Ly] = 0; }
+  const bool want_ecnt = !(Pr.flags & TKFMI_FORWARD_ONLY) && (Pr.flags & TKFMI_WANT_EMIT_COUNTS) &&
+                         Pr.out_ecnt != nullptr;
+  if (want_ecnt)
+    for (int q = threadIdx.x; q < NG * TAB_SZ; q += W * 64) s_ecnt[q] = 0.0;
 
   const double* __restrict__ pm = pmat + pd.pm_off;
   const double* __restrict__ Pf = pm;                    // P[s][k]
@@ -593,6 +600,31 @@ __global__ __launch_bounds__(W * 64) void fbv2_kernel(V2Params Pr, const double*
           for (int b = 0; b < NB; ++b) out_n[b + L::SOFF] = p_start[b] * g[b] * sc;
         }
       }
+      const int xi = valid ? s_x[i >= 1 ? i - 1 : 0] : 0;
+      const int yj = valid ? s_y[j >= 1 ? j - 1 : 0] : 0;
+      // ---- expected emission counts: posterior mass per (group, type) onto
+      // the table entry the cell's emission used
+      if (want_ecnt && valid) {
+        const double pscale = ldexp(inv_mP, EF + EB - EP);
+        double sM[NG], sI[NG], sD[NG];
+#pragma unroll
+        for (int gq = 0; gq < NG; ++gq) { sM[gq] = 0.0; sI[gq] = 0.0; sD[gq] = 0.0; }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const int ty = L::type(b), gq = L::group(b);
+          const double pv = fcur[b] * Bv[b];
+          if (ty == ST_M) sM[gq] += pv;
+          else if (ty == ST_I) sI[gq] += pv;
+          else sD[gq] += pv;
+        }
+#pragma unroll
+        for (int gq = 0; gq < NG; ++gq) {
+          double* tb = s_ecnt + gq * TAB_SZ;
+          if (i >= 1 && j >= 1) atomicAdd(tb + xi * A_PAD + yj, sM[gq] * pscale);
+          if (j >= 1) atomicAdd(tb + A_PAD * A_PAD + yj, sI[gq] * pscale);
+          if (i >= 1) atomicAdd(tb + A_PAD * A_PAD + A_PAD + xi, sD[gq] * pscale);
+        }
+      }
       if (want_post && valid) {
         const double ps_ = ldexp(inv_mP, EF + EB - EP);
         double* dst = post + ((size_t)i * W1 + j) * S + L::SOFF;
@@ -606,8 +638,6 @@ __global__ __launch_bounds__(W * 64) void fbv2_kernel(V2Params Pr, const double*
         }
       }
       // ---- q = e(i,j,k) B(i,j,k) for the predecessor row
-      const int xi = valid ? s_x[i >= 1 ? i - 1 : 0] : 0;
-      const int yj = valid ? s_y[j >= 1 ? j - 1 : 0] : 0;
       const bool eM_ok = valid && i >= 1 && j >= 1, eI_ok = valid && j >= 1, eD_ok = valid && i >= 1;
       double q[NB];
 #pragma unroll
@@ -654,9 +684,16 @@ __global__ __launch_bounds__(W * 64) void fbv2_kernel(V2Params Pr, const double*
       else lds_barrier();
     }
   }
+  __syncthreads();
+  if (want_ecnt) {  // layout group g -> caller group gmap[g]; each entry owned by one thread
+    double* oe = Pr.out_ecnt + (size_t)pd.pair * Pr.G_tab * TAB_SZ;
+    for (int q = threadIdx.x; q < TAB_SZ; q += W * 64) {
+#pragma unroll
+      for (int gq = 0; gq < NG; ++gq) oe[Pr.gmap[gq] * TAB_SZ + q] += s_ecnt[gq * TAB_SZ + q];
+    }
+  }
   if (!want_counts) return;
   // ---- reduce the MFMA accumulators over the waves (fixed order)
-  __syncthreads();
   double* s_red = s_stage;  // [W][16][16]
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
@@ -716,6 +753,11 @@ bool detect(const tkfmi_problem* p, Plan& pl) {
     return true;
   }
   return false;
+}
+
+std::string layout_name(const Plan& pl) {
+  if (pl.lay == Layout::Tkf) return "LTkf";
+  return "LMix<" + std::to_string(pl.K) + "," + std::to_string(pl.F) + ">";
 }
 
 size_t stride_for(int NB) { return (size_t)(((2 * NB * NB + 2 * NB + 1) + 15) / 16 * 16); }
@@ -818,9 +860,11 @@ struct CallPlan {
   std::string why;
   std::vector<int> order;          // processing order (class, then cost)
   std::vector<V2Desc> desc;        // in processing order
+  int64_t cls_cells[4] = {0, 0, 0, 0};
   int cls_begin[4] = {0, 0, 0, 0}, cls_count[4] = {0, 0, 0, 0};
   size_t smem[4] = {0, 0, 0, 0};
   int ring_off[4] = {0, 0, 0, 0}, stage_off[4] = {0, 0, 0, 0}, seq_off[4] = {0, 0, 0, 0};
+  int ecnt_off[4] = {0, 0, 0, 0};
   size_t desc_b = 0, pm_b = 0, lat_b = 0, gring_b = 0;
   size_t total() const { return desc_b + pm_b + lat_b + gring_b; }
 };
@@ -829,10 +873,12 @@ constexpr int kWs[4] = {8, 4, 2, 1};
 int cls_of(int W) { return W == 8 ? 0 : (W == 4 ? 1 : (W == 2 ? 2 : 3)); }
 constexpr size_t kLds = 160 * 1024;
 
-size_t lds_layout(const Plan& pl, int W, int ring_cap, int max_lx, int max_ly, int& ring_off,
-                  int& stage_off, int& seq_off) {
+size_t lds_layout(const Plan& pl, int W, int ring_cap, int max_lx, int max_ly, bool ecnt,
+                  int& ring_off, int& stage_off, int& seq_off, int& ecnt_off) {
   const int NX = nx_of(pl);
   size_t d = (size_t)pl.NG * TAB_SZ + 8;
+  ecnt_off = (int)d;
+  if (ecnt) d += (size_t)pl.NG * TAB_SZ;
   ring_off = (int)d;
   d += (size_t)(W - 1) * 2 * NX + (size_t)ring_cap * NX;
   d = (d + 1) & ~(size_t)1;
@@ -872,16 +918,23 @@ CallPlan plan_call(const tkfmi_problem* p, const Plan& pl, uint32_t flags) {
     max_ring[c] = std::max(max_ring[c], gm[b].ring);
   }
   int ring_cap[4] = {2, 2, 2, 2};
+  const bool ecnt = (flags & TKFMI_WANT_EMIT_COUNTS) && !(flags & TKFMI_FORWARD_ONLY);
   for (int c = 0; c < 4; ++c) {
     if (!cp.cls_count[c]) continue;
-    int ro, so, qo;
-    const size_t base = lds_layout(pl, kWs[c], 0, max_lx[c], max_ly[c], ro, so, qo);
+    int ro, so, qo, eo;
+    const size_t base = lds_layout(pl, kWs[c], 0, max_lx[c], max_ly[c], ecnt, ro, so, qo, eo);
     if (base + (size_t)34 * NX * 8 > kLds) { cp.why = "LDS budget exceeded (sequences too long)"; return cp; }
     const size_t room = (kLds - base) / ((size_t)NX * 8);
     ring_cap[c] = (int)std::min<size_t>((size_t)max_ring[c], room);
-    cp.smem[c] = lds_layout(pl, kWs[c], ring_cap[c], max_lx[c], max_ly[c], cp.ring_off[c],
-                            cp.stage_off[c], cp.seq_off[c]);
+    cp.smem[c] = lds_layout(pl, kWs[c], ring_cap[c], max_lx[c], max_ly[c], ecnt, cp.ring_off[c],
+                            cp.stage_off[c], cp.seq_off[c], cp.ecnt_off[c]);
   }
+  // The 8-wave class holds the longest pairs, whose serial step chains set
+  // the makespan of the call.  When it has few pairs, its workgroups ask for
+  // the whole LDS so that no other workgroup shares their CUs and slows their
+  // steps (the other classes have ample CUs left).
+  static const bool no_reserve = getenv("TKFMI_NO_CU_RESERVE") != nullptr;
+  if (!no_reserve && cp.cls_count[0] > 0 && cp.cls_count[0] <= 128) cp.smem[0] = kLds;
   // descriptors and workspace offsets
   cp.desc.resize(B);
   std::vector<int64_t> lat_off(B), post_off(B);
@@ -907,6 +960,7 @@ CallPlan plan_call(const tkfmi_problem* p, const Plan& pl, uint32_t flags) {
     d.lat_off = lat_off[b]; d.post_off = post_off[b];
     d.pm_off = (p->n_trans > 1 ? (int64_t)b : 0) * (int64_t)stride_for(pl.NB);
     cp.desc[q] = d;
+    cp.cls_cells[cls_of(g.W)] += (int64_t)(d.Lx + 1) * (d.Ly + 1);
   }
   cp.desc_b = align256(sizeof(V2Desc) * (size_t)std::max(B, 1));
   cp.pm_b = align256(stride_for(pl.NB) * 8 * (size_t)p->n_trans);
@@ -974,6 +1028,7 @@ int tkfmi_v2_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, size_
   prm.out_logP = o->log_prob;
   prm.out_n = (flags & TKFMI_NO_COUNTS) ? nullptr : o->n_trans;
   prm.out_post = o->posteriors;
+  prm.out_ecnt = (flags & TKFMI_WANT_EMIT_COUNTS) ? o->emit_counts : nullptr;
   for (int c = 0; c < 4; ++c) {
     if (!cp.cls_count[c]) continue;
     hipStream_t s = sts->s[c];
@@ -981,6 +1036,10 @@ int tkfmi_v2_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, size_
     V2Params pc = prm;
     pc.desc = ddesc + cp.cls_begin[c];
     pc.smem_ring_off = cp.ring_off[c]; pc.smem_stage_off = cp.stage_off[c]; pc.smem_seq_off = cp.seq_off[c];
+    pc.smem_ecnt_off = cp.ecnt_off[c];
+    const std::string kname = std::string("fbv2_kernel<") + layout_name(pl) + "," + std::to_string(kWs[c]) + ">";
+    const int slot = tkfmi_ktime_start(kname.c_str(), cp.cls_count[c], cp.cls_cells[c], s, err);
+    if (slot == -2) return TKFMI_EHIP;
     switch (pl.lay) {
       case Layout::Tkf: rc = launch_class<LTkf>(kWs[c], pc, cp.cls_count[c], cp.smem[c], s, err); break;
       case Layout::Mix11: rc = launch_class<LMix<1, 1>>(kWs[c], pc, cp.cls_count[c], cp.smem[c], s, err); break;
@@ -991,6 +1050,7 @@ int tkfmi_v2_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, size_
       default: rc = TKFMI_EINVAL;
     }
     if (rc != TKFMI_OK) return rc;
+    if (tkfmi_ktime_stop(slot, s, err) != TKFMI_OK) return TKFMI_EHIP;
     if (hipEventRecord(sts->done[c], s) != hipSuccess) { err = "hipEventRecord failed"; return TKFMI_EHIP; }
     if (hipStreamWaitEvent(stream, sts->done[c], 0) != hipSuccess) { err = "hipStreamWaitEvent failed"; return TKFMI_EHIP; }
   }

@@ -14,11 +14,13 @@ import torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libtkfmi.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 FORWARD_ONLY = 1
 WANT_POSTERIORS = 2
 NO_COUNTS = 4
 GENERIC_KERNEL = 8
+WANT_EMIT_COUNTS = 16
+TIME_KERNELS = 32
 A_PAD = 21
 TAB_SZ = A_PAD * A_PAD + 2 * A_PAD
 
@@ -50,6 +52,17 @@ class Outputs(ctypes.Structure):
         ("log_prob", _c_p),
         ("n_trans", _c_p),
         ("posteriors", _c_p),
+        ("emit_counts", _c_p),
+    ]
+
+
+class KernelTime(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 64),
+        ("n_pairs", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("cells", ctypes.c_int64),
+        ("ms", ctypes.c_double),
     ]
 
 
@@ -58,7 +71,8 @@ _lock = threading.Lock()
 
 # Every symbol include/tkfmi.h declares (checked by the CPU test-suite).
 EXPORTED = ("tkfmi_abi_version", "tkfmi_last_error",
-            "tkfmi_fb2d_workspace_bytes", "tkfmi_fb2d_batched")
+            "tkfmi_fb2d_workspace_bytes", "tkfmi_fb2d_batched",
+            "tkfmi_last_kernel_times")
 
 
 class TkfmiError(RuntimeError):
@@ -82,11 +96,25 @@ def lib():
             L.tkfmi_fb2d_batched.restype = ctypes.c_int
             L.tkfmi_fb2d_batched.argtypes = [ctypes.POINTER(Problem), ctypes.POINTER(Outputs),
                                              _c_p, ctypes.c_size_t, _c_p, ctypes.c_uint32]
+            L.tkfmi_last_kernel_times.restype = ctypes.c_int
+            L.tkfmi_last_kernel_times.argtypes = [ctypes.POINTER(KernelTime), ctypes.c_int32]
             v = L.tkfmi_abi_version()
             if v != ABI_VERSION:
                 raise TkfmiError(f"libtkfmi ABI {v} != expected {ABI_VERSION}")
             _lib = L
         return _lib
+
+
+def last_kernel_times(max_entries=16):
+    """Kernel durations of this thread's last call made with TIME_KERNELS:
+    list of dicts (name, n_pairs, cells, ms).  Waits for those kernels."""
+    L = lib()
+    buf = (KernelTime * max_entries)()
+    n = L.tkfmi_last_kernel_times(buf, max_entries)
+    if n < 0:
+        raise TkfmiError(L.tkfmi_last_error().decode() or f"tkfmi error {n}")
+    return [{"name": buf[i].name.decode(), "n_pairs": int(buf[i].n_pairs),
+             "cells": int(buf[i].cells), "ms": float(buf[i].ms)} for i in range(n)]
 
 
 def _ptr(t):
@@ -130,67 +158,24 @@ def normalise_codes(seq):
 
 
 def fb2d_batched(log_trans, state_types, state_group, log_tab, pairs, *,
-                 forward_only=False, want_posteriors=False, want_counts=True, generic=False):
-    """Run the batched forward/backward kernel.
+                 forward_only=False, want_posteriors=False, want_counts=True,
+                 want_emit_counts=False, generic=False):
+    """Run the batched forward/backward kernel once.
 
     log_trans: float64 device tensor (S,S) or (B,S,S) of log transitions.
     state_types, state_group: int sequences of length S.
     log_tab: float64 device tensor (G, TAB_SZ) or (B, G, TAB_SZ).
     pairs: PackedPairs.
-    Returns (log_prob (B,), n (B,S,S) or None, posteriors (flat) or None).
+    Returns a dict: 'log_p' (B,), and unless forward_only 'n' (B,S,S)
+    (want_counts), 'posteriors' (flat, want_posteriors), 'emit_counts'
+    (B,G,TAB_SZ, want_emit_counts).
     """
-    L = lib()
-    dev = log_trans.device
-    if dev.type != "cuda":
-        raise TkfmiError("tkfmi kernels need a CUDA/HIP device tensor")
-    S = int(log_trans.shape[-1])
-    B = pairs.B
-    st = np.ascontiguousarray(np.asarray(state_types, dtype=np.int32))
-    sg = np.ascontiguousarray(np.asarray(state_group, dtype=np.int32))
-    if st.shape != (S,) or sg.shape != (S,):
-        raise ValueError("state_types/state_group must have length S")
-    log_trans = log_trans.to(torch.float64).contiguous()
-    log_tab = log_tab.to(torch.float64).contiguous()
-    n_trans = 1 if log_trans.dim() == 2 else int(log_trans.shape[0])
-    if log_tab.dim() == 2:
-        n_tab, G = 1, int(log_tab.shape[0])
-    else:
-        n_tab, G = int(log_tab.shape[0]), int(log_tab.shape[1])
-    if log_tab.shape[-1] != TAB_SZ:
-        raise ValueError(f"log_tab last dim must be {TAB_SZ}")
-    flags = 0
-    if forward_only:
-        flags |= FORWARD_ONLY
-    if want_posteriors and not forward_only:
-        flags |= WANT_POSTERIORS
-    if not want_counts:
-        flags |= NO_COUNTS
-    if generic:
-        flags |= GENERIC_KERNEL
-    prob = Problem(B, S, _np_ptr(st), _np_ptr(sg), G, n_trans, _ptr(log_trans), n_tab,
-                   _ptr(log_tab), _ptr(pairs.x_cat), _ptr(pairs.y_cat),
-                   _np_ptr(pairs.x_off), _np_ptr(pairs.y_off),
-                   _np_ptr(pairs.x_len), _np_ptr(pairs.y_len))
-    ws_bytes = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(prob), flags)
-    if ws_bytes == 0 and B > 0:
-        # size query failed: run the call anyway to obtain the message
-        rc = L.tkfmi_fb2d_batched(ctypes.byref(prob), ctypes.byref(Outputs()), None, 0, None, flags)
-        raise TkfmiError(L.tkfmi_last_error().decode() or f"tkfmi error {rc}")
-    logp = torch.empty(B, dtype=torch.float64, device=dev)
-    n = None
-    if want_counts and not forward_only:
-        n = torch.zeros((B, S, S), dtype=torch.float64, device=dev)
-    post = None
-    if want_posteriors and not forward_only:
-        post = torch.zeros(int(pairs.cells.sum()) * S + 1, dtype=torch.float64, device=dev)
-    ws = torch.empty(max(int(ws_bytes), 1), dtype=torch.uint8, device=dev)
-    out = Outputs(_ptr(logp), _ptr(n), _ptr(post))
-    stream = torch.cuda.current_stream(dev).cuda_stream
-    rc = L.tkfmi_fb2d_batched(ctypes.byref(prob), ctypes.byref(out), _ptr(ws),
-                              ctypes.c_size_t(int(ws_bytes)), ctypes.c_void_p(stream), flags)
-    if rc != 0:
-        raise TkfmiError(L.tkfmi_last_error().decode() or f"tkfmi error {rc}")
-    return logp, n, post
+    r = BatchRunner(log_trans, state_types, state_group, log_tab, pairs,
+                    forward_only=forward_only, want_posteriors=want_posteriors,
+                    want_counts=want_counts, want_emit_counts=want_emit_counts,
+                    generic=generic)
+    r.run(zero_outputs=False)  # outputs are freshly zeroed
+    return r.results()
 
 
 def split_posteriors(post_flat, pairs, S):
@@ -210,25 +195,36 @@ class BatchRunner:
     per-call host work is the descriptor sort/upload inside the library)."""
 
     def __init__(self, log_trans, state_types, state_group, log_tab, pairs, *,
-                 forward_only=False, want_posteriors=False, want_counts=True, generic=False):
+                 forward_only=False, want_posteriors=False, want_counts=True,
+                 want_emit_counts=False, generic=False, time_kernels=False):
         self.L = lib()
         dev = log_trans.device
+        if dev.type != "cuda":
+            raise TkfmiError("tkfmi kernels need a CUDA/HIP device tensor")
         self.dev = dev
         self.pairs = pairs
         self.S = S = int(log_trans.shape[-1])
         self.st = np.ascontiguousarray(np.asarray(state_types, dtype=np.int32))
         self.sg = np.ascontiguousarray(np.asarray(state_group, dtype=np.int32))
+        if self.st.shape != (S,) or self.sg.shape != (S,):
+            raise ValueError("state_types/state_group must have length S")
         self.log_trans = log_trans.to(torch.float64).contiguous()
         self.log_tab = log_tab.to(torch.float64).contiguous()
+        if self.log_tab.shape[-1] != TAB_SZ:
+            raise ValueError(f"log_tab last dim must be {TAB_SZ}")
         n_trans = 1 if self.log_trans.dim() == 2 else int(self.log_trans.shape[0])
         if self.log_tab.dim() == 2:
             n_tab, G = 1, int(self.log_tab.shape[0])
         else:
             n_tab, G = int(self.log_tab.shape[0]), int(self.log_tab.shape[1])
-        self.flags = ((FORWARD_ONLY if forward_only else 0)
-                      | (WANT_POSTERIORS if (want_posteriors and not forward_only) else 0)
+        self.G = G
+        fwd = bool(forward_only)
+        self.flags = ((FORWARD_ONLY if fwd else 0)
+                      | (WANT_POSTERIORS if (want_posteriors and not fwd) else 0)
                       | (0 if want_counts else NO_COUNTS)
-                      | (GENERIC_KERNEL if generic else 0))
+                      | (WANT_EMIT_COUNTS if (want_emit_counts and not fwd) else 0)
+                      | (GENERIC_KERNEL if generic else 0)
+                      | (TIME_KERNELS if time_kernels else 0))
         B = pairs.B
         self.prob = Problem(B, S, _np_ptr(self.st), _np_ptr(self.sg), G, n_trans,
                             _ptr(self.log_trans), n_tab, _ptr(self.log_tab),
@@ -236,27 +232,41 @@ class BatchRunner:
                             _np_ptr(pairs.y_off), _np_ptr(pairs.x_len), _np_ptr(pairs.y_len))
         self.ws_bytes = int(self.L.tkfmi_fb2d_workspace_bytes(ctypes.byref(self.prob), self.flags))
         if self.ws_bytes == 0 and B > 0:
+            # size query failed: run the call anyway to obtain the message
             rc = self.L.tkfmi_fb2d_batched(ctypes.byref(self.prob), ctypes.byref(Outputs()),
                                            None, 0, None, self.flags)
             raise TkfmiError(self.L.tkfmi_last_error().decode() or f"tkfmi error {rc}")
         self.ws = torch.empty(max(self.ws_bytes, 1), dtype=torch.uint8, device=dev)
         self.log_p = torch.empty(B, dtype=torch.float64, device=dev)
         self.n = (torch.zeros((B, S, S), dtype=torch.float64, device=dev)
-                  if (want_counts and not forward_only) else None)
+                  if (want_counts and not fwd) else None)
         self.post = (torch.zeros(int(pairs.cells.sum()) * S + 1, dtype=torch.float64, device=dev)
                      if (self.flags & WANT_POSTERIORS) else None)
-        self.out = Outputs(_ptr(self.log_p), _ptr(self.n), _ptr(self.post))
+        self.ecnt = (torch.zeros((B, G, TAB_SZ), dtype=torch.float64, device=dev)
+                     if (self.flags & WANT_EMIT_COUNTS) else None)
+        self.out = Outputs(_ptr(self.log_p), _ptr(self.n), _ptr(self.post), _ptr(self.ecnt))
 
-    def run(self, zero_outputs=True):
+    def run(self, zero_outputs=True, stream=None):
+        """Enqueue one call on ``stream`` (default: torch's current stream)."""
         if zero_outputs:
-            if self.n is not None:
-                self.n.zero_()
-            if self.post is not None:
-                self.post.zero_()
-        stream = torch.cuda.current_stream(self.dev).cuda_stream
+            for t in (self.n, self.post, self.ecnt):
+                if t is not None:
+                    t.zero_()
+        if stream is None:
+            stream = torch.cuda.current_stream(self.dev)
         rc = self.L.tkfmi_fb2d_batched(ctypes.byref(self.prob), ctypes.byref(self.out),
                                        _ptr(self.ws), ctypes.c_size_t(self.ws_bytes),
-                                       ctypes.c_void_p(stream), self.flags)
+                                       ctypes.c_void_p(stream.cuda_stream), self.flags)
         if rc != 0:
             raise TkfmiError(self.L.tkfmi_last_error().decode() or f"tkfmi error {rc}")
         return self.log_p, self.n, self.post
+
+    def results(self):
+        out = {"log_p": self.log_p}
+        if self.n is not None:
+            out["n"] = self.n
+        if self.post is not None:
+            out["posteriors"] = self.post
+        if self.ecnt is not None:
+            out["emit_counts"] = self.ecnt
+        return out

@@ -83,14 +83,17 @@ def _types_list(state_types):
 
 
 def fb2d_batch(log_trans, state_types, xs, ys, log_tab, state_group=None, *,
-               forward_only=False, want_posteriors=False, want_counts=True, generic=False):
+               forward_only=False, want_posteriors=False, want_counts=True,
+               want_emit_counts=False, generic=False):
     """One launch over a ragged batch of pairs.
 
     log_trans: (S,S) or (B,S,S) log transitions; log_tab: (G,TAB) or
     (B,G,TAB) log emission tables from ``tkfmi.tables``; xs, ys: sequences
     of integer code arrays (real lengths, no padding).
     Returns dict with 'log_p' (B,), 'n' (B,S,S) (unless forward_only or
-    want_counts=False) and 'posteriors' (list of (Lx+1,Ly+1,S) views).
+    want_counts=False), 'posteriors' (list of (Lx+1,Ly+1,S) views) and
+    'emit_counts' (B,G,TAB): expected uses of each emission-table entry,
+    i.e. d log_p / d log_tab.
     ``generic=True`` forces the any-layout kernel (the TKF and MixDom K*F<=3
     layouts otherwise run on the cell-per-lane kernel).
     """
@@ -102,15 +105,11 @@ def fb2d_batch(log_trans, state_types, xs, ys, log_tab, state_group=None, *,
     if state_group is None:
         state_group = [0] * S
     pairs = _abi.PackedPairs(list(xs), list(ys), dev)
-    logp, n, post = _abi.fb2d_batched(lt, st, state_group, tab, pairs,
-                                      forward_only=forward_only,
-                                      want_posteriors=want_posteriors,
-                                      want_counts=want_counts, generic=generic)
-    out = {"log_p": logp}
-    if n is not None:
-        out["n"] = n
-    if post is not None:
-        out["posteriors"] = _abi.split_posteriors(post, pairs, S)
+    out = _abi.fb2d_batched(lt, st, state_group, tab, pairs, forward_only=forward_only,
+                            want_posteriors=want_posteriors, want_counts=want_counts,
+                            want_emit_counts=want_emit_counts, generic=generic)
+    if "posteriors" in out:
+        out["posteriors"] = _abi.split_posteriors(out["posteriors"], pairs, S)
     return out
 
 
