@@ -1,0 +1,206 @@
+"""Differentiable pair log-likelihoods (reference: train/vjp.py).
+
+The gradient of a pair-HMM log-likelihood with respect to any parameter
+theta is the score identity the reference's custom VJPs use
+(train/vjp.py:156-191, 282-326, 428-455, 770-817):
+
+    d logP / d theta = sum_{k,k'} n[k,k'] d log T[k,k'] / d theta
+                     + sum_{g,e}  c[g,e] d log tab[g,e] / d theta
+
+where n are the expected transition counts and c the expected emission
+counts, both produced by the backward DP kernel.  :class:`FB2DLogProb` is a
+``torch.autograd.Function`` whose backward returns exactly (g n, g c) for the
+log transition matrices and log emission tables it was given; torch autograd
+then carries them through the differentiable builders (tkfmi.params,
+tkfmi.mixdom, tkfmi.tables, matrix_exp) to lambda, mu, t, ext, the domain /
+fragment weights, exchangeabilities, pi and the class distribution.  The DP
+itself is never differentiated by autograd -- the hand-written backward
+sweep is its adjoint.
+
+Every function takes a scalar ``t`` and one (x, y) pair like the reference,
+and has a ``*_batch`` form over a ragged batch with per-pair ``ts`` that runs
+as one kernel call.
+"""
+import torch
+
+from . import _abi, hmm, mixdom, params, tables
+
+
+class _Spec:
+    """Non-tensor inputs of one batched call."""
+
+    def __init__(self, state_types, state_group, xs, ys, device, generic=False):
+        self.state_types = hmm._types_list(state_types)
+        self.state_group = list(state_group) if state_group is not None else [0] * len(self.state_types)
+        self.pairs = _abi.PackedPairs(list(xs), list(ys), device)
+        self.generic = generic
+
+
+class FB2DLogProb(torch.autograd.Function):
+    """log P of each pair of a batch; backward = expected counts.
+
+    log_trans: (S,S) shared or (B,S,S) per pair; log_tab: (G,TAB) shared or
+    (B,G,TAB) per pair.  Returns (B,) log-likelihoods."""
+
+    @staticmethod
+    def forward(ctx, log_trans, log_tab, spec):
+        r = _abi.BatchRunner(log_trans.detach(), spec.state_types, spec.state_group,
+                             log_tab.detach(), spec.pairs, want_emit_counts=True,
+                             generic=spec.generic)
+        r.run(zero_outputs=False)
+        ctx.save_for_backward(r.n, r.ecnt)
+        ctx.shared = (log_trans.dim() == 2, log_tab.dim() == 2)
+        return r.log_p
+
+    @staticmethod
+    def backward(ctx, g):
+        n, ec = ctx.saved_tensors
+        g = g.to(torch.float64)[:, None, None]
+        gt, ge = n * g, ec * g
+        if ctx.shared[0]:
+            gt = gt.sum(0)
+        if ctx.shared[1]:
+            ge = ge.sum(0)
+        return gt, ge, None
+
+
+def log_prob_batch(log_trans, state_types, xs, ys, log_tab, state_group=None, generic=False):
+    """Differentiable (B,) log-likelihoods of a ragged batch (one call)."""
+    dev = log_trans.device if isinstance(log_trans, torch.Tensor) else hmm.device()
+    spec = _Spec(state_types, state_group, xs, ys, dev, generic)
+    return FB2DLogProb.apply(log_trans.to(torch.float64), log_tab.to(torch.float64), spec)
+
+
+# ------------------------------------------------------------ substitution
+def transition_matrix(Q, t):
+    """P(t) = expm(Q t) (core/ctmc.py:31-60, Pade with its canonical
+    tangent); ``t`` may carry leading batch dimensions."""
+    Q = torch.as_tensor(Q, dtype=torch.float64)
+    t = torch.as_tensor(t, dtype=torch.float64, device=Q.device)
+    return torch.linalg.matrix_exp(t[..., None, None] * Q)
+
+
+def build_Q_from_S_pi(S_exch, pi):
+    """GTR rate matrix Q[a,b] = S[a,b] pi[b] off the diagonal, rows summing to
+    zero, no rate normalisation (core/ctmc.py:63-95; train/vjp.py:463-475)."""
+    S_exch = torch.as_tensor(S_exch, dtype=torch.float64)
+    pi = torch.as_tensor(pi, dtype=torch.float64, device=S_exch.device)
+    A = pi.shape[-1]
+    off = 1.0 - torch.eye(A, dtype=torch.float64, device=S_exch.device)
+    Q_off = S_exch * pi[..., None, :] * off
+    return Q_off - torch.diag_embed(Q_off.sum(dim=-1))
+
+
+def _dev():
+    return hmm.device()
+
+
+def _as(x, dev):
+    return x.to(device=dev, dtype=torch.float64) if isinstance(x, torch.Tensor) \
+        else torch.as_tensor(x, dtype=torch.float64, device=dev)
+
+
+# ----------------------------------------------------------------- TKF91/92
+def tkf91_log_prob_batch(ins_rate, del_rate, ts, Q, pi, xs, ys, *, cond=False, generic=False):
+    """(B,) log P(x_b, y_b | lambda, mu, t_b, Q, pi) under TKF91 with the
+    shared emissions of pair_hmm_emissions (train/vjp.py:110-194); ``cond``
+    uses the ancestor-length-conditioned matrix (train/vjp.py:198-249)."""
+    dev = _dev()
+    ts = _as(ts, dev)
+    build = params.tkf91_trans_cond if cond else params.tkf91_trans
+    log_trans = hmm.safe_log(build(_as(ins_rate, dev), _as(del_rate, dev), ts))
+    sub = transition_matrix(_as(Q, dev), ts)                       # (B,A,A)
+    tab = tables.shared_tables(sub, _as(pi, dev).expand(sub.shape[:-1]))  # (B,1,TAB)
+    return log_prob_batch(log_trans, params.TKF_STATE_TYPES, xs, ys, tab, generic=generic)
+
+
+def tkf91_log_prob(ins_rate, del_rate, t, Q, pi, x_seq, y_seq):
+    """Reference surface of train/vjp.py:110-127 (scalar log P)."""
+    return tkf91_log_prob_batch(ins_rate, del_rate, _as(t, _dev()).reshape(1), Q, pi,
+                                [x_seq], [y_seq])[0]
+
+
+def tkf91_log_prob_cond(ins_rate, del_rate, t, Q, pi, x_seq, y_seq):
+    """Reference surface of train/vjp.py:198-205."""
+    return tkf91_log_prob_batch(ins_rate, del_rate, _as(t, _dev()).reshape(1), Q, pi,
+                                [x_seq], [y_seq], cond=True)[0]
+
+
+def tkf92_log_prob_batch(ins_rate, del_rate, ts, ext, Q, pi, xs, ys, *, generic=False):
+    """(B,) TKF92 log-likelihoods (train/vjp.py:253-329)."""
+    dev = _dev()
+    ts = _as(ts, dev)
+    log_trans = hmm.safe_log(params.tkf92_trans(_as(ins_rate, dev), _as(del_rate, dev), ts,
+                                                _as(ext, dev)))
+    sub = transition_matrix(_as(Q, dev), ts)
+    tab = tables.shared_tables(sub, _as(pi, dev).expand(sub.shape[:-1]))
+    return log_prob_batch(log_trans, params.TKF_STATE_TYPES, xs, ys, tab, generic=generic)
+
+
+def tkf92_log_prob(ins_rate, del_rate, t, ext, Q, pi, x_seq, y_seq):
+    """Reference surface of train/vjp.py:253-261."""
+    dev = _dev()
+    return tkf92_log_prob_batch(ins_rate, del_rate, _as(t, dev).reshape(1), ext, Q, pi,
+                                [x_seq], [y_seq])[0]
+
+
+# ------------------------------------------------------------------ MixDom
+def mixdom_log_prob_batch(main_ins_rate, main_del_rate, ts, dom_ins_rates, dom_del_rates,
+                          dom_weights, frag_weights, ext_rates, Q, pi, xs, ys, *, generic=False):
+    """(B,) MixDom log-likelihoods with shared emissions (train/vjp.py:351-458)."""
+    dev = _dev()
+    ts = _as(ts, dev)
+    chi = mixdom.build_nested_trans(_as(main_ins_rate, dev), _as(main_del_rate, dev), ts,
+                                    _as(dom_ins_rates, dev), _as(dom_del_rates, dev),
+                                    _as(dom_weights, dev), _as(frag_weights, dev),
+                                    _as(ext_rates, dev))
+    K, F = _as(frag_weights, dev).shape
+    st = mixdom.state_types(K, F)
+    sub = transition_matrix(_as(Q, dev), ts)
+    tab = tables.shared_tables(sub, _as(pi, dev).expand(sub.shape[:-1]))
+    return log_prob_batch(hmm.safe_log(chi), st, xs, ys, tab, generic=generic)
+
+
+def mixdom_log_prob(main_ins_rate, main_del_rate, t, dom_ins_rates, dom_del_rates, dom_weights,
+                    frag_weights, ext_rates, Q, pi, x_seq, y_seq):
+    """Reference surface of train/vjp.py:351-386."""
+    dev = _dev()
+    return mixdom_log_prob_batch(main_ins_rate, main_del_rate, _as(t, dev).reshape(1),
+                                 dom_ins_rates, dom_del_rates, dom_weights, frag_weights,
+                                 ext_rates, Q, pi, [x_seq], [y_seq])[0]
+
+
+def build_class_subs(class_S_exch, class_pis, ts):
+    """Per-class P_c(t_b) = expm(t_b Q_c), Q_c = GTR(S_c, pi_c), not rate
+    normalised (train/vjp.py:463-510).  Returns (B, C, A, A)."""
+    Qc = build_Q_from_S_pi(class_S_exch, class_pis)               # (C,A,A)
+    return transition_matrix(Qc[None], ts[:, None])               # (B,C,A,A)
+
+
+def mixdom2_log_prob_batch(main_ins_rate, main_del_rate, ts, dom_ins_rates, dom_del_rates,
+                           dom_weights, frag_weights, ext_rates, class_pis, class_S_exch,
+                           class_dist, xs, ys, *, generic=False):
+    """(B,) MixDom2 log-likelihoods: per-(domain, fragment) class-mixture
+    emissions (train/vjp.py:628-820; dp/hmm.py:673-783)."""
+    dev = _dev()
+    ts = _as(ts, dev)
+    chi = mixdom.build_nested_trans(_as(main_ins_rate, dev), _as(main_del_rate, dev), ts,
+                                    _as(dom_ins_rates, dev), _as(dom_del_rates, dev),
+                                    _as(dom_weights, dev), _as(frag_weights, dev),
+                                    _as(ext_rates, dev))
+    K, F = _as(frag_weights, dev).shape
+    st = mixdom.state_types(K, F)
+    subs = build_class_subs(_as(class_S_exch, dev), _as(class_pis, dev), ts)
+    tab = tables.per_class_tables(subs, _as(class_pis, dev), _as(class_dist, dev))  # (B,KF,TAB)
+    groups = tables.state_groups_per_class(len(st), F)
+    return log_prob_batch(hmm.safe_log(chi), st, xs, ys, tab, groups, generic=generic)
+
+
+def mixdom2_log_prob(main_ins_rate, main_del_rate, t, dom_ins_rates, dom_del_rates, dom_weights,
+                     frag_weights, ext_rates, class_pis, class_S_exch, class_dist, x_seq, y_seq):
+    """Reference surface of train/vjp.py:628-679."""
+    dev = _dev()
+    return mixdom2_log_prob_batch(main_ins_rate, main_del_rate, _as(t, dev).reshape(1),
+                                  dom_ins_rates, dom_del_rates, dom_weights, frag_weights,
+                                  ext_rates, class_pis, class_S_exch, class_dist,
+                                  [x_seq], [y_seq])[0]
