@@ -124,6 +124,78 @@ int tkfmi_fb2d_batched(const tkfmi_problem* prob, const tkfmi_outputs* out,
                        void* workspace, size_t workspace_bytes,
                        void* stream /* hipStream_t */, uint32_t flags);
 
+/* ------------------------------------------------------------------------
+ * Null-state count restoration, sufficient statistics and BDI statistics.
+ * ---------------------------------------------------------------------- */
+
+/* Shared MixDom structural parameters (host arrays). */
+typedef struct tkfmi_mixdom_model {
+  int32_t K, F;             /* domains (1..16), fragments per domain (>= 1)   */
+  double main_ins, main_del;
+  const double* dom_ins;    /* host [K]                                      */
+  const double* dom_del;    /* host [K]                                      */
+  const double* dom_w;      /* host [K]                                      */
+  const double* frag_w;     /* host [K,F]                                    */
+  const double* ext;        /* host [K,F,F]; MixDom1 (K,F) rates go on the
+                               diagonal (models/mixdom.py:284-287)            */
+} tkfmi_mixdom_model;
+
+/* Doubles per pair of the statistics vector written by
+ * tkfmi_restore_counts_batched, in this order:
+ *   top_5x5 [5,5] | dom_M_5x5 [K,5,5] | dom_kappa [K] | dom_1mkappa [K] |
+ *   ext [K,F,F] | term [K,F] | dom_w [K] | frag_w [K,F]
+ * (= 25 + 28K + K F (F + 2)). */
+int32_t tkfmi_suffstats_size(int32_t K, int32_t F);
+
+/* Bytes of device workspace tkfmi_restore_counts_batched needs. */
+size_t tkfmi_restore_workspace_bytes(const tkfmi_mixdom_model* m, int32_t B);
+
+/*
+ * Per pair b: rebuild the fully exploded MixDom machine at t[b]
+ * (models/exact_suffstats_batch.py:61-253, N = 8 + 8K + 15KF states),
+ * eliminate its null groups in chain order (:468-482, :272-282), restore the
+ * collapsed expected counts n_chi[b] to the exploded machine by ghost usage
+ * (:285-339) and extract the per-group statistics (:347-458).  Replaces
+ * exact_suffstats_batch (:485-541) and the per-pair loop of
+ * exact_suffstats_per_pair_batch (models/exact_suffstats.py:266-386).
+ *   t       device [B]
+ *   n_chi   device [B, N_c, N_c], N_c = 2 + 5KF (collapsed chi layout, as
+ *           written by tkfmi_fb2d_batched)
+ *   stats   device [B, tkfmi_suffstats_size(K, F)]
+ *   bdi     device [B, 1 + K, 4] or NULL: per level (top, then each domain)
+ *           E[B], E[D], E[S] (core/bdi.py:682-778) and the observation time
+ *           T (top: t * sum top_5x5[:,E]; domain d: t * (sum dom_M_5x5[d][S,:]
+ *           + dom_1mkappa[d]), models/exact_suffstats.py:353-381).
+ * FP64 throughout, one workgroup per pair, stream-ordered.
+ */
+int tkfmi_restore_counts_batched(const tkfmi_mixdom_model* m, int32_t B, const double* t,
+                                 const double* n_chi, double* stats, double* bdi,
+                                 void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * BDI statistics (E[B], E[D], E[S]) of B TKF91-semantics 5x5 count
+ * matrices at shared (lam, mu): tkf91_stats_from_counts_batch
+ * (core/bdi.py:800-888), general or equal-rate (|1 - lam/mu| < 1e-10)
+ * branch.  n5 device [B,5,5], t and T device [B], out device [B,3].
+ */
+int tkfmi_bdi_stats_batched(int32_t B, double lam, double mu, const double* n5, const double* t,
+                            const double* T, double* out, void* stream);
+
+/* Doubles per pair written by tkfmi_tkf92_estep_stats. */
+#define TKFMI_TKF92_STATS 35
+
+/*
+ * TKF92 E-step statistics of B pairs from their chi counts n_chi [B,5,5]
+ * (as written by tkfmi_fb2d_batched for the TKF layout) at per-pair t:
+ * the self-loop split _tkf92_chi_resolve_core (train/tkf92_svi_bw.py:75-123),
+ * the log-kappa coefficients (:160-167) and the clamped BDI statistics of
+ * estep_batch_tkf92 (:263-334).  out device [B, TKFMI_TKF92_STATS]:
+ *   resolved [5,5] | ext_count | notext_count | L | M | E_B | E_D | E_S |
+ *   T (= t) | pad [2]
+ */
+int tkfmi_tkf92_estep_stats(int32_t B, double lam, double mu, double ext, const double* t,
+                            const double* n_chi, double* out, void* stream);
+
 /* One kernel launch of a timed call. */
 typedef struct tkfmi_kernel_time {
   char name[64];    /* kernel, e.g. "fbv2_kernel<LMix<3,1>,8>"            */

@@ -41,9 +41,16 @@ def test_struct_layout_matches_header():
     from tkfmi import _abi
     text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
     for cname, py in (("tkfmi_problem", _abi.Problem), ("tkfmi_outputs", _abi.Outputs),
-                      ("tkfmi_kernel_time", _abi.KernelTime)):
+                      ("tkfmi_kernel_time", _abi.KernelTime),
+                      ("tkfmi_mixdom_model", _abi.MixdomModel)):
         body = re.search(r"typedef struct %s \{(.*?)\}" % cname, text, flags=re.S).group(1)
-        fields = re.findall(r"\*?\s*(\w+)(?:\[\d+\])?;", body)
+        fields = []
+        for decl in body.split(";"):
+            parts = [x.strip() for x in decl.strip().split(",") if x.strip()]
+            if not parts:
+                continue
+            fields.append(re.findall(r"(\w+)(?:\[\w+\])?$", parts[0])[0])
+            fields += [re.sub(r"[\s*]|\[\w+\]", "", x) for x in parts[1:]]
         assert fields == [f[0] for f in py._fields_], cname
     flags = dict(re.findall(r"#define TKFMI_(\w+) (\d+)u", text))
     for k in ("FORWARD_ONLY", "WANT_POSTERIORS", "NO_COUNTS", "GENERIC_KERNEL", "WANT_EMIT_COUNTS",

@@ -616,6 +616,9 @@ int launch(const Plan& pl, DevParams dp, int B, size_t smem, hipStream_t stream)
 
 }  // namespace
 
+int tkfmi_fail(int code, const std::string& msg) { return fail(code, msg); }
+void tkfmi_clear_error() { g_err.clear(); }
+
 int tkfmi_stage_upload(void* dst, const void* src, size_t bytes, hipStream_t stream,
                        std::string& err) {
   Stage& st = g_stage;

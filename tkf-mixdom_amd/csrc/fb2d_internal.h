@@ -22,6 +22,11 @@ bool tkfmi_v2_workspace(const tkfmi_problem* p, uint32_t flags, size_t* bytes);
 int tkfmi_v2_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, size_t ws_bytes,
                  hipStream_t stream, uint32_t flags, std::string& err);
 
+// Records `msg` as this thread's tkfmi_last_error() and returns `code`.
+int tkfmi_fail(int code, const std::string& msg);
+// Clears this thread's error message.
+void tkfmi_clear_error();
+
 // Per-kernel timing (TKFMI_TIME_KERNELS): begin_call arms or disarms the
 // recorder for this thread's call; start/stop bracket one kernel launch on
 // stream `s` (start returns a slot, -1 when disarmed, -2 on error).

@@ -56,6 +56,23 @@ class Outputs(ctypes.Structure):
     ]
 
 
+class MixdomModel(ctypes.Structure):
+    _fields_ = [
+        ("K", ctypes.c_int32),
+        ("F", ctypes.c_int32),
+        ("main_ins", ctypes.c_double),
+        ("main_del", ctypes.c_double),
+        ("dom_ins", _c_p),
+        ("dom_del", _c_p),
+        ("dom_w", _c_p),
+        ("frag_w", _c_p),
+        ("ext", _c_p),
+    ]
+
+
+TKF92_STATS = 35
+
+
 class KernelTime(ctypes.Structure):
     _fields_ = [
         ("name", ctypes.c_char * 64),
@@ -72,7 +89,9 @@ _lock = threading.Lock()
 # Every symbol include/tkfmi.h declares (checked by the CPU test-suite).
 EXPORTED = ("tkfmi_abi_version", "tkfmi_last_error",
             "tkfmi_fb2d_workspace_bytes", "tkfmi_fb2d_batched",
-            "tkfmi_last_kernel_times")
+            "tkfmi_last_kernel_times", "tkfmi_suffstats_size",
+            "tkfmi_restore_workspace_bytes", "tkfmi_restore_counts_batched",
+            "tkfmi_bdi_stats_batched", "tkfmi_tkf92_estep_stats")
 
 
 class TkfmiError(RuntimeError):
@@ -98,6 +117,20 @@ def lib():
                                              _c_p, ctypes.c_size_t, _c_p, ctypes.c_uint32]
             L.tkfmi_last_kernel_times.restype = ctypes.c_int
             L.tkfmi_last_kernel_times.argtypes = [ctypes.POINTER(KernelTime), ctypes.c_int32]
+            L.tkfmi_suffstats_size.restype = ctypes.c_int32
+            L.tkfmi_suffstats_size.argtypes = [ctypes.c_int32, ctypes.c_int32]
+            L.tkfmi_restore_workspace_bytes.restype = ctypes.c_size_t
+            L.tkfmi_restore_workspace_bytes.argtypes = [ctypes.POINTER(MixdomModel), ctypes.c_int32]
+            L.tkfmi_restore_counts_batched.restype = ctypes.c_int
+            L.tkfmi_restore_counts_batched.argtypes = [ctypes.POINTER(MixdomModel), ctypes.c_int32,
+                                                       _c_p, _c_p, _c_p, _c_p, _c_p,
+                                                       ctypes.c_size_t, _c_p]
+            L.tkfmi_bdi_stats_batched.restype = ctypes.c_int
+            L.tkfmi_bdi_stats_batched.argtypes = [ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                                  _c_p, _c_p, _c_p, _c_p, _c_p]
+            L.tkfmi_tkf92_estep_stats.restype = ctypes.c_int
+            L.tkfmi_tkf92_estep_stats.argtypes = [ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                                  ctypes.c_double, _c_p, _c_p, _c_p, _c_p]
             v = L.tkfmi_abi_version()
             if v != ABI_VERSION:
                 raise TkfmiError(f"libtkfmi ABI {v} != expected {ABI_VERSION}")
@@ -270,3 +303,13 @@ class BatchRunner:
         if self.ecnt is not None:
             out["emit_counts"] = self.ecnt
         return out
+
+
+def check(rc):
+    """Raise TkfmiError with the library's message for a nonzero return."""
+    if rc != 0:
+        raise TkfmiError(lib().tkfmi_last_error().decode() or f"tkfmi error {rc}")
+
+
+def current_stream_ptr(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
