@@ -196,6 +196,23 @@ int tkfmi_bdi_stats_batched(int32_t B, double lam, double mu, const double* n5, 
 int tkfmi_tkf92_estep_stats(int32_t B, double lam, double mu, double ext, const double* t,
                             const double* n_chi, double* out, void* stream);
 
+/*
+ * Holmes-Rubin weighted statistics per (pair, class): expected dwell times
+ * W [A] and substitution counts U [A,A] of the CTMC paths behind the pair's
+ * expected match counts (core/ctmc.py:173-326 holmes_rubin_weighted_stats;
+ * the per-pair-per-domain batch of train_pfam.py:1991-2046).  The caller
+ * supplies, per class c, the eigendecomposition S_c = V diag(l) V^T of the
+ * symmetrised rate matrix S_c = D^1/2 Q_c D^-1/2 (D = diag(pi_c), made
+ * exactly symmetric as (S + S^T)/2), shared by all pairs.
+ *   eigvec device [C,A,A] (row a, column k = component a of eigenvector k)
+ *   eigval device [C,A], pi device [C,A], Q device [C,A,A], t device [B],
+ *   mc device [B,C,A,A] (match counts), W device [B,C,A], U device [B,C,A,A].
+ * A <= 32.
+ */
+int tkfmi_hr_stats_batched(int32_t B, int32_t C, int32_t A, const double* eigvec,
+                           const double* eigval, const double* pi, const double* Q, const double* t,
+                           const double* mc, double* W, double* U, void* stream);
+
 /* One kernel launch of a timed call. */
 typedef struct tkfmi_kernel_time {
   char name[64];    /* kernel, e.g. "fbv2_kernel<LMix<3,1>,8>"            */

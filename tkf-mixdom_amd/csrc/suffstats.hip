@@ -420,6 +420,88 @@ __global__ void tkf92_estep_kernel(int B, double lam, double mu, double ext,
   o[34] = 0.0;
 }
 
+
+// Holmes-Rubin weighted statistics for one (pair, class) per workgroup
+// (core/ctmc.py:173-326).  With S = D^1/2 Q D^-1/2 = V diag(l) V^T shared by
+// every pair of a class, the weighted sums over the A^4 integral tensor fold
+// into A^3 products:
+//   R~ = V^T (mc / M * sqrt(pi_b / pi_a)) V,  Y = V (J o R~) V^T,
+//   W_i = Y_ii,  U_ij = Q_ij sqrt(pi_i / pi_j) Y_ij (i != j),
+// with J_kl = (e^{l_k t} - e^{l_l t}) / (l_k - l_l), t e^{l_k t} when
+// |l_k - l_l| < 1e-10, and M = D^-1/2 V e^{l t} V^T D^1/2.
+constexpr int kHrMaxA = 32;
+
+__global__ __launch_bounds__(256) void hr_kernel(int C, int A, const double* __restrict__ V_all,
+                                                 const double* __restrict__ lam_all,
+                                                 const double* __restrict__ pi_all,
+                                                 const double* __restrict__ Q_all,
+                                                 const double* __restrict__ t_all,
+                                                 const double* __restrict__ mc_all,
+                                                 double* __restrict__ W_out, double* __restrict__ U_out) {
+  __shared__ double sV[kHrMaxA * kHrMaxA], sX[kHrMaxA * kHrMaxA], sT[kHrMaxA * kHrMaxA];
+  __shared__ double sl[kHrMaxA], se[kHrMaxA], ssp[kHrMaxA];
+  const int bc = blockIdx.x;
+  const int c = bc % C;
+  const int tid = threadIdx.x;
+  const double t = t_all[bc / C];
+  const double* V = V_all + (size_t)c * A * A;
+  const double* Q = Q_all + (size_t)c * A * A;
+  const double* mc = mc_all + (size_t)bc * A * A;
+  const int AA = A * A;
+  for (int q = tid; q < AA; q += 256) sV[q] = V[q];
+  for (int q = tid; q < A; q += 256) {
+    sl[q] = lam_all[c * A + q];
+    se[q] = exp(sl[q] * t);
+    ssp[q] = sqrt(pi_all[c * A + q]);
+  }
+  __syncthreads();
+  // R'_ab = mc_ab / M_ab * sqrt(pi_b / pi_a)
+  for (int q = tid; q < AA; q += 256) {
+    const int a = q / A, b = q % A;
+    double m = 0.0;
+    for (int k = 0; k < A; ++k) m = fma(sV[a * A + k] * se[k], sV[b * A + k], m);
+    m *= ssp[b] / ssp[a];
+    const double ms = (fabs(m) < 1e-30) ? 1.0 : m;
+    sX[q] = mc[q] / ms * (ssp[b] / ssp[a]);
+  }
+  __syncthreads();
+  for (int q = tid; q < AA; q += 256) {  // sT = R' V
+    const int a = q / A, l = q % A;
+    double acc = 0.0;
+    for (int b = 0; b < A; ++b) acc = fma(sX[a * A + b], sV[b * A + l], acc);
+    sT[q] = acc;
+  }
+  __syncthreads();
+  for (int q = tid; q < AA; q += 256) {  // sX = J o (V^T sT)
+    const int k = q / A, l = q % A;
+    double acc = 0.0;
+    for (int a = 0; a < A; ++a) acc = fma(sV[a * A + k], sT[a * A + l], acc);
+    const double d = sl[k] - sl[l];
+    const double J = (fabs(d) < 1e-10) ? t * se[k] : (se[k] - se[l]) / d;
+    sX[q] = J * acc;
+  }
+  __syncthreads();
+  for (int q = tid; q < AA; q += 256) {  // sT = V sX
+    const int i = q / A, l = q % A;
+    double acc = 0.0;
+    for (int k = 0; k < A; ++k) acc = fma(sV[i * A + k], sX[k * A + l], acc);
+    sT[q] = acc;
+  }
+  __syncthreads();
+  double* U = U_out + (size_t)bc * AA;
+  for (int q = tid; q < AA; q += 256) {  // Y = sT V^T
+    const int i = q / A, j = q % A;
+    double y = 0.0;
+    for (int l = 0; l < A; ++l) y = fma(sT[i * A + l], sV[j * A + l], y);
+    if (i == j) {
+      W_out[(size_t)bc * A + i] = y;
+      U[q] = 0.0;
+    } else {
+      U[q] = Q[q] * (ssp[i] / ssp[j]) * y;
+    }
+  }
+}
+
 // ------------------------------------------------------------- host side
 // State numbering of the exploded machine (the order of the reference's
 // name list, exact_suffstats_batch.py:107-159).
@@ -804,6 +886,22 @@ int tkfmi_tkf92_estep_stats(int32_t B, double lam, double mu, double ext, const 
                      t, n_chi, out);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return tkfmi_fail(TKFMI_EHIP, std::string("tkf92 estep launch: ") + hipGetErrorString(e));
+  return TKFMI_OK;
+}
+
+int tkfmi_hr_stats_batched(int32_t B, int32_t C, int32_t A, const double* eigvec,
+                           const double* eigval, const double* pi, const double* Q, const double* t,
+                           const double* mc, double* W, double* U, void* stream_v) {
+  tkfmi_clear_error();
+  if (B < 0 || C < 1 || A < 1 || A > kHrMaxA) return tkfmi_fail(TKFMI_EINVAL, "need B >= 0, C >= 1, 1 <= A <= 32");
+  if (B == 0) return TKFMI_OK;
+  if (!eigvec || !eigval || !pi || !Q || !t || !mc || !W || !U)
+    return tkfmi_fail(TKFMI_EINVAL, "device pointer missing");
+  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_v);
+  hipLaunchKernelGGL(hr_kernel, dim3((unsigned)B * (unsigned)C), dim3(256), 0, stream, C, A, eigvec,
+                     eigval, pi, Q, t, mc, W, U);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return tkfmi_fail(TKFMI_EHIP, std::string("hr launch: ") + hipGetErrorString(e));
   return TKFMI_OK;
 }
 

@@ -91,7 +91,7 @@ EXPORTED = ("tkfmi_abi_version", "tkfmi_last_error",
             "tkfmi_fb2d_workspace_bytes", "tkfmi_fb2d_batched",
             "tkfmi_last_kernel_times", "tkfmi_suffstats_size",
             "tkfmi_restore_workspace_bytes", "tkfmi_restore_counts_batched",
-            "tkfmi_bdi_stats_batched", "tkfmi_tkf92_estep_stats")
+            "tkfmi_bdi_stats_batched", "tkfmi_tkf92_estep_stats", "tkfmi_hr_stats_batched")
 
 
 class TkfmiError(RuntimeError):
@@ -131,6 +131,9 @@ def lib():
             L.tkfmi_tkf92_estep_stats.restype = ctypes.c_int
             L.tkfmi_tkf92_estep_stats.argtypes = [ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                                   ctypes.c_double, _c_p, _c_p, _c_p, _c_p]
+            L.tkfmi_hr_stats_batched.restype = ctypes.c_int
+            L.tkfmi_hr_stats_batched.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                                 _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p]
             v = L.tkfmi_abi_version()
             if v != ABI_VERSION:
                 raise TkfmiError(f"libtkfmi ABI {v} != expected {ABI_VERSION}")
