@@ -589,13 +589,18 @@ size_t plan_ws_bytes(const Plan& pl) {
 
 // Pinned staging buffer for the per-pair descriptors (reused across calls;
 // an event guards reuse until the previous upload has completed).
+// Pinned staging buffers for host -> device uploads (per-pair descriptors,
+// statistic programs).  A buffer is reused only once the copy out of it has
+// completed (its event), so an upload never waits for earlier work on any
+// stream; the pool grows while every buffer is still in flight.
 struct Stage {
   void* buf = nullptr;
   size_t cap = 0;
   hipEvent_t ev = nullptr;
   bool pending = false;
 };
-thread_local Stage g_stage;
+constexpr int kMaxStages = 64;
+thread_local std::vector<Stage> g_stages;
 
 template <int GL, int NB, int W>
 int launch(const Plan& pl, DevParams dp, int B, size_t smem, hipStream_t stream) {
@@ -621,30 +626,46 @@ void tkfmi_clear_error() { g_err.clear(); }
 
 int tkfmi_stage_upload(void* dst, const void* src, size_t bytes, hipStream_t stream,
                        std::string& err) {
-  Stage& st = g_stage;
-  if (st.pending) {
-    hipError_t e = hipEventSynchronize(st.ev);
-    if (e != hipSuccess) { err = std::string("hipEventSynchronize: ") + hipGetErrorString(e); return TKFMI_EHIP; }
-    st.pending = false;
+  Stage* st = nullptr;
+  for (Stage& c : g_stages) {  // a finished buffer, large enough if possible
+    if (c.pending) {
+      const hipError_t q = hipEventQuery(c.ev);
+      if (q == hipErrorNotReady) continue;
+      if (q != hipSuccess) { err = std::string("hipEventQuery: ") + hipGetErrorString(q); return TKFMI_EHIP; }
+      c.pending = false;
+    }
+    if (!st || (st->cap < bytes && c.cap >= bytes)) st = &c;
+    if (st->cap >= bytes) break;
   }
-  if (st.cap < bytes) {
-    if (st.buf) (void)hipHostFree(st.buf);
-    st.buf = nullptr;
-    st.cap = 0;
-    hipError_t e = hipHostMalloc(&st.buf, bytes, hipHostMallocDefault);
+  if (!st) {
+    if ((int)g_stages.size() < kMaxStages) {
+      g_stages.emplace_back();
+      st = &g_stages.back();
+    } else {  // every buffer in flight: wait for the oldest
+      st = &g_stages.front();
+      hipError_t e = hipEventSynchronize(st->ev);
+      if (e != hipSuccess) { err = std::string("hipEventSynchronize: ") + hipGetErrorString(e); return TKFMI_EHIP; }
+      st->pending = false;
+    }
+  }
+  if (st->cap < bytes) {
+    if (st->buf) (void)hipHostFree(st->buf);
+    st->buf = nullptr;
+    st->cap = 0;
+    hipError_t e = hipHostMalloc(&st->buf, bytes, hipHostMallocDefault);
     if (e != hipSuccess) { err = std::string("hipHostMalloc: ") + hipGetErrorString(e); return TKFMI_EHIP; }
-    st.cap = bytes;
+    st->cap = bytes;
   }
-  if (!st.ev) {
-    hipError_t e = hipEventCreateWithFlags(&st.ev, hipEventDisableTiming);
+  if (!st->ev) {
+    hipError_t e = hipEventCreateWithFlags(&st->ev, hipEventDisableTiming);
     if (e != hipSuccess) { err = std::string("hipEventCreate: ") + hipGetErrorString(e); return TKFMI_EHIP; }
   }
-  memcpy(st.buf, src, bytes);
-  hipError_t e = hipMemcpyAsync(dst, st.buf, bytes, hipMemcpyHostToDevice, stream);
+  memcpy(st->buf, src, bytes);
+  hipError_t e = hipMemcpyAsync(dst, st->buf, bytes, hipMemcpyHostToDevice, stream);
   if (e != hipSuccess) { err = std::string("hipMemcpyAsync: ") + hipGetErrorString(e); return TKFMI_EHIP; }
-  e = hipEventRecord(st.ev, stream);
+  e = hipEventRecord(st->ev, stream);
   if (e != hipSuccess) { err = std::string("hipEventRecord: ") + hipGetErrorString(e); return TKFMI_EHIP; }
-  st.pending = true;
+  st->pending = true;
   return TKFMI_OK;
 }
 

@@ -19,6 +19,14 @@ def _t(x, like=None):
     return torch.as_tensor(x, dtype=torch.float64, device=dev)
 
 
+def _common(*xs):
+    """All arguments as FP64 tensors on the device of the first tensor among
+    them (scalars broadcast to it)."""
+    dev = next((x.device for x in xs if isinstance(x, torch.Tensor)), None)
+    return tuple(x.to(device=dev, dtype=torch.float64) if isinstance(x, torch.Tensor)
+                 else torch.as_tensor(x, dtype=torch.float64, device=dev) for x in xs)
+
+
 def tkf_alpha(mu, t):
     return torch.exp(-_t(mu) * _t(t))
 
@@ -71,14 +79,14 @@ def _assemble91(a, b, g, k, cond=False):
 
 
 def tkf91_trans(lam, mu, t):
-    lam, mu, t = _t(lam), _t(mu, lam), _t(t, lam)
+    lam, mu, t = _common(lam, mu, t)
     return _assemble91(tkf_alpha(mu, t), tkf_beta(lam, mu, t), tkf_gamma(lam, mu, t),
                        tkf_kappa(lam, mu))
 
 
 def tkf91_trans_cond(lam, mu, t):
     """core/params.py:57-89 (kappa and 1-kappa factors removed)."""
-    lam, mu, t = _t(lam), _t(mu, lam), _t(t, lam)
+    lam, mu, t = _common(lam, mu, t)
     return _assemble91(tkf_alpha(mu, t), tkf_beta(lam, mu, t), tkf_gamma(lam, mu, t),
                        tkf_kappa(lam, mu), cond=True)
 
@@ -87,7 +95,8 @@ def tkf92_trans(lam, mu, t, ext):
     """core/params.py:92-108: S row unchanged; M/I/D rows scaled by (1-ext)
     with ext added on the diagonal."""
     t91 = tkf91_trans(lam, mu, t)
-    ext = _t(ext, t91)
+    (ext,) = _common(ext) if isinstance(ext, torch.Tensor) else (torch.as_tensor(ext, dtype=torch.float64, device=t91.device),)
+    ext = ext.to(t91.device)
     ext = ext.expand(t91.shape[:-2]) if ext.dim() <= t91.dim() - 2 else ext
     e = ext[..., None, None]
     scale = torch.ones_like(t91)

@@ -1,0 +1,271 @@
+"""Continuous-batching request loop for pair scoring and E-step requests.
+
+Requests (a pair x, y at branch length t, under TKF92 or MixDom2, either
+"score" = log-likelihood only or "estep" = log-likelihood, expected
+transition counts and the model's sufficient statistics) arrive over time.
+One scheduler thread:
+
+* admits arrived requests in arrival order while the HBM budget holds their
+  lattices (the forward lattice of a pair is (Lx+64)(Ly+64) x 128 bytes;
+  scoring needs none) and the batch has room;
+* launches each admitted set as one forward/backward call per (model, kind)
+  group on one of ``slots`` streams, followed for E-step groups by the
+  statistics kernel (TKF92 self-loop split + BDI, or MixDom2 null-state
+  restoration + BDI), then records an event;
+* polls the events of in-flight batches; a finished batch releases its
+  memory and slot, and its requests' latencies (completion - arrival) are
+  recorded.
+
+So work that arrives while earlier batches are on the GPU is admitted as
+soon as a slot and memory free up, and overlaps with them on the device.
+Parameters stay device-resident; each call builds its per-pair transition
+matrices and emission tables for the pairs' own t on the device.
+
+A pair's results do not depend on the batch it rides in (the kernels give
+every pair its own workgroup and fixed summation orders), which
+``tests/test_gpu_serve.py`` checks request by request.
+"""
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _abi, mixdom, params, tables, vjp
+
+LATTICE_BYTES_PER_CELL = 128
+
+
+@dataclass
+class Request:
+    rid: int
+    model: str        # "tkf92" or "mixdom2"
+    kind: str         # "score" or "estep"
+    x: np.ndarray
+    y: np.ndarray
+    t: float
+    arrival: float = 0.0  # seconds from the start of the run
+
+    def footprint(self):
+        if self.kind == "score":
+            return 0
+        return (len(self.x) + 64) * (len(self.y) + 64) * LATTICE_BYTES_PER_CELL
+
+    def cell_states(self, S):
+        return (len(self.x) + 1) * (len(self.y) + 1) * S
+
+
+@dataclass
+class Result:
+    log_p: float
+    n: np.ndarray = None
+    stats: np.ndarray = None
+    latency: float = 0.0
+
+
+class _Model:
+    """Device-resident parameters of one model and its per-t builders."""
+
+    def __init__(self, name, p, dev):
+        self.name = name
+        self.dev = dev
+        d = lambda v: torch.as_tensor(np.asarray(v, dtype=np.float64), device=dev)
+        if name == "tkf92":
+            self.lam, self.mu, self.ext = float(p["lam"]), float(p["mu"]), float(p["ext"])
+            self.Q, self.pi = d(p["Q"]), d(p["pi"])
+            self.st = list(params.TKF_STATE_TYPES)
+            self.groups = [0] * 5
+        elif name == "mixdom2":
+            self.p = p
+            self.K, self.F = np.asarray(p["frag_w"]).shape
+            self.Sx, self.cpis, self.cd = d(p["class_S"]), d(p["class_pi"]), d(p["class_dist"])
+            self.st = mixdom.state_types(self.K, self.F)
+            self.groups = tables.state_groups_per_class(len(self.st), self.F)
+            self.model = None
+        else:
+            raise ValueError(f"unknown model {name}")
+        self.S = len(self.st)
+
+    def build(self, ts):
+        """(log_trans (B,S,S), log_tab (B,G,TAB)) at per-pair t."""
+        t = torch.as_tensor(np.asarray(ts, dtype=np.float64), device=self.dev)
+        if self.name == "tkf92":
+            tau = params.tkf92_trans(self.lam, self.mu, t, self.ext)
+            sub = vjp.transition_matrix(self.Q, t)
+            tab = tables.shared_tables(sub, self.pi.expand(sub.shape[:-1]))
+        else:
+            p = self.p
+            tau = mixdom.build_nested_trans(p["main_lam"], p["main_mu"], t, p["dom_lam"], p["dom_mu"],
+                                            p["dom_w"], p["frag_w"], p["ext"])
+            subs = vjp.build_class_subs(self.Sx, self.cpis, t)
+            tab = tables.per_class_tables(subs, self.cpis, self.cd)
+        return torch.log(torch.clamp(tau, min=1e-30)), tab, t
+
+
+@dataclass
+class _Call:
+    reqs: list
+    runner: object
+    t: torch.Tensor = None
+    stats: tuple = ()      # device outputs of the statistics kernel
+    keep: tuple = ()       # other device buffers the queued kernels use
+
+
+@dataclass
+class _Batch:
+    calls: list
+    event: object
+    bytes: int
+    slot: int
+    keep: list = field(default_factory=list)
+
+
+class ServingEngine:
+    def __init__(self, tkf92_params, mixdom2_params, device=None, budget_bytes=16 << 30,
+                 max_batch_pairs=512, slots=3):
+        self.dev = device or torch.device("cuda", torch.cuda.current_device())
+        self.models = {"tkf92": _Model("tkf92", tkf92_params, self.dev),
+                       "mixdom2": _Model("mixdom2", mixdom2_params, self.dev)}
+        self.budget = int(budget_bytes)
+        self.max_batch_pairs = int(max_batch_pairs)
+        self.streams = [torch.cuda.Stream(device=self.dev) for _ in range(slots)]
+
+    # ---------------------------------------------------------------- launch
+    def _launch_group(self, model, kind, reqs, stream):
+        m = self.models[model]
+        with torch.cuda.stream(stream):
+            lt, tab, t = m.build([r.t for r in reqs])
+            pairs = _abi.PackedPairs([r.x for r in reqs], [r.y for r in reqs], self.dev)
+            r = _abi.BatchRunner(lt, m.st, m.groups, tab, pairs, forward_only=(kind == "score"))
+            r.run(zero_outputs=False, stream=stream)
+            call = _Call(reqs, r, t)
+            if kind == "estep":
+                call.stats, call.keep = self._estep_stats(m, r, t, stream)
+        return call
+
+    def _estep_stats(self, m, r, t, stream):
+        """Statistics kernel of an E-step group; returns (outputs, buffers)."""
+        L = _abi.lib()
+        sp = _abi.ctypes.c_void_p(stream.cuda_stream)
+        B = r.pairs.B
+        if m.name == "tkf92":
+            out = torch.empty((B, _abi.TKF92_STATS), dtype=torch.float64, device=self.dev)
+            _abi.check(L.tkfmi_tkf92_estep_stats(B, m.lam, m.mu, m.ext, _abi._ptr(t), _abi._ptr(r.n),
+                                                 _abi._ptr(out), sp))
+            return (out,), ()
+        from .suffstats import _Model as SSModel
+        p = m.p
+        if m.model is None:
+            m.model = SSModel(p["main_lam"], p["main_mu"], p["dom_lam"], p["dom_mu"], p["dom_w"],
+                              p["frag_w"], p["ext"])
+        mm = m.model
+        nst = int(L.tkfmi_suffstats_size(mm.K, mm.F))
+        wsb = int(L.tkfmi_restore_workspace_bytes(_abi.ctypes.byref(mm.c), B))
+        ws = torch.empty(wsb, dtype=torch.uint8, device=self.dev)
+        st_buf = torch.empty((B, nst), dtype=torch.float64, device=self.dev)
+        bdi_buf = torch.empty((B, (mm.K + 1) * 4), dtype=torch.float64, device=self.dev)
+        _abi.check(L.tkfmi_restore_counts_batched(_abi.ctypes.byref(mm.c), B, _abi._ptr(t),
+                                                  _abi._ptr(r.n), _abi._ptr(st_buf), _abi._ptr(bdi_buf),
+                                                  _abi._ptr(ws), _abi.ctypes.c_size_t(wsb), sp))
+        return (st_buf, bdi_buf), (ws,)
+
+    def _launch(self, reqs, slot):
+        groups = {}
+        for r in reqs:
+            groups.setdefault((r.model, r.kind), []).append(r)
+        stream = self.streams[slot]
+        calls = [self._launch_group(mdl, kind, rs, stream) for (mdl, kind), rs in groups.items()]
+        ev = torch.cuda.Event()
+        ev.record(stream)
+        return _Batch(calls, ev, sum(r.footprint() for r in reqs), slot)
+
+    def _collect(self, batch, results, t0, clock):
+        now = clock() - t0
+        for c in batch.calls:
+            lp = c.runner.log_p.cpu().numpy()
+            n = c.runner.n.cpu().numpy() if c.runner.n is not None else None
+            st = np.concatenate([x.cpu().numpy() for x in c.stats], axis=1) if c.stats else None
+            for i, r in enumerate(c.reqs):
+                results[r.rid] = Result(float(lp[i]), None if n is None else n[i],
+                                        None if st is None else st[i], now - r.arrival)
+
+    # ----------------------------------------------------------------- serve
+    def serve(self, requests, realtime=True, clock=time.perf_counter):
+        """Serve ``requests`` (sorted by arrival).  With ``realtime`` a
+        request becomes visible at its arrival time on the wall clock;
+        otherwise all are visible at once.  Returns {rid: Result}."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        for r in reqs:
+            if r.footprint() > self.budget:
+                raise ValueError(f"request {r.rid} needs {r.footprint()} B > budget {self.budget}")
+        results = {}
+        inflight = []
+        free_slots = list(range(len(self.streams)))
+        used = 0
+        nxt = 0
+        pending = []
+        t0 = clock()
+        while len(results) < len(reqs):
+            now = clock() - t0
+            while nxt < len(reqs) and (not realtime or reqs[nxt].arrival <= now):
+                pending.append(reqs[nxt])
+                nxt += 1
+            # completions
+            still = []
+            for b in inflight:
+                if b.event.query():
+                    self._collect(b, results, t0, clock)
+                    used -= b.bytes
+                    free_slots.append(b.slot)
+                else:
+                    still.append(b)
+            inflight = still
+            # admission
+            if pending and free_slots:
+                take, nbytes = [], 0
+                while pending and len(take) < self.max_batch_pairs:
+                    fp = pending[0].footprint()
+                    if used + nbytes + fp > self.budget:
+                        break
+                    take.append(pending.pop(0))
+                    nbytes += fp
+                if take:
+                    slot = free_slots.pop(0)
+                    inflight.append(self._launch(take, slot))
+                    used += nbytes
+                    continue
+            if not inflight and not pending and nxt < len(reqs):
+                time.sleep(max(0.0, min(0.001, reqs[nxt].arrival - (clock() - t0))))
+            elif inflight:
+                time.sleep(0.0001)
+        return results
+
+
+def poisson_requests(seed, n, rate, tkf_frac=0.5, score_frac=0.5, median=250, sigma=0.5,
+                     lo=20, hi=2000, t_range=(0.05, 2.0), A=20):
+    """Synthetic request stream: Poisson arrivals at ``rate`` per second,
+    lengths round(LogNormal(ln median, sigma)) clipped to [lo, hi] for both
+    sequences (the descendant drawn around its ancestor's length), a
+    ``tkf_frac`` share of TKF92 requests (the rest MixDom2) and a
+    ``score_frac`` share of scoring requests (the rest E-steps)."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, size=n)
+    arr = np.cumsum(gaps) - gaps[0]
+    Lx = np.clip(np.round(np.exp(np.log(median) + sigma * rng.standard_normal(n))), lo, hi).astype(int)
+    Ly = np.clip(np.round(Lx * np.exp(0.1 * rng.standard_normal(n))), lo, hi).astype(int)
+    out = []
+    for i in range(n):
+        out.append(Request(i, "tkf92" if rng.random() < tkf_frac else "mixdom2",
+                           "score" if rng.random() < score_frac else "estep",
+                           rng.integers(0, A, Lx[i]), rng.integers(0, A, Ly[i]),
+                           float(rng.uniform(*t_range)), float(arr[i])))
+    return out
+
+
+def summarize(requests, results, wall, S_of):
+    lat = np.array([results[r.rid].latency for r in requests])
+    cs = sum(r.cell_states(S_of[r.model]) for r in requests)
+    return {"requests": len(requests), "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3), "max_ms": float(lat.max() * 1e3),
+            "wall_s": wall, "achieved_req_per_s": len(requests) / wall,
+            "cell_states_per_s": cs / wall}
