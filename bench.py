@@ -95,6 +95,36 @@ def plain_per_pair(w, n_pairs=48, emit=True):
             "ms_per_pair": 1e3 * el / len(idx)}
 
 
+def serve_bench(loads, n_requests):
+    """Serving: a Poisson stream of ``n_requests`` scoring / E-step requests,
+    50/50 TKF92 (5 states) / MixDom2 K=3 (17 states), lengths
+    round(LogNormal(ln 250, 0.5)) clipped to [20, 2000], at each offered
+    load (requests/s), served in real time by the continuous-batching loop
+    after a warm-up stream.  Reports p50 / p99 latency per load."""
+    import numpy as np
+    from tkfmi import serve, synth
+    rng = np.random.default_rng(4)
+    Q, pi, _ = synth.random_Q_pi(rng)
+    eng = serve.ServingEngine(dict(lam=0.04, mu=0.05, ext=float(rng.uniform(0.5, 0.95)), Q=Q, pi=pi),
+                              synth.mixdom2_params(rng, 3, 1, 1))
+    eng.serve(serve.poisson_requests(99, 300, rate=1e6), realtime=False)  # warm-up
+    S_of = {"tkf92": 5, "mixdom2": 17}
+    out = []
+    for load in loads:
+        reqs = serve.poisson_requests(4, n_requests, rate=load)
+        t0 = time.perf_counter()
+        res = eng.serve(reqs, realtime=True)
+        wall = time.perf_counter() - t0
+        s = serve.summarize(reqs, res, wall, S_of)
+        s["offered_req_per_s"] = load
+        _log(f"serving at {load} req/s: p50 {s['p50_ms']:.1f} ms p99 {s['p99_ms']:.1f} ms, "
+             f"{s['achieved_req_per_s']:.0f} req/s, {s['cell_states_per_s']:.3e} cell-states/s")
+        out.append(s)
+    return {"workload": f"{n_requests} Poisson requests, 50/50 TKF92 (S=5) / MixDom2 K=3 (S=17), "
+                        "50/50 scoring / E-step, lengths LogNormal(ln250,0.5) in [20,2000]",
+            "loads": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,8 +134,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--serve-loads", default="1000,3000,10000",
+                    help="offered loads (requests/s) for the serving measurement; empty to skip")
+    ap.add_argument("--serve-requests", type=int, default=5000)
     ap.add_argument("--no-emit", action="store_true",
                     help="transition counts only (no expected emission counts)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (x5), to calibrate "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE against known traffic")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -115,6 +151,16 @@ def main():
     import numpy as np
     import torch
     from tkfmi import _abi, synth
+
+    if args.copy_probe_mib:
+        n = args.copy_probe_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n * 8, "launches": 5}), flush=True)
+        return
 
     _log("generating c1 workload")
     w = synth.make_mixdom2(seed=1, B=args.pairs)
@@ -182,6 +228,10 @@ def main():
         pl = plain_per_pair(w, emit=not args.no_emit)
         out["plain_per_pair"] = pl
         out["batched_over_plain"] = value / pl["value"]
+    loads = [float(v) for v in args.serve_loads.split(",") if v.strip()]
+    if loads:
+        _log("serving measurement")
+        out["serving"] = serve_bench(loads, args.serve_requests)
     if not args.no_cpu:
         _log("timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(w, seconds=args.cpu_seconds)

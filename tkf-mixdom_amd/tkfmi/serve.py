@@ -8,12 +8,15 @@ One scheduler thread:
 * admits arrived requests in arrival order while the HBM budget holds their
   lattices (the forward lattice of a pair is (Lx+64)(Ly+64) x 128 bytes;
   scoring needs none) and the batch has room;
-* launches each admitted set as one forward/backward call per (model, kind)
-  group on one of ``slots`` streams, followed for E-step groups by the
-  statistics kernel (TKF92 self-loop split + BDI, or MixDom2 null-state
-  restoration + BDI), then records an event;
-* polls the events of in-flight batches; a finished batch releases its
-  memory and slot, and its requests' latencies (completion - arrival) are
+* groups requests by (model, kind, size bucket) -- a call finishes when its
+  longest pair does, so short pairs never ride with long ones -- and, when
+  a stream slot is free, launches the group of the oldest waiting request
+  (up to ``max_batch_pairs`` of its requests, in arrival order) as one
+  forward/backward call, followed for E-steps by the statistics kernel
+  (TKF92 self-loop split + BDI, or MixDom2 null-state restoration + BDI),
+  then records an event;
+* polls the events of in-flight calls; a finished call releases its memory
+  and slot, and its requests' latencies (completion - arrival) are
   recorded.
 
 So work that arrives while earlier batches are on the GPU is admitted as
@@ -45,6 +48,15 @@ class Request:
     y: np.ndarray
     t: float
     arrival: float = 0.0  # seconds from the start of the run
+
+    def bucket(self):
+        """Size class: the number of waves the kernel gives the pair grows
+        with its length, and so does its serial step chain."""
+        c = (len(self.x) + 1) * (len(self.y) + 1)
+        return 0 if c <= 160 * 160 else (1 if c <= 400 * 400 else (2 if c <= 900 * 900 else 3))
+
+    def key(self):
+        return (self.model, self.kind, self.bucket())
 
     def footprint(self):
         if self.kind == "score":
@@ -122,7 +134,7 @@ class _Batch:
 
 class ServingEngine:
     def __init__(self, tkf92_params, mixdom2_params, device=None, budget_bytes=16 << 30,
-                 max_batch_pairs=512, slots=3):
+                 max_batch_pairs=512, slots=6):
         self.dev = device or torch.device("cuda", torch.cuda.current_device())
         self.models = {"tkf92": _Model("tkf92", tkf92_params, self.dev),
                        "mixdom2": _Model("mixdom2", mixdom2_params, self.dev)}
@@ -170,14 +182,11 @@ class ServingEngine:
         return (st_buf, bdi_buf), (ws,)
 
     def _launch(self, reqs, slot):
-        groups = {}
-        for r in reqs:
-            groups.setdefault((r.model, r.kind), []).append(r)
         stream = self.streams[slot]
-        calls = [self._launch_group(mdl, kind, rs, stream) for (mdl, kind), rs in groups.items()]
+        call = self._launch_group(reqs[0].model, reqs[0].kind, reqs, stream)
         ev = torch.cuda.Event()
         ev.record(stream)
-        return _Batch(calls, ev, sum(r.footprint() for r in reqs), slot)
+        return _Batch([call], ev, sum(r.footprint() for r in reqs), slot)
 
     def _collect(self, batch, results, t0, clock):
         now = clock() - t0
@@ -220,15 +229,19 @@ class ServingEngine:
                 else:
                     still.append(b)
             inflight = still
-            # admission
+            # admission: the group of the oldest waiting request
             if pending and free_slots:
-                take, nbytes = [], 0
-                while pending and len(take) < self.max_batch_pairs:
-                    fp = pending[0].footprint()
-                    if used + nbytes + fp > self.budget:
-                        break
-                    take.append(pending.pop(0))
-                    nbytes += fp
+                key = pending[0].key()
+                take, rest, nbytes = [], [], 0
+                for r in pending:
+                    fp = r.footprint()
+                    if (r.key() == key and len(take) < self.max_batch_pairs
+                            and used + nbytes + fp <= self.budget):
+                        take.append(r)
+                        nbytes += fp
+                    else:
+                        rest.append(r)
+                pending = rest
                 if take:
                     slot = free_slots.pop(0)
                     inflight.append(self._launch(take, slot))
