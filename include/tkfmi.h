@@ -213,6 +213,49 @@ int tkfmi_hr_stats_batched(int32_t B, int32_t C, int32_t A, const double* eigvec
                            const double* eigval, const double* pi, const double* Q, const double* t,
                            const double* mc, double* W, double* U, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Per-pair transition matrices and emission tables on the device.
+ * ---------------------------------------------------------------------- */
+#define TKFMI_MODEL_TKF91 0
+#define TKFMI_MODEL_TKF91_COND 1
+#define TKFMI_MODEL_TKF92 2
+#define TKFMI_MODEL_MIXDOM 3
+#define TKFMI_EMIT_SHARED 0      /* log(x + 1e-30), one table (dp/hmm.py:628-670)      */
+#define TKFMI_EMIT_PER_DOMAIN 1  /* log(max(x, 1e-30)), group = domain (:786-851)      */
+#define TKFMI_EMIT_PER_CLASS 2   /* class mixture per (domain, fragment) (:673-783)    */
+
+typedef struct tkfmi_builder {
+  int32_t model;      /* TKFMI_MODEL_*                                       */
+  int32_t emission;   /* TKFMI_EMIT_*                                        */
+  int32_t log_floor;  /* transitions: 0 = safe_log (NEG_INF for 0), 1 = log(max(x, 1e-30)) */
+  int32_t C;          /* substitution components: 1 shared, K per-domain, classes per-class */
+  int32_t A;          /* alphabet (<= 20)                                    */
+  int32_t pad;
+  double lam, mu, ext;                 /* TKF models                          */
+  const tkfmi_mixdom_model* mix;       /* host, TKFMI_MODEL_MIXDOM            */
+  const double* eigvec;  /* device [C,A,A]: eigenvectors of (S + S^T)/2,
+                            S = D^1/2 Q D^-1/2 (Q reversible w.r.t. pi)      */
+  const double* eigval;  /* device [C,A]                                     */
+  const double* pi;      /* device [C,A]                                     */
+  const double* class_dist;  /* device [K,F,C] (per-class) or NULL           */
+} tkfmi_builder;
+
+/* Bytes of device workspace tkfmi_build_params_batched needs. */
+size_t tkfmi_build_workspace_bytes(const tkfmi_builder* b, int32_t B);
+
+/*
+ * For every pair b at t[b]: log_trans [B,S,S] (TKF91 / TKF91-cond / TKF92,
+ * core/params.py:29-108, or the collapsed MixDom chi, models/mixdom.py:252-423,
+ * obtained as the reduced matrix of the exploded machine) and log_tab
+ * [B,G,TKFMI_TAB_SZ] built from P_c(t) = expm(Q_c t) (core/ctmc.py:31-60).
+ * Replaces the per-pair builders of train_pfam.py:1613-1664, 1940-1988 and
+ * train/tkf92_svi_bw.py:148-152.  G = 1 (shared), K (per-domain) or K*F
+ * (per-class).  One workgroup per pair.
+ */
+int tkfmi_build_params_batched(const tkfmi_builder* b, int32_t B, const double* t,
+                               double* log_trans, double* log_tab, void* workspace,
+                               size_t workspace_bytes, void* stream);
+
 /* One kernel launch of a timed call. */
 typedef struct tkfmi_kernel_time {
   char name[64];    /* kernel, e.g. "fbv2_kernel<LMix<3,1>,8>"            */

@@ -42,7 +42,8 @@ def test_struct_layout_matches_header():
     text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
     for cname, py in (("tkfmi_problem", _abi.Problem), ("tkfmi_outputs", _abi.Outputs),
                       ("tkfmi_kernel_time", _abi.KernelTime),
-                      ("tkfmi_mixdom_model", _abi.MixdomModel)):
+                      ("tkfmi_mixdom_model", _abi.MixdomModel),
+                      ("tkfmi_builder", _abi.Builder)):
         body = re.search(r"typedef struct %s \{(.*?)\}" % cname, text, flags=re.S).group(1)
         fields = []
         for decl in body.split(";"):

@@ -190,14 +190,14 @@ struct RParams {
   double* ws;
 };
 
-__global__ __launch_bounds__(kThreads) void restore_kernel(RParams P) {
-  __shared__ double s_tau[(kMaxK + 1) * 25];
-  __shared__ double s_fac[1024];
-  const int b = blockIdx.x;
+// Assembles the exploded machine of one pair at time t in its workspace
+// slice and runs the six eliminations.  Afterwards T_s sits at steps[s].T_off
+// (s = 0..5), the C and W factors at steps[s].C_off / W_off, and the reduced
+// matrix T_6 -- the collapsed chi in its state order -- right after T_5.
+__device__ void explode_and_eliminate(const RParams& P, double* ws, double t, double* s_tau,
+                                      double* s_fac) {
   const int tid = threadIdx.x;
   const int K = P.K;
-  double* ws = P.ws + (size_t)b * P.slice;
-  const double t = P.t[b];
   if (tid <= K) tkf91_trans(P.lam[tid], P.mu[tid], t, s_tau + tid * 25);
 
   // ---- exploded matrix
@@ -265,6 +265,18 @@ __global__ __launch_bounds__(kThreads) void restore_kernel(RParams P) {
     }
     __syncthreads();
   }
+}
+
+__global__ __launch_bounds__(kThreads) void restore_kernel(RParams P) {
+  __shared__ double s_tau[(kMaxK + 1) * 25];
+  __shared__ double s_fac[1024];
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int K = P.K;
+  double* ws = P.ws + (size_t)b * P.slice;
+  const double t = P.t[b];
+  explode_and_eliminate(P, ws, t, s_tau, s_fac);
+  const int N0 = P.N0;
 
   // ---- restoration (n_6 = n_chi, then back to n_0)
   double* nb0 = ws + P.nbuf_off;
@@ -499,6 +511,114 @@ __global__ __launch_bounds__(256) void hr_kernel(int C, int A, const double* __r
     } else {
       U[q] = Q[q] * (ssp[i] / ssp[j]) * y;
     }
+  }
+}
+
+
+// ----------------------------------------------------------- parameter build
+// Per pair at its own t: the log transition matrix (TKF91 / conditioned
+// TKF91 / TKF92 closed forms, or the collapsed MixDom chi as the reduced
+// matrix of the exploded machine) and the log emission tables from
+// P_c(t) = D^-1/2 V e^{l t} V^T D^1/2 (shared, per-domain or per-class).
+constexpr int kMaxSub = 16;
+constexpr int kMaxA = 20;
+constexpr double kNegInf = -1e30;
+
+struct BParams {
+  RParams R;  // MixDom elimination program (model == TKFMI_MODEL_MIXDOM)
+  int32_t model, emission, log_floor, C;
+  int32_t A, S, G, F;
+  double lam, mu, ext;
+  const double* eigvec;
+  const double* eigval;
+  const double* pi;
+  const double* cd;  // [K, F, C] class distribution (per-class emissions)
+  double* log_trans;
+  double* log_tab;
+};
+
+__device__ inline double floor_log(double x, int mode) {
+  return mode ? log(fmax(x, 1e-30)) : (x > 0.0 ? log(x) : kNegInf);
+}
+__device__ inline double lse2(double a, double b) {  // logaddexp
+  const double m = fmax(a, b);
+  return m + log1p(exp(-fabs(a - b)));
+}
+
+__global__ __launch_bounds__(kThreads) void build_kernel(BParams B) {
+  __shared__ double s_tau[(kMaxK + 1) * 25];
+  __shared__ double s_fac[1024];
+  __shared__ double s_P[kMaxSub * kMaxA * kMaxA];
+  __shared__ double s_e[kMaxSub * kMaxA];
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const double t = B.R.t[b];
+  const int S = B.S, A = B.A, C = B.C;
+  double* lt = B.log_trans + (size_t)b * S * S;
+  if (B.model == TKFMI_MODEL_MIXDOM) {
+    double* ws = B.R.ws + (size_t)b * B.R.slice;
+    explode_and_eliminate(B.R, ws, t, s_tau, s_fac);
+    const Step st = B.R.steps[5];
+    const double* chi = ws + st.T_off + (int64_t)st.n * st.n;
+    for (int q = tid; q < S * S; q += kThreads) lt[q] = floor_log(chi[q], B.log_floor);
+  } else if (tid == 0) {
+    double tau[25];
+    tkf91_trans(B.lam, B.mu, t, tau);
+    if (B.model == TKFMI_MODEL_TKF91_COND) {  // kappa and 1 - kappa divided out (core/params.py:57-89)
+      const double kappa = B.lam / B.mu;
+      for (int r = S_; r <= D_; ++r) {
+        tau[r * 5 + M_] /= kappa;
+        tau[r * 5 + D_] /= kappa;
+        tau[r * 5 + E_] /= (1.0 - kappa);
+      }
+    } else if (B.model == TKFMI_MODEL_TKF92) {  // core/params.py:92-108
+      for (int r = M_; r <= D_; ++r) {
+        for (int c = 0; c < 5; ++c) tau[r * 5 + c] *= (1.0 - B.ext);
+        tau[r * 5 + r] += B.ext;
+      }
+    }
+    for (int q = 0; q < 25; ++q) lt[q] = floor_log(tau[q], B.log_floor);
+  }
+  // ---- substitution matrices P_c(t)
+  for (int q = tid; q < C * A; q += kThreads) s_e[q] = exp(B.eigval[q] * t);
+  __syncthreads();
+  for (int q = tid; q < C * A * A; q += kThreads) {
+    const int c = q / (A * A), ab = q % (A * A), a = ab / A, bb = ab % A;
+    const double* V = B.eigvec + (size_t)c * A * A;
+    double acc = 0.0;
+    for (int k = 0; k < A; ++k) acc = fma(V[a * A + k] * s_e[c * A + k], V[bb * A + k], acc);
+    s_P[q] = acc * sqrt(B.pi[c * A + bb] / B.pi[c * A + a]);
+  }
+  __syncthreads();
+  // ---- tables [G][21x21 match | 21 insert | 21 delete]
+  double* tab = B.log_tab + (size_t)b * B.G * TKFMI_TAB_SZ;
+  const int AP = TKFMI_A_PAD;
+  for (int q = tid; q < B.G * TKFMI_TAB_SZ; q += kThreads) {
+    const int g = q / TKFMI_TAB_SZ, e = q % TKFMI_TAB_SZ;
+    int a, bb;  // characters of the entry (-1: not used)
+    if (e < AP * AP) { a = e / AP; bb = e % AP; }
+    else if (e < AP * AP + AP) { a = -1; bb = e - AP * AP; }
+    else { a = e - AP * AP - AP; bb = -1; }
+    double v;
+    if (B.emission == TKFMI_EMIT_PER_CLASS) {
+      v = kNegInf;
+      for (int c = 0; c < C; ++c) {
+        const double w = B.cd[g * C + c];
+        const double lw = (w > 0.0) ? log(fmax(w, 1e-300)) : kNegInf;
+        const int ch = (a >= 0) ? a : bb;  // pi factor: x for match / delete, y for insert
+        double term = (ch < A) ? log(fmax(B.pi[c * A + ch], 1e-30)) : 0.0;
+        if (a >= 0 && bb >= 0 && a < A && bb < A) term += log(fmax(s_P[(c * A + a) * A + bb], 1e-30));
+        v = lse2(v, term + lw);
+      }
+    } else {
+      const int c = (B.emission == TKFMI_EMIT_PER_DOMAIN) ? g : 0;
+      const bool shared = B.emission == TKFMI_EMIT_SHARED;
+      auto fl = [&](double x) { return shared ? log(x + 1e-30) : log(fmax(x, 1e-30)); };
+      const int ch = (a >= 0) ? a : bb;
+      v = (ch < A) ? fl(B.pi[c * A + ch]) : 0.0;
+      if (a >= 0 && bb >= 0 && a < A && bb < A) v += fl(s_P[(c * A + a) * A + bb]);
+    }
+    tab[q] = v;
   }
 }
 
@@ -902,6 +1022,108 @@ int tkfmi_hr_stats_batched(int32_t B, int32_t C, int32_t A, const double* eigvec
                      eigval, pi, Q, t, mc, W, U);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return tkfmi_fail(TKFMI_EHIP, std::string("hr launch: ") + hipGetErrorString(e));
+  return TKFMI_OK;
+}
+
+size_t tkfmi_build_workspace_bytes(const tkfmi_builder* bl, int32_t B) {
+  if (!bl || B < 0) return 0;
+  if (bl->model != TKFMI_MODEL_MIXDOM) return 256;
+  Program pr;
+  std::string err;
+  if (build_program(bl->mix, pr, err) != TKFMI_OK) return 0;
+  return program_bytes(pr) + (size_t)std::max(B, 1) * (size_t)pr.slice * 8;
+}
+
+int tkfmi_build_params_batched(const tkfmi_builder* bl, int32_t B, const double* t,
+                               double* log_trans, double* log_tab, void* workspace,
+                               size_t workspace_bytes, void* stream_v) {
+  tkfmi_clear_error();
+  if (!bl) return tkfmi_fail(TKFMI_EINVAL, "null builder");
+  if (B < 0) return tkfmi_fail(TKFMI_EINVAL, "B < 0");
+  if (bl->model < TKFMI_MODEL_TKF91 || bl->model > TKFMI_MODEL_MIXDOM)
+    return tkfmi_fail(TKFMI_EINVAL, "unknown model");
+  if (bl->emission < TKFMI_EMIT_SHARED || bl->emission > TKFMI_EMIT_PER_CLASS)
+    return tkfmi_fail(TKFMI_EINVAL, "unknown emission mode");
+  if (bl->A < 1 || bl->A > kMaxA || bl->C < 1 || bl->C > kMaxSub)
+    return tkfmi_fail(TKFMI_EINVAL, "need 1 <= A <= 20 and 1 <= C <= 16");
+  if (!(bl->mu > 0.0) && bl->model != TKFMI_MODEL_MIXDOM) return tkfmi_fail(TKFMI_EINVAL, "mu must be positive");
+  if (B == 0) return TKFMI_OK;
+  if (!t || !log_trans || !log_tab || !bl->eigvec || !bl->eigval || !bl->pi)
+    return tkfmi_fail(TKFMI_EINVAL, "device pointer missing");
+  BParams P;
+  memset(&P, 0, sizeof(P));
+  std::string err;
+  int K = 1, F = 1;
+  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_v);
+  char* wsb = static_cast<char*>(workspace);
+  if (bl->model == TKFMI_MODEL_MIXDOM) {
+    Program pr;
+    int rc = build_program(bl->mix, pr, err);
+    if (rc != TKFMI_OK) return tkfmi_fail(rc, err);
+    const size_t need = program_bytes(pr) + (size_t)B * (size_t)pr.slice * 8;
+    if (!workspace || workspace_bytes < need)
+      return tkfmi_fail(TKFMI_EWORKSPACE, "workspace too small: need " + std::to_string(need));
+    std::vector<char> blob(program_bytes(pr), 0);
+    size_t o = 0;
+    const size_t o_ent = o;
+    memcpy(blob.data() + o, pr.entries.data(), pr.entries.size() * sizeof(Entry));
+    o += align256(pr.entries.size() * sizeof(Entry));
+    const size_t o_idx = o;
+    memcpy(blob.data() + o, pr.idx.data(), pr.idx.size() * 4);
+    rc = tkfmi_stage_upload(wsb, blob.data(), blob.size(), stream, err);
+    if (rc != TKFMI_OK) return tkfmi_fail(rc, err);
+    K = bl->mix->K;
+    F = bl->mix->F;
+    P.R.K = K;
+    P.R.F = F;
+    P.R.N0 = pr.L.N0;
+    P.R.Nc = pr.L.Nc;
+    P.R.n_entries = (int32_t)pr.entries.size();
+    P.R.lam[0] = bl->mix->main_ins;
+    P.R.mu[0] = bl->mix->main_del;
+    for (int k = 0; k < K; ++k) {
+      P.R.lam[1 + k] = bl->mix->dom_ins[k];
+      P.R.mu[1 + k] = bl->mix->dom_del[k];
+    }
+    for (int s = 0; s < 6; ++s) P.R.steps[s] = pr.steps[s];
+    P.R.gj_off = pr.gj_off;
+    P.R.slice = pr.slice;
+    P.R.entries = reinterpret_cast<const Entry*>(wsb + o_ent);
+    P.R.idx = reinterpret_cast<const int32_t*>(wsb + o_idx);
+    P.R.ws = reinterpret_cast<double*>(wsb + program_bytes(pr));
+    P.S = pr.L.Nc;
+  } else {
+    P.S = 5;
+  }
+  if (bl->emission == TKFMI_EMIT_PER_CLASS) {
+    if (!bl->class_dist) return tkfmi_fail(TKFMI_EINVAL, "class_dist missing");
+    P.G = K * F;
+  } else if (bl->emission == TKFMI_EMIT_PER_DOMAIN) {
+    if (bl->C != K) return tkfmi_fail(TKFMI_EINVAL, "per-domain emissions need C == K");
+    P.G = K;
+  } else {
+    if (bl->C != 1) return tkfmi_fail(TKFMI_EINVAL, "shared emissions need C == 1");
+    P.G = 1;
+  }
+  P.R.t = t;
+  P.model = bl->model;
+  P.emission = bl->emission;
+  P.log_floor = bl->log_floor;
+  P.C = bl->C;
+  P.A = bl->A;
+  P.F = F;
+  P.lam = bl->lam;
+  P.mu = bl->mu;
+  P.ext = bl->ext;
+  P.eigvec = bl->eigvec;
+  P.eigval = bl->eigval;
+  P.pi = bl->pi;
+  P.cd = bl->class_dist;
+  P.log_trans = log_trans;
+  P.log_tab = log_tab;
+  hipLaunchKernelGGL(build_kernel, dim3(B), dim3(kThreads), 0, stream, P);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return tkfmi_fail(TKFMI_EHIP, std::string("build launch: ") + hipGetErrorString(e));
   return TKFMI_OK;
 }
 

@@ -73,6 +73,25 @@ class MixdomModel(ctypes.Structure):
 TKF92_STATS = 35
 
 
+class Builder(ctypes.Structure):
+    _fields_ = [
+        ("model", ctypes.c_int32),
+        ("emission", ctypes.c_int32),
+        ("log_floor", ctypes.c_int32),
+        ("C", ctypes.c_int32),
+        ("A", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("lam", ctypes.c_double),
+        ("mu", ctypes.c_double),
+        ("ext", ctypes.c_double),
+        ("mix", ctypes.POINTER(MixdomModel)),
+        ("eigvec", _c_p),
+        ("eigval", _c_p),
+        ("pi", _c_p),
+        ("class_dist", _c_p),
+    ]
+
+
 class KernelTime(ctypes.Structure):
     _fields_ = [
         ("name", ctypes.c_char * 64),
@@ -91,7 +110,8 @@ EXPORTED = ("tkfmi_abi_version", "tkfmi_last_error",
             "tkfmi_fb2d_workspace_bytes", "tkfmi_fb2d_batched",
             "tkfmi_last_kernel_times", "tkfmi_suffstats_size",
             "tkfmi_restore_workspace_bytes", "tkfmi_restore_counts_batched",
-            "tkfmi_bdi_stats_batched", "tkfmi_tkf92_estep_stats", "tkfmi_hr_stats_batched")
+            "tkfmi_bdi_stats_batched", "tkfmi_tkf92_estep_stats", "tkfmi_hr_stats_batched",
+            "tkfmi_build_workspace_bytes", "tkfmi_build_params_batched")
 
 
 class TkfmiError(RuntimeError):
@@ -134,6 +154,11 @@ def lib():
             L.tkfmi_hr_stats_batched.restype = ctypes.c_int
             L.tkfmi_hr_stats_batched.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                  _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p]
+            L.tkfmi_build_workspace_bytes.restype = ctypes.c_size_t
+            L.tkfmi_build_workspace_bytes.argtypes = [ctypes.POINTER(Builder), ctypes.c_int32]
+            L.tkfmi_build_params_batched.restype = ctypes.c_int
+            L.tkfmi_build_params_batched.argtypes = [ctypes.POINTER(Builder), ctypes.c_int32, _c_p, _c_p,
+                                                     _c_p, _c_p, ctypes.c_size_t, _c_p]
             v = L.tkfmi_abi_version()
             if v != ABI_VERSION:
                 raise TkfmiError(f"libtkfmi ABI {v} != expected {ABI_VERSION}")

@@ -20,7 +20,7 @@ import ctypes
 import numpy as np
 import torch
 
-from . import _abi, hmm, hr, mixdom, params, suffstats, tables, vjp
+from . import _abi, build as dbuild, hmm, hr, mixdom, params, suffstats, tables, vjp
 
 AA = 20
 _NM = tables.A_PAD * tables.A_PAD
@@ -56,12 +56,11 @@ def _split_ecnt(ec, A=AA):
 # ------------------------------------------------------------------- TKF92
 def _tkf92_fb(xs, ys, ts, lam, mu, ext, Q, pi, forward_only):
     dev = hmm.device()
-    ts = _dev64(ts, dev).reshape(-1)
-    tau = params.tkf92_trans(_dev64(lam, dev), _dev64(mu, dev), ts, _dev64(ext, dev))
-    sub = vjp.transition_matrix(_dev64(Q, dev), ts)
-    tab = tables.shared_tables(sub, _dev64(pi, dev).expand(sub.shape[:-1]))
+    b = dbuild.DeviceBuilder("tkf92", "shared", _np(Q), _np(pi), lam=float(lam), mu=float(mu),
+                             ext=float(ext), log_floor=True, device=dev)
+    lt, tab, ts = b.build(_np(ts))
     pairs = _abi.PackedPairs(xs, ys, dev)
-    r = _abi.BatchRunner(_floor_log(tau), params.TKF_STATE_TYPES, [0] * 5, tab, pairs,
+    r = _abi.BatchRunner(lt, params.TKF_STATE_TYPES, [0] * 5, tab, pairs,
                          forward_only=forward_only)
     r.run(zero_outputs=False)
     return r, ts
@@ -205,21 +204,20 @@ def process_pairs_batched_2d(batch_pairs, chi_params, st, Q_lg, pi_lg, N, dom_Qs
     xs = [np.asarray(it[0], dtype=np.int64) for it in batch_pairs]
     ys = [np.asarray(it[1], dtype=np.int64) for it in batch_pairs]
     t32 = np.array([it[5] for it in batch_pairs], dtype=np.float32).astype(np.float64)
-    t = torch.as_tensor(t32, device=dev)
     cp = chi_params
-    chi = mixdom.build_nested_trans(cp["main_ins"], cp["main_del"], t, cp["dom_ins"], cp["dom_del"],
-                                    cp["dom_weights"], cp["frag_weights"], cp["ext_rates"])
     if dom_Qs is not None and dom_pis is not None:
-        Qs, pis = _dev64(_np(dom_Qs), dev), _dev64(_np(dom_pis), dev)
+        Qs_n, pis_n = _np(dom_Qs).astype(np.float64), _np(dom_pis).astype(np.float64)
     else:
-        Qs = _dev64(_np(Q_lg), dev)[None].expand(n_dom, -1, -1).contiguous()
-        pis = _dev64(_np(pi_lg), dev)[None].expand(n_dom, -1).contiguous()
-    subs = vjp.transition_matrix(Qs[None], t[:, None])               # (B, K, A, A)
-    tab = tables.per_domain_tables(subs, pis.expand(subs.shape[:-1]))  # (B, K, TAB)
-    groups = [0, 0] + [(s - 2) // (5 * n_frag) for s in range(2, len(st))]
+        Qs_n = np.tile(_np(Q_lg).astype(np.float64)[None], (n_dom, 1, 1))
+        pis_n = np.tile(_np(pi_lg).astype(np.float64)[None], (n_dom, 1))
+    mix = dict(main_ins=cp["main_ins"], main_del=cp["main_del"], dom_ins=cp["dom_ins"],
+               dom_del=cp["dom_del"], dom_w=cp["dom_weights"], frag_w=cp["frag_weights"],
+               ext=cp["ext_rates"])
+    b = dbuild.DeviceBuilder("mixdom", "per_domain", Qs_n, pis_n, mix=mix, log_floor=True, device=dev)
+    lt, tab, t = b.build(t32)                                         # chi and (B, K, TAB) tables
+    Qs, pis = _dev64(Qs_n, dev), _dev64(pis_n, dev)
     pairs = _abi.PackedPairs(xs, ys, dev)
-    r = _abi.BatchRunner(_floor_log(chi), list(np.asarray(st)), groups, tab, pairs,
-                         want_emit_counts=True)
+    r = _abi.BatchRunner(lt, b.state_types, b.state_group, tab, pairs, want_emit_counts=True)
     r.run(zero_outputs=False)
     ok = torch.isfinite(r.log_p)
     dm, di, dd = _split_ecnt(r.ecnt)                                  # (B, K, 20, 20) ...

@@ -34,7 +34,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from . import _abi, mixdom, params, tables, vjp
+from . import _abi, build as dbuild, mixdom, params, tables
 
 LATTICE_BYTES_PER_CELL = 128
 
@@ -75,43 +75,43 @@ class Result:
     latency: float = 0.0
 
 
+def gtr_Q(S_exch, pi):
+    """Q[a,b] = S[a,b] pi[b] off the diagonal, rows summing to zero."""
+    Q = np.asarray(S_exch, np.float64) * np.asarray(pi, np.float64)[..., None, :]
+    A = Q.shape[-1]
+    Q = Q * (1.0 - np.eye(A))
+    return Q - np.eye(A) * Q.sum(-1)[..., :, None]
+
+
 class _Model:
-    """Device-resident parameters of one model and its per-t builders."""
+    """Device-resident parameters of one model; its per-pair transition
+    matrices (log floor 1e-30) and emission tables are built on the device
+    for each call's t (tkfmi.build)."""
 
     def __init__(self, name, p, dev):
         self.name = name
         self.dev = dev
-        d = lambda v: torch.as_tensor(np.asarray(v, dtype=np.float64), device=dev)
         if name == "tkf92":
             self.lam, self.mu, self.ext = float(p["lam"]), float(p["mu"]), float(p["ext"])
-            self.Q, self.pi = d(p["Q"]), d(p["pi"])
-            self.st = list(params.TKF_STATE_TYPES)
-            self.groups = [0] * 5
+            self.builder = dbuild.DeviceBuilder("tkf92", "shared", p["Q"], p["pi"], lam=self.lam,
+                                                mu=self.mu, ext=self.ext, log_floor=True, device=dev)
         elif name == "mixdom2":
             self.p = p
-            self.K, self.F = np.asarray(p["frag_w"]).shape
-            self.Sx, self.cpis, self.cd = d(p["class_S"]), d(p["class_pi"]), d(p["class_dist"])
-            self.st = mixdom.state_types(self.K, self.F)
-            self.groups = tables.state_groups_per_class(len(self.st), self.F)
+            mix = dict(main_ins=p["main_lam"], main_del=p["main_mu"], dom_ins=p["dom_lam"],
+                       dom_del=p["dom_mu"], dom_w=p["dom_w"], frag_w=p["frag_w"], ext=p["ext"])
+            self.builder = dbuild.DeviceBuilder("mixdom", "per_class", gtr_Q(p["class_S"], p["class_pi"]),
+                                                p["class_pi"], mix=mix, class_dist=p["class_dist"],
+                                                log_floor=True, device=dev)
             self.model = None
         else:
             raise ValueError(f"unknown model {name}")
+        self.st = self.builder.state_types
+        self.groups = self.builder.state_group
         self.S = len(self.st)
 
     def build(self, ts):
-        """(log_trans (B,S,S), log_tab (B,G,TAB)) at per-pair t."""
-        t = torch.as_tensor(np.asarray(ts, dtype=np.float64), device=self.dev)
-        if self.name == "tkf92":
-            tau = params.tkf92_trans(self.lam, self.mu, t, self.ext)
-            sub = vjp.transition_matrix(self.Q, t)
-            tab = tables.shared_tables(sub, self.pi.expand(sub.shape[:-1]))
-        else:
-            p = self.p
-            tau = mixdom.build_nested_trans(p["main_lam"], p["main_mu"], t, p["dom_lam"], p["dom_mu"],
-                                            p["dom_w"], p["frag_w"], p["ext"])
-            subs = vjp.build_class_subs(self.Sx, self.cpis, t)
-            tab = tables.per_class_tables(subs, self.cpis, self.cd)
-        return torch.log(torch.clamp(tau, min=1e-30)), tab, t
+        """(log_trans (B,S,S), log_tab (B,G,TAB), t) at per-pair t."""
+        return self.builder.build(ts)
 
 
 @dataclass
