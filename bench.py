@@ -31,10 +31,11 @@ for _p in (ROOT, os.path.join(ROOT, "tkf-mixdom_amd")):
 METRIC = "fwd+bwd+counts DP cell-states/s, MixDom2 K=3 batched vs plain; p99 latency"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_STATE = 16.0
-# HBM bytes per launch of the dominant kernel from rocprofv3 --pmc FETCH_SIZE /
-# WRITE_SIZE (KiB per dispatch, scaled by the copy-kernel calibration); filled
-# in from the committed profile summaries.
-TRAFFIC = {}
+# HBM bytes per launch, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
+# workload (KiB per dispatch, scaled by the copy-kernel calibration: fetch x2.00,
+# write x1.00); profiles/fbv2_kernel_c1_mixdom2_k3.md.
+TRAFFIC = {"fbv2_kernel<LMix<3,1>,8>": 3.13e9, "fbv2_kernel<LMix<3,1>,4>": 8.28e9,
+           "fbv2_kernel<LMix<3,1>,2>": 5.14e9, "fbv2_kernel<LMix<3,1>,1>": 0.665e9}
 
 
 def _log(msg):
