@@ -587,8 +587,6 @@ size_t plan_ws_bytes(const Plan& pl) {
   return pl.desc_bytes + align_up(pl.lat_doubles * 8, 256) + align_up(pl.row_doubles * 8, 256);
 }
 
-// Pinned staging buffer for the per-pair descriptors (reused across calls;
-// an event guards reuse until the previous upload has completed).
 // Pinned staging buffers for host -> device uploads (per-pair descriptors,
 // statistic programs).  A buffer is reused only once the copy out of it has
 // completed (its event), so an upload never waits for earlier work on any
@@ -679,6 +677,7 @@ size_t tkfmi_fb2d_workspace_bytes(const tkfmi_problem* prob, uint32_t flags) {
   size_t v2b = 0;
   if (!(flags & TKFMI_GENERIC_KERNEL) && tkfmi_v2_workspace(prob, flags, &v2b))
     return v2b > 0 ? v2b : 1;
+  if (prob && tkfmi_wide_applies(prob)) return tkfmi_wide_workspace(prob, flags);
   Plan pl;
   if (make_plan(prob, flags, pl) != TKFMI_OK) return 0;
   return plan_ws_bytes(pl);
@@ -688,6 +687,18 @@ int tkfmi_fb2d_batched(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws,
                        size_t ws_bytes, void* stream_v, uint32_t flags) {
   g_err.clear();
   tkfmi_ktime_begin_call((flags & TKFMI_TIME_KERNELS) != 0);
+  if (p && tkfmi_wide_applies(p)) {
+    if (!o || !o->log_prob) return fail(TKFMI_EINVAL, "outputs.log_prob missing");
+    if ((flags & TKFMI_WANT_POSTERIORS) && !(flags & TKFMI_FORWARD_ONLY) && !o->posteriors)
+      return fail(TKFMI_EINVAL, "TKFMI_WANT_POSTERIORS without a posterior buffer");
+    if ((flags & TKFMI_WANT_EMIT_COUNTS) && !(flags & TKFMI_FORWARD_ONLY) && !o->emit_counts)
+      return fail(TKFMI_EINVAL, "TKFMI_WANT_EMIT_COUNTS without an emission-count buffer");
+    if (p->B > 0 && (!p->log_trans || !p->log_tab || !p->x_cat || !p->y_cat))
+      return fail(TKFMI_EINVAL, "device input pointer missing");
+    std::string err;
+    const int rcw = tkfmi_wide_run(p, o, ws, ws_bytes, reinterpret_cast<hipStream_t>(stream_v), flags, err);
+    return rcw == TKFMI_OK ? TKFMI_OK : fail(rcw, err);
+  }
   Plan pl;
   int rc = make_plan(p, flags, pl);
   if (rc != TKFMI_OK) return rc;
