@@ -16,6 +16,12 @@ inputs already resident in HBM.  Prints one JSON line.
                 OpenMP over pairs) on a bounded sample of the same pairs
   plain       = the per-pair drop-in (one launch per pair, as the reference's
                 per-pair forward_backward_2d call) on a sample of pairs
+  configs     = the other workloads, each timed on its own: c0 (TKF91
+                forward-only, 64 pairs), c2 (MixDom2 K=8 F=4 C=4, 162 states,
+                wide kernel), c3 (gradients of the summed MixDom2 K=3 F=1 C=2
+                log-likelihood over 200 pairs, forward + adjoint), fb1d (the
+                1D alignment-constrained forward/backward + counts over the
+                c1 pairs' generating alignments)
 """
 import argparse
 import json
@@ -126,6 +132,118 @@ def serve_bench(loads, n_requests):
             "loads": out}
 
 
+def _time_calls(fn, steps, warmup=1):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return 1e3 * (time.perf_counter() - t0) / steps
+
+
+def _kernel_ms(runner_fn):
+    from tkfmi import _abi
+    runner_fn()
+    return {k["name"]: round(k["ms"], 4) for k in _abi.last_kernel_times()}
+
+
+def fb1d_bench(w, steps):
+    """1D E-step DP over the alignments the c1 pairs were generated from:
+    per-pair chi and per-class tables of the c1 workload, emissions looked up
+    on the device, transition + emission counts."""
+    import torch
+    from tkfmi import _abi, synth
+    dev = torch.device("cuda", 0)
+    cts, ancs, descs = synth.alignment_columns(w)
+    cols = _abi.PackedColumns(cts, ancs, descs, dev)
+    r = _abi.Runner1D(w.log_trans.to(dev), init_state=0, final_state=1, columns=cols,
+                      state_types=w.state_types, state_group=w.state_group,
+                      log_tab=w.log_tab.to(dev), want_emit_counts=True)
+    ms = _time_calls(r.run, steps)
+    r.flags |= _abi.TIME_KERNELS
+    kern = _kernel_ms(r.run)
+    r.flags &= ~_abi.TIME_KERNELS
+    col_states = cols.rows * w.S
+    return {"value": col_states / (ms / 1e3), "unit": "column-states/s", "ms": ms,
+            "pairs": w.B, "columns": cols.rows, "states": w.S, "kernels_ms": kern,
+            "workload": "c1 pairs' generating alignments, MixDom2 K=3 (17 states), "
+                        "fwd+bwd+transition and emission counts"}
+
+
+def c0_bench(steps):
+    """c0: TKF91 forward-only log-likelihood of 64 pairs (lengths U{50..150},
+    t = 0.5)."""
+    import torch
+    from tkfmi import _abi, synth
+    w = synth.make_tkf(seed=0, B=64)
+    dev = torch.device("cuda", 0)
+    r = _abi.BatchRunner(w.log_trans.to(dev), w.state_types, w.state_group, w.log_tab.to(dev),
+                         _abi.PackedPairs(w.xs, w.ys, dev), forward_only=True)
+    ms = _time_calls(r.run, steps)
+    cs = w.cell_states()
+    return {"value": cs / (ms / 1e3), "unit": "cell-states/s (forward only)", "ms": ms,
+            "pairs": w.B, "cell_states": cs}
+
+
+def c2_bench(pairs):
+    """c2: MixDom2 K=8 F=4 C=4 (162 states) fwd+bwd+counts on the wide
+    kernel, lengths U{100..1000}; one timed call."""
+    import numpy as np
+    import torch
+    from tkfmi import _abi, synth
+    rng = np.random.default_rng(2)
+    w = synth.make_mixdom2(seed=2, B=pairs, K=8, F=4, C=4, lengths=rng.integers(100, 1001, pairs),
+                           name="c2")
+    dev = torch.device("cuda", 0)
+    r = _abi.BatchRunner(w.log_trans.to(dev), w.state_types, w.state_group, w.log_tab.to(dev),
+                         _abi.PackedPairs(w.xs, w.ys, dev), time_kernels=True)
+    ms = _time_calls(r.run, 1)
+    kern = {k["name"]: round(k["ms"], 2) for k in _abi.last_kernel_times()}
+    cs = w.cell_states()
+    # 6 S flops per cell-state (forward, backward, count products)
+    tf = 6.0 * w.S * cs / (ms / 1e3) / 1e12
+    out = {"value": cs / (ms / 1e3), "unit": "cell-states/s", "ms": ms, "pairs": w.B,
+           "cell_states": cs, "fp64_tflops": tf, "frac_of_fp64_mfma_peak": tf / 78.6,
+           "kernels_ms": kern, "workspace_GiB": r.ws_bytes / 2**30}
+    del r
+    torch.cuda.empty_cache()
+    return out
+
+
+def c3_bench(steps):
+    """c3: gradient of the summed MixDom2 K=3 F=1 C=2 log-likelihood of 200
+    pairs (LogNormal median 200) w.r.t. every model parameter and t:
+    torch builders + the forward/backward kernel + autograd composition."""
+    import numpy as np
+    import torch
+    from tkfmi import synth, vjp
+    w = synth.make_mixdom2(seed=3, B=200, K=3, F=1, C=2, name="c3")
+    pr = w.params
+    dev = torch.device("cuda", 0)
+    p = {"lam": pr["main_lam"], "mu": pr["main_mu"], "dlam": pr["dom_lam"], "dmu": pr["dom_mu"],
+         "dw": pr["dom_w"], "frag_w": pr["frag_w"], "ext": pr["ext"], "pis": pr["class_pi"],
+         "S": pr["class_S"], "cd": pr["class_dist"]}
+    tp = {k: torch.tensor(np.asarray(v, dtype=np.float64), device=dev, requires_grad=True)
+          for k, v in p.items()}
+    tt = torch.tensor(w.ts, dtype=torch.float64, device=dev, requires_grad=True)
+
+    def step():
+        for v in list(tp.values()) + [tt]:
+            v.grad = None
+        lp = vjp.mixdom2_log_prob_batch(tp["lam"], tp["mu"], tt, tp["dlam"], tp["dmu"], tp["dw"],
+                                        tp["frag_w"], tp["ext"], tp["pis"], tp["S"], tp["cd"],
+                                        w.xs, w.ys)
+        lp.sum().backward()
+
+    ms = _time_calls(step, steps)
+    cs = w.cell_states()
+    return {"value": cs / (ms / 1e3), "unit": "cell-states/s (value + full gradient)", "ms": ms,
+            "pairs": w.B, "cell_states": cs}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +258,10 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=5000)
     ap.add_argument("--no-emit", action="store_true",
                     help="transition counts only (no expected emission counts)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the c0 / c2 / c3 / fb1d measurements")
+    ap.add_argument("--c2-pairs", type=int, default=256,
+                    help="pairs of the c2 (162-state) measurement; 0 skips it")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (x5), to calibrate "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE against known traffic")
@@ -229,6 +351,21 @@ def main():
         pl = plain_per_pair(w, emit=not args.no_emit)
         out["plain_per_pair"] = pl
         out["batched_over_plain"] = value / pl["value"]
+    if not args.no_configs:
+        cfg = {}
+        _log("1D alignment-constrained E-step DP")
+        cfg["fb1d"] = fb1d_bench(w, args.steps)
+        _log(f"fb1d: {cfg['fb1d']['ms']:.3f} ms, {cfg['fb1d']['value']:.3e} column-states/s")
+        _log("c0 TKF91 forward-only")
+        cfg["c0"] = c0_bench(args.steps)
+        _log("c3 gradients, 200 pairs")
+        cfg["c3"] = c3_bench(max(1, args.steps // 2))
+        _log(f"c3: {cfg['c3']['ms']:.1f} ms per gradient")
+        if args.c2_pairs > 0:
+            _log(f"c2 162-state wide kernel, {args.c2_pairs} pairs")
+            cfg["c2"] = c2_bench(args.c2_pairs)
+            _log(f"c2: {cfg['c2']['ms']:.0f} ms, {cfg['c2']['fp64_tflops']:.2f} TF/s")
+        out["configs"] = cfg
     loads = [float(v) for v in args.serve_loads.split(",") if v.strip()]
     if loads:
         _log("serving measurement")

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define TKFMI_ABI_VERSION 2
+#define TKFMI_ABI_VERSION 3
 
 /* error codes */
 #define TKFMI_OK 0
@@ -121,6 +121,66 @@ size_t tkfmi_fb2d_workspace_bytes(const tkfmi_problem* prob, uint32_t flags);
  * the same quantities.
  */
 int tkfmi_fb2d_batched(const tkfmi_problem* prob, const tkfmi_outputs* out,
+                       void* workspace, size_t workspace_bytes,
+                       void* stream /* hipStream_t */, uint32_t flags);
+
+/* ------------------------------------------------------------------------
+ * 1D alignment-constrained forward/backward (the MSA-guided DP).
+ * ---------------------------------------------------------------------- */
+
+/* A ragged batch of 1D problems: pair b has h_len[b] alignment columns,
+ * rows h_off[b] .. h_off[b] + h_len[b] - 1 of the row-indexed inputs.
+ * Emissions come either densely (log_emit != NULL: the reference's
+ * log_emissions argument, NEG_INF = -1e30 where a state cannot emit) or
+ * from the column types and characters plus per-group tables (log_emit ==
+ * NULL: pair_hmm_emissions_constrained, dp/hmm.py:854-1100, computed on the
+ * device from the same tables the 2D path uses). */
+typedef struct tkfmi_problem1d {
+  int32_t B;                   /* number of pairs                           */
+  int32_t S;                   /* number of states, 1..256                  */
+  int32_t s_idx;               /* init state (the reference's init_state)   */
+  int32_t e_idx;               /* final state (final_state)                 */
+  int32_t n_trans;             /* 1 (shared) or B (per pair)                */
+  int32_t pad0;
+  const double* log_trans;     /* device [n_trans,S,S] log transitions      */
+  const double* log_emit;      /* device [rows,S] log emissions, or NULL    */
+  const int32_t* state_types;  /* host [S] (constrained mode): 0=S 1=M 2=I 3=D 4=E */
+  const int32_t* state_group;  /* host [S] (constrained mode)               */
+  int32_t n_groups;            /* G (constrained mode)                      */
+  int32_t n_tab;               /* 1 or B (constrained mode)                 */
+  const double* log_tab;       /* device [n_tab,G,TKFMI_TAB_SZ] (constrained) */
+  const uint8_t* col_type;     /* device [rows]: 1=M 2=I 3=D (constrained)  */
+  const uint8_t* anc;          /* device [rows]: ancestor code at M/D columns */
+  const uint8_t* desc;         /* device [rows]: descendant code at M/I columns */
+  const int64_t* h_off;        /* host [B] first row of each pair           */
+  const int32_t* h_len;        /* host [B] columns L of each pair           */
+} tkfmi_problem1d;
+
+/* Bytes of device workspace tkfmi_fb1d_batched needs for `prob`/`flags`. */
+size_t tkfmi_fb1d_workspace_bytes(const tkfmi_problem1d* prob, uint32_t flags);
+
+/*
+ * Batched 1D forward/backward + expected transition counts.  Replaces, per
+ * pair, forward_backward_1d_padded (dp/hmm.py:471-607, with forward_1d_scan
+ * :404-437 and backward_1d_scan :440-468; TKFMI_FORWARD_ONLY gives the
+ * forward-only fast path) and its vmapped caller batched_fb_per_chi
+ * (train_pfam.py:1576-1610, used by _process_pairs_batched
+ * train_pfam.py:2049-2603), at each pair's real length (no _pad_to_bin).
+ *   log_prob[b]: logsumexp_k alpha[L-1,k] + T[k,final]; L = 0 gives
+ *     T[init,final] and n[init,final] = 1 (dp/hmm.py:598-607).
+ *   n_trans[b] (S,S): interior, terminal and initial terms of
+ *     dp/hmm.py:549-596; the backward uses T with T[final,final] = 1
+ *     (dp/hmm.py:532) as the reference does.
+ *   posteriors (TKFMI_WANT_POSTERIORS): ragged [sum_b L_b, S], pair b at row
+ *     sum_{b'<b} L_b', exp(alpha + beta - logP) (dp/hmm.py:547).
+ *   emit_counts (TKFMI_WANT_EMIT_COUNTS, constrained mode only): [B,G,TAB_SZ]
+ *     posterior mass per table entry, the input of the reference's
+ *     per-domain / per-class count reductions (train_pfam.py:2338-2546).
+ * Outputs n_trans / posteriors / emit_counts are accumulated (+=) and must
+ * be zero-filled by the caller.  An impossible pair gets log_prob -1e30
+ * and zero counts.
+ */
+int tkfmi_fb1d_batched(const tkfmi_problem1d* prob, const tkfmi_outputs* out,
                        void* workspace, size_t workspace_bytes,
                        void* stream /* hipStream_t */, uint32_t flags);
 

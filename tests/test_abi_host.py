@@ -41,6 +41,7 @@ def test_struct_layout_matches_header():
     from tkfmi import _abi
     text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
     for cname, py in (("tkfmi_problem", _abi.Problem), ("tkfmi_outputs", _abi.Outputs),
+                      ("tkfmi_problem1d", _abi.Problem1D),
                       ("tkfmi_kernel_time", _abi.KernelTime),
                       ("tkfmi_mixdom_model", _abi.MixdomModel),
                       ("tkfmi_builder", _abi.Builder)):
@@ -97,6 +98,58 @@ def test_invalid_problems_rejected_with_message():
     p, k = _problem([0, 1, 2, 3, 4], [0] * 5, [-1], [4])
     rc = L.tkfmi_fb2d_batched(ctypes.byref(p), ctypes.byref(_abi.Outputs()), None, 0, None, 0)
     assert rc == -1
+
+
+def _problem1d(S, lens, s_idx=0, e_idx=1, constrained=True, G=1, sg=None):
+    from tkfmi import _abi
+    B = len(lens)
+    keep = dict(st=np.asarray([0, 4] + [1, 2, 3, 2, 3] * ((S - 2) // 5), np.int32)[:S],
+                sg=np.asarray(sg if sg is not None else [0] * S, np.int32),
+                ln=np.asarray(lens, np.int32), off=np.zeros(B, np.int64))
+    if B:
+        keep["off"][1:] = np.cumsum(keep["ln"][:-1])
+    fake = ctypes.c_void_p(16)  # never dereferenced by the host-side checks
+    p = _abi.Problem1D(B, S, s_idx, e_idx, 1, 0, fake, None if constrained else fake,
+                       _abi._np_ptr(keep["st"]) if constrained else None,
+                       _abi._np_ptr(keep["sg"]) if constrained else None, G, 1,
+                       fake if constrained else None, fake, fake, fake,
+                       _abi._np_ptr(keep["off"]), _abi._np_ptr(keep["ln"]))
+    return p, keep
+
+
+def test_fb1d_workspace_query_and_validation():
+    from tkfmi import _abi
+    L = _abi.lib()
+    p_small, k1 = _problem1d(17, [50, 80])
+    p_big, k2 = _problem1d(17, [500, 800])
+    a = L.tkfmi_fb1d_workspace_bytes(ctypes.byref(p_small), 0)
+    b = L.tkfmi_fb1d_workspace_bytes(ctypes.byref(p_big), 0)
+    c = L.tkfmi_fb1d_workspace_bytes(ctypes.byref(p_big), _abi.FORWARD_ONLY)
+    d = L.tkfmi_fb1d_workspace_bytes(ctypes.byref(p_big), _abi.WANT_EMIT_COUNTS)
+    assert 0 < a < b and 0 < c < b < d
+    dense, k3 = _problem1d(162, [10], constrained=False)
+    assert L.tkfmi_fb1d_workspace_bytes(ctypes.byref(dense), 0) > 0
+    for bad in (_problem1d(300, [4], constrained=False),          # S above 256
+                _problem1d(17, [4], s_idx=17),                   # init state out of range
+                _problem1d(17, [-2]),                            # negative length
+                _problem1d(17, [4], G=1, sg=[0] * 16 + [3])):    # group out of range
+        p, k = bad
+        assert L.tkfmi_fb1d_workspace_bytes(ctypes.byref(p), 0) == 0
+        rc = L.tkfmi_fb1d_batched(ctypes.byref(p), ctypes.byref(_abi.Outputs()), None, 0, None, 0)
+        assert rc == -1 and L.tkfmi_last_error().decode()
+
+
+def test_packed_columns_validate_alignments():
+    import torch
+    from tkfmi import _abi
+    pc = _abi.PackedColumns([np.array([1, 2, 3]), np.array([], int)],
+                            [np.array([5, 0, 25]), np.array([], int)],
+                            [np.array([6, 7, 0]), np.array([], int)], torch.device("cpu"))
+    assert list(pc.len) == [3, 0] and list(pc.off) == [0, 3] and pc.rows == 3
+    assert pc.col.tolist()[:3] == [1, 2, 3] and pc.anc.tolist()[:3] == [5, 0, 20]
+    with pytest.raises(ValueError):
+        _abi.PackedColumns([np.array([1, 4])], [np.array([0, 0])], [np.array([0, 0])],
+                           torch.device("cpu"))
 
 
 def test_packed_pairs_and_code_normalisation():

@@ -14,7 +14,7 @@ import torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libtkfmi.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 FORWARD_ONLY = 1
 WANT_POSTERIORS = 2
 NO_COUNTS = 4
@@ -44,6 +44,29 @@ class Problem(ctypes.Structure):
         ("h_y_off", _c_p),
         ("h_x_len", _c_p),
         ("h_y_len", _c_p),
+    ]
+
+
+class Problem1D(ctypes.Structure):
+    _fields_ = [
+        ("B", ctypes.c_int32),
+        ("S", ctypes.c_int32),
+        ("s_idx", ctypes.c_int32),
+        ("e_idx", ctypes.c_int32),
+        ("n_trans", ctypes.c_int32),
+        ("pad0", ctypes.c_int32),
+        ("log_trans", _c_p),
+        ("log_emit", _c_p),
+        ("state_types", _c_p),
+        ("state_group", _c_p),
+        ("n_groups", ctypes.c_int32),
+        ("n_tab", ctypes.c_int32),
+        ("log_tab", _c_p),
+        ("col_type", _c_p),
+        ("anc", _c_p),
+        ("desc", _c_p),
+        ("h_off", _c_p),
+        ("h_len", _c_p),
     ]
 
 
@@ -108,6 +131,7 @@ _lock = threading.Lock()
 # Every symbol include/tkfmi.h declares (checked by the CPU test-suite).
 EXPORTED = ("tkfmi_abi_version", "tkfmi_last_error",
             "tkfmi_fb2d_workspace_bytes", "tkfmi_fb2d_batched",
+            "tkfmi_fb1d_workspace_bytes", "tkfmi_fb1d_batched",
             "tkfmi_last_kernel_times", "tkfmi_suffstats_size",
             "tkfmi_restore_workspace_bytes", "tkfmi_restore_counts_batched",
             "tkfmi_bdi_stats_batched", "tkfmi_tkf92_estep_stats", "tkfmi_hr_stats_batched",
@@ -134,6 +158,11 @@ def lib():
             L.tkfmi_fb2d_workspace_bytes.argtypes = [ctypes.POINTER(Problem), ctypes.c_uint32]
             L.tkfmi_fb2d_batched.restype = ctypes.c_int
             L.tkfmi_fb2d_batched.argtypes = [ctypes.POINTER(Problem), ctypes.POINTER(Outputs),
+                                             _c_p, ctypes.c_size_t, _c_p, ctypes.c_uint32]
+            L.tkfmi_fb1d_workspace_bytes.restype = ctypes.c_size_t
+            L.tkfmi_fb1d_workspace_bytes.argtypes = [ctypes.POINTER(Problem1D), ctypes.c_uint32]
+            L.tkfmi_fb1d_batched.restype = ctypes.c_int
+            L.tkfmi_fb1d_batched.argtypes = [ctypes.POINTER(Problem1D), ctypes.POINTER(Outputs),
                                              _c_p, ctypes.c_size_t, _c_p, ctypes.c_uint32]
             L.tkfmi_last_kernel_times.restype = ctypes.c_int
             L.tkfmi_last_kernel_times.argtypes = [ctypes.POINTER(KernelTime), ctypes.c_int32]
@@ -341,3 +370,144 @@ def check(rc):
 
 def current_stream_ptr(device):
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+class PackedColumns:
+    """A ragged batch of alignments resident on the device: per column the
+    type (1=M, 2=I, 3=D) and the ancestor / descendant codes (0..20; the
+    code of the side a column does not consume is ignored)."""
+
+    def __init__(self, col_types, ancs, descs, device):
+        if not (len(col_types) == len(ancs) == len(descs)):
+            raise ValueError("col_types, ancs and descs must have the same length")
+        self.B = len(col_types)
+        cts = [np.asarray(c.detach().cpu() if isinstance(c, torch.Tensor) else c).astype(np.int64).ravel()
+               for c in col_types]
+        for c in cts:
+            if c.size and (c.min() < 1 or c.max() > 3):
+                raise ValueError("column types must be 1 (M), 2 (I) or 3 (D)")
+        an = [normalise_codes(a) for a in ancs]
+        de = [normalise_codes(d) for d in descs]
+        self.len = np.array([len(c) for c in cts], dtype=np.int32)
+        for c, a, d in zip(cts, an, de):
+            if len(a) != len(c) or len(d) != len(c):
+                raise ValueError("anc / desc must have one code per column")
+        self.off = np.zeros(self.B, dtype=np.int64)
+        if self.B:
+            self.off[1:] = np.cumsum(self.len[:-1], dtype=np.int64)
+        cat = lambda arrs: torch.from_numpy(
+            np.concatenate([x.astype(np.uint8) for x in arrs] + [np.zeros(1, np.uint8)])).to(device)
+        self.col = cat(cts)
+        self.anc = cat(an)
+        self.desc = cat(de)
+        self.rows = int(self.len.sum())
+
+
+class Runner1D:
+    """Launcher of the batched 1D forward/backward (tkfmi_fb1d_batched).
+
+    log_trans: float64 device tensor (S,S) or (B,S,S).  Emissions either
+    dense (``log_emit``: float64 device tensor (rows, S) with rows =
+    sum of lengths, pair b's columns consecutive from ``offsets[b]``) or
+    constrained (``columns``: PackedColumns, with ``state_types``,
+    ``state_group`` and ``log_tab`` (G, TAB_SZ) or (B, G, TAB_SZ))."""
+
+    def __init__(self, log_trans, *, init_state, final_state, lengths=None, offsets=None,
+                 log_emit=None, columns=None, state_types=None, state_group=None, log_tab=None,
+                 forward_only=False, want_posteriors=False, want_counts=True,
+                 want_emit_counts=False, time_kernels=False):
+        self.L = lib()
+        dev = log_trans.device
+        if dev.type != "cuda":
+            raise TkfmiError("tkfmi kernels need a CUDA/HIP device tensor")
+        self.dev = dev
+        self.log_trans = log_trans.to(torch.float64).contiguous()
+        S = self.S = int(self.log_trans.shape[-1])
+        n_trans = 1 if self.log_trans.dim() == 2 else int(self.log_trans.shape[0])
+        self.keep = []
+        if (log_emit is None) == (columns is None):
+            raise ValueError("give exactly one of log_emit and columns")
+        if columns is not None:
+            B = columns.B
+            self.h_len, self.h_off = columns.len, columns.off
+            st = np.ascontiguousarray(np.asarray(state_types, dtype=np.int32))
+            sg = np.ascontiguousarray(np.asarray(state_group, dtype=np.int32))
+            if st.shape != (S,) or sg.shape != (S,):
+                raise ValueError("state_types/state_group must have length S")
+            self.log_tab = log_tab.to(torch.float64).contiguous()
+            if self.log_tab.shape[-1] != TAB_SZ:
+                raise ValueError(f"log_tab last dim must be {TAB_SZ}")
+            if self.log_tab.dim() == 2:
+                n_tab, G = 1, int(self.log_tab.shape[0])
+            else:
+                n_tab, G = int(self.log_tab.shape[0]), int(self.log_tab.shape[1])
+            self.G = G
+            self.keep += [st, sg, columns]
+            prob_args = dict(log_emit=None, state_types=_np_ptr(st), state_group=_np_ptr(sg),
+                             n_groups=G, n_tab=n_tab, log_tab=_ptr(self.log_tab),
+                             col_type=_ptr(columns.col), anc=_ptr(columns.anc), desc=_ptr(columns.desc))
+        else:
+            self.h_len = np.ascontiguousarray(np.asarray(lengths, dtype=np.int32))
+            B = len(self.h_len)
+            if offsets is None:
+                offsets = np.zeros(B, dtype=np.int64)
+                if B:
+                    offsets[1:] = np.cumsum(self.h_len[:-1], dtype=np.int64)
+            self.h_off = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
+            self.log_emit = log_emit.to(torch.float64).contiguous()
+            if self.log_emit.dim() != 2 or self.log_emit.shape[1] != S:
+                raise ValueError("log_emit must be (rows, S)")
+            if B and int((self.h_off + self.h_len).max()) > self.log_emit.shape[0]:
+                raise ValueError("log_emit has fewer rows than the lengths / offsets need")
+            self.G = 0
+            prob_args = dict(log_emit=_ptr(self.log_emit), state_types=None, state_group=None,
+                             n_groups=0, n_tab=0, log_tab=None, col_type=None, anc=None, desc=None)
+        self.B = B
+        fwd = bool(forward_only)
+        self.flags = ((FORWARD_ONLY if fwd else 0)
+                      | (WANT_POSTERIORS if (want_posteriors and not fwd) else 0)
+                      | (0 if want_counts else NO_COUNTS)
+                      | (WANT_EMIT_COUNTS if (want_emit_counts and not fwd and columns is not None) else 0)
+                      | (TIME_KERNELS if time_kernels else 0))
+        self.prob = Problem1D(B, S, int(init_state), int(final_state), n_trans, 0,
+                              _ptr(self.log_trans), prob_args["log_emit"], prob_args["state_types"],
+                              prob_args["state_group"], prob_args["n_groups"], prob_args["n_tab"],
+                              prob_args["log_tab"], prob_args["col_type"], prob_args["anc"],
+                              prob_args["desc"], _np_ptr(self.h_off), _np_ptr(self.h_len))
+        self.ws_bytes = int(self.L.tkfmi_fb1d_workspace_bytes(ctypes.byref(self.prob), self.flags))
+        if self.ws_bytes == 0:
+            raise TkfmiError(self.L.tkfmi_last_error().decode() or "invalid 1D problem")
+        self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=dev)
+        rows = int(self.h_len.sum())
+        self.rows = rows
+        self.log_p = torch.empty(B, dtype=torch.float64, device=dev)
+        self.n = (torch.zeros((B, S, S), dtype=torch.float64, device=dev)
+                  if (want_counts and not fwd) else None)
+        self.post = (torch.zeros(rows * S + 1, dtype=torch.float64, device=dev)
+                     if (self.flags & WANT_POSTERIORS) else None)
+        self.ecnt = (torch.zeros((B, self.G, TAB_SZ), dtype=torch.float64, device=dev)
+                     if (self.flags & WANT_EMIT_COUNTS) else None)
+        self.out = Outputs(_ptr(self.log_p), _ptr(self.n), _ptr(self.post), _ptr(self.ecnt))
+
+    def run(self, zero_outputs=True, stream=None):
+        if zero_outputs:
+            for t in (self.n, self.post, self.ecnt):
+                if t is not None:
+                    t.zero_()
+        if stream is None:
+            stream = torch.cuda.current_stream(self.dev)
+        rc = self.L.tkfmi_fb1d_batched(ctypes.byref(self.prob), ctypes.byref(self.out),
+                                       _ptr(self.ws), ctypes.c_size_t(self.ws_bytes),
+                                       ctypes.c_void_p(stream.cuda_stream), self.flags)
+        if rc != 0:
+            raise TkfmiError(self.L.tkfmi_last_error().decode() or f"tkfmi error {rc}")
+        return self.log_p
+
+    def posteriors(self):
+        """Per-pair (L_b, S) views of the posterior buffer (pair order)."""
+        out, off = [], 0
+        for b in range(self.B):
+            n = int(self.h_len[b]) * self.S
+            out.append(self.post[off:off + n].view(int(self.h_len[b]), self.S))
+            off += n
+        return out

@@ -5,6 +5,8 @@
 * ``e_step`` / ``e_step_mixdom2`` (train/likelihood.py:49-369)
 * ``process_pairs_batched_2d`` -- the ``delta`` dict of
   ``_process_pairs_batched_2d`` (python/train_pfam.py:2672-2903)
+* ``process_pairs_batched`` -- the ``delta`` dict of the alignment-constrained
+  (1D) ``_process_pairs_batched`` (python/train_pfam.py:2049-2603)
 
 Each runs the batched forward/backward kernel once over its pairs (no
 geometric-bin bucketing: every pair at its real lengths), then the
@@ -247,5 +249,115 @@ def process_pairs_batched_2d(batch_pairs, chi_params, st, Q_lg, pi_lg, N, dom_Qs
     acc.update(top_E_B=float(b[0, 0]), top_E_D=float(b[0, 1]), top_E_S=float(b[0, 2]),
                top_T_obs=float(b[0, 3]), dom_E_B=b[1:, 0].copy(), dom_E_D=b[1:, 1].copy(),
                dom_E_S=b[1:, 2].copy(), dom_T_obs=b[1:, 3].copy())
+    delta["exact_ss"] = acc
+    return delta
+
+
+# ---------------------------------------- batched 1D (MSA-guided) E-step delta
+def process_pairs_batched(batch_pairs, chi_params, st, Q_lg, pi_lg, N, alignment_to_states=None,
+                          dom_Qs=None, dom_pis=None, n_dom=1, n_frag=1, classdist_params=None,
+                          fb_scan_mode="associative"):
+    """Sufficient-statistics delta of a minibatch of aligned pairs
+    (train_pfam.py:2049-2603), on the 1D alignment-constrained DP.
+
+    batch_pairs: (x, y, states, anc_chars, desc_chars, t_est) tuples; the
+    column types ``states`` and the compact character lists define each
+    pair's alignment (x, y are unused, as in the reference).  As the
+    reference does: chi is built at t_est rounded to float32 (:2321) and the
+    substitution matrices at the FP64 t_est (:2119); Holmes-Rubin and the
+    per-pair count restoration use the float32 t (:2364, :2589); emissions
+    are per-domain expm(Q_d t) (LG tiled when dom_Qs is None), or per-class
+    mixtures when ``classdist_params`` holds 'classdist', 'class_S_exch',
+    'class_pis' (:2129-2157).  Every pair is accumulated (no non-finite
+    filter on this path).  The reference rounds the log emissions to float32
+    before its DP (:2261); here they stay FP64.  Its per-column 1e-30
+    cut-offs in the class responsibilities (:2451-2466) drop contributions
+    below 1e-30 that are kept here.  ``alignment_to_states`` and
+    ``fb_scan_mode`` are accepted for signature parity (unused there too /
+    same result in both modes).  Returns the delta dict or None."""
+    if not batch_pairs:
+        return None
+    dev = hmm.device()
+    t64 = np.array([it[5] for it in batch_pairs], dtype=np.float64)
+    t32 = t64.astype(np.float32).astype(np.float64)
+    cp = chi_params
+    K, F = int(n_dom), int(n_frag)
+    mix = dict(main_ins=cp["main_ins"], main_del=cp["main_del"], dom_ins=cp["dom_ins"],
+               dom_del=cp["dom_del"], dom_w=cp["dom_weights"], frag_w=cp["frag_weights"],
+               ext=cp["ext_rates"])
+    use_cd = classdist_params is not None and "classdist" in classdist_params
+    if dom_Qs is not None and dom_pis is not None:
+        Qs_n, pis_n = _np(dom_Qs).astype(np.float64), _np(dom_pis).astype(np.float64)
+    else:
+        Qs_n = np.tile(_np(Q_lg).astype(np.float64)[None], (K, 1, 1))
+        pis_n = np.tile(_np(pi_lg).astype(np.float64)[None], (K, 1))
+    if use_cd:
+        S_exch = _np(classdist_params["class_S_exch"]).astype(np.float64)
+        cpis_n = _np(classdist_params["class_pis"]).astype(np.float64)
+        cd_n = _np(classdist_params["classdist"]).astype(np.float64)
+        Qc_n = S_exch * cpis_n[:, None, :]
+        for c in range(Qc_n.shape[0]):
+            np.fill_diagonal(Qc_n[c], 0.0)
+            Qc_n[c][np.diag_indices(Qc_n.shape[1])] = -Qc_n[c].sum(axis=1)
+        b = dbuild.DeviceBuilder("mixdom", "per_class", Qc_n, cpis_n, mix=mix, class_dist=cd_n,
+                                 log_floor=True, device=dev)
+    else:
+        b = dbuild.DeviceBuilder("mixdom", "per_domain", Qs_n, pis_n, mix=mix, log_floor=True,
+                                 device=dev)
+    if b.S != N:
+        raise ValueError(f"N = {N} but the model has {b.S} states")
+    lt, _, t_dev32 = b.build(t32)                    # chi at the float32 t
+    _, tab, _ = b.build(t64)                         # emission tables at the FP64 t
+    cts, ancs, descs = [], [], []
+    for it in batch_pairs:
+        ct = np.asarray(it[2], dtype=np.int64).ravel()
+        a = np.zeros(len(ct), dtype=np.int64)
+        d = np.zeros(len(ct), dtype=np.int64)
+        a[(ct == 1) | (ct == 3)] = np.asarray(it[3], dtype=np.int64).ravel()
+        d[(ct == 1) | (ct == 2)] = np.asarray(it[4], dtype=np.int64).ravel()
+        cts.append(ct)
+        ancs.append(a)
+        descs.append(d)
+    from . import hmm1d
+    res = hmm1d.fb1d_constrained_batch(lt, b.state_types, b.state_group, tab, cts, ancs, descs,
+                                       init_state=0, final_state=1, want_emit_counts=True)
+    n, lp, ec = res["n"], res["log_p"], res["emit_counts"]
+    Bn = len(batch_pairs)
+    ec_dom = ec.reshape(Bn, K, -1, tables.TAB_SZ).sum(2) if use_cd else ec   # (B, K, TAB)
+    dm, di, dd = _split_ecnt(ec_dom)
+    delta = {
+        "agg_n_chi": n.sum(0).cpu().numpy(),
+        "total_ll": float(lp.sum()),
+        "n_pairs": Bn,
+        "n_files": 1,
+        "match_counts": dm.sum((0, 1)).cpu().numpy(),
+        "dom_match_counts": dm.sum(0).cpu().numpy(),
+        "dom_insert_counts": di.sum(0).cpu().numpy(),
+        "dom_delete_counts": dd.sum(0).cpu().numpy(),
+    }
+    Qs, pis = _dev64(Qs_n, dev), _dev64(pis_n, dev)
+    W, U = hr.per_pair_hr_stats(Qs, pis, t_dev32, dm.contiguous())
+    delta["dom_W"] = W.sum(0).cpu().numpy()
+    delta["dom_U"] = U.sum(0).cpu().numpy()
+    if use_cd:
+        cpis = _dev64(cpis_n, dev)
+        cdv = _dev64(cd_n, dev)
+        subs = vjp.build_class_subs(_dev64(S_exch, dev), cpis, _dev64(t64, dev))  # (B, C, A, A)
+        cm, ci, cdl, cdc = _class_counts(ec, subs, cpis, cdv, K, F)
+        Wc, Uc = hr.per_pair_hr_stats(_dev64(Qc_n, dev), cpis, t_dev32, cm.contiguous())
+        delta.update(class_match_counts=cm.sum(0).cpu().numpy(),
+                     class_insert_counts=ci.sum(0).cpu().numpy(),
+                     class_delete_counts=cdl.sum(0).cpu().numpy(),
+                     classdist_counts=cdc.sum(0).cpu().numpy(),
+                     class_W=Wc.sum(0).cpu().numpy(), class_U=Uc.sum(0).cpu().numpy())
+    ss = suffstats.exact_suffstats_batch(n, cp["main_ins"], cp["main_del"], t_dev32,
+                                         cp["dom_ins"], cp["dom_del"], cp["dom_weights"],
+                                         cp["frag_weights"], cp["ext_rates"], with_bdi=True)
+    acc = {k: v.sum(0).cpu().numpy() for k, v in ss.items() if k not in ("dom_M_5x5", "bdi")}
+    acc["dom_M_5x5"] = [v.sum(0).cpu().numpy() for v in ss["dom_M_5x5"]]
+    bb = ss["bdi"].sum(0).cpu().numpy()
+    acc.update(top_E_B=float(bb[0, 0]), top_E_D=float(bb[0, 1]), top_E_S=float(bb[0, 2]),
+               top_T_obs=float(bb[0, 3]), dom_E_B=bb[1:, 0].copy(), dom_E_D=bb[1:, 1].copy(),
+               dom_E_S=bb[1:, 2].copy(), dom_T_obs=bb[1:, 3].copy())
     delta["exact_ss"] = acc
     return delta

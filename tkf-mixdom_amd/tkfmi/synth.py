@@ -77,12 +77,14 @@ def _structural_backward(T, types, x, pi_x_marg):
     return B, ls
 
 
-def sample_descendant(rng, x, T, types, emit_y_match, emit_y_ins, pi_x_marg):
+def sample_descendant(rng, x, T, types, emit_y_match, emit_y_ins, pi_x_marg, path=None):
     """Backward-guided conditional sampling of y given x.
 
     emit_y_match(state, a_char) -> distribution over y for a match state;
     emit_y_ins(state) -> distribution over y for an insert state;
-    pi_x_marg[a] = marginal emission weight of consuming x[a]."""
+    pi_x_marg[a] = marginal emission weight of consuming x[a].  When ``path``
+    is a list, the column type (M/I/D) of every step is appended to it: the
+    alignment the pair was generated from."""
     types = np.asarray(types)
     N = T.shape[0]
     L = len(x)
@@ -108,6 +110,8 @@ def sample_descendant(rng, x, T, types, emit_y_match, emit_y_ins, pi_x_marg):
         if s2 == e_idx:
             break
         ty = types[s2]
+        if path is not None:
+            path.append(int(ty))
         if ty == params.M:
             p = emit_y_match(s2, x[a])
             y.append(int(np.searchsorted(np.cumsum(p), rng.random() * p.sum(), side="right")))
@@ -209,7 +213,7 @@ def make_mixdom2(seed=1, B=1024, K=3, F=1, C=1, median=200, sigma=0.5, lo=30, hi
     subs = np.stack([expm_batch(pr["class_Q"][c], ts) for c in range(C)], axis=1)  # (B,C,A,A)
     cpi = pr["class_pi"]
     cd = pr["class_dist"]
-    xs, ys = [], []
+    xs, ys, alns = [], [], []
     for b in range(B):
         # ancestor from the class-mixture equilibrium of a random (d,f)
         wx = cd.reshape(-1, C).mean(axis=0) @ cpi
@@ -221,11 +225,33 @@ def make_mixdom2(seed=1, B=1024, K=3, F=1, C=1, median=200, sigma=0.5, lo=30, hi
 
         def em_ins(s):
             return cd[dom[s], frag[s]] @ cpi
-        y = sample_descendant(rng, x, chi[b], types, em_match, em_ins, wx[x] / wx.sum())
+        path = []
+        y = sample_descendant(rng, x, chi[b], types, em_match, em_ins, wx[x] / wx.sum(), path=path)
         xs.append(x.astype(np.int64))
         ys.append(y)
+        alns.append(np.asarray(path, dtype=np.int64))
     log_trans = torch.log(torch.clamp(torch.as_tensor(chi), min=1e-30))
     log_tab = tables.per_class_tables(torch.as_tensor(subs), torch.as_tensor(cpi), torch.as_tensor(cd))
     groups = tables.state_groups_per_class(N, F)
-    pr.update(ts=ts, class_subs=subs)
+    pr.update(ts=ts, class_subs=subs, alignments=alns)
     return Workload(name, N, list(types), groups, xs, ys, ts, log_trans, log_tab, pr)
+
+
+def alignment_columns(w):
+    """Per pair (col_types, anc_full, desc_full) of the alignment each pair of
+    a make_mixdom2 workload was generated from (the sampler stops at the
+    first E, so the path consumes a prefix of y when y was clipped)."""
+    cts, ancs, descs = [], [], []
+    for x, y, ct in zip(w.xs, w.ys, w.params["alignments"]):
+        a = np.zeros(len(ct), dtype=np.int64)
+        d = np.zeros(len(ct), dtype=np.int64)
+        ia = np.cumsum((ct == params.M) | (ct == params.D)) - 1
+        idd = np.cumsum((ct == params.M) | (ct == params.I)) - 1
+        mx = (ct == params.M) | (ct == params.D)
+        my = (ct == params.M) | (ct == params.I)
+        a[mx] = np.asarray(x)[ia[mx]]
+        d[my] = np.asarray(y)[idd[my]]
+        cts.append(ct)
+        ancs.append(a)
+        descs.append(d)
+    return cts, ancs, descs
