@@ -263,15 +263,15 @@ def process_pairs_batched(batch_pairs, chi_params, st, Q_lg, pi_lg, N, alignment
     batch_pairs: (x, y, states, anc_chars, desc_chars, t_est) tuples; the
     column types ``states`` and the compact character lists define each
     pair's alignment (x, y are unused, as in the reference).  As the
-    reference does: chi is built at t_est rounded to float32 (:2321) and the
-    substitution matrices at the FP64 t_est (:2119); Holmes-Rubin and the
-    per-pair count restoration use the float32 t (:2364, :2589); emissions
+    reference does: chi is built at t_est rounded to float32 (:2322) and the
+    substitution matrices at the FP64 t_est (:2112); Holmes-Rubin and the
+    per-pair count restoration use the float32 t (:2364, :2343); emissions
     are per-domain expm(Q_d t) (LG tiled when dom_Qs is None), or per-class
     mixtures when ``classdist_params`` holds 'classdist', 'class_S_exch',
-    'class_pis' (:2129-2157).  Every pair is accumulated (no non-finite
+    'class_pis' (:2137-2157).  Every pair is accumulated (no non-finite
     filter on this path).  The reference rounds the log emissions to float32
-    before its DP (:2261); here they stay FP64.  Its per-column 1e-30
-    cut-offs in the class responsibilities (:2451-2466) drop contributions
+    before its DP (:2243); here they stay FP64.  Its per-column 1e-30
+    cut-offs in the class responsibilities (:2455-2462) drop contributions
     below 1e-30 that are kept here.  ``alignment_to_states`` and
     ``fb_scan_mode`` are accepted for signature parity (unused there too /
     same result in both modes).  Returns the delta dict or None."""
