@@ -44,6 +44,15 @@ TRAFFIC = {"fbv2_kernel<LMix<3,1>,8>": 3.13e9, "fbv2_kernel<LMix<3,1>,4>": 8.28e
            "fbv2_kernel<LMix<3,1>,2>": 5.14e9, "fbv2_kernel<LMix<3,1>,1>": 0.665e9}
 
 
+def _bytes_per_cell_state(kernel_name):
+    """Algorithmic lattice bytes per cell-state of one launch: the forward
+    writes the lattice (8 B), the backward reads it (8 B); a kernel that does
+    both in one launch moves 16 B."""
+    if kernel_name.startswith(("fbc_fwd", "fbc_bwd")):
+        return BYTES_PER_CELL_STATE / 2
+    return BYTES_PER_CELL_STATE
+
+
 def _log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -262,6 +271,9 @@ def main():
                     help="skip the c0 / c2 / c3 / fb1d measurements")
     ap.add_argument("--c2-pairs", type=int, default=256,
                     help="pairs of the c2 (162-state) measurement; 0 skips it")
+    ap.add_argument("--kernel", default="chain", choices=["chain", "class", "generic"],
+                    help="2D kernel: chained workgroups per pair (default), one workgroup per "
+                         "pair, or the any-layout kernel")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (x5), to calibrate "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE against known traffic")
@@ -291,7 +303,8 @@ def main():
     torch.cuda.set_device(dev)
     pairs = _abi.PackedPairs(w.xs, w.ys, dev)
     runner = _abi.BatchRunner(w.log_trans.to(dev), w.state_types, w.state_group,
-                              w.log_tab.to(dev), pairs, want_emit_counts=not args.no_emit)
+                              w.log_tab.to(dev), pairs, want_emit_counts=not args.no_emit,
+                              kernel=args.kernel)
     cell_states = w.cell_states()
     _log(f"pairs={w.B} S={w.S} cell-states={cell_states:.4e} workspace={runner.ws_bytes/2**30:.2f} GiB")
     for _ in range(args.warmup):
@@ -322,8 +335,10 @@ def main():
     for name, recs in per_kernel.items():
         ms = float(np.mean([r["ms"] for r in recs]))
         cs = recs[0]["cells"] * w.S
+        bpc = _bytes_per_cell_state(name)
         kernels.append({"name": name, "pairs": recs[0]["n_pairs"], "cell_states": cs, "ms": ms,
-                        "achieved_GBs": BYTES_PER_CELL_STATE * cs / (ms / 1e3) / 1e9})
+                        "bytes_per_cell_state": bpc,
+                        "achieved_GBs": bpc * cs / (ms / 1e3) / 1e9})
     kernels.sort(key=lambda k: -k["ms"])
     dom = kernels[0]
     _log(f"step {step_ms:.3f} ms; kernels: " +
@@ -344,7 +359,7 @@ def main():
                      "unit": "GB/s", "frac": dom["achieved_GBs"] / HBM_PEAK_GBS,
                      "traffic": TRAFFIC.get(dom["name"]),
                      "kernel": dom["name"], "kernel_ms": dom["ms"],
-                     "bytes_per_cell_state": BYTES_PER_CELL_STATE, "kernels": kernels},
+                     "bytes_per_cell_state": dom["bytes_per_cell_state"], "kernels": kernels},
     }
     if not args.no_plain:
         _log("timing the per-pair drop-in")

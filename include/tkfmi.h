@@ -45,6 +45,8 @@ extern "C" {
 #define TKFMI_GENERIC_KERNEL 8u  /* force the generic (any-layout) kernel  */
 #define TKFMI_WANT_EMIT_COUNTS 16u /* expected emission counts (see below)  */
 #define TKFMI_TIME_KERNELS 32u   /* record per-kernel HIP-event timings    */
+#define TKFMI_CLASS_KERNEL 64u   /* force the one-workgroup-per-pair
+                                    cell-per-lane kernel (see below)        */
 
 /* emission tables: per group g, 21x21 match block then 21 insert then 21
  * delete entries (index 20 = wildcard), all natural-log values. */
@@ -116,9 +118,14 @@ size_t tkfmi_fb2d_workspace_bytes(const tkfmi_problem* prob, uint32_t flags);
  * Requirements: state 0 has type S, some state has type E, at most 15
  * emitting (M/I/D) states, 0 <= Lx, Ly, Lx + Ly <= 32000.
  * The TKF (S,M,I,D,E) layout and the MixDom layout (core/params.py:22,
- * models/mixdom.py:69-77) with K*F <= 3 run on the cell-per-lane kernel;
- * other layouts (or TKFMI_GENERIC_KERNEL) on the generic one.  Both compute
- * the same quantities.
+ * models/mixdom.py:69-77) with K*F <= 3 run on the cell-per-lane kernels:
+ * by default as chains of workgroups per pair (a pair's 64-row bands spread
+ * over several compute units, one forward and one backward launch per
+ * wave-count class, workgroups ticketed in dependency order), or with
+ * TKFMI_CLASS_KERNEL one workgroup per pair.  Other layouts (or
+ * TKFMI_GENERIC_KERNEL) run on the generic kernel.  All compute the same
+ * quantities.  If a chained launch ever times out on an internal wait, the
+ * affected pairs' log_prob is NaN.
  */
 int tkfmi_fb2d_batched(const tkfmi_problem* prob, const tkfmi_outputs* out,
                        void* workspace, size_t workspace_bytes,

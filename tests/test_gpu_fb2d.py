@@ -26,14 +26,15 @@ def _oracle_pair(log_trans, types, emit):
     return dp2d.forward_backward(log_trans, np.asarray(types), emit)
 
 
-def _run(w, forward_only=False, want_post=True, generic=False):
+def _run(w, forward_only=False, want_post=True, kernel=None):
     from tkfmi import hmm
     return hmm.fb2d_batch(w.log_trans, w.state_types, w.xs, w.ys, w.log_tab, w.state_group,
                           forward_only=forward_only, want_posteriors=want_post and not forward_only,
-                          generic=generic)
+                          kernel=kernel)
 
 
-KERNELS = [pytest.param(False, id="cell-per-lane"), pytest.param(True, id="generic")]
+KERNELS = [pytest.param("chain", id="chain"), pytest.param("class", id="class"),
+           pytest.param("generic", id="generic")]
 
 
 def _tkf_workload(lengths_x, lengths_y, seed=0, ext=None):
@@ -57,11 +58,11 @@ EDGE_X = [0, 0, 3, 1, 2, 37, 64, 65, 70, 5, 130]
 EDGE_Y = [0, 4, 0, 1, 2, 64, 37, 3, 1, 140, 96]
 
 
-@pytest.mark.parametrize("generic", KERNELS)
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("ext", [None, 0.6])
-def test_tkf_edge_lengths_match_oracle(ext, generic):
+def test_tkf_edge_lengths_match_oracle(ext, kernel):
     w = _tkf_workload(EDGE_X + [300, 20], EDGE_Y + [520, 700], seed=3, ext=ext)
-    res = _run(w, generic=generic)
+    res = _run(w, kernel=kernel)
     lt = w.log_trans.numpy()
     for b, (x, y) in enumerate(zip(w.xs, w.ys)):
         print(f"pair {b} Lx={len(x)} Ly={len(y)}", flush=True)
@@ -74,11 +75,11 @@ def test_tkf_edge_lengths_match_oracle(ext, generic):
         assert ok, (b, worst)
 
 
-@pytest.mark.parametrize("generic", KERNELS)
-def test_tkf91_forward_only_matches_oracle(generic):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_tkf91_forward_only_matches_oracle(kernel):
     from oracle import dp2d
     w = _tkf_workload([10, 50, 150, 0], [12, 47, 160, 9], seed=5)
-    res = _run(w, forward_only=True, generic=generic)
+    res = _run(w, forward_only=True, kernel=kernel)
     assert "n" not in res
     lt = w.log_trans.numpy()
     for b, (x, y) in enumerate(zip(w.xs, w.ys)):
@@ -94,9 +95,9 @@ def _mixdom_oracle_emit(w, b, x, y):
                                                   p["class_dist"], p["K"], p["F"])
 
 
-@pytest.mark.parametrize("generic", KERNELS)
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("K,F,C", [(3, 1, 1), (3, 1, 2), (1, 3, 2), (2, 1, 3), (1, 2, 1), (1, 1, 2)])
-def test_mixdom2_matches_oracle(K, F, C, generic):
+def test_mixdom2_matches_oracle(K, F, C, kernel):
     from tkfmi import synth
     lengths = [0, 1, 31, 63, 64, 65, 130, 200, 140]
     w = synth.make_mixdom2(seed=11 + K + F + C, B=len(lengths), K=K, F=F, C=C, lengths=lengths)
@@ -106,7 +107,7 @@ def test_mixdom2_matches_oracle(K, F, C, generic):
     rng = np.random.default_rng(K * 100 + F * 10 + C)
     w.ys[7] = rng.integers(0, 20, size=560)
     w.ys[8] = rng.integers(0, 20, size=300)
-    res = _run(w, generic=generic)
+    res = _run(w, kernel=kernel)
     for b, (x, y) in enumerate(zip(w.xs, w.ys)):
         print(f"pair {b} Lx={len(x)} Ly={len(y)}", flush=True)
         lt = w.log_trans[b].numpy()
@@ -119,27 +120,27 @@ def test_mixdom2_matches_oracle(K, F, C, generic):
         assert ok, (b, worst)
 
 
-@pytest.mark.parametrize("generic", KERNELS)
-def test_batched_equals_alone(generic):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_batched_equals_alone(kernel):
     """A pair's results do not depend on the rest of the batch (bitwise)."""
     from tkfmi import hmm, synth
     w = synth.make_mixdom2(seed=4, B=12, lengths=[40, 200, 7, 90, 150, 33, 64, 65, 2, 180, 120, 99])
-    full = _run(w, want_post=False, generic=generic)
+    full = _run(w, want_post=False, kernel=kernel)
     for b in (0, 3, 8):
         alone = hmm.fb2d_batch(w.log_trans[b], w.state_types, [w.xs[b]], [w.ys[b]],
-                               w.log_tab[b], w.state_group, generic=generic)
+                               w.log_tab[b], w.state_group, kernel=kernel)
         assert float(alone["log_p"][0]) == float(full["log_p"][b])
         assert np.array_equal(alone["n"][0].cpu().numpy(), full["n"][b].cpu().numpy())
 
 
-@pytest.mark.parametrize("generic", KERNELS)
-def test_count_identities_full_size(generic):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_count_identities_full_size(kernel):
     """Size-independent identities at the benchmark's length range:
     sum of counts into M/D states == Lx, into M/I states == Ly, start row sums
     to 1, and flow conservation for body states."""
     from tkfmi import synth
     w = synth.make_mixdom2(seed=7, B=6, lengths=[800, 30, 400, 650, 200, 777])
-    res = _run(w, want_post=False, generic=generic)
+    res = _run(w, want_post=False, kernel=kernel)
     types = np.asarray(w.state_types)
     n = res["n"].cpu().numpy()
     for b, (x, y) in enumerate(zip(w.xs, w.ys)):
@@ -156,8 +157,8 @@ def test_count_identities_full_size(generic):
         assert np.isfinite(float(res["log_p"][b]))
 
 
-@pytest.mark.parametrize("generic", KERNELS)
-def test_long_rows_global_row_buffer_matches_golden(generic):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_long_rows_global_row_buffer_matches_golden(kernel):
     """Descendants too long for the LDS row buffer (1200 and 2000 residues)
     use the global one, in one launch with a short pair that uses LDS.
     Expected values: tests/golden/long_pairs.npz, the oracle in 80-bit long
@@ -169,7 +170,7 @@ def test_long_rows_global_row_buffer_matches_golden(generic):
     xs = [g[f"x{b}"] for b in range(B)]
     ys = [g[f"y{b}"] for b in range(B)]
     res = hmm.fb2d_batch(g["log_trans"], g["state_types"], xs, ys, g["log_tab"],
-                         list(g["state_group"]), want_posteriors=True, generic=generic)
+                         list(g["state_group"]), want_posteriors=True, kernel=kernel)
     stride = int(g["post_stride"])
     for b in range(B):
         print(f"pair {b} Lx={len(xs[b])} Ly={len(ys[b])}", flush=True)
@@ -183,18 +184,21 @@ def test_long_rows_global_row_buffer_matches_golden(generic):
         assert ok, (b, worst)
 
 
-def test_cell_per_lane_matches_generic_kernel_long_rows():
-    """Pairs whose wrap-around ring does not fit in LDS (8 waves, Ly >= 1500)
-    use a global-memory ring; compare both kernels and the exact count
-    identities (sum into M/D == Lx, into M/I == Ly)."""
+@pytest.mark.parametrize("kernel", ["chain", "class"])
+def test_cell_per_lane_matches_generic_kernel_long_rows(kernel):
+    """Long rows: in the one-workgroup kernel the wrap-around ring does not
+    fit in LDS (8 waves, Ly >= 1500) and goes through global memory; in the
+    chained kernel pairs of 10-12 bands span 3 workgroups.  Compare with the
+    generic kernel and the exact count identities (sum into M/D == Lx, into
+    M/I == Ly)."""
     from tkfmi import synth
     w = synth.make_mixdom2(seed=33, B=4, lengths=[600, 700, 130, 520])
     rng = np.random.default_rng(5)
     w.ys[0] = rng.integers(0, 20, size=2000)
     w.ys[1] = rng.integers(0, 20, size=1500)
     w.ys[2] = rng.integers(0, 20, size=900)
-    a = _run(w, want_post=True)
-    b = _run(w, want_post=True, generic=True)
+    a = _run(w, want_post=True, kernel=kernel)
+    b = _run(w, want_post=True, kernel="generic")
     types = np.asarray(w.state_types)
     for p in range(w.B):
         print(f"pair {p} Lx={len(w.xs[p])} Ly={len(w.ys[p])}", flush=True)
@@ -209,21 +213,22 @@ def test_cell_per_lane_matches_generic_kernel_long_rows():
         assert abs(na[:, np.isin(types, (1, 2))].sum() - len(w.ys[p])) < 1e-9 * len(w.ys[p])
 
 
-def test_c1_subset_both_kernels_agree():
+@pytest.mark.parametrize("kernel", ["chain", "class"])
+def test_c1_subset_both_kernels_agree(kernel):
     """The benchmark workload's first 96 pairs: cell-per-lane == generic."""
     from tkfmi import synth
     w = synth.make_mixdom2(seed=1, B=96)
-    a = _run(w, want_post=False)
-    b = _run(w, want_post=False, generic=True)
+    a = _run(w, want_post=False, kernel=kernel)
+    b = _run(w, want_post=False, kernel="generic")
     la, lb = a["log_p"].cpu().numpy(), b["log_p"].cpu().numpy()
     assert np.all(np.abs(la - lb) <= 1e-12 * np.abs(lb))
     ok, worst = _close(a["n"].cpu().numpy(), b["n"].cpu().numpy())
     assert ok, worst
 
 
-@pytest.mark.parametrize("generic", KERNELS)
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("model", ["tkf92", "mixdom2_k3c2", "mixdom2_k1f3"])
-def test_emit_counts_match_oracle(model, generic):
+def test_emit_counts_match_oracle(model, kernel):
     """Expected emission counts (d logP / d log_tab) vs the oracle's
     posterior reduction, over edge and long lengths."""
     from oracle import emissions
@@ -237,7 +242,7 @@ def test_emit_counts_match_oracle(model, generic):
         w = synth.make_mixdom2(seed=21, B=7, K=K, F=F, C=C, lengths=[0, 1, 40, 64, 65, 200, 330])
         emit = lambda b, x, y: _mixdom_oracle_emit(w, b, x, y)
         lt_of = lambda b: w.log_trans[b].numpy()
-    res = _run_ec(w, generic)
+    res = _run_ec(w, kernel)
     G = res["emit_counts"].shape[1]
     ec = res["emit_counts"].cpu().numpy()
     for b, (x, y) in enumerate(zip(w.xs, w.ys)):
@@ -249,19 +254,19 @@ def test_emit_counts_match_oracle(model, generic):
         assert abs(float(res["log_p"][b]) - lp_o) <= RTOL * abs(lp_o)
 
 
-def _run_ec(w, generic):
+def _run_ec(w, kernel):
     from tkfmi import hmm
     return hmm.fb2d_batch(w.log_trans, w.state_types, w.xs, w.ys, w.log_tab, w.state_group,
-                          want_emit_counts=True, generic=generic)
+                          want_emit_counts=True, kernel=kernel)
 
 
-@pytest.mark.parametrize("generic", KERNELS)
-def test_emit_counts_full_size_identities(generic):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_emit_counts_full_size_identities(kernel):
     """At the benchmark's lengths: emission totals per type equal the
     expected visits in n, and every residue of x (of y) is emitted once."""
     from tkfmi import synth
     w = synth.make_mixdom2(seed=8, B=5, lengths=[800, 650, 30, 420, 777])
-    res = _run_ec(w, generic)
+    res = _run_ec(w, kernel)
     types = np.asarray(w.state_types)
     n = res["n"].cpu().numpy()
     ec = res["emit_counts"].cpu().numpy()

@@ -50,7 +50,7 @@ def _oracle_sum_logp(p, ts, xs, ys):
     return tot
 
 
-def _gpu_sum_logp(p, ts, xs, ys, grad=False, generic=False):
+def _gpu_sum_logp(p, ts, xs, ys, grad=False, kernel=None):
     from tkfmi import vjp
     dev = torch.device("cuda", 0)
     tp = {k: torch.tensor(v, dtype=torch.float64, device=dev, requires_grad=grad)
@@ -58,7 +58,7 @@ def _gpu_sum_logp(p, ts, xs, ys, grad=False, generic=False):
     tt = torch.tensor(ts, dtype=torch.float64, device=dev, requires_grad=grad)
     lp = vjp.mixdom2_log_prob_batch(tp["lam"], tp["mu"], tt, tp["dlam"], tp["dmu"], tp["dw"],
                                     tp["frag_w"], tp["ext"], tp["pis"], tp["S"], tp["cd"], xs, ys,
-                                    generic=generic)
+                                    kernel=kernel)
     tot = lp.sum()
     if not grad:
         return float(tot)
@@ -74,15 +74,15 @@ def _shift(p, key, idx, d):
     return q
 
 
-@pytest.mark.parametrize("generic", [False, True], ids=["cell-per-lane", "generic"])
-def test_mixdom2_gradients_match_oracle_fd(generic):
+@pytest.mark.parametrize("kernel", ["chain", "class", "generic"])
+def test_mixdom2_gradients_match_oracle_fd(kernel):
     rng = np.random.default_rng(41)
     p = _mix2_params(rng, 3, 1, 2)
     lens = [(0, 3), (7, 0), (25, 31), (40, 38), (64, 70)]
     xs = [rng.integers(0, A, a) for a, _ in lens]
     ys = [rng.integers(0, A, b) for _, b in lens]
     ts = rng.uniform(0.1, 1.5, len(lens))
-    tot, g = _gpu_sum_logp(p, ts, xs, ys, grad=True, generic=generic)
+    tot, g = _gpu_sum_logp(p, ts, xs, ys, grad=True, kernel=kernel)
     assert tot == pytest.approx(_oracle_sum_logp(p, ts, xs, ys), rel=1e-10)
     checks = [("lam", ()), ("mu", ()), ("dlam", (2,)), ("dmu", (0,)), ("dw", (1,)),
               ("frag_w", (0, 0)), ("ext", (1, 0)), ("pis", (0, 5)), ("pis", (1, int(xs[3][0]))),

@@ -675,6 +675,8 @@ const char* tkfmi_last_error(void) { return g_err.c_str(); }
 
 size_t tkfmi_fb2d_workspace_bytes(const tkfmi_problem* prob, uint32_t flags) {
   size_t v2b = 0;
+  if (!(flags & (TKFMI_GENERIC_KERNEL | TKFMI_CLASS_KERNEL)) && tkfmi_chain_workspace(prob, flags, &v2b))
+    return v2b > 0 ? v2b : 1;
   if (!(flags & TKFMI_GENERIC_KERNEL) && tkfmi_v2_workspace(prob, flags, &v2b))
     return v2b > 0 ? v2b : 1;
   if (prob && tkfmi_wide_applies(prob)) return tkfmi_wide_workspace(prob, flags);
@@ -711,6 +713,11 @@ int tkfmi_fb2d_batched(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws,
   if (!p->log_trans || !p->log_tab || !p->x_cat || !p->y_cat)
     return fail(TKFMI_EINVAL, "device input pointer missing");
   size_t v2b = 0;
+  if (!(flags & (TKFMI_GENERIC_KERNEL | TKFMI_CLASS_KERNEL)) && tkfmi_chain_workspace(p, flags, &v2b)) {
+    std::string err;
+    const int rc2 = tkfmi_chain_run(p, o, ws, ws_bytes, reinterpret_cast<hipStream_t>(stream_v), flags, err);
+    return rc2 == TKFMI_OK ? TKFMI_OK : fail(rc2, err);
+  }
   if (!(flags & TKFMI_GENERIC_KERNEL) && tkfmi_v2_workspace(p, flags, &v2b)) {
     std::string err;
     const int rc2 = tkfmi_v2_run(p, o, ws, ws_bytes, reinterpret_cast<hipStream_t>(stream_v), flags, err);

@@ -1,0 +1,905 @@
+// tkfmi: chained cell-per-lane 2D pair-HMM forward/backward + expected
+// counts for gfx950 (MI355X), TKF91/TKF92 (S,M,I,D,E) and MixDom
+// (2 + 5KF states, K*F <= 3).  Reference algorithm:
+// python/tkfmixdom/jax/dp/hmm.py (forward hmm.py:1186-1292, backward
+// hmm.py:1295-1437, posteriors and counts hmm.py:1930-1971).
+//
+// The per-cell arithmetic is the one of the single-workgroup kernel in
+// fb2d_v2.hip (one lattice cell per lane, probability space with a binary
+// exponent per cell, DPP row shifts, P as an SGPR operand, MFMA counts).
+// What differs is how a pair is spread over the machine:
+//  * A pair's bands of 64 rows are cut into a chain of G workgroups of at
+//    most W waves, one band per wave, so a long pair runs on G compute units
+//    at one wave per SIMD instead of on one compute unit at two.
+//  * Inside a workgroup, waves run 64 steps apart in lock-step and pass the
+//    boundary row through LDS (as in the single-workgroup kernel).  Between
+//    workgroups of a chain the boundary row goes through global memory:
+//    the producer publishes every CQ columns with a release store of a
+//    progress counter, the consumer acquires it and copies the next CQ
+//    entries into LDS.
+//  * Workgroups take tickets from a counter in launch order, and a pair's
+//    chain is ticketed in dependency order (top band first in the forward,
+//    bottom band first in the backward).  A workgroup therefore waits only
+//    on workgroups that already hold a ticket, i.e. are running: no
+//    deadlock whatever the occupancy.  Every wait is bounded; on a time-out
+//    the launch raises an abort flag, all waits end, and the affected pairs
+//    get a NaN log-likelihood.
+//  * The forward and the backward are separate launches on the class's
+//    stream; log P and its binary exponent pass through a per-pair slot.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fb2d_cell.h"
+#include "fb2d_internal.h"
+#include "tkfmi.h"
+
+namespace tkfmi_v2 {
+
+constexpr int CQ = 16;                 // boundary entries per published / fetched chunk
+constexpr int kSpinMax = 1 << 22;      // bounded waits (each spin sleeps ~0.2 us + a load)
+
+struct CDesc {
+  int32_t pair, Lx, Ly, band0;  // pair geometry; first band of this workgroup
+  int32_t nw, g, G, T;          // waves holding a band (<= W), chain index, chain length, steps
+  int64_t x_off, y_off;
+  int64_t lat_off, post_off, pm_off;
+  int64_t xin_off, xout_off;    // doubles into the boundary region (-1: none)
+  int32_t fin, fout;            // progress counters (ints into the flag region; -1: none)
+};
+
+struct CParams {
+  int32_t S, flags, n_tab, G_tab;
+  int32_t gmap[16];  // layout group -> caller table group
+  const double* log_tab;
+  const uint8_t* x_cat;
+  const uint8_t* y_cat;
+  double* lattice;
+  double* xring;      // boundary rows between the workgroups of a chain
+  int32_t* flagv;     // progress counters
+  int32_t* ticket;    // this launch's ticket counter
+  int32_t* abortf;    // raised by a timed-out wait
+  double* pinfo;      // [B][2]: mantissa and binary exponent of P(x, y)
+  double* out_logP;
+  double* out_n;
+  double* out_post;
+  double* out_ecnt;
+};
+
+__device__ __forceinline__ int ld_acquire(int* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits (whole wave, uniform) until *flag >= need.  Returns false when the
+// launch was aborted, by this wait timing out or by another one.
+__device__ bool wait_ge(int* flag, int need, int* abortf) {
+  int v = __builtin_amdgcn_readfirstlane(ld_acquire(flag));
+  int spins = 0;
+  while (v < need) {
+    if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+      return false;
+    if (++spins > kSpinMax) {
+      __hip_atomic_store(abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(8);
+    v = __builtin_amdgcn_readfirstlane(ld_acquire(flag));
+  }
+  return true;
+}
+
+// Publishes `count` boundary entries (this lane's stores before it) to the
+// next workgroup of the chain.
+__device__ __forceinline__ void publish(int* flag, int count) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __hip_atomic_store(flag, count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Copies boundary entries [e, e_end) into the LDS chunk buffer (whole wave).
+__device__ __forceinline__ void fetch_chunk(double* s_xc, const double* src, int n) {
+  for (int q = threadIdx.x & 63; q < n; q += 64) s_xc[q] = src[q];
+  wave_lds_sync();
+}
+
+__device__ __forceinline__ int take_ticket(int32_t* counter) {
+  __shared__ int s_ticket;
+  if (threadIdx.x == 0) s_ticket = atomicAdd(counter, 1);
+  __syncthreads();
+  return __builtin_amdgcn_readfirstlane(s_ticket);
+}
+
+// LDS layout (doubles): tables | emission counts (backward) | short rings |
+// chunk buffer | MFMA staging (backward) | bytes: x rows of the workgroup, y.
+template <class L, int W>
+struct CLds {
+  static constexpr int NX = count_type<L, ST_M>() + count_type<L, ST_D>() + 1;
+  static constexpr int tab = 0;
+  static constexpr int misc = L::NG * TAB_SZ;  // p_start[NB], p_end[NB], P[S->E]
+  static constexpr int ecnt = misc + 32;
+  static constexpr int ring(bool bwd) { return ecnt + (bwd ? L::NG * TAB_SZ : 0); }
+  static constexpr int xc(bool bwd) { return ring(bwd) + (W - 1) * 2 * NX; }
+  static constexpr int stage(bool bwd) { return ((xc(bwd) + CQ * NX) + 1) & ~1; }
+  static constexpr int seq_bytes(bool bwd) { return 8 * (stage(bwd) + (bwd ? W * 1024 : 0)); }
+  static constexpr int xbytes = 64 * W + 8;
+  static size_t total(bool bwd, int max_ly) { return (size_t)seq_bytes(bwd) + xbytes + ((max_ly + 8) & ~7) + 16; }
+};
+
+// Stages the exponentiated emission tables and the workgroup's x rows / y.
+// The start / end transitions go to LDS too: a global load inside the sweep
+// would make the compiler wait for every outstanding lattice store (stores
+// and loads share one counter) at the next use of its register.
+template <class L, int W>
+__device__ __forceinline__ void stage_inputs(const CParams& Pr, const CDesc& pd, const double* pm,
+                                             double* s_tab, uint8_t* s_x, uint8_t* s_y) {
+  constexpr int NG = L::NG, NB = L::NB;
+  static_assert(2 * NB + 1 <= 32, "start / end staging");
+  double* s_misc = s_tab + NG * TAB_SZ;
+  for (int q = threadIdx.x; q < 2 * NB + 1; q += W * 64) s_misc[q] = pm[2 * NB * NB + q];
+  const double* ltab = Pr.log_tab + (Pr.n_tab > 1 ? (size_t)pd.pair * Pr.G_tab * TAB_SZ : 0);
+  for (int q = threadIdx.x; q < NG * TAB_SZ; q += W * 64) {
+    const int g = q / TAB_SZ, o = q - g * TAB_SZ;
+    s_tab[q] = exp(ltab[Pr.gmap[g] * TAB_SZ + o]);
+  }
+  // s_x[q] = x[64 band0 + q - 1] (the character of row 64 band0 + q)
+  const int xb = 64 * pd.band0 - 1;
+  for (int q = threadIdx.x; q < 64 * W + 1; q += W * 64) {
+    const int xi = xb + q;
+    s_x[q] = (xi >= 0 && xi < pd.Lx) ? (uint8_t)min((int)Pr.x_cat[pd.x_off + xi], A_PAD - 1) : 0;
+  }
+  for (int q = threadIdx.x; q < pd.Ly; q += W * 64) s_y[q] = min((int)Pr.y_cat[pd.y_off + q], A_PAD - 1);
+  if (threadIdx.x == 0) s_y[pd.Ly] = 0;
+}
+
+// ============================================================== forward
+template <class L, int W>
+__global__ __launch_bounds__(W * 64) void fbc_fwd_kernel(CParams Pr, const double* __restrict__ pmat,
+                                                         const CDesc* __restrict__ descs) {
+  constexpr int NB = L::NB, NG = L::NG;
+  constexpr int nM = count_type<L, ST_M>();
+  constexpr int nD = count_type<L, ST_D>();
+  using LY = CLds<L, W>;
+  constexpr int NX = LY::NX;
+  constexpr int EIDX = L::EIDX;
+  static_assert(NB <= SLOTS - 1, "exponent slot");
+  extern __shared__ double smem[];
+
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const CDesc pd = descs[take_ticket(Pr.ticket)];
+  const int Lx = pd.Lx, Ly = pd.Ly, NW = pd.nw;
+  const int S = Pr.S;
+  const int BSTRIDE = Ly + 64;
+
+  double* s_tab = smem + LY::tab;
+  double* s_ring = smem + LY::ring(false);
+  double* s_xc = smem + LY::xc(false);
+  uint8_t* s_x = reinterpret_cast<uint8_t*>(smem) + LY::seq_bytes(false);
+  uint8_t* s_y = s_x + LY::xbytes;
+  stage_inputs<L, W>(Pr, pd, pmat + pd.pm_off, s_tab, s_x, s_y);
+
+  const double* __restrict__ pm = pmat + pd.pm_off;
+  const double* p_start = smem + LY::misc;
+  const double* p_end = p_start + NB;
+  double prow[NB];  // prow[s] in lane l: P[s][l % 16] (0 past NB)
+#pragma unroll
+  for (int s = 0; s < NB; ++s) prow[s] = valu_copy((lane & 15) < NB ? pm[s * NB + (lane & 15)] : 0.0);
+
+  const bool want_lat = !(Pr.flags & TKFMI_FORWARD_ONLY);
+  const bool want_counts = want_lat && !(Pr.flags & TKFMI_NO_COUNTS) && Pr.out_n != nullptr;
+  double* out_n = want_counts ? Pr.out_n + (size_t)pd.pair * S * S : nullptr;
+  double* lat = Pr.lattice + pd.lat_off;
+  const bool has_band = w < NW;
+  const int band = pd.band0 + w;
+  const int i_band = 64 * band + lane;
+  const bool xin = (w == 0) && pd.fin >= 0;
+  const bool xout = (w == NW - 1) && pd.fout >= 0;
+  bool ok_run = true;
+  __syncthreads();
+
+  double hp[NB];
+  double rM_last[nM];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) hp[b] = 0.0;
+#pragma unroll
+  for (int m = 0; m < nM; ++m) rM_last[m] = 0.0;
+  int eH = EXP_NONE, eM_last = EXP_NONE;
+  const int T = pd.T;
+  for (int t = 0; t < T; ++t) {
+    if (has_band) {
+      int i = i_band;
+      asm volatile("" : "+v"(i));  // keeps row predicates inside the loop (no SGPR masks)
+      const int j = t - 64 * w - lane;
+      const bool valid = j >= 0 && j <= Ly && i <= Lx;
+      // ---- boundary row of the previous workgroup: fetch the next chunk
+      if (xin && t <= Ly && (t % CQ) == 0) {
+        const int e_end = min(t + CQ, Ly + 1);
+        if (ok_run) ok_run = wait_ge(Pr.flagv + pd.fin, e_end, Pr.abortf);
+        fetch_chunk(s_xc, Pr.xring + pd.xin_off + (size_t)t * NX, (e_end - t) * NX);
+      }
+      // ---- receive the predecessor row's h (lane-1, or a ring for lane 0)
+      double rD[nD > 0 ? nD : 1], rM[nM > 0 ? nM : 1];
+      int eR;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        if (L::type(b) == ST_D) rD[rank_in_type<L, ST_D>(b)] = dpp_f64<DPP_WAVE_SHR1>(hp[b]);
+        if (L::type(b) == ST_M) rM[rank_in_type<L, ST_M>(b)] = dpp_f64<DPP_WAVE_SHR1>(hp[b]);
+      }
+      eR = dpp_i32<DPP_WAVE_SHR1>(eH);
+      if (lane == 0 && valid && i >= 1) {
+        const double* e = (w >= 1) ? s_ring + ((size_t)(w - 1) * 2 + (j & 1)) * NX
+                                   : s_xc + (size_t)(j % CQ) * NX;
+#pragma unroll
+        for (int m = 0; m < nM; ++m) rM[m] = e[m];
+#pragma unroll
+        for (int d = 0; d < nD; ++d) rD[d] = e[nM + d];
+        eR = (int)e[NX - 1];
+      }
+      // ---- emissions at (i, j)
+      const int xi = valid ? s_x[i - 64 * pd.band0] : 0;
+      const int yj = valid ? s_y[j >= 1 ? j - 1 : 0] : 0;
+      double emM[NG], emI[NG], emD[NG];
+#pragma unroll
+      for (int g = 0; g < NG; ++g) {
+        emM[g] = s_tab[g * TAB_SZ + xi * A_PAD + yj];
+        emI[g] = s_tab[g * TAB_SZ + A_PAD * A_PAD + yj];
+        emD[g] = s_tab[g * TAB_SZ + A_PAD * A_PAD + A_PAD + xi];
+      }
+      const bool okM = valid && i >= 1 && j >= 1;
+      const bool okI = valid && j >= 1;
+      const bool okD = valid && i >= 1;
+      // ---- new cell: raw products and per-type maxima
+      double f[NB];
+      double mxM = 0.0, mxI = 0.0, mxD = 0.0;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int ty = L::type(b), g = L::group(b);
+        double v;
+        if (ty == ST_M) { v = okM ? rM_last[rank_in_type<L, ST_M>(b)] * emM[g] : 0.0; mxM = fmax(mxM, v); }
+        else if (ty == ST_I) { v = okI ? hp[b] * emI[g] : 0.0; mxI = fmax(mxI, v); }
+        else { v = okD ? rD[rank_in_type<L, ST_D>(b)] * emD[g] : 0.0; mxD = fmax(mxD, v); }
+        f[b] = v;
+      }
+      const int cM = (mxM > 0.0) ? eM_last + frexp_exp(mxM) : EXP_NONE;
+      const int cI = (mxI > 0.0) ? eH + frexp_exp(mxI) : EXP_NONE;
+      const int cD = (mxD > 0.0) ? eR + frexp_exp(mxD) : EXP_NONE;
+      int E = max(cM, max(cI, cD));
+      if (E == EXP_NONE) E = 0;
+      const int shM = eM_last - E, shI = eH - E, shD = eR - E;
+      double F[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int ty = L::type(b);
+        F[b] = ldexp(f[b], ty == ST_M ? shM : (ty == ST_I ? shI : shD));
+      }
+      const bool origin = valid && i == 0 && j == 0;
+      if (origin) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) F[b] = 0.0;
+        E = 0;
+      }
+      // ---- store the forward lattice (band-local step u = j + lane)
+      if (want_lat && valid) {
+        double* dst = lat + (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + lane;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) dst[b * 64] = F[b];
+        dst[(SLOTS - 1) * 64] = (double)E;
+      }
+      // ---- push: h[k] = sum_s F[s] P[s][k]
+      double h[NB];
+#pragma unroll
+      for (int k = 0; k < NB; ++k) h[k] = 0.0;
+      row_contract<NB>(h, prow, F);
+      if (origin) {
+#pragma unroll
+        for (int k = 0; k < NB; ++k) h[k] = p_start[k];
+      }
+      // ---- corner: log P, its mantissa / exponent for the backward, the E column
+      if (valid && i == Lx && j == Ly) {
+        double tot = 0.0;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) tot = fma(F[b], p_end[b], tot);
+        double lp;
+        if (origin) { tot = p_end[NB]; lp = log(tot); }
+        else lp = log(tot) + (double)E * LN2;
+        const int ep = (tot > 0.0) ? E + frexp_exp(tot) : 0;
+        const double mp = (tot > 0.0) ? ldexp(tot, -frexp_exp(tot)) : 0.0;
+        Pr.out_logP[pd.pair] = (tot > 0.0) ? lp : -1e30;
+        Pr.pinfo[2 * pd.pair] = mp;
+        Pr.pinfo[2 * pd.pair + 1] = (double)ep;
+        if (want_counts && tot > 0.0) {
+          if (origin) out_n[EIDX] = 1.0;
+          else {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) out_n[(b + L::SOFF) * S + EIDX] = F[b] * p_end[b] / tot;
+          }
+        }
+      }
+      // ---- publish this row's h for the band below (lane 63)
+      if (lane == 63 && valid) {
+        double* e = xout ? Pr.xring + pd.xout_off + (size_t)j * NX
+                         : s_ring + ((size_t)w * 2 + (j & 1)) * NX;
+        if (xout || w < NW - 1) {
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            if (L::type(b) == ST_M) e[rank_in_type<L, ST_M>(b)] = h[b];
+            if (L::type(b) == ST_D) e[nM + rank_in_type<L, ST_D>(b)] = h[b];
+          }
+          e[NX - 1] = (double)E;
+          if (xout && (((j + 1) % CQ) == 0 || j == Ly)) publish(Pr.flagv + pd.fout, j + 1);
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < NB; ++b) hp[b] = h[b];
+#pragma unroll
+      for (int m = 0; m < nM; ++m) rM_last[m] = rM[m];
+      eH = E;
+      eM_last = eR;
+    }
+    lds_barrier();
+  }
+  if (!ok_run) {
+    __hip_atomic_store(Pr.abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) Pr.out_logP[pd.pair] = __builtin_nan("");
+  }
+}
+
+// ============================================================= backward
+template <class L, int W>
+__global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const double* __restrict__ pmat,
+                                                         const CDesc* __restrict__ descs) {
+  constexpr int NB = L::NB, NG = L::NG;
+  constexpr int nM = count_type<L, ST_M>();
+  constexpr int nD = count_type<L, ST_D>();
+  using LY = CLds<L, W>;
+  constexpr int NX = LY::NX;
+  extern __shared__ double smem[];
+
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const CDesc pd = descs[take_ticket(Pr.ticket)];
+  const int Lx = pd.Lx, Ly = pd.Ly, NW = pd.nw;
+  const int S = Pr.S;
+  const int W1 = Ly + 1;
+  const int BSTRIDE = Ly + 64;
+
+  double* s_tab = smem + LY::tab;
+  double* s_ecnt = smem + LY::ecnt;
+  double* s_ring = smem + LY::ring(true);
+  double* s_xc = smem + LY::xc(true);
+  double* s_stage = smem + LY::stage(true);
+  uint8_t* s_x = reinterpret_cast<uint8_t*>(smem) + LY::seq_bytes(true);
+  uint8_t* s_y = s_x + LY::xbytes;
+  stage_inputs<L, W>(Pr, pd, pmat + pd.pm_off, s_tab, s_x, s_y);
+  const bool want_ecnt = (Pr.flags & TKFMI_WANT_EMIT_COUNTS) && Pr.out_ecnt != nullptr;
+  if (want_ecnt)
+    for (int q = threadIdx.x; q < NG * TAB_SZ; q += W * 64) s_ecnt[q] = 0.0;
+
+  const double* __restrict__ pm = pmat + pd.pm_off;
+  const double* __restrict__ Pf = pm;
+  const double* p_start = smem + LY::misc;
+  const double* p_end = p_start + NB;
+  double prowT[NB];  // prowT[k'] in lane l: P[l % 16][k'] (0 past NB)
+#pragma unroll
+  for (int k2 = 0; k2 < NB; ++k2)
+    prowT[k2] = valu_copy((lane & 15) < NB ? pm[NB * NB + k2 * NB + (lane & 15)] : 0.0);
+
+  const bool want_counts = !(Pr.flags & TKFMI_NO_COUNTS) && Pr.out_n != nullptr;
+  const bool want_post = (Pr.flags & TKFMI_WANT_POSTERIORS) && Pr.out_post != nullptr;
+  double* out_n = want_counts ? Pr.out_n + (size_t)pd.pair * S * S : nullptr;
+  const double* lat = Pr.lattice + pd.lat_off;
+  const double mP = Pr.pinfo[2 * pd.pair];
+  const int EP = (int)Pr.pinfo[2 * pd.pair + 1];
+  if (!(mP > 0.0)) return;  // impossible pair (uniform over the workgroup and its chain)
+  const double inv_mP = 1.0 / mP;
+  const bool has_band = w < NW;
+  const int band = pd.band0 + NW - 1 - w;  // wave 0 holds the bottom band
+  const int i_band = 64 * band + lane;
+  const bool xin = (w == 0) && pd.fin >= 0;
+  const bool xout = (w == NW - 1) && pd.fout >= 0;
+  bool ok_run = true;
+  d4 cacc = {0.0, 0.0, 0.0, 0.0}, cacc2 = {0.0, 0.0, 0.0, 0.0};
+  double* stage_f = s_stage + (size_t)w * 1024;  // [32 cells][16] F-scaled
+  double* stage_g = stage_f + 512;               // [32 cells][16] g
+  double* post = want_post ? Pr.out_post + pd.post_off : nullptr;
+  __syncthreads();
+
+  double qp[NB];
+  double rM_last[nM];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) qp[b] = 0.0;
+#pragma unroll
+  for (int m = 0; m < nM; ++m) rM_last[m] = 0.0;
+  int eQ = EXP_NONE, eM_last = EXP_NONE;
+  const int off = 64 * w + (63 - lane);  // position p = t - off, column j = Ly - p
+  auto f_addr = [&](int p, int i) -> const double* {
+    const bool v = has_band && p >= 0 && p <= Ly && i <= Lx;
+    return v ? lat + (((size_t)band * BSTRIDE + (Ly - p + lane)) * SLOTS) * 64 + lane : nullptr;
+  };
+  double fnext[SLOTS];
+  {
+    const double* a = f_addr(-off, i_band);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) fnext[b] = a ? a[b * 64] : 0.0;
+    fnext[SLOTS - 1] = a ? a[(SLOTS - 1) * 64] : 0.0;
+  }
+  const int T = pd.T;
+  for (int t = 0; t < T; ++t) {
+    if (has_band) {
+      int i = i_band;
+      asm volatile("" : "+v"(i));  // keeps row predicates inside the loop (no SGPR masks)
+      const int p = t - off;
+      const int j = Ly - p;
+      const bool valid = p >= 0 && p <= Ly && i <= Lx;
+      double fcur[SLOTS];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) fcur[b] = fnext[b];
+      fcur[SLOTS - 1] = fnext[SLOTS - 1];
+      {
+        const double* a = f_addr(p + 1, i);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) fnext[b] = a ? a[b * 64] : 0.0;
+        fnext[SLOTS - 1] = a ? a[(SLOTS - 1) * 64] : 0.0;
+      }
+      // ---- boundary row of the next workgroup (below): fetch the next chunk
+      if (xin && t <= Ly && (t % CQ) == 0) {
+        const int e_end = min(t + CQ, Ly + 1);
+        if (ok_run) ok_run = wait_ge(Pr.flagv + pd.fin, e_end, Pr.abortf);
+        fetch_chunk(s_xc, Pr.xring + pd.xin_off + (size_t)t * NX, (e_end - t) * NX);
+      }
+      // ---- receive the successor row's q (lane+1, or a ring for lane 63)
+      double rD[nD > 0 ? nD : 1], rM[nM > 0 ? nM : 1];
+      int eR;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        if (L::type(b) == ST_D) rD[rank_in_type<L, ST_D>(b)] = dpp_f64<DPP_WAVE_SHL1>(qp[b]);
+        if (L::type(b) == ST_M) rM[rank_in_type<L, ST_M>(b)] = dpp_f64<DPP_WAVE_SHL1>(qp[b]);
+      }
+      eR = dpp_i32<DPP_WAVE_SHL1>(eQ);
+      if (lane == 63 && valid && i < Lx) {
+        const double* e = (w >= 1) ? s_ring + ((size_t)(w - 1) * 2 + (p & 1)) * NX
+                                   : s_xc + (size_t)(p % CQ) * NX;
+#pragma unroll
+        for (int m = 0; m < nM; ++m) rM[m] = e[m];
+#pragma unroll
+        for (int d = 0; d < nD; ++d) rD[d] = e[nM + d];
+        eR = (int)e[NX - 1];
+      }
+      const bool okM = valid && i < Lx && j < Ly;
+      const bool okI = valid && j < Ly;
+      const bool okD = valid && i < Lx;
+      const int cM = okM ? eM_last : EXP_NONE;
+      const int cI = okI ? eQ : EXP_NONE;
+      const int cD = okD ? eR : EXP_NONE;
+      int Eg = max(cM, max(cI, cD));
+      if (Eg == EXP_NONE) Eg = 0;
+      const int shM = eM_last - Eg, shI = eQ - Eg, shD = eR - Eg;
+      double g[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int ty = L::type(b);
+        double v;
+        if (ty == ST_M) v = okM ? ldexp(rM_last[rank_in_type<L, ST_M>(b)], shM) : 0.0;
+        else if (ty == ST_I) v = okI ? ldexp(qp[b], shI) : 0.0;
+        else v = okD ? ldexp(rD[rank_in_type<L, ST_D>(b)], shD) : 0.0;
+        g[b] = v;
+      }
+      // ---- B[k] = sum_k' P[k][k'] g[k']
+      double Bv[NB];
+#pragma unroll
+      for (int k = 0; k < NB; ++k) Bv[k] = 0.0;
+      row_contract<NB>(Bv, prowT, g);
+      const bool corner = valid && i == Lx && j == Ly;
+      int EBraw = Eg;
+      if (corner) {
+#pragma unroll
+        for (int k = 0; k < NB; ++k) Bv[k] = p_end[k];
+        EBraw = 0;
+      }
+      double mx = 0.0;
+#pragma unroll
+      for (int k = 0; k < NB; ++k) mx = fmax(mx, Bv[k]);
+      const int sh = (mx > 0.0) ? frexp_exp(mx) : 0;
+      const int EB = EBraw + sh;
+#pragma unroll
+      for (int k = 0; k < NB; ++k) Bv[k] = ldexp(Bv[k], -sh);
+      // ---- counts: stage F-scaled and g, then MFMA (K = 4 cells per op)
+      const int EF = (int)fcur[SLOTS - 1];
+      const double fscale = valid ? ldexp(inv_mP, EF + Eg - EP) : 0.0;
+      if (want_counts) {
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+          if ((lane >> 5) == half) {
+            const int c = lane & 31;
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+              const int o = c * 16 + (s ^ (c & 15));
+              stage_f[o] = (s < NB) ? fcur[s < NB ? s : 0] * fscale : 0.0;
+              stage_g[o] = (s < NB) ? g[s < NB ? s : 0] : 0.0;
+            }
+          }
+          wave_lds_sync();
+#pragma unroll
+          for (int q = 0; q < 8; q += 2) {  // two accumulator chains
+            const int c0 = 4 * q + (lane >> 4), c1 = c0 + 4;
+            const int o0 = c0 * 16 + ((lane & 15) ^ (c0 & 15));
+            const int o1 = c1 * 16 + ((lane & 15) ^ (c1 & 15));
+            cacc = __builtin_amdgcn_mfma_f64_16x16x4f64(stage_f[o0], stage_g[o0], cacc, 0, 0, 0);
+            cacc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(stage_f[o1], stage_g[o1], cacc2, 0, 0, 0);
+          }
+          wave_lds_sync();
+        }
+        if (valid && i == 0 && j == 0) {  // start state row: n[S, k]
+          const double sc = ldexp(inv_mP, Eg - EP);
+#pragma unroll
+          for (int b = 0; b < NB; ++b) out_n[b + L::SOFF] = p_start[b] * g[b] * sc;
+        }
+      }
+      const int xi = valid ? s_x[i - 64 * pd.band0] : 0;
+      const int yj = valid ? s_y[j >= 1 ? j - 1 : 0] : 0;
+      // ---- expected emission counts: posterior mass per (group, type)
+      if (want_ecnt && valid) {
+        const double pscale = ldexp(inv_mP, EF + EB - EP);
+        double sM[NG], sI[NG], sD[NG];
+#pragma unroll
+        for (int gq = 0; gq < NG; ++gq) { sM[gq] = 0.0; sI[gq] = 0.0; sD[gq] = 0.0; }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const int ty = L::type(b), gq = L::group(b);
+          const double pv = fcur[b] * Bv[b];
+          if (ty == ST_M) sM[gq] += pv;
+          else if (ty == ST_I) sI[gq] += pv;
+          else sD[gq] += pv;
+        }
+#pragma unroll
+        for (int gq = 0; gq < NG; ++gq) {
+          double* tb = s_ecnt + gq * TAB_SZ;
+          if (i >= 1 && j >= 1) atomicAdd(tb + xi * A_PAD + yj, sM[gq] * pscale);
+          if (j >= 1) atomicAdd(tb + A_PAD * A_PAD + yj, sI[gq] * pscale);
+          if (i >= 1) atomicAdd(tb + A_PAD * A_PAD + A_PAD + xi, sD[gq] * pscale);
+        }
+      }
+      if (want_post && valid) {
+        const double ps_ = ldexp(inv_mP, EF + EB - EP);
+        double* dst = post + ((size_t)i * W1 + j) * S + L::SOFF;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) dst[b] = fcur[b] * Bv[b] * ps_;
+        if (i == 0 && j == 0) {
+          double b00 = 0.0;
+#pragma unroll
+          for (int b = 0; b < NB; ++b) b00 = fma(p_start[b], g[b], b00);
+          post[0] = corner ? 1.0 : b00 * ldexp(inv_mP, Eg - EP);
+        }
+      }
+      // ---- q = e(i,j,k) B(i,j,k) for the predecessor row
+      const bool eM_ok = valid && i >= 1 && j >= 1, eI_ok = valid && j >= 1, eD_ok = valid && i >= 1;
+      double q[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int ty = L::type(b), gq = L::group(b);
+        double em;
+        bool ok;
+        if (ty == ST_M) { em = s_tab[gq * TAB_SZ + xi * A_PAD + yj]; ok = eM_ok; }
+        else if (ty == ST_I) { em = s_tab[gq * TAB_SZ + A_PAD * A_PAD + yj]; ok = eI_ok; }
+        else { em = s_tab[gq * TAB_SZ + A_PAD * A_PAD + A_PAD + xi]; ok = eD_ok; }
+        q[b] = ok ? em * Bv[b] : 0.0;
+      }
+      // ---- publish this row's q for the band above (lane 0)
+      if (lane == 0 && valid) {
+        double* e = xout ? Pr.xring + pd.xout_off + (size_t)p * NX
+                         : s_ring + ((size_t)w * 2 + (p & 1)) * NX;
+        if (xout || w < NW - 1) {
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            if (L::type(b) == ST_M) e[rank_in_type<L, ST_M>(b)] = q[b];
+            if (L::type(b) == ST_D) e[nM + rank_in_type<L, ST_D>(b)] = q[b];
+          }
+          e[NX - 1] = (double)EB;
+          if (xout && (((p + 1) % CQ) == 0 || p == Ly)) publish(Pr.flagv + pd.fout, p + 1);
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < NB; ++b) qp[b] = q[b];
+#pragma unroll
+      for (int m = 0; m < nM; ++m) rM_last[m] = rM[m];
+      eQ = EB;
+      eM_last = eR;
+    }
+    lds_barrier();
+  }
+  __syncthreads();
+  if (!ok_run) {
+    __hip_atomic_store(Pr.abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) Pr.out_logP[pd.pair] = __builtin_nan("");
+  }
+  if (want_ecnt) {  // layout group g -> caller group gmap[g]
+    double* oe = Pr.out_ecnt + (size_t)pd.pair * Pr.G_tab * TAB_SZ;
+    for (int q = threadIdx.x; q < NG * TAB_SZ; q += W * 64) {
+      const int gq = q / TAB_SZ, o = q - gq * TAB_SZ;
+      const double v = s_ecnt[q];
+      if (v != 0.0) unsafeAtomicAdd(oe + Pr.gmap[gq] * TAB_SZ + o, v);
+    }
+  }
+  if (!want_counts) return;
+  // ---- reduce the MFMA accumulators over the waves (fixed order), then add
+  double* s_red = s_stage;  // [W][16][16]
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int row = 4 * v + (lane >> 4), col = lane & 15;  // accumulator layout
+    s_red[(w * 16 + row) * 16 + col] = cacc[v] + cacc2[v];
+  }
+  __syncthreads();
+  for (int qq = threadIdx.x; qq < NB * NB; qq += W * 64) {
+    const int k = qq / NB, k2 = qq % NB;
+    double acc = 0.0;
+    for (int ww = 0; ww < NW; ++ww) acc += s_red[(ww * 16 + k) * 16 + k2];
+    const double pr = Pf[k * NB + k2];
+    if (pr > 0.0 && acc != 0.0) unsafeAtomicAdd(out_n + (k + L::SOFF) * S + (k2 + L::SOFF), pr * acc);
+  }
+}
+
+__global__ void zero_ints_kernel(int32_t* p, int n) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) p[q] = 0;
+}
+
+// ---------------------------------------------------------------- host side
+constexpr int kCW[3] = {4, 2, 1};
+int ccls_of(int nb) { return nb >= 4 ? 0 : (nb >= 2 ? 1 : 2); }
+
+struct ChainPlan {
+  bool ok = false;
+  std::string why;
+  std::vector<CDesc> fwd[3], bwd[3];
+  int64_t cls_cells[3] = {0, 0, 0};
+  int cls_pairs[3] = {0, 0, 0};
+  int max_ly[3] = {0, 0, 0};
+  size_t desc_b = 0, pm_b = 0, lat_b = 0, ring_b = 0, flag_b = 0, pinfo_b = 0;
+  int n_flags = 0;  // ints: 3 classes x 2 tickets, abort, pad, then 2 per boundary
+  size_t total() const { return desc_b + pm_b + lat_b + ring_b + flag_b + pinfo_b; }
+};
+
+ChainPlan plan_chain(const tkfmi_problem* p, const Plan& pl, uint32_t flags) {
+  ChainPlan cp;
+  const int B = p->B;
+  const int NX = nx_of(pl);
+  std::vector<int> order(B);
+  for (int b = 0; b < B; ++b) {
+    const int lx = p->h_x_len[b], ly = p->h_y_len[b];
+    if (lx < 0 || ly < 0 || lx + ly > 32000) { cp.why = "sequence lengths out of range"; return cp; }
+    order[b] = b;
+  }
+  // longest chains first inside each class (the ticket order)
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const int64_t ca = (int64_t)(p->h_x_len[a] + 1) * (p->h_y_len[a] + 1);
+    const int64_t cb = (int64_t)(p->h_x_len[b] + 1) * (p->h_y_len[b] + 1);
+    return ca > cb;
+  });
+  std::vector<int64_t> lat_off(B), post_off(B);
+  int64_t al = 0, ap = 0;
+  for (int b = 0; b < B; ++b) {
+    lat_off[b] = al;
+    if (!(flags & TKFMI_FORWARD_ONLY)) al += (int64_t)((p->h_x_len[b] + 64) / 64) * (p->h_y_len[b] + 64) * SLOTS * 64;
+    post_off[b] = ap;
+    ap += (int64_t)(p->h_x_len[b] + 1) * (p->h_y_len[b] + 1) * p->S;
+  }
+  int64_t aring = 0;
+  int nfl = 8;
+  for (int q = 0; q < B; ++q) {
+    const int b = order[q];
+    const int Lx = p->h_x_len[b], Ly = p->h_y_len[b];
+    const int nb = (Lx + 64) / 64;
+    const int c = ccls_of(nb);
+    const int W = kCW[c];
+    const int G = (nb + W - 1) / W;
+    ++cp.cls_pairs[c];
+    cp.cls_cells[c] += (int64_t)(Lx + 1) * (Ly + 1);
+    cp.max_ly[c] = std::max(cp.max_ly[c], Ly);
+    const int64_t ring0 = aring;
+    const int flag0 = nfl;
+    aring += (int64_t)(G - 1) * (Ly + 1) * NX;
+    nfl += 2 * (G - 1);
+    std::vector<CDesc> ch(G);
+    for (int g = 0; g < G; ++g) {
+      CDesc d;
+      memset(&d, 0, sizeof(d));
+      d.pair = b; d.Lx = Lx; d.Ly = Ly; d.band0 = g * W;
+      d.nw = std::min(W, nb - g * W); d.g = g; d.G = G;
+      d.x_off = p->h_x_off[b]; d.y_off = p->h_y_off[b];
+      d.lat_off = lat_off[b]; d.post_off = post_off[b];
+      d.pm_off = (p->n_trans > 1 ? (int64_t)b : 0) * (int64_t)stride_for(pl.NB);
+      ch[g] = d;
+    }
+    for (int g = 0; g < G; ++g) {  // forward: boundary g-1 in, g out
+      CDesc d = ch[g];
+      const int rl = std::min(63, Lx - 64 * (d.band0 + d.nw - 1));
+      d.T = Ly + 64 * (d.nw - 1) + rl + 1;
+      d.xin_off = g > 0 ? ring0 + (int64_t)(g - 1) * (Ly + 1) * NX : -1;
+      d.fin = g > 0 ? flag0 + 2 * (g - 1) : -1;
+      d.xout_off = g < G - 1 ? ring0 + (int64_t)g * (Ly + 1) * NX : -1;
+      d.fout = g < G - 1 ? flag0 + 2 * g : -1;
+      cp.fwd[c].push_back(d);
+    }
+    for (int g = G - 1; g >= 0; --g) {  // backward: boundary g in, g-1 out
+      CDesc d = ch[g];
+      d.T = Ly + 64 * (d.nw - 1) + 64;
+      d.xin_off = g < G - 1 ? ring0 + (int64_t)g * (Ly + 1) * NX : -1;
+      d.fin = g < G - 1 ? flag0 + 2 * g + 1 : -1;
+      d.xout_off = g > 0 ? ring0 + (int64_t)(g - 1) * (Ly + 1) * NX : -1;
+      d.fout = g > 0 ? flag0 + 2 * (g - 1) + 1 : -1;
+      cp.bwd[c].push_back(d);
+    }
+  }
+  size_t nd = 0;
+  for (int c = 0; c < 3; ++c) nd += cp.fwd[c].size() + cp.bwd[c].size();
+  cp.desc_b = align256(sizeof(CDesc) * std::max<size_t>(nd, 1));
+  cp.pm_b = align256(stride_for(pl.NB) * 8 * (size_t)p->n_trans);
+  cp.lat_b = align256((size_t)al * 8);
+  cp.ring_b = align256((size_t)std::max<int64_t>(aring, 1) * 8);
+  cp.n_flags = nfl;
+  cp.flag_b = align256((size_t)nfl * 4);
+  cp.pinfo_b = align256((size_t)std::max(B, 1) * 16);
+  for (int c = 0; c < 3; ++c)
+    if (cp.cls_pairs[c] && CLds<LMix<3, 1>, 4>::total(true, cp.max_ly[c]) > 160 * 1024 - 512) {
+      cp.why = "LDS budget exceeded (sequences too long)";
+      return cp;
+    }
+  cp.ok = true;
+  return cp;
+}
+
+// Workgroups per CU that each class's launches leave room for (through their
+// LDS request); 0 = as many as fit.  TKFMI_CHAIN_WPC="f4,b4,f2,b2,f1,b1".
+void chain_wpc(int wpc[3][2]) {
+  static int v[6] = {0, 0, 0, 0, 0, 0};
+  static bool init = false;
+  if (!init) {
+    const char* e = getenv("TKFMI_CHAIN_WPC");
+    if (e) sscanf(e, "%d,%d,%d,%d,%d,%d", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5]);
+    init = true;
+  }
+  for (int c = 0; c < 3; ++c) { wpc[c][0] = v[2 * c]; wpc[c][1] = v[2 * c + 1]; }
+}
+
+template <class L, int W>
+int launch_chain(bool bwd, const CParams& prm, const double* pmat, const CDesc* d, int n, int max_ly,
+                 int wpc, hipStream_t s, std::string& err) {
+  const void* fn = bwd ? (const void*)fbc_bwd_kernel<L, W> : (const void*)fbc_fwd_kernel<L, W>;
+  size_t smem = CLds<L, W>::total(bwd, max_ly);
+  if (wpc > 0) smem = std::max(smem, (size_t)(160 * 1024 / wpc - 512) & ~(size_t)255);
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  if (e != hipSuccess) { err = std::string("hipFuncSetAttribute: ") + hipGetErrorString(e); return TKFMI_EHIP; }
+  if (bwd) hipLaunchKernelGGL((fbc_bwd_kernel<L, W>), dim3(n), dim3(W * 64), smem, s, prm, pmat, d);
+  else hipLaunchKernelGGL((fbc_fwd_kernel<L, W>), dim3(n), dim3(W * 64), smem, s, prm, pmat, d);
+  e = hipGetLastError();
+  if (e != hipSuccess) { err = std::string("fbc launch: ") + hipGetErrorString(e); return TKFMI_EHIP; }
+  return TKFMI_OK;
+}
+
+template <class L>
+int launch_chain_cls(int c, bool bwd, const CParams& prm, const double* pmat, const CDesc* d, int n,
+                     int max_ly, int wpc, hipStream_t s, std::string& err) {
+  switch (kCW[c]) {
+    case 4: return launch_chain<L, 4>(bwd, prm, pmat, d, n, max_ly, wpc, s, err);
+    case 2: return launch_chain<L, 2>(bwd, prm, pmat, d, n, max_ly, wpc, s, err);
+    default: return launch_chain<L, 1>(bwd, prm, pmat, d, n, max_ly, wpc, s, err);
+  }
+}
+
+int dispatch_chain(const Plan& pl, int c, bool bwd, const CParams& prm, const double* pmat,
+                   const CDesc* d, int n, int max_ly, int wpc, hipStream_t s, std::string& err) {
+  switch (pl.lay) {
+    case Layout::Tkf: return launch_chain_cls<LTkf>(c, bwd, prm, pmat, d, n, max_ly, wpc, s, err);
+    case Layout::Mix11: return launch_chain_cls<LMix<1, 1>>(c, bwd, prm, pmat, d, n, max_ly, wpc, s, err);
+    case Layout::Mix21: return launch_chain_cls<LMix<2, 1>>(c, bwd, prm, pmat, d, n, max_ly, wpc, s, err);
+    case Layout::Mix31: return launch_chain_cls<LMix<3, 1>>(c, bwd, prm, pmat, d, n, max_ly, wpc, s, err);
+    case Layout::Mix12: return launch_chain_cls<LMix<1, 2>>(c, bwd, prm, pmat, d, n, max_ly, wpc, s, err);
+    case Layout::Mix13: return launch_chain_cls<LMix<1, 3>>(c, bwd, prm, pmat, d, n, max_ly, wpc, s, err);
+    default: err = "layout not handled"; return TKFMI_EINVAL;
+  }
+}
+
+int prep_dispatch(const Plan& pl, const tkfmi_problem* p, double* pmat, hipStream_t s, std::string& err) {
+  switch (pl.lay) {
+    case Layout::Tkf: return launch_prep<LTkf>(p, pmat, s, err);
+    case Layout::Mix11: return launch_prep<LMix<1, 1>>(p, pmat, s, err);
+    case Layout::Mix21: return launch_prep<LMix<2, 1>>(p, pmat, s, err);
+    case Layout::Mix31: return launch_prep<LMix<3, 1>>(p, pmat, s, err);
+    case Layout::Mix12: return launch_prep<LMix<1, 2>>(p, pmat, s, err);
+    case Layout::Mix13: return launch_prep<LMix<1, 3>>(p, pmat, s, err);
+    default: err = "layout not handled"; return TKFMI_EINVAL;
+  }
+}
+
+}  // namespace tkfmi_v2
+
+using namespace tkfmi_v2;
+
+bool tkfmi_chain_workspace(const tkfmi_problem* p, uint32_t flags, size_t* bytes) {
+  Plan pl;
+  if (!p || p->B < 0 || !p->state_types || !p->state_group || !detect(p, pl)) return false;
+  if (p->B > 0 && (!p->h_x_len || !p->h_y_len || !p->h_x_off || !p->h_y_off)) return false;
+  if (p->n_trans != 1 && p->n_trans != p->B) return false;
+  const ChainPlan cp = plan_chain(p, pl, flags);
+  if (!cp.ok) return false;
+  *bytes = cp.total();
+  return true;
+}
+
+int tkfmi_chain_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, size_t ws_bytes,
+                    hipStream_t stream, uint32_t flags, std::string& err) {
+  Plan pl;
+  if (!detect(p, pl)) { err = "layout not handled"; return TKFMI_EINVAL; }
+  const ChainPlan cp = plan_chain(p, pl, flags);
+  if (!cp.ok) { err = cp.why; return TKFMI_EINVAL; }
+  if (!ws || ws_bytes < cp.total()) {
+    err = "workspace too small: need " + std::to_string(cp.total());
+    return TKFMI_EWORKSPACE;
+  }
+  if (p->B == 0) return TKFMI_OK;
+  char* wsb = static_cast<char*>(ws);
+  CDesc* ddesc = reinterpret_cast<CDesc*>(wsb);
+  double* pmat = reinterpret_cast<double*>(wsb + cp.desc_b);
+  double* lattice = reinterpret_cast<double*>(wsb + cp.desc_b + cp.pm_b);
+  double* xring = reinterpret_cast<double*>(wsb + cp.desc_b + cp.pm_b + cp.lat_b);
+  int32_t* flagv = reinterpret_cast<int32_t*>(wsb + cp.desc_b + cp.pm_b + cp.lat_b + cp.ring_b);
+  double* pinfo = reinterpret_cast<double*>(wsb + cp.desc_b + cp.pm_b + cp.lat_b + cp.ring_b + cp.flag_b);
+  // descriptors: [class][fwd | bwd]
+  std::vector<CDesc> all;
+  size_t base[3][2];
+  for (int c = 0; c < 3; ++c) {
+    base[c][0] = all.size();
+    all.insert(all.end(), cp.fwd[c].begin(), cp.fwd[c].end());
+    base[c][1] = all.size();
+    all.insert(all.end(), cp.bwd[c].begin(), cp.bwd[c].end());
+  }
+  Streams* sts = nullptr;
+  int rc = get_streams(sts, err);
+  if (rc != TKFMI_OK) return rc;
+  rc = tkfmi_stage_upload(ddesc, all.data(), sizeof(CDesc) * all.size(), stream, err);
+  if (rc != TKFMI_OK) return rc;
+  hipLaunchKernelGGL(zero_ints_kernel, dim3(std::min(1024, (cp.n_flags + 255) / 256)), dim3(256), 0, stream,
+                     flagv, cp.n_flags);
+  if (hipGetLastError() != hipSuccess) { err = "zero launch failed"; return TKFMI_EHIP; }
+  rc = prep_dispatch(pl, p, pmat, stream, err);
+  if (rc != TKFMI_OK) return rc;
+  if (hipEventRecord(sts->start, stream) != hipSuccess) { err = "hipEventRecord failed"; return TKFMI_EHIP; }
+  CParams prm;
+  memset(&prm, 0, sizeof(prm));
+  prm.S = p->S; prm.flags = (int32_t)flags; prm.n_tab = p->n_tab; prm.G_tab = p->n_groups;
+  for (int g = 0; g < 16; ++g) prm.gmap[g] = pl.gmap[g];
+  prm.log_tab = p->log_tab; prm.x_cat = p->x_cat; prm.y_cat = p->y_cat;
+  prm.lattice = lattice; prm.xring = xring; prm.flagv = flagv; prm.abortf = flagv + 6;
+  prm.pinfo = pinfo;
+  prm.out_logP = o->log_prob;
+  prm.out_n = (flags & TKFMI_NO_COUNTS) ? nullptr : o->n_trans;
+  prm.out_post = o->posteriors;
+  prm.out_ecnt = (flags & TKFMI_WANT_EMIT_COUNTS) ? o->emit_counts : nullptr;
+  int wpc[3][2];
+  chain_wpc(wpc);
+  const bool fwd_only = (flags & TKFMI_FORWARD_ONLY) != 0;
+  for (int c = 0; c < 3; ++c) {
+    if (!cp.cls_pairs[c]) continue;
+    hipStream_t s = sts->s[c];
+    if (hipStreamWaitEvent(s, sts->start, 0) != hipSuccess) { err = "hipStreamWaitEvent failed"; return TKFMI_EHIP; }
+    for (int dir = 0; dir < (fwd_only ? 1 : 2); ++dir) {
+      CParams pc = prm;
+      pc.ticket = flagv + 2 * c + dir;
+      const int n = (int)(dir ? cp.bwd[c].size() : cp.fwd[c].size());
+      const std::string kname = std::string(dir ? "fbc_bwd_kernel<" : "fbc_fwd_kernel<") + layout_name(pl) +
+                                "," + std::to_string(kCW[c]) + ">";
+      const int slot = tkfmi_ktime_start(kname.c_str(), cp.cls_pairs[c], cp.cls_cells[c], s, err);
+      if (slot == -2) return TKFMI_EHIP;
+      rc = dispatch_chain(pl, c, dir == 1, pc, pmat, ddesc + base[c][dir], n, cp.max_ly[c], wpc[c][dir], s, err);
+      if (rc != TKFMI_OK) return rc;
+      if (tkfmi_ktime_stop(slot, s, err) != TKFMI_OK) return TKFMI_EHIP;
+    }
+    if (hipEventRecord(sts->done[c], s) != hipSuccess) { err = "hipEventRecord failed"; return TKFMI_EHIP; }
+    if (hipStreamWaitEvent(stream, sts->done[c], 0) != hipSuccess) { err = "hipStreamWaitEvent failed"; return TKFMI_EHIP; }
+  }
+  return TKFMI_OK;
+}

@@ -22,6 +22,12 @@ bool tkfmi_v2_workspace(const tkfmi_problem* p, uint32_t flags, size_t* bytes);
 int tkfmi_v2_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, size_t ws_bytes,
                  hipStream_t stream, uint32_t flags, std::string& err);
 
+// Chained cell-per-lane path (same layouts, a pair's bands spread over a
+// chain of workgroups).
+bool tkfmi_chain_workspace(const tkfmi_problem* p, uint32_t flags, size_t* bytes);
+int tkfmi_chain_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, size_t ws_bytes,
+                    hipStream_t stream, uint32_t flags, std::string& err);
+
 // Wide path (16..160 emitting states).
 bool tkfmi_wide_applies(const tkfmi_problem* p);
 size_t tkfmi_wide_workspace(const tkfmi_problem* p, uint32_t flags);

@@ -21,6 +21,7 @@ NO_COUNTS = 4
 GENERIC_KERNEL = 8
 WANT_EMIT_COUNTS = 16
 TIME_KERNELS = 32
+CLASS_KERNEL = 64
 A_PAD = 21
 TAB_SZ = A_PAD * A_PAD + 2 * A_PAD
 
@@ -247,9 +248,26 @@ def normalise_codes(seq):
     return np.clip(a, 0, A_PAD - 1).astype(np.uint8)
 
 
+def kernel_flags(kernel=None, generic=False):
+    """Flag bits that select the 2D forward/backward kernel.
+
+    kernel: None or "chain" (default: chained workgroups per pair on the
+    cell-per-lane path where the state layout allows, else the generic
+    kernel), "class" (one workgroup per pair, cell-per-lane) or "generic"
+    (any layout).  ``generic=True`` means kernel="generic".
+    """
+    if generic or kernel == "generic":
+        return GENERIC_KERNEL
+    if kernel == "class":
+        return CLASS_KERNEL
+    if kernel not in (None, "chain"):
+        raise ValueError(f"unknown kernel {kernel!r}")
+    return 0
+
+
 def fb2d_batched(log_trans, state_types, state_group, log_tab, pairs, *,
                  forward_only=False, want_posteriors=False, want_counts=True,
-                 want_emit_counts=False, generic=False):
+                 want_emit_counts=False, generic=False, kernel=None):
     """Run the batched forward/backward kernel once.
 
     log_trans: float64 device tensor (S,S) or (B,S,S) of log transitions.
@@ -263,7 +281,7 @@ def fb2d_batched(log_trans, state_types, state_group, log_tab, pairs, *,
     r = BatchRunner(log_trans, state_types, state_group, log_tab, pairs,
                     forward_only=forward_only, want_posteriors=want_posteriors,
                     want_counts=want_counts, want_emit_counts=want_emit_counts,
-                    generic=generic)
+                    generic=generic, kernel=kernel)
     r.run(zero_outputs=False)  # outputs are freshly zeroed
     return r.results()
 
@@ -286,7 +304,7 @@ class BatchRunner:
 
     def __init__(self, log_trans, state_types, state_group, log_tab, pairs, *,
                  forward_only=False, want_posteriors=False, want_counts=True,
-                 want_emit_counts=False, generic=False, time_kernels=False):
+                 want_emit_counts=False, generic=False, time_kernels=False, kernel=None):
         self.L = lib()
         dev = log_trans.device
         if dev.type != "cuda":
@@ -313,7 +331,7 @@ class BatchRunner:
                       | (WANT_POSTERIORS if (want_posteriors and not fwd) else 0)
                       | (0 if want_counts else NO_COUNTS)
                       | (WANT_EMIT_COUNTS if (want_emit_counts and not fwd) else 0)
-                      | (GENERIC_KERNEL if generic else 0)
+                      | kernel_flags(kernel, generic)
                       | (TIME_KERNELS if time_kernels else 0))
         B = pairs.B
         self.prob = Problem(B, S, _np_ptr(self.st), _np_ptr(self.sg), G, n_trans,

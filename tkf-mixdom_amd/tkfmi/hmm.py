@@ -84,7 +84,7 @@ def _types_list(state_types):
 
 def fb2d_batch(log_trans, state_types, xs, ys, log_tab, state_group=None, *,
                forward_only=False, want_posteriors=False, want_counts=True,
-               want_emit_counts=False, generic=False):
+               want_emit_counts=False, generic=False, kernel=None):
     """One launch over a ragged batch of pairs.
 
     log_trans: (S,S) or (B,S,S) log transitions; log_tab: (G,TAB) or
@@ -94,8 +94,10 @@ def fb2d_batch(log_trans, state_types, xs, ys, log_tab, state_group=None, *,
     want_counts=False), 'posteriors' (list of (Lx+1,Ly+1,S) views) and
     'emit_counts' (B,G,TAB): expected uses of each emission-table entry,
     i.e. d log_p / d log_tab.
-    ``generic=True`` forces the any-layout kernel (the TKF and MixDom K*F<=3
-    layouts otherwise run on the cell-per-lane kernel).
+    ``generic=True`` (or kernel="generic") forces the any-layout kernel; the
+    TKF and MixDom K*F<=3 layouts otherwise run on the cell-per-lane path,
+    chained workgroups per pair by default or one workgroup per pair with
+    kernel="class".
     """
     dev = device()
     lt = _dev64(log_trans, dev)
@@ -107,7 +109,8 @@ def fb2d_batch(log_trans, state_types, xs, ys, log_tab, state_group=None, *,
     pairs = _abi.PackedPairs(list(xs), list(ys), dev)
     out = _abi.fb2d_batched(lt, st, state_group, tab, pairs, forward_only=forward_only,
                             want_posteriors=want_posteriors, want_counts=want_counts,
-                            want_emit_counts=want_emit_counts, generic=generic)
+                            want_emit_counts=want_emit_counts, generic=generic,
+                            kernel=kernel)
     if "posteriors" in out:
         out["posteriors"] = _abi.split_posteriors(out["posteriors"], pairs, S)
     return out

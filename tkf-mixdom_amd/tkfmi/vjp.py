@@ -29,11 +29,12 @@ from . import _abi, hmm, mixdom, params, tables
 class _Spec:
     """Non-tensor inputs of one batched call."""
 
-    def __init__(self, state_types, state_group, xs, ys, device, generic=False):
+    def __init__(self, state_types, state_group, xs, ys, device, generic=False, kernel=None):
         self.state_types = hmm._types_list(state_types)
         self.state_group = list(state_group) if state_group is not None else [0] * len(self.state_types)
         self.pairs = _abi.PackedPairs(list(xs), list(ys), device)
         self.generic = generic
+        self.kernel = kernel
 
 
 class FB2DLogProb(torch.autograd.Function):
@@ -46,7 +47,7 @@ class FB2DLogProb(torch.autograd.Function):
     def forward(ctx, log_trans, log_tab, spec):
         r = _abi.BatchRunner(log_trans.detach(), spec.state_types, spec.state_group,
                              log_tab.detach(), spec.pairs, want_emit_counts=True,
-                             generic=spec.generic)
+                             generic=spec.generic, kernel=spec.kernel)
         r.run(zero_outputs=False)
         ctx.save_for_backward(r.n, r.ecnt)
         ctx.shared = (log_trans.dim() == 2, log_tab.dim() == 2)
@@ -64,10 +65,12 @@ class FB2DLogProb(torch.autograd.Function):
         return gt, ge, None
 
 
-def log_prob_batch(log_trans, state_types, xs, ys, log_tab, state_group=None, generic=False):
-    """Differentiable (B,) log-likelihoods of a ragged batch (one call)."""
+def log_prob_batch(log_trans, state_types, xs, ys, log_tab, state_group=None, generic=False,
+                   kernel=None):
+    """Differentiable (B,) log-likelihoods of a ragged batch (one call).
+    ``generic`` / ``kernel`` select the kernel as in tkfmi.hmm.fb2d_batch."""
     dev = log_trans.device if isinstance(log_trans, torch.Tensor) else hmm.device()
-    spec = _Spec(state_types, state_group, xs, ys, dev, generic)
+    spec = _Spec(state_types, state_group, xs, ys, dev, generic, kernel)
     return FB2DLogProb.apply(log_trans.to(torch.float64), log_tab.to(torch.float64), spec)
 
 
@@ -101,7 +104,7 @@ def _as(x, dev):
 
 
 # ----------------------------------------------------------------- TKF91/92
-def tkf91_log_prob_batch(ins_rate, del_rate, ts, Q, pi, xs, ys, *, cond=False, generic=False):
+def tkf91_log_prob_batch(ins_rate, del_rate, ts, Q, pi, xs, ys, *, cond=False, generic=False, kernel=None):
     """(B,) log P(x_b, y_b | lambda, mu, t_b, Q, pi) under TKF91 with the
     shared emissions of pair_hmm_emissions (train/vjp.py:110-194); ``cond``
     uses the ancestor-length-conditioned matrix (train/vjp.py:198-249)."""
@@ -111,7 +114,7 @@ def tkf91_log_prob_batch(ins_rate, del_rate, ts, Q, pi, xs, ys, *, cond=False, g
     log_trans = hmm.safe_log(build(_as(ins_rate, dev), _as(del_rate, dev), ts))
     sub = transition_matrix(_as(Q, dev), ts)                       # (B,A,A)
     tab = tables.shared_tables(sub, _as(pi, dev).expand(sub.shape[:-1]))  # (B,1,TAB)
-    return log_prob_batch(log_trans, params.TKF_STATE_TYPES, xs, ys, tab, generic=generic)
+    return log_prob_batch(log_trans, params.TKF_STATE_TYPES, xs, ys, tab, generic=generic, kernel=kernel)
 
 
 def tkf91_log_prob(ins_rate, del_rate, t, Q, pi, x_seq, y_seq):
@@ -126,7 +129,7 @@ def tkf91_log_prob_cond(ins_rate, del_rate, t, Q, pi, x_seq, y_seq):
                                 [x_seq], [y_seq], cond=True)[0]
 
 
-def tkf92_log_prob_batch(ins_rate, del_rate, ts, ext, Q, pi, xs, ys, *, generic=False):
+def tkf92_log_prob_batch(ins_rate, del_rate, ts, ext, Q, pi, xs, ys, *, generic=False, kernel=None):
     """(B,) TKF92 log-likelihoods (train/vjp.py:253-329)."""
     dev = _dev()
     ts = _as(ts, dev)
@@ -134,7 +137,7 @@ def tkf92_log_prob_batch(ins_rate, del_rate, ts, ext, Q, pi, xs, ys, *, generic=
                                                 _as(ext, dev)))
     sub = transition_matrix(_as(Q, dev), ts)
     tab = tables.shared_tables(sub, _as(pi, dev).expand(sub.shape[:-1]))
-    return log_prob_batch(log_trans, params.TKF_STATE_TYPES, xs, ys, tab, generic=generic)
+    return log_prob_batch(log_trans, params.TKF_STATE_TYPES, xs, ys, tab, generic=generic, kernel=kernel)
 
 
 def tkf92_log_prob(ins_rate, del_rate, t, ext, Q, pi, x_seq, y_seq):
@@ -146,7 +149,7 @@ def tkf92_log_prob(ins_rate, del_rate, t, ext, Q, pi, x_seq, y_seq):
 
 # ------------------------------------------------------------------ MixDom
 def mixdom_log_prob_batch(main_ins_rate, main_del_rate, ts, dom_ins_rates, dom_del_rates,
-                          dom_weights, frag_weights, ext_rates, Q, pi, xs, ys, *, generic=False):
+                          dom_weights, frag_weights, ext_rates, Q, pi, xs, ys, *, generic=False, kernel=None):
     """(B,) MixDom log-likelihoods with shared emissions (train/vjp.py:351-458)."""
     dev = _dev()
     ts = _as(ts, dev)
@@ -158,7 +161,7 @@ def mixdom_log_prob_batch(main_ins_rate, main_del_rate, ts, dom_ins_rates, dom_d
     st = mixdom.state_types(K, F)
     sub = transition_matrix(_as(Q, dev), ts)
     tab = tables.shared_tables(sub, _as(pi, dev).expand(sub.shape[:-1]))
-    return log_prob_batch(hmm.safe_log(chi), st, xs, ys, tab, generic=generic)
+    return log_prob_batch(hmm.safe_log(chi), st, xs, ys, tab, generic=generic, kernel=kernel)
 
 
 def mixdom_log_prob(main_ins_rate, main_del_rate, t, dom_ins_rates, dom_del_rates, dom_weights,
@@ -179,7 +182,7 @@ def build_class_subs(class_S_exch, class_pis, ts):
 
 def mixdom2_log_prob_batch(main_ins_rate, main_del_rate, ts, dom_ins_rates, dom_del_rates,
                            dom_weights, frag_weights, ext_rates, class_pis, class_S_exch,
-                           class_dist, xs, ys, *, generic=False):
+                           class_dist, xs, ys, *, generic=False, kernel=None):
     """(B,) MixDom2 log-likelihoods: per-(domain, fragment) class-mixture
     emissions (train/vjp.py:628-820; dp/hmm.py:673-783)."""
     dev = _dev()
@@ -193,7 +196,7 @@ def mixdom2_log_prob_batch(main_ins_rate, main_del_rate, ts, dom_ins_rates, dom_
     subs = build_class_subs(_as(class_S_exch, dev), _as(class_pis, dev), ts)
     tab = tables.per_class_tables(subs, _as(class_pis, dev), _as(class_dist, dev))  # (B,KF,TAB)
     groups = tables.state_groups_per_class(len(st), F)
-    return log_prob_batch(hmm.safe_log(chi), st, xs, ys, tab, groups, generic=generic)
+    return log_prob_batch(hmm.safe_log(chi), st, xs, ys, tab, groups, generic=generic, kernel=kernel)
 
 
 def mixdom2_log_prob(main_ins_rate, main_del_rate, t, dom_ins_rates, dom_del_rates, dom_weights,
