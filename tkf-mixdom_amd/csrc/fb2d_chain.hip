@@ -114,8 +114,9 @@ __device__ __forceinline__ int take_ticket(int32_t* counter) {
   return __builtin_amdgcn_readfirstlane(s_ticket);
 }
 
-// LDS layout (doubles): tables | emission counts (backward) | short rings |
-// chunk buffer | MFMA staging (backward) | bytes: x rows of the workgroup, y.
+// LDS layout (doubles): tables | start / end transitions | emission counts
+// (backward) | short rings | chunk buffer | MFMA staging (backward, 64 cells x
+// 32 doubles per wave) | bytes: x rows of the workgroup, y.
 template <class L, int W>
 struct CLds {
   static constexpr int NX = count_type<L, ST_M>() + count_type<L, ST_D>() + 1;
@@ -125,7 +126,7 @@ struct CLds {
   static constexpr int ring(bool bwd) { return ecnt + (bwd ? L::NG * TAB_SZ : 0); }
   static constexpr int xc(bool bwd) { return ring(bwd) + (W - 1) * 2 * NX; }
   static constexpr int stage(bool bwd) { return ((xc(bwd) + CQ * NX) + 1) & ~1; }
-  static constexpr int seq_bytes(bool bwd) { return 8 * (stage(bwd) + (bwd ? W * 1024 : 0)); }
+  static constexpr int seq_bytes(bool bwd) { return 8 * (stage(bwd) + (bwd ? W * 2048 : 0)); }
   static constexpr int xbytes = 64 * W + 8;
   static size_t total(bool bwd, int max_ly) { return (size_t)seq_bytes(bwd) + xbytes + ((max_ly + 8) & ~7) + 16; }
 };
@@ -210,8 +211,12 @@ __global__ __launch_bounds__(W * 64) void fbc_fwd_kernel(CParams Pr, const doubl
   for (int m = 0; m < nM; ++m) rM_last[m] = 0.0;
   int eH = EXP_NONE, eM_last = EXP_NONE;
   const int T = pd.T;
+  // A wave's band is in range for steps 64 w .. 64 w + 63 + Ly; outside
+  // them every lane is invalid and the wave only keeps the barrier (its
+  // carried values are zero there, as the skipped steps would leave them).
+  const int t_on = 64 * w, t_off = 64 * w + 63 + Ly;
   for (int t = 0; t < T; ++t) {
-    if (has_band) {
+    if (has_band && t >= t_on && t <= t_off) {
       int i = i_band;
       asm volatile("" : "+v"(i));  // keeps row predicates inside the loop (no SGPR masks)
       const int j = t - 64 * w - lane;
@@ -349,6 +354,37 @@ __global__ __launch_bounds__(W * 64) void fbc_fwd_kernel(CParams Pr, const doubl
   }
 }
 
+// Expected-count MFMAs over 64 staged cells: acc (+acc2) += F^T g, K = 4
+// cells per v_mfma_f64_16x16x4f64, alternating two accumulator chains.
+__device__ __forceinline__ void count_mfma(const double* stage_f, const double* stage_g, int lane, d4& acc,
+                                           d4& acc2) {
+#pragma unroll
+  for (int q = 0; q < 16; q += 2) {
+    const int c0 = 4 * q + (lane >> 4), c1 = c0 + 4;
+    const int o0 = c0 * 16 + ((lane & 15) ^ (c0 & 15));
+    const int o1 = c1 * 16 + ((lane & 15) ^ (c1 & 15));
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(stage_f[o0], stage_g[o0], acc, 0, 0, 0);
+    acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(stage_f[o1], stage_g[o1], acc2, 0, 0, 0);
+  }
+}
+
+// Bv[k] = sum_k' bcast_k(prowT[k']) g[k'] interleaved with the 16 count
+// MFMAs of the previously staged cells (one per transition row), so that the
+// matrix core works while the VALU issues the row's DPP FMAs.
+template <int NB>
+__device__ __forceinline__ void contract_and_count(double (&Bv)[NB], const double (&prowT)[NB],
+                                                   const double (&g)[NB], const double* stage_f,
+                                                   const double* stage_g, int lane, d4& acc, d4& acc2) {
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    if (q < NB) row_fmac<NB>(Bv, prowT[q < NB ? q : 0], g[q < NB ? q : 0]);
+    const int c = 4 * q + (lane >> 4);
+    const int o = c * 16 + ((lane & 15) ^ (c & 15));
+    if (q & 1) acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(stage_f[o], stage_g[o], acc2, 0, 0, 0);
+    else acc = __builtin_amdgcn_mfma_f64_16x16x4f64(stage_f[o], stage_g[o], acc, 0, 0, 0);
+  }
+}
+
 // ============================================================= backward
 template <class L, int W>
 __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const double* __restrict__ pmat,
@@ -404,8 +440,18 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
   const bool xout = (w == NW - 1) && pd.fout >= 0;
   bool ok_run = true;
   d4 cacc = {0.0, 0.0, 0.0, 0.0}, cacc2 = {0.0, 0.0, 0.0, 0.0};
-  double* stage_f = s_stage + (size_t)w * 1024;  // [32 cells][16] F-scaled
-  double* stage_g = stage_f + 512;               // [32 cells][16] g
+  // Count staging, one step deep: at step t the wave runs the MFMAs of the
+  // cells it staged at step t-1 (interleaved with the transition FMAs), then
+  // stages its cells of step t in the same buffer.  A wave's LDS operations execute in issue order, so the
+  // reads of step t-1's cells precede their overwrite without a wait, and no
+  // write -> read round trip sits inside a step.
+  double* stage_f = s_stage + (size_t)w * 2048;  // [64 cells][16] F-scaled
+  double* stage_g = stage_f + 1024;              // [64 cells][16] g
+  double accD[NG];   // D-type emission mass of this lane's row (x fixed per lane)
+  double runI[NG];   // I-type mass of the column passing up through the lanes
+#pragma unroll
+  for (int gq = 0; gq < NG; ++gq) { accD[gq] = 0.0; runI[gq] = 0.0; }
+  for (int q = lane; q < 2048; q += 64) stage_f[q] = 0.0;  // step 0 counts an empty stage
   double* post = want_post ? Pr.out_post + pd.post_off : nullptr;
   __syncthreads();
 
@@ -423,14 +469,18 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
   };
   double fnext[SLOTS];
   {
-    const double* a = f_addr(-off, i_band);
+    const double* a = f_addr(64 * w - off, i_band);  // the first active step
 #pragma unroll
     for (int b = 0; b < NB; ++b) fnext[b] = a ? a[b * 64] : 0.0;
     fnext[SLOTS - 1] = a ? a[(SLOTS - 1) * 64] : 0.0;
   }
   const int T = pd.T;
+  // A wave's band is in range for steps 64 w .. 64 w + 63 + Ly; outside
+  // them every lane is invalid and the wave only keeps the barrier (its
+  // carried values are zero there, as the skipped steps would leave them).
+  const int t_on = 64 * w, t_off = 64 * w + 63 + Ly;
   for (int t = 0; t < T; ++t) {
-    if (has_band) {
+    if (has_band && t >= t_on && t <= t_off) {
       int i = i_band;
       asm volatile("" : "+v"(i));  // keeps row predicates inside the loop (no SGPR masks)
       const int p = t - off;
@@ -493,7 +543,7 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
       double Bv[NB];
 #pragma unroll
       for (int k = 0; k < NB; ++k) Bv[k] = 0.0;
-      row_contract<NB>(Bv, prowT, g);
+      contract_and_count<NB>(Bv, prowT, g, stage_f, stage_g, lane, cacc, cacc2);
       const bool corner = valid && i == Lx && j == Ly;
       int EBraw = Eg;
       if (corner) {
@@ -511,29 +561,13 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
       // ---- counts: stage F-scaled and g, then MFMA (K = 4 cells per op)
       const int EF = (int)fcur[SLOTS - 1];
       const double fscale = valid ? ldexp(inv_mP, EF + Eg - EP) : 0.0;
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {  // stage this step's cells (counted next step)
+        const int o = lane * 16 + (s ^ (lane & 15));
+        stage_f[o] = (s < NB) ? fcur[s < NB ? s : 0] * fscale : 0.0;
+        stage_g[o] = (s < NB) ? g[s < NB ? s : 0] : 0.0;
+      }
       if (want_counts) {
-#pragma unroll
-        for (int half = 0; half < 2; ++half) {
-          if ((lane >> 5) == half) {
-            const int c = lane & 31;
-#pragma unroll
-            for (int s = 0; s < 16; ++s) {
-              const int o = c * 16 + (s ^ (c & 15));
-              stage_f[o] = (s < NB) ? fcur[s < NB ? s : 0] * fscale : 0.0;
-              stage_g[o] = (s < NB) ? g[s < NB ? s : 0] : 0.0;
-            }
-          }
-          wave_lds_sync();
-#pragma unroll
-          for (int q = 0; q < 8; q += 2) {  // two accumulator chains
-            const int c0 = 4 * q + (lane >> 4), c1 = c0 + 4;
-            const int o0 = c0 * 16 + ((lane & 15) ^ (c0 & 15));
-            const int o1 = c1 * 16 + ((lane & 15) ^ (c1 & 15));
-            cacc = __builtin_amdgcn_mfma_f64_16x16x4f64(stage_f[o0], stage_g[o0], cacc, 0, 0, 0);
-            cacc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(stage_f[o1], stage_g[o1], cacc2, 0, 0, 0);
-          }
-          wave_lds_sync();
-        }
         if (valid && i == 0 && j == 0) {  // start state row: n[S, k]
           const double sc = ldexp(inv_mP, Eg - EP);
 #pragma unroll
@@ -543,25 +577,39 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
       const int xi = valid ? s_x[i - 64 * pd.band0] : 0;
       const int yj = valid ? s_y[j >= 1 ? j - 1 : 0] : 0;
       // ---- expected emission counts: posterior mass per (group, type)
-      if (want_ecnt && valid) {
-        const double pscale = ldexp(inv_mP, EF + EB - EP);
-        double sM[NG], sI[NG], sD[NG];
+      // M-type mass goes to its (x, y) entry with an LDS atomic; D-type mass
+      // accumulates per lane (the row, hence x, is fixed); I-type mass rides
+      // a running sum up the column (lane l+1 -> l, as q does) and lane 0
+      // adds the column total of the band to its y entry.
+      if (want_ecnt) {
+        double cI[NG];
 #pragma unroll
-        for (int gq = 0; gq < NG; ++gq) { sM[gq] = 0.0; sI[gq] = 0.0; sD[gq] = 0.0; }
+        for (int gq = 0; gq < NG; ++gq) { runI[gq] = dpp_f64<DPP_WAVE_SHL1>(runI[gq]); cI[gq] = 0.0; }
+        if (valid) {
+          const double pscale = ldexp(inv_mP, EF + EB - EP);
+          double sM[NG], sI[NG], sD[NG];
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          const int ty = L::type(b), gq = L::group(b);
-          const double pv = fcur[b] * Bv[b];
-          if (ty == ST_M) sM[gq] += pv;
-          else if (ty == ST_I) sI[gq] += pv;
-          else sD[gq] += pv;
+          for (int gq = 0; gq < NG; ++gq) { sM[gq] = 0.0; sI[gq] = 0.0; sD[gq] = 0.0; }
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            const int ty = L::type(b), gq = L::group(b);
+            const double pv = fcur[b] * Bv[b];
+            if (ty == ST_M) sM[gq] += pv;
+            else if (ty == ST_I) sI[gq] += pv;
+            else sD[gq] += pv;
+          }
+#pragma unroll
+          for (int gq = 0; gq < NG; ++gq) {
+            if (i >= 1 && j >= 1) atomicAdd(s_ecnt + gq * TAB_SZ + xi * A_PAD + yj, sM[gq] * pscale);
+            if (j >= 1) cI[gq] = sI[gq] * pscale;
+            if (i >= 1) accD[gq] = fma(sD[gq], pscale, accD[gq]);
+          }
         }
 #pragma unroll
-        for (int gq = 0; gq < NG; ++gq) {
-          double* tb = s_ecnt + gq * TAB_SZ;
-          if (i >= 1 && j >= 1) atomicAdd(tb + xi * A_PAD + yj, sM[gq] * pscale);
-          if (j >= 1) atomicAdd(tb + A_PAD * A_PAD + yj, sI[gq] * pscale);
-          if (i >= 1) atomicAdd(tb + A_PAD * A_PAD + A_PAD + xi, sD[gq] * pscale);
+        for (int gq = 0; gq < NG; ++gq) runI[gq] += cI[gq];
+        if (lane == 0 && valid && j >= 1) {
+#pragma unroll
+          for (int gq = 0; gq < NG; ++gq) atomicAdd(s_ecnt + gq * TAB_SZ + A_PAD * A_PAD + yj, runI[gq]);
         }
       }
       if (want_post && valid) {
@@ -612,6 +660,13 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
     }
     lds_barrier();
   }
+  if (has_band) count_mfma(stage_f, stage_g, lane, cacc, cacc2);
+  if (has_band && want_ecnt && i_band >= 1 && i_band <= Lx) {
+    const int xi = s_x[i_band - 64 * pd.band0];
+#pragma unroll
+    for (int gq = 0; gq < NG; ++gq)
+      if (accD[gq] != 0.0) atomicAdd(s_ecnt + gq * TAB_SZ + A_PAD * A_PAD + A_PAD + xi, accD[gq]);
+  }
   __syncthreads();
   if (!ok_run) {
     __hip_atomic_store(Pr.abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -648,18 +703,45 @@ __global__ void zero_ints_kernel(int32_t* p, int n) {
 }
 
 // ---------------------------------------------------------------- host side
-constexpr int kCW[3] = {4, 2, 1};
-int ccls_of(int nb) { return nb >= 4 ? 0 : (nb >= 2 ? 1 : 2); }
+// Launch classes, each with its own stream: 0 = the longest 4-wave pairs,
+// 1 = the other 4-wave pairs, 2 = 2-wave, 3 = 1-wave workgroups.  The
+// longest pairs set the makespan (their backward waits for their class's
+// forward launch), so they get a class of their own that starts first.
+constexpr int NCLS = 4;
+constexpr int kCW[NCLS] = {4, 4, 2, 1};
+// Workgroup size class of a pair with nb bands: 4 waves from 4 bands on, 2
+// for 2-3 bands, 1 for one band.  TKFMI_CHAIN_MAXW (4, 2 or 1) caps the
+// workgroup size, so that longer pairs form longer chains of smaller
+// workgroups.
+int ccls_of(int nb) {
+  static const int maxw = [] {
+    const char* e = getenv("TKFMI_CHAIN_MAXW");
+    const int v = e ? atoi(e) : 4;
+    return v >= 4 ? 4 : (v >= 2 ? 2 : 1);
+  }();
+  if (nb >= 4 && maxw >= 4) return 1;
+  if (nb >= 2 && maxw >= 2) return 2;
+  return 3;
+}
+// A 4-wave pair joins the long class when its cells are at least this
+// fraction of the batch's largest pair (TKFMI_CHAIN_LONG, default 0.3).
+double long_frac() {
+  static const double v = [] {
+    const char* e = getenv("TKFMI_CHAIN_LONG");
+    return e ? atof(e) : 0.3;
+  }();
+  return v;
+}
 
 struct ChainPlan {
   bool ok = false;
   std::string why;
-  std::vector<CDesc> fwd[3], bwd[3];
-  int64_t cls_cells[3] = {0, 0, 0};
-  int cls_pairs[3] = {0, 0, 0};
-  int max_ly[3] = {0, 0, 0};
+  std::vector<CDesc> fwd[NCLS], bwd[NCLS];
+  int64_t cls_cells[NCLS] = {0, 0, 0, 0};
+  int cls_pairs[NCLS] = {0, 0, 0, 0};
+  int max_ly[NCLS] = {0, 0, 0, 0};
   size_t desc_b = 0, pm_b = 0, lat_b = 0, ring_b = 0, flag_b = 0, pinfo_b = 0;
-  int n_flags = 0;  // ints: 3 classes x 2 tickets, abort, pad, then 2 per boundary
+  int n_flags = 0;  // ints: NCLS classes x 2 tickets, abort, pad, then 2 per boundary
   size_t total() const { return desc_b + pm_b + lat_b + ring_b + flag_b + pinfo_b; }
 };
 
@@ -688,12 +770,14 @@ ChainPlan plan_chain(const tkfmi_problem* p, const Plan& pl, uint32_t flags) {
     ap += (int64_t)(p->h_x_len[b] + 1) * (p->h_y_len[b] + 1) * p->S;
   }
   int64_t aring = 0;
-  int nfl = 8;
+  int nfl = 2 * NCLS + 8;
+  const int64_t cells_max = B > 0 ? (int64_t)(p->h_x_len[order[0]] + 1) * (p->h_y_len[order[0]] + 1) : 0;
   for (int q = 0; q < B; ++q) {
     const int b = order[q];
     const int Lx = p->h_x_len[b], Ly = p->h_y_len[b];
     const int nb = (Lx + 64) / 64;
-    const int c = ccls_of(nb);
+    int c = ccls_of(nb);
+    if (c == 1 && (double)(Lx + 1) * (Ly + 1) >= long_frac() * (double)cells_max) c = 0;
     const int W = kCW[c];
     const int G = (nb + W - 1) / W;
     ++cp.cls_pairs[c];
@@ -735,7 +819,7 @@ ChainPlan plan_chain(const tkfmi_problem* p, const Plan& pl, uint32_t flags) {
     }
   }
   size_t nd = 0;
-  for (int c = 0; c < 3; ++c) nd += cp.fwd[c].size() + cp.bwd[c].size();
+  for (int c = 0; c < NCLS; ++c) nd += cp.fwd[c].size() + cp.bwd[c].size();
   cp.desc_b = align256(sizeof(CDesc) * std::max<size_t>(nd, 1));
   cp.pm_b = align256(stride_for(pl.NB) * 8 * (size_t)p->n_trans);
   cp.lat_b = align256((size_t)al * 8);
@@ -743,7 +827,7 @@ ChainPlan plan_chain(const tkfmi_problem* p, const Plan& pl, uint32_t flags) {
   cp.n_flags = nfl;
   cp.flag_b = align256((size_t)nfl * 4);
   cp.pinfo_b = align256((size_t)std::max(B, 1) * 16);
-  for (int c = 0; c < 3; ++c)
+  for (int c = 0; c < NCLS; ++c)
     if (cp.cls_pairs[c] && CLds<LMix<3, 1>, 4>::total(true, cp.max_ly[c]) > 160 * 1024 - 512) {
       cp.why = "LDS budget exceeded (sequences too long)";
       return cp;
@@ -753,16 +837,16 @@ ChainPlan plan_chain(const tkfmi_problem* p, const Plan& pl, uint32_t flags) {
 }
 
 // Workgroups per CU that each class's launches leave room for (through their
-// LDS request); 0 = as many as fit.  TKFMI_CHAIN_WPC="f4,b4,f2,b2,f1,b1".
-void chain_wpc(int wpc[3][2]) {
-  static int v[6] = {0, 0, 0, 0, 0, 0};
+// LDS request); 0 = as many as fit.  TKFMI_CHAIN_WPC="f0,b0,f1,b1,f2,b2,f3,b3".
+void chain_wpc(int wpc[NCLS][2]) {
+  static int v[2 * NCLS] = {0, 0, 0, 0, 0, 0, 0, 0};
   static bool init = false;
   if (!init) {
     const char* e = getenv("TKFMI_CHAIN_WPC");
-    if (e) sscanf(e, "%d,%d,%d,%d,%d,%d", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5]);
+    if (e) sscanf(e, "%d,%d,%d,%d,%d,%d,%d,%d", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7]);
     init = true;
   }
-  for (int c = 0; c < 3; ++c) { wpc[c][0] = v[2 * c]; wpc[c][1] = v[2 * c + 1]; }
+  for (int c = 0; c < NCLS; ++c) { wpc[c][0] = v[2 * c]; wpc[c][1] = v[2 * c + 1]; }
 }
 
 template <class L, int W>
@@ -850,8 +934,8 @@ int tkfmi_chain_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, si
   double* pinfo = reinterpret_cast<double*>(wsb + cp.desc_b + cp.pm_b + cp.lat_b + cp.ring_b + cp.flag_b);
   // descriptors: [class][fwd | bwd]
   std::vector<CDesc> all;
-  size_t base[3][2];
-  for (int c = 0; c < 3; ++c) {
+  size_t base[NCLS][2];
+  for (int c = 0; c < NCLS; ++c) {
     base[c][0] = all.size();
     all.insert(all.end(), cp.fwd[c].begin(), cp.fwd[c].end());
     base[c][1] = all.size();
@@ -873,16 +957,16 @@ int tkfmi_chain_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, si
   prm.S = p->S; prm.flags = (int32_t)flags; prm.n_tab = p->n_tab; prm.G_tab = p->n_groups;
   for (int g = 0; g < 16; ++g) prm.gmap[g] = pl.gmap[g];
   prm.log_tab = p->log_tab; prm.x_cat = p->x_cat; prm.y_cat = p->y_cat;
-  prm.lattice = lattice; prm.xring = xring; prm.flagv = flagv; prm.abortf = flagv + 6;
+  prm.lattice = lattice; prm.xring = xring; prm.flagv = flagv; prm.abortf = flagv + 2 * NCLS;
   prm.pinfo = pinfo;
   prm.out_logP = o->log_prob;
   prm.out_n = (flags & TKFMI_NO_COUNTS) ? nullptr : o->n_trans;
   prm.out_post = o->posteriors;
   prm.out_ecnt = (flags & TKFMI_WANT_EMIT_COUNTS) ? o->emit_counts : nullptr;
-  int wpc[3][2];
+  int wpc[NCLS][2];
   chain_wpc(wpc);
   const bool fwd_only = (flags & TKFMI_FORWARD_ONLY) != 0;
-  for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < NCLS; ++c) {
     if (!cp.cls_pairs[c]) continue;
     hipStream_t s = sts->s[c];
     if (hipStreamWaitEvent(s, sts->start, 0) != hipSuccess) { err = "hipStreamWaitEvent failed"; return TKFMI_EHIP; }
@@ -891,7 +975,7 @@ int tkfmi_chain_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, si
       pc.ticket = flagv + 2 * c + dir;
       const int n = (int)(dir ? cp.bwd[c].size() : cp.fwd[c].size());
       const std::string kname = std::string(dir ? "fbc_bwd_kernel<" : "fbc_fwd_kernel<") + layout_name(pl) +
-                                "," + std::to_string(kCW[c]) + ">";
+                                "," + std::to_string(kCW[c]) + ">" + (c == 0 ? " long" : "");
       const int slot = tkfmi_ktime_start(kname.c_str(), cp.cls_pairs[c], cp.cls_cells[c], s, err);
       if (slot == -2) return TKFMI_EHIP;
       rc = dispatch_chain(pl, c, dir == 1, pc, pmat, ddesc + base[c][dir], n, cp.max_ly[c], wpc[c][dir], s, err);
