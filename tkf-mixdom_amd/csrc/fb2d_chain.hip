@@ -210,6 +210,15 @@ __global__ __launch_bounds__(W * 64) void fbc_fwd_kernel(CParams Pr, const doubl
 #pragma unroll
   for (int m = 0; m < nM; ++m) rM_last[m] = 0.0;
   int eH = EXP_NONE, eM_last = EXP_NONE;
+  // The lane's row, hence x and its D emissions, is fixed; the column's y
+  // moves one lane down per step with the cells (DPP), lane 0 reading the
+  // next one from LDS a step ahead.  One LDS round trip per step remains.
+  const int xi = s_x[64 * w + lane];
+  double emD[NG];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) emD[g] = s_tab[g * TAB_SZ + A_PAD * A_PAD + A_PAD + xi];
+  int yj_prev = 0;
+  int y0_next = s_y[0];
   const int T = pd.T;
   // A wave's band is in range for steps 64 w .. 64 w + 63 + Ly; outside
   // them every lane is invalid and the wave only keeps the barrier (its
@@ -246,14 +255,13 @@ __global__ __launch_bounds__(W * 64) void fbc_fwd_kernel(CParams Pr, const doubl
         eR = (int)e[NX - 1];
       }
       // ---- emissions at (i, j)
-      const int xi = valid ? s_x[i - 64 * pd.band0] : 0;
-      const int yj = valid ? s_y[j >= 1 ? j - 1 : 0] : 0;
-      double emM[NG], emI[NG], emD[NG];
+      int yj = dpp_i32<DPP_WAVE_SHR1>(yj_prev);
+      if (lane == 0) yj = y0_next;
+      double emM[NG], emI[NG];
 #pragma unroll
       for (int g = 0; g < NG; ++g) {
         emM[g] = s_tab[g * TAB_SZ + xi * A_PAD + yj];
         emI[g] = s_tab[g * TAB_SZ + A_PAD * A_PAD + yj];
-        emD[g] = s_tab[g * TAB_SZ + A_PAD * A_PAD + A_PAD + xi];
       }
       const bool okM = valid && i >= 1 && j >= 1;
       const bool okI = valid && j >= 1;
@@ -345,6 +353,8 @@ __global__ __launch_bounds__(W * 64) void fbc_fwd_kernel(CParams Pr, const doubl
       for (int m = 0; m < nM; ++m) rM_last[m] = rM[m];
       eH = E;
       eM_last = eR;
+      yj_prev = yj;
+      y0_next = s_y[min(max(t - 64 * w, 0), Ly)];  // lane 0's y_{j-1} of the next step
     }
     lds_barrier();
   }
@@ -462,18 +472,20 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
 #pragma unroll
   for (int m = 0; m < nM; ++m) rM_last[m] = 0.0;
   int eQ = EXP_NONE, eM_last = EXP_NONE;
+  // x fixed per lane; y passes up one lane per step (lane 63 reads ahead).
+  const int xi = s_x[64 * (NW - 1 - w) + lane];
+  double emD[NG];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) emD[g] = s_tab[g * TAB_SZ + A_PAD * A_PAD + A_PAD + xi];
+  int yj_prev = 0;
+  int y63_next = s_y[max(Ly - 1, 0)];
   const int off = 64 * w + (63 - lane);  // position p = t - off, column j = Ly - p
+  // Lattice slot of this lane's cell at position p; an invalid cell reads
+  // the pair's first slot (inside the allocation) and is zeroed after.
   auto f_addr = [&](int p, int i) -> const double* {
     const bool v = has_band && p >= 0 && p <= Ly && i <= Lx;
-    return v ? lat + (((size_t)band * BSTRIDE + (Ly - p + lane)) * SLOTS) * 64 + lane : nullptr;
+    return lat + (v ? (((size_t)band * BSTRIDE + (Ly - p + lane)) * SLOTS) * 64 : 0) + lane;
   };
-  double fnext[SLOTS];
-  {
-    const double* a = f_addr(64 * w - off, i_band);  // the first active step
-#pragma unroll
-    for (int b = 0; b < NB; ++b) fnext[b] = a ? a[b * 64] : 0.0;
-    fnext[SLOTS - 1] = a ? a[(SLOTS - 1) * 64] : 0.0;
-  }
   const int T = pd.T;
   // A wave's band is in range for steps 64 w .. 64 w + 63 + Ly; outside
   // them every lane is invalid and the wave only keeps the barrier (its
@@ -486,15 +498,14 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
       const int p = t - off;
       const int j = Ly - p;
       const bool valid = p >= 0 && p <= Ly && i <= Lx;
+      // The cell's forward values: loaded here, first used after the
+      // transition contraction, which covers the load latency.
       double fcur[SLOTS];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) fcur[b] = fnext[b];
-      fcur[SLOTS - 1] = fnext[SLOTS - 1];
       {
-        const double* a = f_addr(p + 1, i);
+        const double* a = f_addr(p, i);
 #pragma unroll
-        for (int b = 0; b < NB; ++b) fnext[b] = a ? a[b * 64] : 0.0;
-        fnext[SLOTS - 1] = a ? a[(SLOTS - 1) * 64] : 0.0;
+        for (int b = 0; b < NB; ++b) fcur[b] = a[b * 64];
+        fcur[SLOTS - 1] = a[(SLOTS - 1) * 64];
       }
       // ---- boundary row of the next workgroup (below): fetch the next chunk
       if (xin && t <= Ly && (t % CQ) == 0) {
@@ -559,6 +570,8 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
 #pragma unroll
       for (int k = 0; k < NB; ++k) Bv[k] = ldexp(Bv[k], -sh);
       // ---- counts: stage F-scaled and g, then MFMA (K = 4 cells per op)
+#pragma unroll
+      for (int b = 0; b < SLOTS; ++b) fcur[b] = valid ? fcur[b] : 0.0;
       const int EF = (int)fcur[SLOTS - 1];
       const double fscale = valid ? ldexp(inv_mP, EF + Eg - EP) : 0.0;
 #pragma unroll
@@ -574,8 +587,14 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
           for (int b = 0; b < NB; ++b) out_n[b + L::SOFF] = p_start[b] * g[b] * sc;
         }
       }
-      const int xi = valid ? s_x[i - 64 * pd.band0] : 0;
-      const int yj = valid ? s_y[j >= 1 ? j - 1 : 0] : 0;
+      int yj = dpp_i32<DPP_WAVE_SHL1>(yj_prev);
+      if (lane == 63) yj = y63_next;
+      double emM[NG], emI[NG];
+#pragma unroll
+      for (int g = 0; g < NG; ++g) {
+        emM[g] = s_tab[g * TAB_SZ + xi * A_PAD + yj];
+        emI[g] = s_tab[g * TAB_SZ + A_PAD * A_PAD + yj];
+      }
       // ---- expected emission counts: posterior mass per (group, type)
       // M-type mass goes to its (x, y) entry with an LDS atomic; D-type mass
       // accumulates per lane (the row, hence x, is fixed); I-type mass rides
@@ -632,9 +651,9 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
         const int ty = L::type(b), gq = L::group(b);
         double em;
         bool ok;
-        if (ty == ST_M) { em = s_tab[gq * TAB_SZ + xi * A_PAD + yj]; ok = eM_ok; }
-        else if (ty == ST_I) { em = s_tab[gq * TAB_SZ + A_PAD * A_PAD + yj]; ok = eI_ok; }
-        else { em = s_tab[gq * TAB_SZ + A_PAD * A_PAD + A_PAD + xi]; ok = eD_ok; }
+        if (ty == ST_M) { em = emM[gq]; ok = eM_ok; }
+        else if (ty == ST_I) { em = emI[gq]; ok = eI_ok; }
+        else { em = emD[gq]; ok = eD_ok; }
         q[b] = ok ? em * Bv[b] : 0.0;
       }
       // ---- publish this row's q for the band above (lane 0)
@@ -657,12 +676,13 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
       for (int m = 0; m < nM; ++m) rM_last[m] = rM[m];
       eQ = EB;
       eM_last = eR;
+      yj_prev = yj;
+      y63_next = s_y[min(max(Ly - (t + 1 - 64 * w) - 1, 0), Ly)];  // lane 63's y_{j-1} of the next step
     }
     lds_barrier();
   }
   if (has_band) count_mfma(stage_f, stage_g, lane, cacc, cacc2);
   if (has_band && want_ecnt && i_band >= 1 && i_band <= Lx) {
-    const int xi = s_x[i_band - 64 * pd.band0];
 #pragma unroll
     for (int gq = 0; gq < NG; ++gq)
       if (accD[gq] != 0.0) atomicAdd(s_ecnt + gq * TAB_SZ + A_PAD * A_PAD + A_PAD + xi, accD[gq]);
