@@ -86,9 +86,11 @@ __global__ void prep_kernel(const double* __restrict__ log_trans, int S, int n_t
 }
 
 // --------------------------------------------------------------- helpers
+// DPP lane shift; a lane whose source is outside the wave reads 0
+// (bound_ctrl), so no zero-initialised destination is needed.
 template <int CTRL>
 __device__ __forceinline__ int dpp_i32(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, true);
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v) {

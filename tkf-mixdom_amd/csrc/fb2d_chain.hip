@@ -43,6 +43,14 @@
 namespace tkfmi_v2 {
 
 constexpr int CQ = 16;                 // boundary entries per published / fetched chunk
+#ifndef TKFMI_COUNT_DEFER
+#define TKFMI_COUNT_DEFER 1
+#endif
+// Backward count staging: 1 = one step deep (64 cells per wave, no LDS round
+// trip inside a step, 16 KB per wave); 0 = two 32-cell halves counted in the
+// step (8 KB per wave, so that two 4-wave workgroups fit in a compute unit).
+constexpr bool kCountDefer = TKFMI_COUNT_DEFER != 0;
+constexpr int kStageDoubles = kCountDefer ? 2048 : 1024;
 constexpr int kSpinMax = 1 << 22;      // bounded waits (each spin sleeps ~0.2 us + a load)
 
 struct CDesc {
@@ -126,7 +134,7 @@ struct CLds {
   static constexpr int ring(bool bwd) { return ecnt + (bwd ? L::NG * TAB_SZ : 0); }
   static constexpr int xc(bool bwd) { return ring(bwd) + (W - 1) * 2 * NX; }
   static constexpr int stage(bool bwd) { return ((xc(bwd) + CQ * NX) + 1) & ~1; }
-  static constexpr int seq_bytes(bool bwd) { return 8 * (stage(bwd) + (bwd ? W * 2048 : 0)); }
+  static constexpr int seq_bytes(bool bwd) { return 8 * (stage(bwd) + (bwd ? W * kStageDoubles : 0)); }
   static constexpr int xbytes = 64 * W + 8;
   static size_t total(bool bwd, int max_ly) { return (size_t)seq_bytes(bwd) + xbytes + ((max_ly + 8) & ~7) + 16; }
 };
@@ -263,19 +271,19 @@ __global__ __launch_bounds__(W * 64) void fbc_fwd_kernel(CParams Pr, const doubl
         emM[g] = s_tab[g * TAB_SZ + xi * A_PAD + yj];
         emI[g] = s_tab[g * TAB_SZ + A_PAD * A_PAD + yj];
       }
-      const bool okM = valid && i >= 1 && j >= 1;
-      const bool okI = valid && j >= 1;
-      const bool okD = valid && i >= 1;
-      // ---- new cell: raw products and per-type maxima
+      // ---- new cell: raw products and per-type maxima.  No validity masks:
+      // a valid cell's out-of-lattice predecessors (row -1: DPP zero fill;
+      // column -1: cells of negative j, computed from zeros) hold exactly 0,
+      // and invalid cells past Lx or Ly feed only invalid cells.
       double f[NB];
       double mxM = 0.0, mxI = 0.0, mxD = 0.0;
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
         const int ty = L::type(b), g = L::group(b);
         double v;
-        if (ty == ST_M) { v = okM ? rM_last[rank_in_type<L, ST_M>(b)] * emM[g] : 0.0; mxM = fmax(mxM, v); }
-        else if (ty == ST_I) { v = okI ? hp[b] * emI[g] : 0.0; mxI = fmax(mxI, v); }
-        else { v = okD ? rD[rank_in_type<L, ST_D>(b)] * emD[g] : 0.0; mxD = fmax(mxD, v); }
+        if (ty == ST_M) { v = rM_last[rank_in_type<L, ST_M>(b)] * emM[g]; mxM = fmax(mxM, v); }
+        else if (ty == ST_I) { v = hp[b] * emI[g]; mxI = fmax(mxI, v); }
+        else { v = rD[rank_in_type<L, ST_D>(b)] * emD[g]; mxD = fmax(mxD, v); }
         f[b] = v;
       }
       const int cM = (mxM > 0.0) ? eM_last + frexp_exp(mxM) : EXP_NONE;
@@ -290,12 +298,7 @@ __global__ __launch_bounds__(W * 64) void fbc_fwd_kernel(CParams Pr, const doubl
         const int ty = L::type(b);
         F[b] = ldexp(f[b], ty == ST_M ? shM : (ty == ST_I ? shI : shD));
       }
-      const bool origin = valid && i == 0 && j == 0;
-      if (origin) {
-#pragma unroll
-        for (int b = 0; b < NB; ++b) F[b] = 0.0;
-        E = 0;
-      }
+      const bool origin = valid && i == 0 && j == 0;  // F = 0, E = 0 there already
       // ---- store the forward lattice (band-local step u = j + lane)
       if (want_lat && valid) {
         double* dst = lat + (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + lane;
@@ -308,9 +311,11 @@ __global__ __launch_bounds__(W * 64) void fbc_fwd_kernel(CParams Pr, const doubl
 #pragma unroll
       for (int k = 0; k < NB; ++k) h[k] = 0.0;
       row_contract<NB>(h, prow, F);
-      if (origin) {
+      if (pd.band0 == 0 && w == 0 && t == 0) {  // the origin's step (wave-uniform test)
+        if (lane == 0) {
 #pragma unroll
-        for (int k = 0; k < NB; ++k) h[k] = p_start[k];
+          for (int k = 0; k < NB; ++k) h[k] = p_start[k];
+        }
       }
       // ---- corner: log P, its mantissa / exponent for the backward, the E column
       if (valid && i == Lx && j == Ly) {
@@ -364,12 +369,13 @@ __global__ __launch_bounds__(W * 64) void fbc_fwd_kernel(CParams Pr, const doubl
   }
 }
 
-// Expected-count MFMAs over 64 staged cells: acc (+acc2) += F^T g, K = 4
+// Expected-count MFMAs over 4 NQ staged cells: acc (+acc2) += F^T g, K = 4
 // cells per v_mfma_f64_16x16x4f64, alternating two accumulator chains.
+template <int NQ>
 __device__ __forceinline__ void count_mfma(const double* stage_f, const double* stage_g, int lane, d4& acc,
                                            d4& acc2) {
 #pragma unroll
-  for (int q = 0; q < 16; q += 2) {
+  for (int q = 0; q < NQ; q += 2) {
     const int c0 = 4 * q + (lane >> 4), c1 = c0 + 4;
     const int o0 = c0 * 16 + ((lane & 15) ^ (c0 & 15));
     const int o1 = c1 * 16 + ((lane & 15) ^ (c1 & 15));
@@ -455,13 +461,13 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
   // stages its cells of step t in the same buffer.  A wave's LDS operations execute in issue order, so the
   // reads of step t-1's cells precede their overwrite without a wait, and no
   // write -> read round trip sits inside a step.
-  double* stage_f = s_stage + (size_t)w * 2048;  // [64 cells][16] F-scaled
-  double* stage_g = stage_f + 1024;              // [64 cells][16] g
+  double* stage_f = s_stage + (size_t)w * kStageDoubles;  // [64 | 32 cells][16] F-scaled
+  double* stage_g = stage_f + kStageDoubles / 2;          // [64 | 32 cells][16] g
   double accD[NG];   // D-type emission mass of this lane's row (x fixed per lane)
   double runI[NG];   // I-type mass of the column passing up through the lanes
 #pragma unroll
   for (int gq = 0; gq < NG; ++gq) { accD[gq] = 0.0; runI[gq] = 0.0; }
-  for (int q = lane; q < 2048; q += 64) stage_f[q] = 0.0;  // step 0 counts an empty stage
+  for (int q = lane; q < kStageDoubles; q += 64) stage_f[q] = 0.0;  // step 0 counts an empty stage
   double* post = want_post ? Pr.out_post + pd.post_off : nullptr;
   __syncthreads();
 
@@ -491,6 +497,10 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
   // them every lane is invalid and the wave only keeps the barrier (its
   // carried values are zero there, as the skipped steps would leave them).
   const int t_on = 64 * w, t_off = 64 * w + 63 + Ly;
+  // The corner (Lx, Ly) is in wave 0 of the chain's last workgroup, lane
+  // Lx - 64 band, at position 0.
+  const bool corner_wave = has_band && band == (Lx + 64) / 64 - 1;
+  const int t_corner = 64 * w + 63 - (Lx - 64 * band);
   for (int t = 0; t < T; ++t) {
     if (has_band && t >= t_on && t <= t_off) {
       int i = i_band;
@@ -554,13 +564,16 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
       double Bv[NB];
 #pragma unroll
       for (int k = 0; k < NB; ++k) Bv[k] = 0.0;
-      contract_and_count<NB>(Bv, prowT, g, stage_f, stage_g, lane, cacc, cacc2);
+      if constexpr (kCountDefer) contract_and_count<NB>(Bv, prowT, g, stage_f, stage_g, lane, cacc, cacc2);
+      else row_contract<NB>(Bv, prowT, g);
       const bool corner = valid && i == Lx && j == Ly;
       int EBraw = Eg;
-      if (corner) {
+      if (corner_wave && t == t_corner) {  // the corner's step (wave-uniform test)
+        if (corner) {
 #pragma unroll
-        for (int k = 0; k < NB; ++k) Bv[k] = p_end[k];
-        EBraw = 0;
+          for (int k = 0; k < NB; ++k) Bv[k] = p_end[k];
+          EBraw = 0;
+        }
       }
       double mx = 0.0;
 #pragma unroll
@@ -574,11 +587,29 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
       for (int b = 0; b < SLOTS; ++b) fcur[b] = valid ? fcur[b] : 0.0;
       const int EF = (int)fcur[SLOTS - 1];
       const double fscale = valid ? ldexp(inv_mP, EF + Eg - EP) : 0.0;
+      if constexpr (kCountDefer) {
 #pragma unroll
-      for (int s = 0; s < 16; ++s) {  // stage this step's cells (counted next step)
-        const int o = lane * 16 + (s ^ (lane & 15));
-        stage_f[o] = (s < NB) ? fcur[s < NB ? s : 0] * fscale : 0.0;
-        stage_g[o] = (s < NB) ? g[s < NB ? s : 0] : 0.0;
+        for (int s = 0; s < 16; ++s) {  // stage this step's cells (counted next step)
+          const int o = lane * 16 + (s ^ (lane & 15));
+          stage_f[o] = (s < NB) ? fcur[s < NB ? s : 0] * fscale : 0.0;
+          stage_g[o] = (s < NB) ? g[s < NB ? s : 0] : 0.0;
+        }
+      } else {
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {  // stage 32 cells, count them now
+          if ((lane >> 5) == half) {
+            const int c = lane & 31;
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+              const int o = c * 16 + (s ^ (c & 15));
+              stage_f[o] = (s < NB) ? fcur[s < NB ? s : 0] * fscale : 0.0;
+              stage_g[o] = (s < NB) ? g[s < NB ? s : 0] : 0.0;
+            }
+          }
+          wave_lds_sync();
+          count_mfma<8>(stage_f, stage_g, lane, cacc, cacc2);
+          wave_lds_sync();
+        }
       }
       if (want_counts) {
         if (valid && i == 0 && j == 0) {  // start state row: n[S, k]
@@ -681,7 +712,7 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
     }
     lds_barrier();
   }
-  if (has_band) count_mfma(stage_f, stage_g, lane, cacc, cacc2);
+  if (kCountDefer && has_band) count_mfma<16>(stage_f, stage_g, lane, cacc, cacc2);
   if (has_band && want_ecnt && i_band >= 1 && i_band <= Lx) {
 #pragma unroll
     for (int gq = 0; gq < NG; ++gq)
