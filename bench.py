@@ -7,11 +7,12 @@ round(LogNormal(ln 200, 0.5)) clipped to [30, 800], t ~ U(0.05, 2)) with all
 inputs already resident in HBM.  Prints one JSON line.
 
   value       = sum_pairs (Lx+1)(Ly+1) S / step time         (cell-states/s)
-  roofline    = HBM-bound: 16 B of lattice traffic per cell-state
-                (write + read of the forward lattice) of the pairs in the
-                dominant kernel launch (the longest of the per-wave-count
-                class launches), divided by that launch's HIP-event duration
-                on its own stream, against 8 TB/s
+  roofline    = HBM-bound: lattice traffic per cell-state of the pairs in
+                the dominant kernel launch (the longest of the per-class
+                launches: 8 B for a forward launch, which writes the
+                lattice, or a backward launch, which reads it; 16 B for a
+                one-workgroup-per-pair launch doing both), divided by that
+                launch's HIP-event duration on its own stream, against 8 TB/s
   cpu_baseline= the C restatement of the reference recursion (log space,
                 OpenMP over pairs) on a bounded sample of the same pairs
   plain       = the per-pair drop-in (one launch per pair, as the reference's
@@ -39,8 +40,12 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_STATE = 16.0
 # HBM bytes per launch, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
 # workload (KiB per dispatch, scaled by the copy-kernel calibration: fetch x2.00,
-# write x1.00); profiles/fbv2_kernel_c1_mixdom2_k3.md.
-TRAFFIC = {"fbv2_kernel<LMix<3,1>,8>": 3.13e9, "fbv2_kernel<LMix<3,1>,4>": 8.28e9,
+# write x1.00); profiles/fbc_kernels_c1_mixdom2_k3.md (chained kernels) and
+# profiles/fbv2_kernel_c1_mixdom2_k3.md (one-workgroup-per-pair kernel).
+TRAFFIC = {"fbc_bwd_kernel<LMix<3,1>,4>": 5.27e9, "fbc_fwd_kernel<LMix<3,1>,4>": 4.93e9,
+           "fbc_bwd_kernel<LMix<3,1>,4> long": 2.26e9, "fbc_fwd_kernel<LMix<3,1>,4> long": 2.18e9,
+           "fbc_fwd_kernel<LMix<3,1>,2>": 1.26e9, "fbc_bwd_kernel<LMix<3,1>,2>": 1.44e9,
+           "fbv2_kernel<LMix<3,1>,8>": 3.13e9, "fbv2_kernel<LMix<3,1>,4>": 8.28e9,
            "fbv2_kernel<LMix<3,1>,2>": 5.14e9, "fbv2_kernel<LMix<3,1>,1>": 0.665e9}
 
 
