@@ -332,6 +332,15 @@ def main():
         runner.run()
     torch.cuda.synchronize()
     step_ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    # the same call captured once into a HIP graph and replayed (no host work)
+    graph = runner.capture()
+    graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    graph_ms = 1e3 * (time.perf_counter() - t0) / args.steps
     logp = runner.log_p.cpu().numpy()
     n = runner.n.cpu().numpy()
     assert np.all(np.isfinite(logp)), "non-finite log-likelihood"
@@ -358,7 +367,8 @@ def main():
                                "transition and emission counts, 1024 uneven pairs, "
                                "LogNormal(ln200,0.5) lengths in [30,800], t~U(0.05,2), one call",
                    "pairs": w.B, "states": w.S, "cell_states": cell_states,
-                   "emission_counts": not args.no_emit,
+                   "emission_counts": not args.no_emit, "kernel": args.kernel,
+                   "graph_replay_ms_per_step": graph_ms,
                    "sum_logp": float(logp.sum()), "sum_counts": float(n.sum())},
         "roofline": {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dom["achieved_GBs"] / HBM_PEAK_GBS,

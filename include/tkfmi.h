@@ -47,6 +47,12 @@ extern "C" {
 #define TKFMI_TIME_KERNELS 32u   /* record per-kernel HIP-event timings    */
 #define TKFMI_CLASS_KERNEL 64u   /* force the one-workgroup-per-pair
                                     cell-per-lane kernel (see below)        */
+#define TKFMI_REUSE_DESCRIPTORS 128u /* the workspace already holds this
+                                    problem's descriptors from an earlier
+                                    call with the same problem, flags and
+                                    workspace: skip their host-to-device
+                                    upload, so the call is only device
+                                    work (e.g. captured into a HIP graph)  */
 
 /* emission tables: per group g, 21x21 match block then 21 insert then 21
  * delete entries (index 20 = wildcard), all natural-log values. */

@@ -55,7 +55,7 @@ def test_struct_layout_matches_header():
             fields += [re.sub(r"[\s*]|\[\w+\]", "", x) for x in parts[1:]]
         assert fields == [f[0] for f in py._fields_], cname
     flags = dict(re.findall(r"#define TKFMI_(\w+) (\d+)u", text))
-    for k in ("FORWARD_ONLY", "WANT_POSTERIORS", "NO_COUNTS", "GENERIC_KERNEL", "CLASS_KERNEL", "WANT_EMIT_COUNTS",
+    for k in ("FORWARD_ONLY", "WANT_POSTERIORS", "NO_COUNTS", "GENERIC_KERNEL", "CLASS_KERNEL", "REUSE_DESCRIPTORS", "WANT_EMIT_COUNTS",
               "TIME_KERNELS"):
         assert int(flags[k]) == getattr(_abi, k)
 

@@ -768,7 +768,7 @@ int tkfmi_fb2d_batched(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws,
   double* rowbuf = reinterpret_cast<double*>(wsb + pl.desc_bytes + align_up(pl.lat_doubles * 8, 256));
   {
     std::string err;
-    const int rcu = tkfmi_stage_upload(ddesc, hd.data(), dbytes, stream, err);
+    const int rcu = (flags & TKFMI_REUSE_DESCRIPTORS) ? TKFMI_OK : tkfmi_stage_upload(ddesc, hd.data(), dbytes, stream, err);
     if (rcu != TKFMI_OK) return fail(rcu, err);
   }
 

@@ -995,7 +995,7 @@ int tkfmi_chain_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, si
   Streams* sts = nullptr;
   int rc = get_streams(sts, err);
   if (rc != TKFMI_OK) return rc;
-  rc = tkfmi_stage_upload(ddesc, all.data(), sizeof(CDesc) * all.size(), stream, err);
+  rc = (flags & TKFMI_REUSE_DESCRIPTORS) ? TKFMI_OK : tkfmi_stage_upload(ddesc, all.data(), sizeof(CDesc) * all.size(), stream, err);
   if (rc != TKFMI_OK) return rc;
   hipLaunchKernelGGL(zero_ints_kernel, dim3(std::min(1024, (cp.n_flags + 255) / 256)), dim3(256), 0, stream,
                      flagv, cp.n_flags);

@@ -825,7 +825,7 @@ int tkfmi_v2_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, size_
   Streams* sts = nullptr;
   int rc = get_streams(sts, err);
   if (rc != TKFMI_OK) return rc;
-  rc = tkfmi_stage_upload(ddesc, cp.desc.data(), sizeof(V2Desc) * (size_t)B, stream, err);
+  rc = (flags & TKFMI_REUSE_DESCRIPTORS) ? TKFMI_OK : tkfmi_stage_upload(ddesc, cp.desc.data(), sizeof(V2Desc) * (size_t)B, stream, err);
   if (rc != TKFMI_OK) return rc;
   switch (pl.lay) {
     case Layout::Tkf: rc = launch_prep<LTkf>(p, pmat, stream, err); break;

@@ -539,7 +539,7 @@ int tkfmi_wide_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
   memcpy(blob.data() + pl.NBP * 4, pl.btype.data(), pl.NBP * 4);
   memcpy(blob.data() + 2 * pl.NBP * 4, pl.bgrp.data(), pl.NBP * 4);
   memcpy(blob.data() + pl.idx_b, pl.desc.data(), sizeof(WDesc) * B);
-  int rc = tkfmi_stage_upload(wsb, blob.data(), blob.size(), stream, err);
+  int rc = (flags & TKFMI_REUSE_DESCRIPTORS) ? TKFMI_OK : tkfmi_stage_upload(wsb, blob.data(), blob.size(), stream, err);
   if (rc != TKFMI_OK) return rc;
   const int32_t* d_idx = reinterpret_cast<const int32_t*>(wsb);
   double* wsd = reinterpret_cast<double*>(wsb);

@@ -22,6 +22,7 @@ GENERIC_KERNEL = 8
 WANT_EMIT_COUNTS = 16
 TIME_KERNELS = 32
 CLASS_KERNEL = 64
+REUSE_DESCRIPTORS = 128
 A_PAD = 21
 TAB_SZ = A_PAD * A_PAD + 2 * A_PAD
 
@@ -368,6 +369,34 @@ class BatchRunner:
         if rc != 0:
             raise TkfmiError(self.L.tkfmi_last_error().decode() or f"tkfmi error {rc}")
         return self.log_p, self.n, self.post
+
+    def capture(self, zero_outputs=True):
+        """Capture one call into a HIP graph and return it (a
+        ``torch.cuda.CUDAGraph``; ``graph.replay()`` reruns the call with no
+        host work).  The graph holds the output zeroing (torch fill kernels,
+        no memset nodes) and the library's launches on its class streams,
+        which join the capture through events.  The descriptors are uploaded
+        by an ordinary call first; the captured call reuses them
+        (REUSE_DESCRIPTORS), so the runner's inputs must not change shape
+        while the graph is in use."""
+        if self.flags & TIME_KERNELS:
+            raise TkfmiError("per-kernel timing cannot be captured")
+        self.run(zero_outputs=zero_outputs)
+        torch.cuda.synchronize(self.dev)
+        graph = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream(self.dev)
+        flags = self.flags | REUSE_DESCRIPTORS
+        with torch.cuda.graph(graph, stream=side, capture_error_mode="relaxed"):
+            if zero_outputs:
+                for t in (self.n, self.post, self.ecnt):
+                    if t is not None:
+                        t.zero_()
+            rc = self.L.tkfmi_fb2d_batched(ctypes.byref(self.prob), ctypes.byref(self.out),
+                                           _ptr(self.ws), ctypes.c_size_t(self.ws_bytes),
+                                           ctypes.c_void_p(side.cuda_stream), flags)
+        if rc != 0:
+            raise TkfmiError(self.L.tkfmi_last_error().decode() or f"tkfmi error {rc}")
+        return graph
 
     def results(self):
         out = {"log_p": self.log_p}
