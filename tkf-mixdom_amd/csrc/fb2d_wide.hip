@@ -118,32 +118,71 @@ __device__ inline double emit(const double* tab, int ty, int g, int xi, int yj, 
 // C[cells x NBP] = A[cells x NBP] . Bm[NBP x NBP] for the rows i_lo..i_hi of
 // an antidiagonal (A and C row-indexed by i, row length NBP); the result
 // for row i_lo is replaced by `fix` when it is not null.
+//
+// A task is a block of RB x CT output tiles (16 rows x 16 columns each):
+// every A fragment feeds CT MFMAs and every Bm fragment RB MFMAs (0.75 loads
+// per v_mfma_f64_16x16x4f64 at 4 x 2, against 2 for single tiles), and the
+// RB x CT accumulators are independent chains.  Bm (the pair's transition
+// block) does not stay in L2 next to the other pairs' blocks, so reusing its
+// fragments is what matters.  Row blocks past the antidiagonal are skipped
+// wave-uniformly.
 template <int NT>
 __device__ inline void gemm_rows(const double* __restrict__ A, const double* __restrict__ Bm,
                                  double* __restrict__ Cm, int i_lo, int i_hi, const double* fix) {
   constexpr int NBP = 16 * NT;
+  constexpr int CT = (NT % 2 == 0 && NT <= 6) ? 2 : 1;  // registers: the backward keeps its count tiles live
+  constexpr int RB = 4;
+  constexpr int NCG = NT / CT;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int n = i_hi - i_lo + 1;
   const int nrb = (n + 15) >> 4;
-  for (int it = w; it < nrb * NT; it += kWaves) {
-    const int rb = it / NT, ct = it - rb * NT;
-    const int ra = i_lo + 16 * rb + (lane & 15);
-    const bool va = ra <= i_hi;
-    const double* ap = A + (size_t)(va ? ra : i_lo) * NBP + (lane >> 4);
-    const double* bp = Bm + (size_t)(lane >> 4) * NBP + 16 * ct + (lane & 15);
-    d4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 8
+  const int nrg = (nrb + RB - 1) / RB;
+  for (int task = w; task < nrg * NCG; task += kWaves) {
+    const int rg = task / NCG, cg = task - rg * NCG;
+    const int rb0 = rg * RB;
+    const int nrb_here = min(RB, nrb - rb0);  // wave-uniform
+    const double* ap[RB];
+    bool va[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int ra = i_lo + 16 * (rb0 + r) + (lane & 15);
+      va[r] = (r < nrb_here) && ra <= i_hi;
+      ap[r] = A + (size_t)(va[r] ? ra : i_lo) * NBP + (lane >> 4);
+    }
+    const double* bp = Bm + (size_t)(lane >> 4) * NBP + 16 * (cg * CT) + (lane & 15);
+    d4 acc[RB][CT];
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+      for (int c = 0; c < CT; ++c) acc[r][c] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
     for (int kk = 0; kk < NBP / 4; ++kk) {
-      const double a = va ? ap[4 * kk] : 0.0;
-      const double bv = bp[(size_t)4 * kk * NBP];
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
+      double a[RB], b[CT];
+#pragma unroll
+      for (int r = 0; r < RB; ++r) a[r] = va[r] ? ap[r][4 * kk] : 0.0;
+#pragma unroll
+      for (int c = 0; c < CT; ++c) b[c] = bp[(size_t)4 * kk * NBP + 16 * c];
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        if (r < nrb_here) {
+#pragma unroll
+          for (int c = 0; c < CT; ++c) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
+        }
+      }
     }
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int r = i_lo + 16 * rb + 4 * v + (lane >> 4);
-      if (r <= i_hi) {
-        const int col = 16 * ct + (lane & 15);
-        Cm[(size_t)r * NBP + col] = (fix && r == i_lo) ? fix[col] : acc[v];
+    for (int r = 0; r < RB; ++r) {
+      if (r >= nrb_here) continue;
+#pragma unroll
+      for (int c = 0; c < CT; ++c) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = i_lo + 16 * (rb0 + r) + 4 * v + (lane >> 4);
+          if (row <= i_hi) {
+            const int col = 16 * (cg * CT + c) + (lane & 15);
+            Cm[(size_t)row * NBP + col] = (fix && row == i_lo) ? fix[col] : acc[r][c][v];
+          }
+        }
       }
     }
   }
