@@ -279,6 +279,8 @@ def main():
     ap.add_argument("--kernel", default="chain", choices=["chain", "class", "generic"],
                     help="2D kernel: chained workgroups per pair (default), one workgroup per "
                          "pair, or the any-layout kernel")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the chained kernels' autotuning (library defaults)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (x5), to calibrate "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE against known traffic")
@@ -307,9 +309,22 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     pairs = _abi.PackedPairs(w.xs, w.ys, dev)
-    runner = _abi.BatchRunner(w.log_trans.to(dev), w.state_types, w.state_group,
-                              w.log_tab.to(dev), pairs, want_emit_counts=not args.no_emit,
-                              kernel=args.kernel)
+    lt_dev, tab_dev = w.log_trans.to(dev), w.log_tab.to(dev)
+
+    def make_runner():
+        return _abi.BatchRunner(lt_dev, w.state_types, w.state_group, tab_dev, pairs,
+                                want_emit_counts=not args.no_emit, kernel=args.kernel)
+
+    tuned = None
+    if args.kernel == "chain" and not args.no_tune:
+        # untimed set-up: the chained kernels' workgroup widths per length
+        # class, chosen on this batch
+        from tkfmi import autotune
+        _log("autotuning the chained kernels on this batch")
+        tuned = autotune.tune_chain(make_runner)
+        _log(f"autotune: best {tuned['best']}: " +
+             ", ".join(f"{k}: {v:.2f} ms" for k, v in tuned["ms"].items()))
+    runner = make_runner()
     cell_states = w.cell_states()
     _log(f"pairs={w.B} S={w.S} cell-states={cell_states:.4e} workspace={runner.ws_bytes/2**30:.2f} GiB")
     for _ in range(args.warmup):
@@ -369,6 +384,10 @@ def main():
                    "pairs": w.B, "states": w.S, "cell_states": cell_states,
                    "emission_counts": not args.no_emit, "kernel": args.kernel,
                    "graph_replay_ms_per_step": graph_ms,
+                   "chain_config": (None if tuned is None else
+                                    {"long_w": tuned["best"][0], "main_w": tuned["best"][1],
+                                     "long_frac": tuned["best"][2],
+                                     "tuned_ms": {str(k): round(v, 3) for k, v in tuned["ms"].items()}}),
                    "sum_logp": float(logp.sum()), "sum_counts": float(n.sum())},
         "roofline": {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dom["achieved_GBs"] / HBM_PEAK_GBS,

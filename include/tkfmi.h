@@ -349,6 +349,20 @@ typedef struct tkfmi_kernel_time {
  */
 int tkfmi_last_kernel_times(tkfmi_kernel_time* out, int32_t max);
 
+/*
+ * Tuning of the chained 2D kernels; no counterpart in the reference (its
+ * 2D path is one vmapped JAX call per padded shape).  Process-wide, applies
+ * to later calls (a call's workspace size depends on it: query again after
+ * changing it).  A pair with at least long_w bands whose cells are at least
+ * long_frac x the batch's largest pair runs as a chain of long_w-wave
+ * workgroups on its own stream; the other pairs with at least main_w bands
+ * as chains of main_w-wave workgroups; shorter pairs with 2- and 1-wave
+ * workgroups.  long_w in 1..4, main_w in 3..4, long_frac >= 0 (0: no long
+ * class).  Defaults 3, 3, 0.3.  tkfmi.autotune chooses them by timing the
+ * caller's batch.  Returns TKFMI_OK or TKFMI_EINVAL.
+ */
+int tkfmi_chain_config(int32_t long_w, int32_t main_w, double long_frac);
+
 #ifdef __cplusplus
 }
 #endif

@@ -179,3 +179,13 @@ def test_pad_to_bin_matches_reference_bins(monkeypatch):
     assert hmm._pad_to_bin(70) == 96 and hmm._pad_to_bin(97) == 100 and hmm._pad_to_bin(0) == 32
     with pytest.raises(ValueError):
         hmm._pad_to_bin(101)
+
+
+def test_chain_config_validates_arguments():
+    from tkfmi import _abi, autotune
+    L = _abi.lib()
+    assert L.tkfmi_chain_config(5, 3, 0.3) < 0
+    assert L.tkfmi_chain_config(4, 2, 0.3) < 0
+    assert L.tkfmi_chain_config(4, 3, -1.0) < 0
+    assert b"tkfmi_chain_config" in L.tkfmi_last_error()
+    autotune.set_chain_config(*autotune.DEFAULT)

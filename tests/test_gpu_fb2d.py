@@ -280,3 +280,21 @@ def test_emit_counts_full_size_identities(kernel):
                                    np.bincount(np.minimum(x, 20), minlength=21), rtol=1e-10, atol=1e-9)
         np.testing.assert_allclose(m.sum(axis=(0, 1)) + e[:, nm:nm + 21].sum(axis=0),
                                    np.bincount(np.minimum(y, 20), minlength=21), rtol=1e-10, atol=1e-9)
+
+
+@pytest.mark.parametrize("cfg", [(4, 3, 0.3), (4, 4, 0.0), (3, 3, 0.3), (2, 4, 0.05)])
+def test_chain_configs_agree_with_generic(cfg):
+    """Every chained-kernel configuration (workgroup widths per length class)
+    gives the generic kernel's results."""
+    from tkfmi import autotune, synth
+    w = synth.make_mixdom2(seed=12, B=24, lengths=[5, 40, 64, 65, 130, 200, 260, 300, 390, 450, 520, 700] * 2)
+    b = _run(w, want_post=False, kernel="generic")
+    try:
+        autotune.set_chain_config(*cfg)
+        a = _run(w, want_post=False, kernel="chain")
+    finally:
+        autotune.set_chain_config(*autotune.DEFAULT)
+    la, lb = a["log_p"].cpu().numpy(), b["log_p"].cpu().numpy()
+    assert np.all(np.abs(la - lb) <= 1e-12 * np.abs(lb)), cfg
+    ok, worst = _close(a["n"].cpu().numpy(), b["n"].cpu().numpy())
+    assert ok, (cfg, worst)

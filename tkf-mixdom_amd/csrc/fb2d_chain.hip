@@ -33,6 +33,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -754,34 +755,36 @@ __global__ void zero_ints_kernel(int32_t* p, int n) {
 }
 
 // ---------------------------------------------------------------- host side
-// Launch classes, each with its own stream: 0 = the longest 4-wave pairs,
-// 1 = the other 4-wave pairs, 2 = 2-wave, 3 = 1-wave workgroups.  The
-// longest pairs set the makespan (their backward waits for their class's
+// Launch classes, each with its own stream: 0 = the longest pairs, 1 = the
+// other pairs of at least main_w bands, 2 = 2-wave, 3 = 1-wave workgroups.
+// The longest pairs set the makespan (their backward waits for their class's
 // forward launch), so they get a class of their own that starts first.
+// Waves per workgroup of classes 0 and 1 are tuning parameters
+// (tkfmi_chain_config; tkfmi.autotune picks them per workload): a 4-wave
+// backward workgroup needs ~92 KB of LDS (one per compute unit), a 3-wave
+// one ~76 KB (two fit).
 constexpr int NCLS = 4;
-constexpr int kCW[NCLS] = {4, 4, 2, 1};
-// Workgroup size class of a pair with nb bands: 4 waves from 4 bands on, 2
-// for 2-3 bands, 1 for one band.  TKFMI_CHAIN_MAXW (4, 2 or 1) caps the
-// workgroup size, so that longer pairs form longer chains of smaller
-// workgroups.
-int ccls_of(int nb) {
-  static const int maxw = [] {
-    const char* e = getenv("TKFMI_CHAIN_MAXW");
-    const int v = e ? atoi(e) : 4;
-    return v >= 4 ? 4 : (v >= 2 ? 2 : 1);
-  }();
-  if (nb >= 4 && maxw >= 4) return 1;
-  if (nb >= 2 && maxw >= 2) return 2;
-  return 3;
+struct ChainConfig {
+  int w[NCLS] = {3, 3, 2, 1};
+  double long_frac = 0.3;  // cells >= long_frac x the batch's largest pair
+};
+std::mutex g_cfg_mu;
+ChainConfig g_cfg = [] {
+  ChainConfig c;
+  if (const char* e = getenv("TKFMI_CHAIN_LONG")) c.long_frac = atof(e);
+  return c;
+}();
+ChainConfig chain_config() {
+  std::lock_guard<std::mutex> lk(g_cfg_mu);
+  return g_cfg;
 }
-// A 4-wave pair joins the long class when its cells are at least this
-// fraction of the batch's largest pair (TKFMI_CHAIN_LONG, default 0.3).
-double long_frac() {
-  static const double v = [] {
-    const char* e = getenv("TKFMI_CHAIN_LONG");
-    return e ? atof(e) : 0.3;
-  }();
-  return v;
+// Class of a pair with nb bands and `cells` lattice cells.
+int ccls_of(const ChainConfig& cf, int nb, int64_t cells, int64_t cells_max) {
+  if (nb >= cf.w[0] && cf.w[0] > 1 && cf.long_frac > 0.0 && (double)cells >= cf.long_frac * (double)cells_max)
+    return 0;
+  if (nb >= cf.w[1] && cf.w[1] > 2) return 1;
+  if (nb >= 2) return 2;
+  return 3;
 }
 
 struct ChainPlan {
@@ -793,6 +796,7 @@ struct ChainPlan {
   int max_ly[NCLS] = {0, 0, 0, 0};
   size_t desc_b = 0, pm_b = 0, lat_b = 0, ring_b = 0, flag_b = 0, pinfo_b = 0;
   int n_flags = 0;  // ints: NCLS classes x 2 tickets, abort, pad, then 2 per boundary
+  int W[NCLS] = {3, 3, 2, 1};  // waves per workgroup of each class
   size_t total() const { return desc_b + pm_b + lat_b + ring_b + flag_b + pinfo_b; }
 };
 
@@ -823,13 +827,14 @@ ChainPlan plan_chain(const tkfmi_problem* p, const Plan& pl, uint32_t flags) {
   int64_t aring = 0;
   int nfl = 2 * NCLS + 8;
   const int64_t cells_max = B > 0 ? (int64_t)(p->h_x_len[order[0]] + 1) * (p->h_y_len[order[0]] + 1) : 0;
+  const ChainConfig cf = chain_config();
+  for (int c = 0; c < NCLS; ++c) cp.W[c] = cf.w[c];
   for (int q = 0; q < B; ++q) {
     const int b = order[q];
     const int Lx = p->h_x_len[b], Ly = p->h_y_len[b];
     const int nb = (Lx + 64) / 64;
-    int c = ccls_of(nb);
-    if (c == 1 && (double)(Lx + 1) * (Ly + 1) >= long_frac() * (double)cells_max) c = 0;
-    const int W = kCW[c];
+    const int c = ccls_of(cf, nb, (int64_t)(Lx + 1) * (Ly + 1), cells_max);
+    const int W = cf.w[c];
     const int G = (nb + W - 1) / W;
     ++cp.cls_pairs[c];
     cp.cls_cells[c] += (int64_t)(Lx + 1) * (Ly + 1);
@@ -916,24 +921,25 @@ int launch_chain(bool bwd, const CParams& prm, const double* pmat, const CDesc* 
 }
 
 template <class L>
-int launch_chain_cls(int c, bool bwd, const CParams& prm, const double* pmat, const CDesc* d, int n,
+int launch_chain_cls(int W, bool bwd, const CParams& prm, const double* pmat, const CDesc* d, int n,
                      int max_ly, int wpc, hipStream_t s, std::string& err) {
-  switch (kCW[c]) {
+  switch (W) {
     case 4: return launch_chain<L, 4>(bwd, prm, pmat, d, n, max_ly, wpc, s, err);
+    case 3: return launch_chain<L, 3>(bwd, prm, pmat, d, n, max_ly, wpc, s, err);
     case 2: return launch_chain<L, 2>(bwd, prm, pmat, d, n, max_ly, wpc, s, err);
     default: return launch_chain<L, 1>(bwd, prm, pmat, d, n, max_ly, wpc, s, err);
   }
 }
 
-int dispatch_chain(const Plan& pl, int c, bool bwd, const CParams& prm, const double* pmat,
+int dispatch_chain(const Plan& pl, int W, bool bwd, const CParams& prm, const double* pmat,
                    const CDesc* d, int n, int max_ly, int wpc, hipStream_t s, std::string& err) {
   switch (pl.lay) {
-    case Layout::Tkf: return launch_chain_cls<LTkf>(c, bwd, prm, pmat, d, n, max_ly, wpc, s, err);
-    case Layout::Mix11: return launch_chain_cls<LMix<1, 1>>(c, bwd, prm, pmat, d, n, max_ly, wpc, s, err);
-    case Layout::Mix21: return launch_chain_cls<LMix<2, 1>>(c, bwd, prm, pmat, d, n, max_ly, wpc, s, err);
-    case Layout::Mix31: return launch_chain_cls<LMix<3, 1>>(c, bwd, prm, pmat, d, n, max_ly, wpc, s, err);
-    case Layout::Mix12: return launch_chain_cls<LMix<1, 2>>(c, bwd, prm, pmat, d, n, max_ly, wpc, s, err);
-    case Layout::Mix13: return launch_chain_cls<LMix<1, 3>>(c, bwd, prm, pmat, d, n, max_ly, wpc, s, err);
+    case Layout::Tkf: return launch_chain_cls<LTkf>(W, bwd, prm, pmat, d, n, max_ly, wpc, s, err);
+    case Layout::Mix11: return launch_chain_cls<LMix<1, 1>>(W, bwd, prm, pmat, d, n, max_ly, wpc, s, err);
+    case Layout::Mix21: return launch_chain_cls<LMix<2, 1>>(W, bwd, prm, pmat, d, n, max_ly, wpc, s, err);
+    case Layout::Mix31: return launch_chain_cls<LMix<3, 1>>(W, bwd, prm, pmat, d, n, max_ly, wpc, s, err);
+    case Layout::Mix12: return launch_chain_cls<LMix<1, 2>>(W, bwd, prm, pmat, d, n, max_ly, wpc, s, err);
+    case Layout::Mix13: return launch_chain_cls<LMix<1, 3>>(W, bwd, prm, pmat, d, n, max_ly, wpc, s, err);
     default: err = "layout not handled"; return TKFMI_EINVAL;
   }
 }
@@ -1026,15 +1032,25 @@ int tkfmi_chain_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, si
       pc.ticket = flagv + 2 * c + dir;
       const int n = (int)(dir ? cp.bwd[c].size() : cp.fwd[c].size());
       const std::string kname = std::string(dir ? "fbc_bwd_kernel<" : "fbc_fwd_kernel<") + layout_name(pl) +
-                                "," + std::to_string(kCW[c]) + ">" + (c == 0 ? " long" : "");
+                                "," + std::to_string(cp.W[c]) + ">" + (c == 0 ? " long" : "");
       const int slot = tkfmi_ktime_start(kname.c_str(), cp.cls_pairs[c], cp.cls_cells[c], s, err);
       if (slot == -2) return TKFMI_EHIP;
-      rc = dispatch_chain(pl, c, dir == 1, pc, pmat, ddesc + base[c][dir], n, cp.max_ly[c], wpc[c][dir], s, err);
+      rc = dispatch_chain(pl, cp.W[c], dir == 1, pc, pmat, ddesc + base[c][dir], n, cp.max_ly[c], wpc[c][dir], s, err);
       if (rc != TKFMI_OK) return rc;
       if (tkfmi_ktime_stop(slot, s, err) != TKFMI_OK) return TKFMI_EHIP;
     }
     if (hipEventRecord(sts->done[c], s) != hipSuccess) { err = "hipEventRecord failed"; return TKFMI_EHIP; }
     if (hipStreamWaitEvent(stream, sts->done[c], 0) != hipSuccess) { err = "hipStreamWaitEvent failed"; return TKFMI_EHIP; }
   }
+  return TKFMI_OK;
+}
+
+extern "C" int tkfmi_chain_config(int32_t long_w, int32_t main_w, double long_frac) {
+  if (long_w < 1 || long_w > 4 || main_w < 3 || main_w > 4 || !(long_frac >= 0.0))
+    return tkfmi_fail(TKFMI_EINVAL, "tkfmi_chain_config: waves must be 1..4 (main 3..4), long_frac >= 0");
+  std::lock_guard<std::mutex> lk(g_cfg_mu);
+  g_cfg.w[0] = long_w;
+  g_cfg.w[1] = main_w;
+  g_cfg.long_frac = long_frac;
   return TKFMI_OK;
 }

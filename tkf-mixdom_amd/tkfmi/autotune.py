@@ -1,0 +1,58 @@
+"""Autotuner for the chained 2D forward/backward kernels.
+
+The chained kernels split a batch into length classes (the longest pairs,
+the other pairs of several bands, 2-band and 1-band pairs) and run each
+class as chains of workgroups.  How many waves a workgroup of the two long
+classes has trades per-pair parallelism against occupancy (a 4-wave backward
+workgroup needs ~92 KB of LDS, so one fits a compute unit; a 3-wave one needs
+~76 KB, so two fit).  The best choice depends on the batch's length mix, so
+:func:`tune_chain` times a few settings on the caller's own batch (one
+representative call each) and keeps the fastest, through the library's
+process-wide ``tkfmi_chain_config``.
+"""
+import time
+
+import torch
+
+from . import _abi
+
+# (long_w, main_w, long_frac): waves per workgroup of the longest pairs'
+# class, of the other pairs' class, and the cell fraction of the batch's
+# largest pair from which a pair counts as long (0: no long class).
+DEFAULT = (3, 3, 0.3)  # the library's own defaults
+CANDIDATES = ((3, 3, 0.3), (4, 3, 0.3), (4, 4, 0.3), (3, 3, 0.15), (3, 3, 0.5), (3, 3, 0.0), (4, 4, 0.0))
+
+
+def set_chain_config(long_w, main_w, long_frac):
+    """Set the library's chained-kernel configuration (process-wide)."""
+    L = _abi.lib()
+    _abi.check(L.tkfmi_chain_config(int(long_w), int(main_w), float(long_frac)))
+
+
+def _time_runner(runner, steps):
+    runner.run()
+    torch.cuda.synchronize(runner.dev)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        runner.run()
+    torch.cuda.synchronize(runner.dev)
+    return 1e3 * (time.perf_counter() - t0) / steps
+
+
+def tune_chain(make_runner, candidates=CANDIDATES, steps=3):
+    """Time each configuration on the caller's batch and keep the fastest.
+
+    make_runner: callable returning a fresh :class:`tkfmi._abi.BatchRunner`
+    for the batch (a runner's workspace is sized for the configuration in
+    force when it is built).  Returns {"best": (long_w, main_w, long_frac),
+    "ms": {config: ms per call}}; the library is left set to the best.
+    """
+    timings = {}
+    for cfg in candidates:
+        set_chain_config(*cfg)
+        r = make_runner()
+        timings[cfg] = _time_runner(r, steps)
+        del r
+    best = min(timings, key=timings.get)
+    set_chain_config(*best)
+    return {"best": best, "ms": timings}
