@@ -40,13 +40,14 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_STATE = 16.0
 # HBM bytes per launch, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
 # workload (KiB per dispatch, scaled by the copy-kernel calibration: fetch x2.00,
-# write x1.00); profiles/fbc_kernels_c1_mixdom2_k3.md (chained kernels) and
+# write x1.00), keyed by (launch, pairs in it): profiles/fbc_kernels_c1_mixdom2_k3.md
+# (chained kernels, default class configuration) and
 # profiles/fbv2_kernel_c1_mixdom2_k3.md (one-workgroup-per-pair kernel).
-TRAFFIC = {"fbc_bwd_kernel<LMix<3,1>,4>": 5.27e9, "fbc_fwd_kernel<LMix<3,1>,4>": 4.93e9,
-           "fbc_bwd_kernel<LMix<3,1>,4> long": 2.26e9, "fbc_fwd_kernel<LMix<3,1>,4> long": 2.18e9,
-           "fbc_fwd_kernel<LMix<3,1>,2>": 1.26e9, "fbc_bwd_kernel<LMix<3,1>,2>": 1.44e9,
-           "fbv2_kernel<LMix<3,1>,8>": 3.13e9, "fbv2_kernel<LMix<3,1>,4>": 8.28e9,
-           "fbv2_kernel<LMix<3,1>,2>": 5.14e9, "fbv2_kernel<LMix<3,1>,1>": 0.665e9}
+TRAFFIC = {("fbc_bwd_kernel<LMix<3,1>,3>", 769): 6.40e9, ("fbc_fwd_kernel<LMix<3,1>,3>", 769): 5.92e9,
+           ("fbc_bwd_kernel<LMix<3,1>,3> long", 54): 2.26e9, ("fbc_fwd_kernel<LMix<3,1>,3> long", 54): 2.19e9,
+           ("fbc_bwd_kernel<LMix<3,1>,2>", 189): 3.25e8, ("fbc_fwd_kernel<LMix<3,1>,2>", 189): 2.74e8,
+           ("fbv2_kernel<LMix<3,1>,8>", 33): 3.13e9, ("fbv2_kernel<LMix<3,1>,4>", 269): 8.28e9,
+           ("fbv2_kernel<LMix<3,1>,2>", 506): 5.14e9, ("fbv2_kernel<LMix<3,1>,1>", 216): 0.665e9}
 
 
 def _bytes_per_cell_state(kernel_name):
@@ -391,7 +392,7 @@ def main():
                    "sum_logp": float(logp.sum()), "sum_counts": float(n.sum())},
         "roofline": {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dom["achieved_GBs"] / HBM_PEAK_GBS,
-                     "traffic": TRAFFIC.get(dom["name"]),
+                     "traffic": TRAFFIC.get((dom["name"], dom["pairs"])),
                      "kernel": dom["name"], "kernel_ms": dom["ms"],
                      "bytes_per_cell_state": dom["bytes_per_cell_state"], "kernels": kernels},
     }

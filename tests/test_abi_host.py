@@ -185,7 +185,7 @@ def test_chain_config_validates_arguments():
     from tkfmi import _abi, autotune
     L = _abi.lib()
     assert L.tkfmi_chain_config(5, 3, 0.3) < 0
-    assert L.tkfmi_chain_config(4, 2, 0.3) < 0
+    assert L.tkfmi_chain_config(4, 1, 0.3) < 0
     assert L.tkfmi_chain_config(4, 3, -1.0) < 0
     assert b"tkfmi_chain_config" in L.tkfmi_last_error()
     autotune.set_chain_config(*autotune.DEFAULT)

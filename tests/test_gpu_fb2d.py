@@ -282,7 +282,7 @@ def test_emit_counts_full_size_identities(kernel):
                                    np.bincount(np.minimum(y, 20), minlength=21), rtol=1e-10, atol=1e-9)
 
 
-@pytest.mark.parametrize("cfg", [(4, 3, 0.3), (4, 4, 0.0), (3, 3, 0.3), (2, 4, 0.05)])
+@pytest.mark.parametrize("cfg", [(4, 3, 0.3), (4, 4, 0.0), (3, 3, 0.3), (2, 4, 0.05), (3, 2, 0.3)])
 def test_chain_configs_agree_with_generic(cfg):
     """Every chained-kernel configuration (workgroup widths per length class)
     gives the generic kernel's results."""

@@ -782,7 +782,7 @@ ChainConfig chain_config() {
 int ccls_of(const ChainConfig& cf, int nb, int64_t cells, int64_t cells_max) {
   if (nb >= cf.w[0] && cf.w[0] > 1 && cf.long_frac > 0.0 && (double)cells >= cf.long_frac * (double)cells_max)
     return 0;
-  if (nb >= cf.w[1] && cf.w[1] > 2) return 1;
+  if (nb >= cf.w[1] && cf.w[1] >= 2) return 1;
   if (nb >= 2) return 2;
   return 3;
 }
@@ -1046,8 +1046,8 @@ int tkfmi_chain_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, si
 }
 
 extern "C" int tkfmi_chain_config(int32_t long_w, int32_t main_w, double long_frac) {
-  if (long_w < 1 || long_w > 4 || main_w < 3 || main_w > 4 || !(long_frac >= 0.0))
-    return tkfmi_fail(TKFMI_EINVAL, "tkfmi_chain_config: waves must be 1..4 (main 3..4), long_frac >= 0");
+  if (long_w < 1 || long_w > 4 || main_w < 2 || main_w > 4 || !(long_frac >= 0.0))
+    return tkfmi_fail(TKFMI_EINVAL, "tkfmi_chain_config: waves must be 1..4 (main 2..4), long_frac >= 0");
   std::lock_guard<std::mutex> lk(g_cfg_mu);
   g_cfg.w[0] = long_w;
   g_cfg.w[1] = main_w;

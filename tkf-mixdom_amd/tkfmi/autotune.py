@@ -5,7 +5,7 @@ the other pairs of several bands, 2-band and 1-band pairs) and run each
 class as chains of workgroups.  How many waves a workgroup of the two long
 classes has trades per-pair parallelism against occupancy (a 4-wave backward
 workgroup needs ~92 KB of LDS, so one fits a compute unit; a 3-wave one needs
-~76 KB, so two fit).  The best choice depends on the batch's length mix, so
+~76 KB and a 2-wave one ~58 KB, so two fit).  The best choice depends on the batch's length mix, so
 :func:`tune_chain` times a few settings on the caller's own batch (one
 representative call each) and keeps the fastest, through the library's
 process-wide ``tkfmi_chain_config``.
@@ -20,7 +20,8 @@ from . import _abi
 # class, of the other pairs' class, and the cell fraction of the batch's
 # largest pair from which a pair counts as long (0: no long class).
 DEFAULT = (3, 3, 0.3)  # the library's own defaults
-CANDIDATES = ((3, 3, 0.3), (4, 3, 0.3), (4, 4, 0.3), (3, 3, 0.15), (3, 3, 0.5), (3, 3, 0.0), (4, 4, 0.0))
+CANDIDATES = ((3, 3, 0.3), (4, 3, 0.3), (4, 4, 0.3), (3, 3, 0.15), (3, 3, 0.5), (3, 3, 0.0), (4, 4, 0.0),
+              (3, 2, 0.3), (2, 2, 0.0))
 
 
 def set_chain_config(long_w, main_w, long_frac):
@@ -39,13 +40,16 @@ def _time_runner(runner, steps):
     return 1e3 * (time.perf_counter() - t0) / steps
 
 
-def tune_chain(make_runner, candidates=CANDIDATES, steps=3):
+def tune_chain(make_runner, candidates=CANDIDATES, steps=3, margin=0.03):
     """Time each configuration on the caller's batch and keep the fastest.
 
     make_runner: callable returning a fresh :class:`tkfmi._abi.BatchRunner`
     for the batch (a runner's workspace is sized for the configuration in
-    force when it is built).  Returns {"best": (long_w, main_w, long_frac),
-    "ms": {config: ms per call}}; the library is left set to the best.
+    force when it is built).  The first candidate (the library default) is
+    kept unless another is faster by more than ``margin`` (a fraction), so
+    that timing noise does not flip the choice.  Returns {"best": (long_w,
+    main_w, long_frac), "ms": {config: ms per call}}; the library is left set
+    to the best.
     """
     timings = {}
     for cfg in candidates:
@@ -53,6 +57,9 @@ def tune_chain(make_runner, candidates=CANDIDATES, steps=3):
         r = make_runner()
         timings[cfg] = _time_runner(r, steps)
         del r
-    best = min(timings, key=timings.get)
+    best = candidates[0]
+    fastest = min(timings, key=timings.get)
+    if timings[fastest] < (1.0 - margin) * timings[best]:
+        best = fastest
     set_chain_config(*best)
     return {"best": best, "ms": timings}
