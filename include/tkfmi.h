@@ -358,7 +358,7 @@ int tkfmi_last_kernel_times(tkfmi_kernel_time* out, int32_t max);
  * workgroups on its own stream; the other pairs with at least main_w bands
  * as chains of main_w-wave workgroups; shorter pairs with 2- and 1-wave
  * workgroups.  long_w in 1..4, main_w in 2..4, long_frac >= 0 (0: no long
- * class).  Defaults 3, 3, 0.3.  tkfmi.autotune chooses them by timing the
+ * class).  Defaults 3, 3, 0.  tkfmi.autotune chooses them by timing the
  * caller's batch.  Returns TKFMI_OK or TKFMI_EINVAL.
  */
 int tkfmi_chain_config(int32_t long_w, int32_t main_w, double long_frac);

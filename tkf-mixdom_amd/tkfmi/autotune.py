@@ -19,9 +19,9 @@ from . import _abi
 # (long_w, main_w, long_frac): waves per workgroup of the longest pairs'
 # class, of the other pairs' class, and the cell fraction of the batch's
 # largest pair from which a pair counts as long (0: no long class).
-DEFAULT = (3, 3, 0.3)  # the library's own defaults
-CANDIDATES = ((3, 3, 0.3), (4, 3, 0.3), (4, 4, 0.3), (3, 3, 0.15), (3, 3, 0.5), (3, 3, 0.0), (4, 4, 0.0),
-              (3, 2, 0.3), (2, 2, 0.0))
+DEFAULT = (3, 3, 0.0)  # the library's own defaults
+CANDIDATES = ((3, 3, 0.0), (3, 3, 0.3), (4, 3, 0.3), (4, 4, 0.3), (3, 3, 0.15), (3, 3, 0.5), (4, 4, 0.0),
+              (3, 2, 0.0), (2, 2, 0.0))
 
 
 def set_chain_config(long_w, main_w, long_frac):
