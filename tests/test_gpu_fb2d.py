@@ -298,3 +298,43 @@ def test_chain_configs_agree_with_generic(cfg):
     assert np.all(np.abs(la - lb) <= 1e-12 * np.abs(lb)), cfg
     ok, worst = _close(a["n"].cpu().numpy(), b["n"].cpu().numpy())
     assert ok, (cfg, worst)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_tall_and_flat_pairs_match_oracle(kernel):
+    """Pairs whose lattice is one cell wide or high over many bands (the
+    chained kernel's cross-workgroup rows carry a single column)."""
+    w = _tkf_workload([1000, 900, 640, 0, 2, 257], [0, 1, 3, 900, 700, 2], seed=4, ext=0.4)
+    res = _run(w, want_post=False, kernel=kernel)
+    lt = w.log_trans.numpy()
+    for b, (x, y) in enumerate(zip(w.xs, w.ys)):
+        print(f"pair {b} Lx={len(x)} Ly={len(y)}", flush=True)
+        lp_o, _post, n_o, _ = _oracle_pair(lt, w.state_types, _tkf_oracle_emit(w, x, y))
+        lp = float(res["log_p"][b])
+        assert abs(lp - lp_o) <= RTOL * abs(lp_o), (b, lp, lp_o)
+        ok, worst = _close(res["n"][b].cpu().numpy(), n_o)
+        assert ok, (b, worst)
+
+
+def test_chain_wait_timeout_aborts_cleanly(monkeypatch):
+    """A wait that times out (forced from the host with a spin limit of 1)
+    must end the launch: pairs computed without a wait keep their values,
+    pairs whose chain waited either completed or report NaN, never a wrong
+    number; the next call with the normal limit is exact."""
+    from tkfmi import synth
+    w = synth.make_mixdom2(seed=31, B=8, lengths=[40, 700, 20, 650, 60, 800, 30, 500])
+    ref = _run(w, want_post=False, kernel="generic")
+    monkeypatch.setenv("TKFMI_CHAIN_SPIN_LIMIT", "1")
+    bad = _run(w, want_post=False, kernel="chain")
+    monkeypatch.delenv("TKFMI_CHAIN_SPIN_LIMIT")
+    lr, lb = ref["log_p"].cpu().numpy(), bad["log_p"].cpu().numpy()
+    for b in range(w.B):
+        print(f"pair {b} Lx={len(w.xs[b])}: {lb[b]} vs {lr[b]}", flush=True)
+        assert np.isnan(lb[b]) or abs(lb[b] - lr[b]) <= 1e-12 * abs(lr[b]), b
+    for b in (0, 2, 4, 6):  # one band: no cross-workgroup wait
+        assert abs(lb[b] - lr[b]) <= 1e-12 * abs(lr[b]), b
+    good = _run(w, want_post=False, kernel="chain")
+    lg = good["log_p"].cpu().numpy()
+    assert np.all(np.abs(lg - lr) <= 1e-12 * np.abs(lr))
+    ok, worst = _close(good["n"].cpu().numpy(), ref["n"].cpu().numpy())
+    assert ok, worst

@@ -53,6 +53,8 @@ constexpr int CQ = 16;                 // boundary entries per published / fetch
 constexpr bool kCountDefer = TKFMI_COUNT_DEFER != 0;
 constexpr int kStageDoubles = kCountDefer ? 2048 : 1024;
 constexpr int kSpinMax = 1 << 22;      // bounded waits (each spin sleeps ~0.2 us + a load)
+// TKFMI_CHAIN_SPIN_LIMIT overrides kSpinMax; tests set it tiny so that
+// waits time out and the abort path runs with every kernel still ending.
 
 struct CDesc {
   int32_t pair, Lx, Ly, band0;  // pair geometry; first band of this workgroup
@@ -74,6 +76,7 @@ struct CParams {
   int32_t* flagv;     // progress counters
   int32_t* ticket;    // this launch's ticket counter
   int32_t* abortf;    // raised by a timed-out wait
+  int32_t spin_max;   // spins before a wait times out
   double* pinfo;      // [B][2]: mantissa and binary exponent of P(x, y)
   double* out_logP;
   double* out_n;
@@ -87,13 +90,13 @@ __device__ __forceinline__ int ld_acquire(int* p) {
 
 // Waits (whole wave, uniform) until *flag >= need.  Returns false when the
 // launch was aborted, by this wait timing out or by another one.
-__device__ bool wait_ge(int* flag, int need, int* abortf) {
+__device__ bool wait_ge(int* flag, int need, int* abortf, int spin_max) {
   int v = __builtin_amdgcn_readfirstlane(ld_acquire(flag));
   int spins = 0;
   while (v < need) {
     if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
       return false;
-    if (++spins > kSpinMax) {
+    if (++spins > spin_max) {
       __hip_atomic_store(abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return false;
     }
@@ -242,7 +245,7 @@ __global__ __launch_bounds__(W * 64) void fbc_fwd_kernel(CParams Pr, const doubl
       // ---- boundary row of the previous workgroup: fetch the next chunk
       if (xin && t <= Ly && (t % CQ) == 0) {
         const int e_end = min(t + CQ, Ly + 1);
-        if (ok_run) ok_run = wait_ge(Pr.flagv + pd.fin, e_end, Pr.abortf);
+        if (ok_run) ok_run = wait_ge(Pr.flagv + pd.fin, e_end, Pr.abortf, Pr.spin_max);
         fetch_chunk(s_xc, Pr.xring + pd.xin_off + (size_t)t * NX, (e_end - t) * NX);
       }
       // ---- receive the predecessor row's h (lane-1, or a ring for lane 0)
@@ -521,7 +524,7 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
       // ---- boundary row of the next workgroup (below): fetch the next chunk
       if (xin && t <= Ly && (t % CQ) == 0) {
         const int e_end = min(t + CQ, Ly + 1);
-        if (ok_run) ok_run = wait_ge(Pr.flagv + pd.fin, e_end, Pr.abortf);
+        if (ok_run) ok_run = wait_ge(Pr.flagv + pd.fin, e_end, Pr.abortf, Pr.spin_max);
         fetch_chunk(s_xc, Pr.xring + pd.xin_off + (size_t)t * NX, (e_end - t) * NX);
       }
       // ---- receive the successor row's q (lane+1, or a ring for lane 63)
@@ -1019,6 +1022,10 @@ int tkfmi_chain_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, si
   prm.log_tab = p->log_tab; prm.x_cat = p->x_cat; prm.y_cat = p->y_cat;
   prm.lattice = lattice; prm.xring = xring; prm.flagv = flagv; prm.abortf = flagv + 2 * NCLS;
   prm.pinfo = pinfo;
+  {
+    const char* e = getenv("TKFMI_CHAIN_SPIN_LIMIT");
+    prm.spin_max = e ? std::max(0, atoi(e)) : kSpinMax;
+  }
   prm.out_logP = o->log_prob;
   prm.out_n = (flags & TKFMI_NO_COUNTS) ? nullptr : o->n_trans;
   prm.out_post = o->posteriors;
