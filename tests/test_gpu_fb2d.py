@@ -333,6 +333,9 @@ def test_chain_wait_timeout_aborts_cleanly(monkeypatch):
         assert np.isnan(lb[b]) or abs(lb[b] - lr[b]) <= 1e-12 * abs(lr[b]), b
     for b in (0, 2, 4, 6):  # one band: no cross-workgroup wait
         assert abs(lb[b] - lr[b]) <= 1e-12 * abs(lr[b]), b
+    # a chain's first wait comes ~200 steps before its producer can publish,
+    # so with one spin it times out: the abort path did run
+    assert np.isnan(lb[[1, 3, 5, 7]]).any()
     good = _run(w, want_post=False, kernel="chain")
     lg = good["log_p"].cpu().numpy()
     assert np.all(np.abs(lg - lr) <= 1e-12 * np.abs(lr))
