@@ -115,76 +115,87 @@ __device__ inline double emit(const double* tab, int ty, int g, int xi, int yj, 
   return 0.0;
 }
 
+// Rows of the transition block held in LDS (the rest is read from global
+// memory): as many 4-row chunks as fit in 159 KB, each LDS row padded by 16
+// doubles so that the four rows one MFMA operand load touches fall in two
+// bank halves (two passes, no further conflicts).
+__host__ __device__ constexpr int lds_b_stride(int NBP) { return NBP + 16; }
+__host__ __device__ constexpr int lds_b_rows(int NBP) {
+  const int r = (159 * 1024) / (lds_b_stride(NBP) * 8) / 4 * 4;
+  return r < NBP ? r : NBP;
+}
+// Output tiles per GEMM task: one 16-row block across CT column tiles (CT the
+// largest divisor of NT up to 5), so each A fragment (a cell row, read from
+// global memory) feeds CT MFMAs; the B fragments come mostly from LDS.
+__host__ __device__ constexpr int gemm_ct(int NT) {
+  return NT % 5 == 0 ? 5 : (NT % 4 == 0 ? 4 : (NT % 3 == 0 ? 3 : (NT % 2 == 0 ? 2 : 1)));
+}
+
 // C[cells x NBP] = A[cells x NBP] . Bm[NBP x NBP] for the rows i_lo..i_hi of
 // an antidiagonal (A and C row-indexed by i, row length NBP); the result
-// for row i_lo is replaced by `fix` when it is not null.
-//
-// A task is a block of RB x CT output tiles (16 rows x 16 columns each):
-// every A fragment feeds CT MFMAs and every Bm fragment RB MFMAs (0.75 loads
-// per v_mfma_f64_16x16x4f64 at 4 x 2, against 2 for single tiles), and the
-// RB x CT accumulators are independent chains.  Bm (the pair's transition
-// block) does not stay in L2 next to the other pairs' blocks, so reusing its
-// fragments is what matters.  Row blocks past the antidiagonal are skipped
-// wave-uniformly.
+// for row i_lo is replaced by `fix` when it is not null.  Rows 0..KL-1 of Bm
+// are read from the LDS copy sB (row stride lds_b_stride).
 template <int NT>
 __device__ inline void gemm_rows(const double* __restrict__ A, const double* __restrict__ Bm,
-                                 double* __restrict__ Cm, int i_lo, int i_hi, const double* fix) {
+                                 const double* __restrict__ sB, double* __restrict__ Cm, int i_lo, int i_hi,
+                                 const double* fix) {
   constexpr int NBP = 16 * NT;
-  constexpr int CT = (NT % 2 == 0 && NT <= 6) ? 2 : 1;  // registers: the backward keeps its count tiles live
-  constexpr int RB = 4;
+  constexpr int CT = gemm_ct(NT);
   constexpr int NCG = NT / CT;
+  constexpr int KL4 = lds_b_rows(NBP) / 4;  // k-chunks read from LDS
+  constexpr int SB = lds_b_stride(NBP);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int n = i_hi - i_lo + 1;
   const int nrb = (n + 15) >> 4;
-  const int nrg = (nrb + RB - 1) / RB;
-  for (int task = w; task < nrg * NCG; task += kWaves) {
-    const int rg = task / NCG, cg = task - rg * NCG;
-    const int rb0 = rg * RB;
-    const int nrb_here = min(RB, nrb - rb0);  // wave-uniform
-    const double* ap[RB];
-    bool va[RB];
+  for (int task = w; task < nrb * NCG; task += kWaves) {
+    const int rb = task / NCG, cg = task - rb * NCG;
+    const int ra = i_lo + 16 * rb + (lane & 15);
+    const bool va = ra <= i_hi;
+    const double* ap = A + (size_t)(va ? ra : i_lo) * NBP + (lane >> 4);
+    const int col0 = 16 * (cg * CT) + (lane & 15);
+    const double* sbp = sB + (lane >> 4) * SB + col0;
+    const double* bp = Bm + (size_t)(lane >> 4) * NBP + col0;
+    d4 acc[CT];
 #pragma unroll
-    for (int r = 0; r < RB; ++r) {
-      const int ra = i_lo + 16 * (rb0 + r) + (lane & 15);
-      va[r] = (r < nrb_here) && ra <= i_hi;
-      ap[r] = A + (size_t)(va[r] ? ra : i_lo) * NBP + (lane >> 4);
-    }
-    const double* bp = Bm + (size_t)(lane >> 4) * NBP + 16 * (cg * CT) + (lane & 15);
-    d4 acc[RB][CT];
-#pragma unroll
-    for (int r = 0; r < RB; ++r)
-#pragma unroll
-      for (int c = 0; c < CT; ++c) acc[r][c] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int c = 0; c < CT; ++c) acc[c] = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll 4
-    for (int kk = 0; kk < NBP / 4; ++kk) {
-      double a[RB], b[CT];
+    for (int kk = 0; kk < KL4; ++kk) {
+      const double a = va ? ap[4 * kk] : 0.0;
 #pragma unroll
-      for (int r = 0; r < RB; ++r) a[r] = va[r] ? ap[r][4 * kk] : 0.0;
+      for (int c = 0; c < CT; ++c)
+        acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sbp[4 * kk * SB + 16 * c], acc[c], 0, 0, 0);
+    }
+#pragma unroll 2
+    for (int kk = KL4; kk < NBP / 4; ++kk) {
+      const double a = va ? ap[4 * kk] : 0.0;
+      double b[CT];
 #pragma unroll
       for (int c = 0; c < CT; ++c) b[c] = bp[(size_t)4 * kk * NBP + 16 * c];
 #pragma unroll
-      for (int r = 0; r < RB; ++r) {
-        if (r < nrb_here) {
+      for (int c = 0; c < CT; ++c) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[c], acc[c], 0, 0, 0);
+    }
 #pragma unroll
-          for (int c = 0; c < CT; ++c) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
+    for (int c = 0; c < CT; ++c) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = i_lo + 16 * rb + 4 * v + (lane >> 4);
+        if (row <= i_hi) {
+          const int col = 16 * (cg * CT + c) + (lane & 15);
+          Cm[(size_t)row * NBP + col] = (fix && row == i_lo) ? fix[col] : acc[c][v];
         }
       }
     }
-#pragma unroll
-    for (int r = 0; r < RB; ++r) {
-      if (r >= nrb_here) continue;
-#pragma unroll
-      for (int c = 0; c < CT; ++c) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = i_lo + 16 * (rb0 + r) + 4 * v + (lane >> 4);
-          if (row <= i_hi) {
-            const int col = 16 * (cg * CT + c) + (lane & 15);
-            Cm[(size_t)row * NBP + col] = (fix && row == i_lo) ? fix[col] : acc[r][c][v];
-          }
-        }
-      }
-    }
+  }
+}
+
+// Copies rows 0..lds_b_rows-1 of the NBP x NBP block Bm into LDS (whole
+// workgroup; the caller synchronises).
+template <int NT>
+__device__ inline void stage_b(const double* __restrict__ Bm, double* __restrict__ sB) {
+  constexpr int NBP = 16 * NT, KL = lds_b_rows(NBP), SB = lds_b_stride(NBP);
+  for (int q = threadIdx.x; q < KL * NBP; q += kWaves * 64) {
+    const int r = q / NBP, c = q - r * NBP;
+    sB[r * SB + c] = Bm[q];
   }
 }
 
@@ -194,6 +205,7 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
   constexpr int NTILE = (NT * NT + kWaves - 1) / kWaves;  // count tiles per wave
   constexpr int KR = (NBP + 63) / 64;                      // state rounds per lane
   __shared__ double s_misc[4];
+  extern __shared__ double s_B[];  // rows of P (forward) or PT (backward)
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const WDesc pd = Pr.desc[blockIdx.x];
   const int Lx = pd.Lx, Ly = pd.Ly, W1 = Ly + 1, NB = Pr.NB, S = Pr.S;
@@ -230,6 +242,8 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
   auto ychar = [&](int j) { return (j >= 1) ? (int)min((int)ys[j - 1], A_PAD - 1) : 0; };
 
   // ======================================================== forward sweep
+  stage_b<NT>(P, s_B);
+  __syncthreads();
   for (int d = 0; d <= Lx + Ly; ++d) {
     const int i_lo = max(0, d - Ly), i_hi = min(d, Lx);
     const int sd = d % 3, s1 = (d + 2) % 3, s2 = (d + 1) % 3;  // slots of d, d-1, d-2
@@ -311,7 +325,7 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
     }
     __syncthreads();
     // h = F P (the origin pushes p_start)
-    gemm_rows<NT>(cur0, P, ring + (size_t)sd * (Lx + 1) * NBP, i_lo, i_hi, d == 0 ? p_start : nullptr);
+    gemm_rows<NT>(cur0, P, s_B, ring + (size_t)sd * (Lx + 1) * NBP, i_lo, i_hi, d == 0 ? p_start : nullptr);
     __syncthreads();
   }
   if (!want_lat) return;
@@ -326,6 +340,9 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
   for (int q = 0; q < NTILE; ++q) cacc[q] = d4{0.0, 0.0, 0.0, 0.0};
   double* post = want_post ? Pr.out_post + pd.post_off : nullptr;
   double* ecnt = want_ecnt ? Pr.out_ecnt + (size_t)pd.pair * Pr.G * TAB_SZ : nullptr;
+  __syncthreads();
+  stage_b<NT>(PT, s_B);
+  __syncthreads();
   for (int d = Lx + Ly; d >= 0; --d) {
     const int i_lo = max(0, d - Ly), i_hi = min(d, Lx);
     const int sd = d % 3, s1 = (d + 1) % 3, s2 = (d + 2) % 3;  // slots of d, d+1, d+2
@@ -368,7 +385,7 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
     }
     __syncthreads();
     // ---- B = g P^T, counts += Fs^T g
-    gemm_rows<NT>(cur1, PT, cur2, i_lo, i_hi, nullptr);
+    gemm_rows<NT>(cur1, PT, s_B, cur2, i_lo, i_hi, nullptr);
     if (want_counts) {
       const int n = i_hi - i_lo + 1;
       const int nq = (n + 3) >> 2;
@@ -534,17 +551,25 @@ bool plan(const tkfmi_problem* p, uint32_t flags, WPlan& pl, std::string& why) {
   return true;
 }
 
-int launch(int NT, const WParams& prm, int B, hipStream_t s, std::string& err) {
-  switch (NT) {
-    case 2: hipLaunchKernelGGL((wide_kernel<2>), dim3(B), dim3(kWaves * 64), 0, s, prm); break;
-    case 4: hipLaunchKernelGGL((wide_kernel<4>), dim3(B), dim3(kWaves * 64), 0, s, prm); break;
-    case 6: hipLaunchKernelGGL((wide_kernel<6>), dim3(B), dim3(kWaves * 64), 0, s, prm); break;
-    case 8: hipLaunchKernelGGL((wide_kernel<8>), dim3(B), dim3(kWaves * 64), 0, s, prm); break;
-    default: hipLaunchKernelGGL((wide_kernel<10>), dim3(B), dim3(kWaves * 64), 0, s, prm); break;
-  }
-  hipError_t e = hipGetLastError();
+template <int NT>
+int launch_nt(const WParams& prm, int B, hipStream_t s, std::string& err) {
+  const size_t smem = (size_t)lds_b_rows(16 * NT) * lds_b_stride(16 * NT) * 8;
+  hipError_t e = hipFuncSetAttribute((const void*)wide_kernel<NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  if (e != hipSuccess) { err = std::string("hipFuncSetAttribute: ") + hipGetErrorString(e); return TKFMI_EHIP; }
+  hipLaunchKernelGGL((wide_kernel<NT>), dim3(B), dim3(kWaves * 64), smem, s, prm);
+  e = hipGetLastError();
   if (e != hipSuccess) { err = std::string("wide launch: ") + hipGetErrorString(e); return TKFMI_EHIP; }
   return TKFMI_OK;
+}
+
+int launch(int NT, const WParams& prm, int B, hipStream_t s, std::string& err) {
+  switch (NT) {
+    case 2: return launch_nt<2>(prm, B, s, err);
+    case 4: return launch_nt<4>(prm, B, s, err);
+    case 6: return launch_nt<6>(prm, B, s, err);
+    case 8: return launch_nt<8>(prm, B, s, err);
+    default: return launch_nt<10>(prm, B, s, err);
+  }
 }
 
 }  // namespace tkfmi_wide
