@@ -124,9 +124,10 @@ __host__ __device__ constexpr int lds_b_rows(int NBP) {
   const int r = (159 * 1024) / (lds_b_stride(NBP) * 8) / 4 * 4;
   return r < NBP ? r : NBP;
 }
-// Output tiles per GEMM task: one 16-row block across CT column tiles (CT the
-// largest divisor of NT up to 5), so each A fragment (a cell row, read from
-// global memory) feeds CT MFMAs; the B fragments come mostly from LDS.
+// Output tiles per GEMM task: one 16-row block across CT column tiles, so
+// each A fragment (a cell row, read from global memory) feeds CT MFMAs; the
+// B fragments come mostly from LDS.  The backward, whose count tiles are live,
+// uses the largest divisor of NT up to 5; the forward all NT tiles.
 __host__ __device__ constexpr int gemm_ct(int NT) {
   return NT % 5 == 0 ? 5 : (NT % 4 == 0 ? 4 : (NT % 3 == 0 ? 3 : (NT % 2 == 0 ? 2 : 1)));
 }
@@ -135,12 +136,12 @@ __host__ __device__ constexpr int gemm_ct(int NT) {
 // an antidiagonal (A and C row-indexed by i, row length NBP); the result
 // for row i_lo is replaced by `fix` when it is not null.  Rows 0..KL-1 of Bm
 // are read from the LDS copy sB (row stride lds_b_stride).
-template <int NT>
+template <int NT, int CT>
 __device__ inline void gemm_rows(const double* __restrict__ A, const double* __restrict__ Bm,
                                  const double* __restrict__ sB, double* __restrict__ Cm, int i_lo, int i_hi,
                                  const double* fix) {
   constexpr int NBP = 16 * NT;
-  constexpr int CT = gemm_ct(NT);
+  static_assert(NT % CT == 0, "column tiles per task");
   constexpr int NCG = NT / CT;
   constexpr int KL4 = lds_b_rows(NBP) / 4;  // k-chunks read from LDS
   constexpr int SB = lds_b_stride(NBP);
@@ -325,7 +326,8 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
     }
     __syncthreads();
     // h = F P (the origin pushes p_start)
-    gemm_rows<NT>(cur0, P, s_B, ring + (size_t)sd * (Lx + 1) * NBP, i_lo, i_hi, d == 0 ? p_start : nullptr);
+    // the forward has no count tiles live: one task covers all column tiles
+    gemm_rows<NT, NT>(cur0, P, s_B, ring + (size_t)sd * (Lx + 1) * NBP, i_lo, i_hi, d == 0 ? p_start : nullptr);
     __syncthreads();
   }
   if (!want_lat) return;
@@ -385,7 +387,7 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
     }
     __syncthreads();
     // ---- B = g P^T, counts += Fs^T g
-    gemm_rows<NT>(cur1, PT, s_B, cur2, i_lo, i_hi, nullptr);
+    gemm_rows<NT, gemm_ct(NT)>(cur1, PT, s_B, cur2, i_lo, i_hi, nullptr);
     if (want_counts) {
       const int n = i_hi - i_lo + 1;
       const int nq = (n + 3) >> 2;
