@@ -106,17 +106,49 @@ __device__ bool wait_ge(int* flag, int need, int* abortf, int spin_max) {
   return true;
 }
 
-// Publishes `count` boundary entries (this lane's stores before it) to the
-// next workgroup of the chain.
+// Boundary entries between the workgroups of a chain are written and read
+// as device-scope relaxed atomics, which gfx950 performs at the device's
+// coherence point (past the per-XCD L2s).  Publishing then only has to wait
+// until this lane's entry stores have completed before it raises the counter:
+// an agent-scope release fence would also write back every dirty line of the
+// XCD's L2, and there the forward's lattice stores keep megabytes dirty, so
+// each publish would wait for a flush of the whole cache.
+__device__ __forceinline__ void ring_store(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Publishes `count` boundary entries (this lane's ring_store calls before it)
+// to the next workgroup of the chain.
 __device__ __forceinline__ void publish(int* flag, int count) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the entry stores have completed
   __hip_atomic_store(flag, count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Copies boundary entries [e, e_end) into the LDS chunk buffer (whole wave).
+// Copies boundary entries [e, e_end) into the LDS chunk buffer (whole wave),
+// after wait_ge has acquired their progress counter.
 __device__ __forceinline__ void fetch_chunk(double* s_xc, const double* src, int n) {
-  for (int q = threadIdx.x & 63; q < n; q += 64) s_xc[q] = src[q];
+  for (int q = threadIdx.x & 63; q < n; q += 64)
+    s_xc[q] = __hip_atomic_load(src + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   wave_lds_sync();
+}
+
+// Writes one boundary entry (the M and D parts of v, then the exponent) to
+// the LDS ring of the next wave or, for the next workgroup, to global memory.
+template <class L, int NX>
+__device__ __forceinline__ void put_entry(double* e, bool global, const double* v, int E) {
+  constexpr int NB = L::NB, nM = count_type<L, ST_M>();
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    int o = -1;
+    if (L::type(b) == ST_M) o = rank_in_type<L, ST_M>(b);
+    if (L::type(b) == ST_D) o = nM + rank_in_type<L, ST_D>(b);
+    if (o >= 0) {
+      if (global) ring_store(e + o, v[b]);
+      else e[o] = v[b];
+    }
+  }
+  if (global) ring_store(e + NX - 1, (double)E);
+  else e[NX - 1] = (double)E;
 }
 
 __device__ __forceinline__ int take_ticket(int32_t* counter) {
@@ -344,16 +376,11 @@ __global__ __launch_bounds__(W * 64) void fbc_fwd_kernel(CParams Pr, const doubl
       }
       // ---- publish this row's h for the band below (lane 63)
       if (lane == 63 && valid) {
-        double* e = xout ? Pr.xring + pd.xout_off + (size_t)j * NX
-                         : s_ring + ((size_t)w * 2 + (j & 1)) * NX;
-        if (xout || w < NW - 1) {
-#pragma unroll
-          for (int b = 0; b < NB; ++b) {
-            if (L::type(b) == ST_M) e[rank_in_type<L, ST_M>(b)] = h[b];
-            if (L::type(b) == ST_D) e[nM + rank_in_type<L, ST_D>(b)] = h[b];
-          }
-          e[NX - 1] = (double)E;
-          if (xout && (((j + 1) % CQ) == 0 || j == Ly)) publish(Pr.flagv + pd.fout, j + 1);
+        if (xout) {
+          put_entry<L, NX>(Pr.xring + pd.xout_off + (size_t)j * NX, true, h, E);
+          if (((j + 1) % CQ) == 0 || j == Ly) publish(Pr.flagv + pd.fout, j + 1);
+        } else if (w < NW - 1) {
+          put_entry<L, NX>(s_ring + ((size_t)w * 2 + (j & 1)) * NX, false, h, E);
         }
       }
 #pragma unroll
@@ -693,16 +720,11 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
       }
       // ---- publish this row's q for the band above (lane 0)
       if (lane == 0 && valid) {
-        double* e = xout ? Pr.xring + pd.xout_off + (size_t)p * NX
-                         : s_ring + ((size_t)w * 2 + (p & 1)) * NX;
-        if (xout || w < NW - 1) {
-#pragma unroll
-          for (int b = 0; b < NB; ++b) {
-            if (L::type(b) == ST_M) e[rank_in_type<L, ST_M>(b)] = q[b];
-            if (L::type(b) == ST_D) e[nM + rank_in_type<L, ST_D>(b)] = q[b];
-          }
-          e[NX - 1] = (double)EB;
-          if (xout && (((p + 1) % CQ) == 0 || p == Ly)) publish(Pr.flagv + pd.fout, p + 1);
+        if (xout) {
+          put_entry<L, NX>(Pr.xring + pd.xout_off + (size_t)p * NX, true, q, EB);
+          if (((p + 1) % CQ) == 0 || p == Ly) publish(Pr.flagv + pd.fout, p + 1);
+        } else if (w < NW - 1) {
+          put_entry<L, NX>(s_ring + ((size_t)w * 2 + (p & 1)) * NX, false, q, EB);
         }
       }
 #pragma unroll
@@ -793,6 +815,17 @@ int ccls_of(const ChainConfig& cf, int nb, int64_t cells, int64_t cells_max) {
   return 3;
 }
 
+// Ticket order of each class: 1 (default) = critical path first, 0 = pair
+// by pair (largest pairs first, a pair's chain consecutive).
+// TKFMI_CHAIN_ORDER.
+int chain_order() {
+  static const int v = [] {
+    const char* e = getenv("TKFMI_CHAIN_ORDER");
+    return e ? atoi(e) : 1;
+  }();
+  return v;
+}
+
 struct ChainPlan {
   bool ok = false;
   std::string why;
@@ -878,6 +911,27 @@ ChainPlan plan_chain(const tkfmi_problem* p, const Plan& pl, uint32_t flags) {
       d.xout_off = g > 0 ? ring0 + (int64_t)(g - 1) * (Ly + 1) * NX : -1;
       d.fout = g > 0 ? flag0 + 2 * (g - 1) + 1 : -1;
       cp.bwd[c].push_back(d);
+    }
+  }
+  // Critical-path ticket order.  A workgroup's critical path is the number
+  // of steps from its start to the end of its pair's chain, (links after it)
+  // x lag + T, where lag is how far a consumer's first step trails its
+  // producer's (the producer's last wave is 64 (W - 1) steps behind its
+  // first, plus one chunk).  Longest first: a long pair's later links are
+  // dispatched early enough to keep up with their producers, and the short
+  // pairs fill in behind.  The critical path falls along a chain, so
+  // producers keep lower tickets than their consumers.
+  if (chain_order() == 1) {
+    for (int c = 0; c < NCLS; ++c) {
+      const int lag = 64 * (cp.W[c] - 1) + 64 + CQ;
+      for (int dir = 0; dir < 2; ++dir) {
+        std::vector<CDesc>& v = dir ? cp.bwd[c] : cp.fwd[c];
+        auto crit = [&](const CDesc& d) {
+          const int links = dir ? d.g : d.G - 1 - d.g;
+          return (int64_t)links * lag + d.T;
+        };
+        std::stable_sort(v.begin(), v.end(), [&](const CDesc& a, const CDesc& b) { return crit(a) > crit(b); });
+      }
     }
   }
   size_t nd = 0;
