@@ -44,14 +44,9 @@
 namespace tkfmi_v2 {
 
 constexpr int CQ = 16;                 // boundary entries per published / fetched chunk
-#ifndef TKFMI_COUNT_DEFER
-#define TKFMI_COUNT_DEFER 1
-#endif
-// Backward count staging: 1 = one step deep (64 cells per wave, no LDS round
-// trip inside a step, 16 KB per wave); 0 = two 32-cell halves counted in the
-// step (8 KB per wave, so that two 4-wave workgroups fit in a compute unit).
-constexpr bool kCountDefer = TKFMI_COUNT_DEFER != 0;
-constexpr int kStageDoubles = kCountDefer ? 2048 : 1024;
+// Backward count staging per recursion wave: its 64 cells' F-scaled and g
+// vectors, 16 doubles each (16 KB).
+constexpr int kStageDoubles = 2048;
 constexpr int kSpinMax = 1 << 22;      // bounded waits (each spin sleeps ~0.2 us + a load)
 // TKFMI_CHAIN_SPIN_LIMIT overrides kSpinMax; tests set it tiny so that
 // waits time out and the abort path runs with every kernel still ending.
@@ -159,8 +154,8 @@ __device__ __forceinline__ int take_ticket(int32_t* counter) {
 }
 
 // LDS layout (doubles): tables | start / end transitions | emission counts
-// (backward) | short rings | chunk buffer | MFMA staging (backward, 64 cells x
-// 32 doubles per wave) | bytes: x rows of the workgroup, y.
+// (backward) | short rings | chunk buffer | count staging (backward, 64 cells
+// x 32 doubles per recursion wave) | bytes: x rows of the workgroup, y.
 template <class L, int W>
 struct CLds {
   static constexpr int NX = count_type<L, ST_M>() + count_type<L, ST_D>() + 1;
@@ -179,25 +174,25 @@ struct CLds {
 // The start / end transitions go to LDS too: a global load inside the sweep
 // would make the compiler wait for every outstanding lattice store (stores
 // and loads share one counter) at the next use of its register.
-template <class L, int W>
+template <class L, int W, int NTHR = W * 64>
 __device__ __forceinline__ void stage_inputs(const CParams& Pr, const CDesc& pd, const double* pm,
                                              double* s_tab, uint8_t* s_x, uint8_t* s_y) {
   constexpr int NG = L::NG, NB = L::NB;
   static_assert(2 * NB + 1 <= 32, "start / end staging");
   double* s_misc = s_tab + NG * TAB_SZ;
-  for (int q = threadIdx.x; q < 2 * NB + 1; q += W * 64) s_misc[q] = pm[2 * NB * NB + q];
+  for (int q = threadIdx.x; q < 2 * NB + 1; q += NTHR) s_misc[q] = pm[2 * NB * NB + q];
   const double* ltab = Pr.log_tab + (Pr.n_tab > 1 ? (size_t)pd.pair * Pr.G_tab * TAB_SZ : 0);
-  for (int q = threadIdx.x; q < NG * TAB_SZ; q += W * 64) {
+  for (int q = threadIdx.x; q < NG * TAB_SZ; q += NTHR) {
     const int g = q / TAB_SZ, o = q - g * TAB_SZ;
     s_tab[q] = exp(ltab[Pr.gmap[g] * TAB_SZ + o]);
   }
   // s_x[q] = x[64 band0 + q - 1] (the character of row 64 band0 + q)
   const int xb = 64 * pd.band0 - 1;
-  for (int q = threadIdx.x; q < 64 * W + 1; q += W * 64) {
+  for (int q = threadIdx.x; q < 64 * W + 1; q += NTHR) {
     const int xi = xb + q;
     s_x[q] = (xi >= 0 && xi < pd.Lx) ? (uint8_t)min((int)Pr.x_cat[pd.x_off + xi], A_PAD - 1) : 0;
   }
-  for (int q = threadIdx.x; q < pd.Ly; q += W * 64) s_y[q] = min((int)Pr.y_cat[pd.y_off + q], A_PAD - 1);
+  for (int q = threadIdx.x; q < pd.Ly; q += NTHR) s_y[q] = min((int)Pr.y_cat[pd.y_off + q], A_PAD - 1);
   if (threadIdx.x == 0) s_y[pd.Ly] = 0;
 }
 
@@ -400,42 +395,61 @@ __global__ __launch_bounds__(W * 64) void fbc_fwd_kernel(CParams Pr, const doubl
   }
 }
 
-// Expected-count MFMAs over 4 NQ staged cells: acc (+acc2) += F^T g, K = 4
-// cells per v_mfma_f64_16x16x4f64, alternating two accumulator chains.
-template <int NQ>
-__device__ __forceinline__ void count_mfma(const double* stage_f, const double* stage_g, int lane, d4& acc,
-                                           d4& acc2) {
+// Count wave, one step: the expected-count MFMAs of the cells the recursion
+// waves staged at the previous step.  On gfx950 an FP64 MFMA occupies its
+// SIMD's FP64 pipe for 64 cycles, with no overlap with VALU FP64 work, so a
+// separate wave (on its own SIMD) keeps them off the recursion's dependent
+// chain.  Each stage (64 cells) is read in two halves of 32 cells into
+// registers; the reads of the next half are issued before the 8 MFMAs of the
+// current one (4 cells each, two accumulator chains: acc (+acc2) += F^T g),
+// so LDS latency hides behind the matrix pipe.  When both halves of stage v
+// are in registers the wave releases it (s_cflag[v] = t): recursion wave v
+// may then overwrite it with its cells of step t.  Stages of waves that were
+// not active at step t - 1 are read but not counted.
+template <int W>
+__device__ __forceinline__ void count_step(const double* s_stage, int NW, int Ly, int t, int lane, int* s_cflag,
+                                           d4& acc, d4& acc2) {
+  int ln = lane;
+  asm volatile("" : "+v"(ln));  // recompute the addresses here (hoisted, they would take 48 VGPRs)
+  auto load = [&](int i, double (&a)[8], double (&b)[8]) {
+    const double* sf = s_stage + (size_t)(i >> 1) * kStageDoubles;
+    const double* sg = sf + kStageDoubles / 2;
 #pragma unroll
-  for (int q = 0; q < NQ; q += 2) {
-    const int c0 = 4 * q + (lane >> 4), c1 = c0 + 4;
-    const int o0 = c0 * 16 + ((lane & 15) ^ (c0 & 15));
-    const int o1 = c1 * 16 + ((lane & 15) ^ (c1 & 15));
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(stage_f[o0], stage_g[o0], acc, 0, 0, 0);
-    acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(stage_f[o1], stage_g[o1], acc2, 0, 0, 0);
-  }
-}
-
-// Bv[k] = sum_k' bcast_k(prowT[k']) g[k'] interleaved with the 16 count
-// MFMAs of the previously staged cells (one per transition row), so that the
-// matrix core works while the VALU issues the row's DPP FMAs.
-template <int NB>
-__device__ __forceinline__ void contract_and_count(double (&Bv)[NB], const double (&prowT)[NB],
-                                                   const double (&g)[NB], const double* stage_f,
-                                                   const double* stage_g, int lane, d4& acc, d4& acc2) {
+    for (int k = 0; k < 8; ++k) {
+      const int c = 4 * (8 * (i & 1) + k) + (ln >> 4);
+      const int o = c * 16 + ((ln & 15) ^ (c & 15));
+      a[k] = sf[o];
+      b[k] = sg[o];
+    }
+  };
+  double ca[8], cb[8], na[8], nb[8];
+  load(0, ca, cb);
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    if (q < NB) row_fmac<NB>(Bv, prowT[q < NB ? q : 0], g[q < NB ? q : 0]);
-    const int c = 4 * q + (lane >> 4);
-    const int o = c * 16 + ((lane & 15) ^ (c & 15));
-    if (q & 1) acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(stage_f[o], stage_g[o], acc2, 0, 0, 0);
-    else acc = __builtin_amdgcn_mfma_f64_16x16x4f64(stage_f[o], stage_g[o], acc, 0, 0, 0);
+  for (int i = 0; i < 2 * W; ++i) {
+    const int v = i >> 1;
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // half i is in registers
+    if ((i & 1) && lane == 0) __hip_atomic_store(s_cflag + v, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (i + 1 < 2 * W) load(i + 1, na, nb);
+    __builtin_amdgcn_sched_barrier(0);
+    if (v < NW && t >= 64 * v + 1 && t <= 64 * v + 64 + Ly) {  // wave v staged at step t - 1
+#pragma unroll
+      for (int k = 0; k < 8; k += 2) {
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[k], cb[k], acc, 0, 0, 0);
+        acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[k + 1], cb[k + 1], acc2, 0, 0, 0);
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) { ca[k] = na[k]; cb[k] = nb[k]; }
   }
 }
 
 // ============================================================= backward
+// W recursion waves (one band each) plus one count wave (w == W).
 template <class L, int W>
-__global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const double* __restrict__ pmat,
-                                                         const CDesc* __restrict__ descs) {
+__global__ __launch_bounds__((W + 1) * 64, 2) void fbc_bwd_kernel(CParams Pr, const double* __restrict__ pmat,
+                                                               const CDesc* __restrict__ descs) {
+  constexpr int NTHR = (W + 1) * 64;
   constexpr int NB = L::NB, NG = L::NG;
   constexpr int nM = count_type<L, ST_M>();
   constexpr int nD = count_type<L, ST_D>();
@@ -458,10 +472,12 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
   double* s_stage = smem + LY::stage(true);
   uint8_t* s_x = reinterpret_cast<uint8_t*>(smem) + LY::seq_bytes(true);
   uint8_t* s_y = s_x + LY::xbytes;
-  stage_inputs<L, W>(Pr, pd, pmat + pd.pm_off, s_tab, s_x, s_y);
+  __shared__ int s_cflag[W];  // count wave -> recursion wave w: stage read up to step s_cflag[w] - 1
+  stage_inputs<L, W, NTHR>(Pr, pd, pmat + pd.pm_off, s_tab, s_x, s_y);
+  if (threadIdx.x < W) s_cflag[threadIdx.x] = 0;
   const bool want_ecnt = (Pr.flags & TKFMI_WANT_EMIT_COUNTS) && Pr.out_ecnt != nullptr;
   if (want_ecnt)
-    for (int q = threadIdx.x; q < NG * TAB_SZ; q += W * 64) s_ecnt[q] = 0.0;
+    for (int q = threadIdx.x; q < NG * TAB_SZ; q += NTHR) s_ecnt[q] = 0.0;
 
   const double* __restrict__ pm = pmat + pd.pm_off;
   const double* __restrict__ Pf = pm;
@@ -486,19 +502,17 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
   const bool xin = (w == 0) && pd.fin >= 0;
   const bool xout = (w == NW - 1) && pd.fout >= 0;
   bool ok_run = true;
-  d4 cacc = {0.0, 0.0, 0.0, 0.0}, cacc2 = {0.0, 0.0, 0.0, 0.0};
-  // Count staging, one step deep: at step t the wave runs the MFMAs of the
-  // cells it staged at step t-1 (interleaved with the transition FMAs), then
-  // stages its cells of step t in the same buffer.  A wave's LDS operations execute in issue order, so the
-  // reads of step t-1's cells precede their overwrite without a wait, and no
-  // write -> read round trip sits inside a step.
-  double* stage_f = s_stage + (size_t)w * kStageDoubles;  // [64 | 32 cells][16] F-scaled
-  double* stage_g = stage_f + kStageDoubles / 2;          // [64 | 32 cells][16] g
+  const bool counter = w == W;
+  d4 cacc = {0.0, 0.0, 0.0, 0.0}, cacc2 = {0.0, 0.0, 0.0, 0.0};  // count wave's accumulators
+  // Count staging, one step deep: at the end of step t a recursion wave
+  // stages its cells in its buffer (after the count wave has read the cells of
+  // step t-1 from it); during step t+1 the count wave runs their MFMAs.
+  double* stage_f = s_stage + (size_t)(counter ? 0 : w) * kStageDoubles;  // [64 cells][16] F-scaled
+  double* stage_g = stage_f + kStageDoubles / 2;                          // [64 cells][16] g
   double accD[NG];   // D-type emission mass of this lane's row (x fixed per lane)
   double runI[NG];   // I-type mass of the column passing up through the lanes
 #pragma unroll
   for (int gq = 0; gq < NG; ++gq) { accD[gq] = 0.0; runI[gq] = 0.0; }
-  for (int q = lane; q < kStageDoubles; q += 64) stage_f[q] = 0.0;  // step 0 counts an empty stage
   double* post = want_post ? Pr.out_post + pd.post_off : nullptr;
   __syncthreads();
 
@@ -510,7 +524,7 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
   for (int m = 0; m < nM; ++m) rM_last[m] = 0.0;
   int eQ = EXP_NONE, eM_last = EXP_NONE;
   // x fixed per lane; y passes up one lane per step (lane 63 reads ahead).
-  const int xi = s_x[64 * (NW - 1 - w) + lane];
+  const int xi = s_x[64 * max(NW - 1 - w, 0) + lane];
   double emD[NG];
 #pragma unroll
   for (int g = 0; g < NG; ++g) emD[g] = s_tab[g * TAB_SZ + A_PAD * A_PAD + A_PAD + xi];
@@ -533,12 +547,21 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
   const bool corner_wave = has_band && band == (Lx + 64) / 64 - 1;
   const int t_corner = 64 * w + 63 - (Lx - 64 * band);
   for (int t = 0; t < T; ++t) {
+    if (counter && want_counts) count_step<W>(s_stage, NW, Ly, t, lane, s_cflag, cacc, cacc2);
     if (has_band && t >= t_on && t <= t_off) {
       int i = i_band;
       asm volatile("" : "+v"(i));  // keeps row predicates inside the loop (no SGPR masks)
       const int p = t - off;
       const int j = Ly - p;
       const bool valid = p >= 0 && p <= Ly && i <= Lx;
+      // ---- boundary row of the next workgroup (below): fetch the next chunk
+      // (before this step's lattice loads: loads complete in order, so the
+      // chunk's wait would otherwise include them)
+      if (xin && t <= Ly && (t % CQ) == 0) {
+        const int e_end = min(t + CQ, Ly + 1);
+        if (ok_run) ok_run = wait_ge(Pr.flagv + pd.fin, e_end, Pr.abortf, Pr.spin_max);
+        fetch_chunk(s_xc, Pr.xring + pd.xin_off + (size_t)t * NX, (e_end - t) * NX);
+      }
       // The cell's forward values: loaded here, first used after the
       // transition contraction, which covers the load latency.
       double fcur[SLOTS];
@@ -547,12 +570,6 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
 #pragma unroll
         for (int b = 0; b < NB; ++b) fcur[b] = a[b * 64];
         fcur[SLOTS - 1] = a[(SLOTS - 1) * 64];
-      }
-      // ---- boundary row of the next workgroup (below): fetch the next chunk
-      if (xin && t <= Ly && (t % CQ) == 0) {
-        const int e_end = min(t + CQ, Ly + 1);
-        if (ok_run) ok_run = wait_ge(Pr.flagv + pd.fin, e_end, Pr.abortf, Pr.spin_max);
-        fetch_chunk(s_xc, Pr.xring + pd.xin_off + (size_t)t * NX, (e_end - t) * NX);
       }
       // ---- receive the successor row's q (lane+1, or a ring for lane 63)
       double rD[nD > 0 ? nD : 1], rM[nM > 0 ? nM : 1];
@@ -595,8 +612,7 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
       double Bv[NB];
 #pragma unroll
       for (int k = 0; k < NB; ++k) Bv[k] = 0.0;
-      if constexpr (kCountDefer) contract_and_count<NB>(Bv, prowT, g, stage_f, stage_g, lane, cacc, cacc2);
-      else row_contract<NB>(Bv, prowT, g);
+      row_contract<NB>(Bv, prowT, g);
       const bool corner = valid && i == Lx && j == Ly;
       int EBraw = Eg;
       if (corner_wave && t == t_corner) {  // the corner's step (wave-uniform test)
@@ -618,28 +634,17 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
       for (int b = 0; b < SLOTS; ++b) fcur[b] = valid ? fcur[b] : 0.0;
       const int EF = (int)fcur[SLOTS - 1];
       const double fscale = valid ? ldexp(inv_mP, EF + Eg - EP) : 0.0;
-      if constexpr (kCountDefer) {
+      if (want_counts) {  // stage this step's cells (counted during the next step)
+        int spins = 0;
+        while (__hip_atomic_load(s_cflag + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < t) {
+          __builtin_amdgcn_s_sleep(1);
+          if (++spins > kSpinMax) { ok_run = false; break; }
+        }
 #pragma unroll
-        for (int s = 0; s < 16; ++s) {  // stage this step's cells (counted next step)
+        for (int s = 0; s < 16; ++s) {
           const int o = lane * 16 + (s ^ (lane & 15));
           stage_f[o] = (s < NB) ? fcur[s < NB ? s : 0] * fscale : 0.0;
           stage_g[o] = (s < NB) ? g[s < NB ? s : 0] : 0.0;
-        }
-      } else {
-#pragma unroll
-        for (int half = 0; half < 2; ++half) {  // stage 32 cells, count them now
-          if ((lane >> 5) == half) {
-            const int c = lane & 31;
-#pragma unroll
-            for (int s = 0; s < 16; ++s) {
-              const int o = c * 16 + (s ^ (c & 15));
-              stage_f[o] = (s < NB) ? fcur[s < NB ? s : 0] * fscale : 0.0;
-              stage_g[o] = (s < NB) ? g[s < NB ? s : 0] : 0.0;
-            }
-          }
-          wave_lds_sync();
-          count_mfma<8>(stage_f, stage_g, lane, cacc, cacc2);
-          wave_lds_sync();
         }
       }
       if (want_counts) {
@@ -738,7 +743,7 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
     }
     lds_barrier();
   }
-  if (kCountDefer && has_band) count_mfma<16>(stage_f, stage_g, lane, cacc, cacc2);
+  if (counter && want_counts) count_step<W>(s_stage, NW, Ly, T, lane, s_cflag, cacc, cacc2);  // last step's cells
   if (has_band && want_ecnt && i_band >= 1 && i_band <= Lx) {
 #pragma unroll
     for (int gq = 0; gq < NG; ++gq)
@@ -751,25 +756,26 @@ __global__ __launch_bounds__(W * 64) void fbc_bwd_kernel(CParams Pr, const doubl
   }
   if (want_ecnt) {  // layout group g -> caller group gmap[g]
     double* oe = Pr.out_ecnt + (size_t)pd.pair * Pr.G_tab * TAB_SZ;
-    for (int q = threadIdx.x; q < NG * TAB_SZ; q += W * 64) {
+    for (int q = threadIdx.x; q < NG * TAB_SZ; q += NTHR) {
       const int gq = q / TAB_SZ, o = q - gq * TAB_SZ;
       const double v = s_ecnt[q];
       if (v != 0.0) unsafeAtomicAdd(oe + Pr.gmap[gq] * TAB_SZ + o, v);
     }
   }
   if (!want_counts) return;
-  // ---- reduce the MFMA accumulators over the waves (fixed order), then add
-  double* s_red = s_stage;  // [W][16][16]
+  // ---- the count wave's accumulators through LDS, then add
+  double* s_red = s_stage;  // [16][16] (the count wave has read every stage)
+  if (counter) {
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int row = 4 * v + (lane >> 4), col = lane & 15;  // accumulator layout
-    s_red[(w * 16 + row) * 16 + col] = cacc[v] + cacc2[v];
+    for (int v = 0; v < 4; ++v) {
+      const int row = 4 * v + (lane >> 4), col = lane & 15;  // accumulator layout
+      s_red[row * 16 + col] = cacc[v] + cacc2[v];
+    }
   }
   __syncthreads();
-  for (int qq = threadIdx.x; qq < NB * NB; qq += W * 64) {
+  for (int qq = threadIdx.x; qq < NB * NB; qq += NTHR) {
     const int k = qq / NB, k2 = qq % NB;
-    double acc = 0.0;
-    for (int ww = 0; ww < NW; ++ww) acc += s_red[(ww * 16 + k) * 16 + k2];
+    const double acc = s_red[k * 16 + k2];
     const double pr = Pf[k * NB + k2];
     if (pr > 0.0 && acc != 0.0) unsafeAtomicAdd(out_n + (k + L::SOFF) * S + (k2 + L::SOFF), pr * acc);
   }
@@ -973,7 +979,7 @@ int launch_chain(bool bwd, const CParams& prm, const double* pmat, const CDesc* 
   if (wpc > 0) smem = std::max(smem, (size_t)(160 * 1024 / wpc - 512) & ~(size_t)255);
   hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
   if (e != hipSuccess) { err = std::string("hipFuncSetAttribute: ") + hipGetErrorString(e); return TKFMI_EHIP; }
-  if (bwd) hipLaunchKernelGGL((fbc_bwd_kernel<L, W>), dim3(n), dim3(W * 64), smem, s, prm, pmat, d);
+  if (bwd) hipLaunchKernelGGL((fbc_bwd_kernel<L, W>), dim3(n), dim3((W + 1) * 64), smem, s, prm, pmat, d);
   else hipLaunchKernelGGL((fbc_fwd_kernel<L, W>), dim3(n), dim3(W * 64), smem, s, prm, pmat, d);
   e = hipGetLastError();
   if (e != hipSuccess) { err = std::string("fbc launch: ") + hipGetErrorString(e); return TKFMI_EHIP; }
