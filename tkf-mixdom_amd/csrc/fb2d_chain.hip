@@ -531,11 +531,12 @@ __global__ __launch_bounds__((W + 1) * 64, 2) void fbc_bwd_kernel(CParams Pr, co
   int yj_prev = 0;
   int y63_next = s_y[max(Ly - 1, 0)];
   const int off = 64 * w + (63 - lane);  // position p = t - off, column j = Ly - p
-  // Lattice slot of this lane's cell at position p; an invalid cell reads
-  // the pair's first slot (inside the allocation) and is zeroed after.
+  // Lattice slot of this lane's cell at position p.  An invalid cell reads
+  // the origin's slot (cell (0, 0), always written by the forward, so finite);
+  // its count operands are then scaled by 0.
   auto f_addr = [&](int p, int i) -> const double* {
     const bool v = has_band && p >= 0 && p <= Ly && i <= Lx;
-    return lat + (v ? (((size_t)band * BSTRIDE + (Ly - p + lane)) * SLOTS) * 64 : 0) + lane;
+    return lat + (v ? (((size_t)band * BSTRIDE + (Ly - p + lane)) * SLOTS) * 64 + lane : 0);
   };
   const int T = pd.T;
   // A wave's band is in range for steps 64 w .. 64 w + 63 + Ly; outside
@@ -630,8 +631,6 @@ __global__ __launch_bounds__((W + 1) * 64, 2) void fbc_bwd_kernel(CParams Pr, co
 #pragma unroll
       for (int k = 0; k < NB; ++k) Bv[k] = ldexp(Bv[k], -sh);
       // ---- counts: stage F-scaled and g, then MFMA (K = 4 cells per op)
-#pragma unroll
-      for (int b = 0; b < SLOTS; ++b) fcur[b] = valid ? fcur[b] : 0.0;
       const int EF = (int)fcur[SLOTS - 1];
       const double fscale = valid ? ldexp(inv_mP, EF + Eg - EP) : 0.0;
       if (want_counts) {  // stage this step's cells (counted during the next step)
@@ -711,17 +710,16 @@ __global__ __launch_bounds__((W + 1) * 64, 2) void fbc_bwd_kernel(CParams Pr, co
         }
       }
       // ---- q = e(i,j,k) B(i,j,k) for the predecessor row
-      const bool eM_ok = valid && i >= 1 && j >= 1, eI_ok = valid && j >= 1, eD_ok = valid && i >= 1;
+      // No masks: an invalid cell's g is 0 (its ok flags are false), so its q
+      // is 0; a valid cell in row 0 or column 0 gets a q for the types whose
+      // emission is not allowed there, but that q feeds only cells outside
+      // the lattice (row -1, column -1), which no valid cell reads.
       double q[NB];
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
         const int ty = L::type(b), gq = L::group(b);
-        double em;
-        bool ok;
-        if (ty == ST_M) { em = emM[gq]; ok = eM_ok; }
-        else if (ty == ST_I) { em = emI[gq]; ok = eI_ok; }
-        else { em = emD[gq]; ok = eD_ok; }
-        q[b] = ok ? em * Bv[b] : 0.0;
+        const double em = ty == ST_M ? emM[gq] : (ty == ST_I ? emI[gq] : emD[gq]);
+        q[b] = em * Bv[b];
       }
       // ---- publish this row's q for the band above (lane 0)
       if (lane == 0 && valid) {
