@@ -53,6 +53,20 @@ extern "C" {
                                     workspace: skip their host-to-device
                                     upload, so the call is only device
                                     work (e.g. captured into a HIP graph)  */
+#define TKFMI_CHECKPOINT 256u    /* antidiagonal checkpointing (the
+                                    reference's gradient path recomputes
+                                    the lattice under jax.checkpoint,
+                                    python/tkfmixdom/jax/train/vjp.py
+                                    remat): the forward keeps the restart
+                                    state of every K-th antidiagonal, K ~
+                                    sqrt(2 (Lx+Ly+1)), and the backward
+                                    recomputes one K-antidiagonal segment
+                                    at a time; identical results, one extra
+                                    forward sweep, lattice memory about
+                                    3 K / (Ly+1) of the full one.  Applies
+                                    to the 17..162-state kernel; the other
+                                    kernels keep their 128-B-per-cell
+                                    lattice and ignore it                  */
 
 /* emission tables: per group g, 21x21 match block then 21 insert then 21
  * delete entries (index 20 = wildcard), all natural-log values. */

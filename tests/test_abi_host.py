@@ -56,7 +56,7 @@ def test_struct_layout_matches_header():
         assert fields == [f[0] for f in py._fields_], cname
     flags = dict(re.findall(r"#define TKFMI_(\w+) (\d+)u", text))
     for k in ("FORWARD_ONLY", "WANT_POSTERIORS", "NO_COUNTS", "GENERIC_KERNEL", "CLASS_KERNEL", "REUSE_DESCRIPTORS", "WANT_EMIT_COUNTS",
-              "TIME_KERNELS"):
+              "TIME_KERNELS", "CHECKPOINT"):
         assert int(flags[k]) == getattr(_abi, k)
 
 
@@ -84,6 +84,29 @@ def test_workspace_query_is_host_only_and_monotone():
     c = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p_big), _abi.FORWARD_ONLY)
     g = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p_big), _abi.GENERIC_KERNEL)
     assert 0 < a < b and 0 < c < b and g > 0
+
+
+def test_checkpoint_shrinks_the_wide_workspace():
+    """TKFMI_CHECKPOINT on the 162-state layout: the lattice part shrinks from
+    (Ly+1) antidiagonal-row equivalents per pair to 2 (Lx+Ly)/K + K + 3 with
+    K = ceil(sqrt(2 (Lx+Ly+1))); the cell-per-lane layouts ignore the flag."""
+    from tkfmi import _abi, mixdom
+    L = _abi.lib()
+    st = list(mixdom.state_types(8, 4))
+    p, k = _problem(st, [0] * len(st), [500, 800], [600, 700], G=1)
+    full = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), 0)
+    ck = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), _abi.CHECKPOINT)
+    fo = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), _abi.FORWARD_ONLY)
+    lat_full = sum((lx + 1) * (ly + 1) * 161 * 8 for lx, ly in ((500, 600), (800, 700)))
+    assert abs((full - fo) - lat_full) < 4096
+    for lx, ly in ((500, 600), (800, 700)):
+        K = int(np.ceil(np.sqrt(2.0 * (lx + ly + 1))))
+        lat_full -= ((lx + ly) // K + 1) * 2 * (lx + 1) * 161 * 8 + (K + 3) * (lx + 1) * 161 * 8
+    assert abs((full - ck) - lat_full) < 4096 and ck < full / 4
+    st17 = list(mixdom.state_types(3, 1))
+    p17, k17 = _problem(st17, [0] * 17, [500, 800], [600, 700], G=1)
+    assert L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p17), _abi.CHECKPOINT) == \
+        L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p17), 0)
 
 
 def test_invalid_problems_rejected_with_message():

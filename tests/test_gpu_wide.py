@@ -81,3 +81,34 @@ def test_wide_count_identities_long_pairs():
         assert abs(nb[0].sum() - 1.0) < 1e-11
         body = np.isin(types, (1, 2, 3))
         np.testing.assert_allclose(nb[:, body].sum(0), nb[body, :].sum(1), rtol=1e-9, atol=1e-9)
+
+
+def test_wide_checkpoint_matches_full_lattice():
+    """TKFMI_CHECKPOINT: the backward recomputes each K-antidiagonal segment
+    from its checkpoint with the forward's own arithmetic, so log P, counts
+    and posteriors equal the full-lattice run bit for bit (emission counts to
+    the FP64 atomics' summation order); a small layout also against the
+    oracle."""
+    from oracle import emissions
+    from tkfmi import hmm, synth
+    w = synth.make_mixdom2(seed=21, B=5, K=8, F=4, C=4, lengths=[60, 3, 90, 0, 150])
+    kw = dict(want_posteriors=True, want_emit_counts=True)
+    full = hmm.fb2d_batch(w.log_trans, w.state_types, w.xs, w.ys, w.log_tab, w.state_group, **kw)
+    ck = hmm.fb2d_batch(w.log_trans, w.state_types, w.xs, w.ys, w.log_tab, w.state_group, checkpoint=True, **kw)
+    assert np.array_equal(full["log_p"].cpu().numpy(), ck["log_p"].cpu().numpy())
+    assert np.array_equal(full["n"].cpu().numpy(), ck["n"].cpu().numpy())
+    for b in range(5):
+        print(f"K8 F4 C4 pair {b} Lx={len(w.xs[b])} Ly={len(w.ys[b])}", flush=True)
+        assert np.array_equal(full["posteriors"][b].cpu().numpy(), ck["posteriors"][b].cpu().numpy()), b
+    ok, worst = _close(ck["emit_counts"].cpu().numpy(), full["emit_counts"].cpu().numpy(), rtol=1e-12)
+    assert ok, worst
+    w = synth.make_mixdom2(seed=77, B=3, K=2, F=2, C=2, lengths=[0, 17, 40])
+    res = hmm.fb2d_batch(w.log_trans, w.state_types, w.xs, w.ys, w.log_tab, w.state_group, checkpoint=True, **kw)
+    G = res["emit_counts"].shape[1]
+    for b, (x, y) in enumerate(zip(w.xs, w.ys)):
+        lp_o, post_o, n_o, _ = _oracle(w, b, x, y)
+        assert abs(float(res["log_p"][b]) - lp_o) <= RTOL * abs(lp_o), b
+        assert _close(res["n"][b].cpu().numpy(), n_o)[0], b
+        assert _close(res["posteriors"][b].cpu().numpy(), post_o)[0], b
+        ec_o = emissions.emission_counts(post_o, w.state_types, w.state_group, x, y, G)
+        assert _close(res["emit_counts"][b].cpu().numpy(), ec_o)[0], b

@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -44,8 +45,12 @@ constexpr double LN2 = 0.69314718055994530942;
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 struct WDesc {
-  int32_t pair, Lx, Ly, pad;
-  int64_t lat_off;   // doubles: forward lattice [(Lx+1)(Ly+1)][NBP], then exponents
+  int32_t pair, Lx, Ly;
+  int32_t ck;        // checkpoint interval K in antidiagonals (0: full lattice)
+  int64_t lat_off;   // doubles: forward lattice [(Lx+1)(Ly+1)][NBP], then exponents;
+                     // with checkpoints: checkpoints [nck][2][Lx+1][NBP+1], then the
+                     // segment lattice [K][Lx+1][NBP] + [K][Lx+1], then a ring
+                     // [3][Lx+1][NBP] + [3][Lx+1] for the recomputation
   int64_t ring_off;  // doubles: ring [3][Lx+1][NBP], exponents [3][Lx+1]
   int64_t cur_off;   // doubles: 3 antidiagonal buffers [Lx+1][NBP]
   int64_t pm_off;    // doubles: transition block
@@ -243,9 +248,24 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
   auto ychar = [&](int j) { return (j >= 1) ? (int)min((int)ys[j - 1], A_PAD - 1) : 0; };
 
   // ======================================================== forward sweep
-  stage_b<NT>(P, s_B);
-  __syncthreads();
-  for (int d = 0; d <= Lx + Ly; ++d) {
+  // One antidiagonal d of the forward: F from the ring slots of d-1 and d-2
+  // (rg, rgE), written to the lattice (F of cell i at fb + i*fs, its
+  // exponent at eb + i*es; fb null: not kept) and to cur0, then h = F P into
+  // the ring slot of d.  `main`: the first sweep, which also produces log P
+  // and the E column; a recomputation inside the backward skips them.
+  // With checkpoints (pd.ck = K > 0) the first sweep keeps only the restart
+  // state of every K-th antidiagonal, and the backward recomputes each
+  // segment of K antidiagonals from its checkpoint when it enters it: the
+  // same arithmetic in the same order, so the values are those of the full
+  // lattice.
+  const int ck = want_lat ? pd.ck : 0;
+  const int nck = ck ? (Lx + Ly) / ck + 1 : 0;
+  double* ckp = lat;                                               // checkpoints
+  double* seg = lat + (size_t)nck * 2 * (Lx + 1) * (NBP + 1);      // segment lattice
+  double* segE = seg + (size_t)ck * (Lx + 1) * NBP;
+  double* rring = segE + (size_t)ck * (Lx + 1);                    // recomputation ring
+  double* rringE = rring + 3 * (size_t)(Lx + 1) * NBP;
+  auto fstep = [&](int d, double* rg, double* rgE, double* fb, size_t fs, double* eb, size_t es, bool main) {
     const int i_lo = max(0, d - Ly), i_hi = min(d, Lx);
     const int sd = d % 3, s1 = (d + 2) % 3, s2 = (d + 1) % 3;  // slots of d, d-1, d-2
     for (int i = i_lo + w; i <= i_hi; i += kWaves) {
@@ -253,18 +273,18 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
       const int xi = xchar(i), yj = ychar(j);
       // predecessor exponents: M from (i-1, j-1) [d-2], I from (i, j-1) [d-1], D from (i-1, j) [d-1]
       const bool okM = i >= 1 && j >= 1, okI = j >= 1, okD = i >= 1;
-      const int eM = okM ? (int)ringE[s2 * (Lx + 1) + (i - 1)] : EXP_NONE;
-      const int eI = okI ? (int)ringE[s1 * (Lx + 1) + i] : EXP_NONE;
-      const int eD = okD ? (int)ringE[s1 * (Lx + 1) + (i - 1)] : EXP_NONE;
+      const int eM = okM ? (int)rgE[s2 * (Lx + 1) + (i - 1)] : EXP_NONE;
+      const int eI = okI ? (int)rgE[s1 * (Lx + 1) + i] : EXP_NONE;
+      const int eD = okD ? (int)rgE[s1 * (Lx + 1) + (i - 1)] : EXP_NONE;
       double raw[KR];
       double mM = 0.0, mI = 0.0, mD = 0.0;
 #pragma unroll
       for (int r = 0; r < KR; ++r) {
         const int k = lane + 64 * r;
         double v = 0.0;
-        if (ty[r] == ST_M && okM) { v = ring[((size_t)s2 * (Lx + 1) + i - 1) * NBP + k] * emit(tab, ST_M, gr[r], xi, yj, i, j); mM = fmax(mM, v); }
-        else if (ty[r] == ST_I && okI) { v = ring[((size_t)s1 * (Lx + 1) + i) * NBP + k] * emit(tab, ST_I, gr[r], xi, yj, i, j); mI = fmax(mI, v); }
-        else if (ty[r] == ST_D && okD) { v = ring[((size_t)s1 * (Lx + 1) + i - 1) * NBP + k] * emit(tab, ST_D, gr[r], xi, yj, i, j); mD = fmax(mD, v); }
+        if (ty[r] == ST_M && okM) { v = rg[((size_t)s2 * (Lx + 1) + i - 1) * NBP + k] * emit(tab, ST_M, gr[r], xi, yj, i, j); mM = fmax(mM, v); }
+        else if (ty[r] == ST_I && okI) { v = rg[((size_t)s1 * (Lx + 1) + i) * NBP + k] * emit(tab, ST_I, gr[r], xi, yj, i, j); mI = fmax(mI, v); }
+        else if (ty[r] == ST_D && okD) { v = rg[((size_t)s1 * (Lx + 1) + i - 1) * NBP + k] * emit(tab, ST_D, gr[r], xi, yj, i, j); mD = fmax(mD, v); }
         raw[r] = v;
       }
       mM = wave_max(mM);
@@ -283,20 +303,19 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
         F[r] = (origin || raw[r] == 0.0) ? 0.0 : ldexp(raw[r], sh);
       }
       if (origin) E = 0;
-      const size_t cell = (size_t)i * W1 + j;
 #pragma unroll
       for (int r = 0; r < KR; ++r) {
         const int k = lane + 64 * r;
         if (k < NBP) {
           cur0[(size_t)i * NBP + k] = F[r];
-          if (want_lat) lat[cell * NBP + k] = F[r];
+          if (fb) fb[(size_t)i * fs + k] = F[r];
         }
       }
       if (lane == 0) {
-        ringE[sd * (Lx + 1) + i] = (double)E;
-        if (want_lat) latE[cell] = (double)E;
+        rgE[sd * (Lx + 1) + i] = (double)E;
+        if (eb) eb[(size_t)i * es] = (double)E;
       }
-      if (i == Lx && j == Ly) {  // log P and the E column
+      if (main && i == Lx && j == Ly) {  // log P and the E column
         double part = 0.0;
 #pragma unroll
         for (int r = 0; r < KR; ++r) {
@@ -326,9 +345,37 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
     }
     __syncthreads();
     // h = F P (the origin pushes p_start)
-    // the forward has no count tiles live: one task covers all column tiles
-    gemm_rows<NT, NT>(cur0, P, s_B, ring + (size_t)sd * (Lx + 1) * NBP, i_lo, i_hi, d == 0 ? p_start : nullptr);
+    if (main)  // no count tiles live: one task covers all column tiles
+      gemm_rows<NT, NT>(cur0, P, s_B, rg + (size_t)sd * (Lx + 1) * NBP, i_lo, i_hi, d == 0 ? p_start : nullptr);
+    else
+      gemm_rows<NT, gemm_ct(NT)>(cur0, P, s_B, rg + (size_t)sd * (Lx + 1) * NBP, i_lo, i_hi, d == 0 ? p_start : nullptr);
     __syncthreads();
+  };
+  // The state a forward restarts from at antidiagonal d: the ring slots of d-1
+  // and d-2 (h values and exponents).
+  auto ck_copy = [&](int d, double* rg, double* rgE, double* c, bool save) {
+    for (int h = 0; h < 2; ++h) {
+      const int sl = (d + 2 - h) % 3;  // h = 0: slot of d-1, h = 1: slot of d-2
+      double* cs = c + (size_t)h * (Lx + 1) * (NBP + 1);
+      for (int q = threadIdx.x; q < (Lx + 1) * NBP; q += kWaves * 64) {
+        double* r = rg + (size_t)sl * (Lx + 1) * NBP + q;
+        if (save) cs[q] = *r;
+        else *r = cs[q];
+      }
+      for (int q = threadIdx.x; q <= Lx; q += kWaves * 64) {
+        double* r = rgE + sl * (Lx + 1) + q;
+        if (save) cs[(size_t)(Lx + 1) * NBP + q] = *r;
+        else *r = cs[(size_t)(Lx + 1) * NBP + q];
+      }
+    }
+    __syncthreads();
+  };
+  stage_b<NT>(P, s_B);
+  __syncthreads();
+  for (int d = 0; d <= Lx + Ly; ++d) {
+    if (ck && d > 0 && d % ck == 0) ck_copy(d, ring, ringE, ckp + (size_t)(d / ck) * 2 * (Lx + 1) * (NBP + 1), true);
+    fstep(d, ring, ringE, (want_lat && !ck) ? lat + (size_t)d * NBP : nullptr, (size_t)Ly * NBP,
+          (want_lat && !ck) ? latE + d : nullptr, (size_t)Ly, true);
   }
   if (!want_lat) return;
 
@@ -348,6 +395,23 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
   for (int d = Lx + Ly; d >= 0; --d) {
     const int i_lo = max(0, d - Ly), i_hi = min(d, Lx);
     const int sd = d % 3, s1 = (d + 1) % 3, s2 = (d + 2) % 3;  // slots of d, d+1, d+2
+    // ---- the forward values of antidiagonal d: the lattice, or (checkpoints)
+    // the segment lattice of d's segment, recomputed when the sweep enters it
+    const int d0 = ck ? d / ck * ck : 0;
+    if (ck && d == min(Lx + Ly, d0 + ck - 1)) {
+      if (d0 > 0) ck_copy(d0, rring, rringE, ckp + (size_t)(d0 / ck) * 2 * (Lx + 1) * (NBP + 1), false);
+      stage_b<NT>(P, s_B);
+      __syncthreads();
+      for (int dd = d0; dd <= d; ++dd)
+        fstep(dd, rring, rringE, seg + (size_t)(dd - d0) * (Lx + 1) * NBP, (size_t)NBP,
+              segE + (size_t)(dd - d0) * (Lx + 1), 1, false);
+      stage_b<NT>(PT, s_B);
+      __syncthreads();
+    }
+    const double* fb = ck ? seg + (size_t)(d - d0) * (Lx + 1) * NBP : lat + (size_t)d * NBP;
+    const size_t fs = ck ? (size_t)NBP : (size_t)Ly * NBP;
+    const double* eb = ck ? segE + (size_t)(d - d0) * (Lx + 1) : latE + d;
+    const size_t es = ck ? 1 : (size_t)Ly;
     // ---- g (successor-pulled q), scaled forward values
     for (int i = i_lo + w; i <= i_hi; i += kWaves) {
       const int j = d - i;
@@ -357,8 +421,7 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
       const int eD = okD ? (int)ringE[s1 * (Lx + 1) + (i + 1)] : EXP_NONE;
       int Eg = max(eM, max(eI, eD));
       if (Eg == EXP_NONE) Eg = 0;
-      const size_t cell = (size_t)i * W1 + j;
-      const int EF = (int)latE[cell];
+      const int EF = (int)eb[(size_t)i * es];
       const double fsc = ldexp(inv_mP, EF + Eg - EP);
 #pragma unroll
       for (int r = 0; r < KR; ++r) {
@@ -369,7 +432,7 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
         else if (ty[r] == ST_I && okI) g = ldexp(ring[((size_t)s1 * (Lx + 1) + i) * NBP + k], eI - Eg);
         else if (ty[r] == ST_D && okD) g = ldexp(ring[((size_t)s1 * (Lx + 1) + i + 1) * NBP + k], eD - Eg);
         cur1[(size_t)i * NBP + k] = g;
-        cur0[(size_t)i * NBP + k] = lat[cell * NBP + k] * fsc;
+        cur0[(size_t)i * NBP + k] = fb[(size_t)i * fs + k] * fsc;
         if (want_counts && i == 0 && j == 0 && k < NB)
           out_n[Pr.bidx[k]] = p_start[k] * g * ldexp(inv_mP, Eg - EP);
       }
@@ -425,7 +488,7 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
       const int sh = (mx > 0.0) ? frexp_exp(mx) : 0;
       const int EB = (corner ? 0 : (int)ringE[sd * (Lx + 1) + i]) + sh;
       const size_t cell = (size_t)i * W1 + j;
-      const int EF = (int)latE[cell];
+      const int EF = (int)eb[(size_t)i * es];
       const double psc = ldexp(inv_mP, EF + EB - EP);
 #pragma unroll
       for (int r = 0; r < KR; ++r) {
@@ -435,7 +498,7 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
         const double e = (k < NB) ? emit(tab, ty[r], gr[r], xi, yj, i, j) : 0.0;
         ring[((size_t)sd * (Lx + 1) + i) * NBP + k] = e * b;
         if (k < NB && (want_post || want_ecnt)) {
-          const double pv = lat[cell * NBP + k] * b * psc;
+          const double pv = fb[(size_t)i * fs + k] * b * psc;
           if (want_post) post[cell * S + Pr.bidx[k]] = pv;
           if (want_ecnt && pv != 0.0) {
             double* tb = ecnt + (size_t)gr[r] * TAB_SZ;
@@ -541,7 +604,17 @@ bool plan(const tkfmi_problem* p, uint32_t flags, WPlan& pl, std::string& why) {
     d.post_off = post_off[b];
     const int64_t cells = (int64_t)(lx + 1) * (ly + 1);
     d.lat_off = off;
-    if (!(flags & TKFMI_FORWARD_ONLY)) off += cells * pl.NBP + cells;
+    if (!(flags & TKFMI_FORWARD_ONLY)) {
+      if (flags & TKFMI_CHECKPOINT) {
+        // K ~ sqrt(2 (Lx + Ly + 1)) minimises checkpoints (2 rows each) + segment
+        const int K = std::max(4, (int)std::ceil(std::sqrt(2.0 * (lx + ly + 1))));
+        const int64_t nck = (lx + ly) / K + 1, row = (int64_t)(lx + 1) * (pl.NBP + 1);
+        d.ck = K;
+        off += nck * 2 * row + (int64_t)K * row + 3 * row;
+      } else {
+        off += cells * pl.NBP + cells;
+      }
+    }
     d.ring_off = off;
     off += 3 * (int64_t)(lx + 1) * pl.NBP + 3 * (int64_t)(lx + 1);
     d.cur_off = off;

@@ -23,6 +23,7 @@ WANT_EMIT_COUNTS = 16
 TIME_KERNELS = 32
 CLASS_KERNEL = 64
 REUSE_DESCRIPTORS = 128
+CHECKPOINT = 256
 A_PAD = 21
 TAB_SZ = A_PAD * A_PAD + 2 * A_PAD
 
@@ -270,7 +271,7 @@ def kernel_flags(kernel=None, generic=False):
 
 def fb2d_batched(log_trans, state_types, state_group, log_tab, pairs, *,
                  forward_only=False, want_posteriors=False, want_counts=True,
-                 want_emit_counts=False, generic=False, kernel=None):
+                 want_emit_counts=False, generic=False, kernel=None, checkpoint=False):
     """Run the batched forward/backward kernel once.
 
     log_trans: float64 device tensor (S,S) or (B,S,S) of log transitions.
@@ -279,12 +280,14 @@ def fb2d_batched(log_trans, state_types, state_group, log_tab, pairs, *,
     pairs: PackedPairs.
     Returns a dict: 'log_p' (B,), and unless forward_only 'n' (B,S,S)
     (want_counts), 'posteriors' (flat, want_posteriors), 'emit_counts'
-    (B,G,TAB_SZ, want_emit_counts).
+    (B,G,TAB_SZ, want_emit_counts).  checkpoint: antidiagonal checkpointing
+    (TKFMI_CHECKPOINT: less lattice memory, one extra forward sweep; used by
+    the 17..162-state kernel).
     """
     r = BatchRunner(log_trans, state_types, state_group, log_tab, pairs,
                     forward_only=forward_only, want_posteriors=want_posteriors,
                     want_counts=want_counts, want_emit_counts=want_emit_counts,
-                    generic=generic, kernel=kernel)
+                    generic=generic, kernel=kernel, checkpoint=checkpoint)
     r.run(zero_outputs=False)  # outputs are freshly zeroed
     return r.results()
 
@@ -307,7 +310,8 @@ class BatchRunner:
 
     def __init__(self, log_trans, state_types, state_group, log_tab, pairs, *,
                  forward_only=False, want_posteriors=False, want_counts=True,
-                 want_emit_counts=False, generic=False, time_kernels=False, kernel=None):
+                 want_emit_counts=False, generic=False, time_kernels=False, kernel=None,
+                 checkpoint=False):
         self.L = lib()
         dev = log_trans.device
         if dev.type != "cuda":
@@ -335,6 +339,7 @@ class BatchRunner:
                       | (0 if want_counts else NO_COUNTS)
                       | (WANT_EMIT_COUNTS if (want_emit_counts and not fwd) else 0)
                       | kernel_flags(kernel, generic)
+                      | (CHECKPOINT if checkpoint else 0)
                       | (TIME_KERNELS if time_kernels else 0))
         B = pairs.B
         self.prob = Problem(B, S, _np_ptr(self.st), _np_ptr(self.sg), G, n_trans,

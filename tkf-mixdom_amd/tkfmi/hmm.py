@@ -84,7 +84,7 @@ def _types_list(state_types):
 
 def fb2d_batch(log_trans, state_types, xs, ys, log_tab, state_group=None, *,
                forward_only=False, want_posteriors=False, want_counts=True,
-               want_emit_counts=False, generic=False, kernel=None):
+               want_emit_counts=False, generic=False, kernel=None, checkpoint=False):
     """One launch over a ragged batch of pairs.
 
     log_trans: (S,S) or (B,S,S) log transitions; log_tab: (G,TAB) or
@@ -97,7 +97,9 @@ def fb2d_batch(log_trans, state_types, xs, ys, log_tab, state_group=None, *,
     ``generic=True`` (or kernel="generic") forces the any-layout kernel; the
     TKF and MixDom K*F<=3 layouts otherwise run on the cell-per-lane path,
     chained workgroups per pair by default or one workgroup per pair with
-    kernel="class".
+    kernel="class".  ``checkpoint=True`` keeps only every K-th antidiagonal's
+    forward state and recomputes the rest in the backward (the 17..162-state
+    kernel; same results, less lattice memory).
     """
     dev = device()
     lt = _dev64(log_trans, dev)
@@ -110,7 +112,7 @@ def fb2d_batch(log_trans, state_types, xs, ys, log_tab, state_group=None, *,
     out = _abi.fb2d_batched(lt, st, state_group, tab, pairs, forward_only=forward_only,
                             want_posteriors=want_posteriors, want_counts=want_counts,
                             want_emit_counts=want_emit_counts, generic=generic,
-                            kernel=kernel)
+                            kernel=kernel, checkpoint=checkpoint)
     if "posteriors" in out:
         out["posteriors"] = _abi.split_posteriors(out["posteriors"], pairs, S)
     return out
