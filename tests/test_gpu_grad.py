@@ -50,7 +50,7 @@ def _oracle_sum_logp(p, ts, xs, ys):
     return tot
 
 
-def _gpu_sum_logp(p, ts, xs, ys, grad=False, kernel=None):
+def _gpu_sum_logp(p, ts, xs, ys, grad=False, kernel=None, checkpoint=False):
     from tkfmi import vjp
     dev = torch.device("cuda", 0)
     tp = {k: torch.tensor(v, dtype=torch.float64, device=dev, requires_grad=grad)
@@ -58,7 +58,7 @@ def _gpu_sum_logp(p, ts, xs, ys, grad=False, kernel=None):
     tt = torch.tensor(ts, dtype=torch.float64, device=dev, requires_grad=grad)
     lp = vjp.mixdom2_log_prob_batch(tp["lam"], tp["mu"], tt, tp["dlam"], tp["dmu"], tp["dw"],
                                     tp["frag_w"], tp["ext"], tp["pis"], tp["S"], tp["cd"], xs, ys,
-                                    kernel=kernel)
+                                    kernel=kernel, checkpoint=checkpoint)
     tot = lp.sum()
     if not grad:
         return float(tot)
@@ -169,3 +169,21 @@ def test_tkf_reference_surfaces_gradients():
             want = (oracle(**bp) - oracle(**bm)) / (2 * H)
             got = float(np.asarray(th[name].grad.cpu().numpy())[idx])
             assert got == pytest.approx(want, rel=2e-5, abs=2e-6), (ext, name)
+
+
+def test_wide_gradients_with_checkpointing_equal_full_lattice():
+    """MixDom2 K=4 F=2 C=2 (42 states, the wide kernel): log P and every
+    parameter gradient are the same with antidiagonal checkpointing as with
+    the full lattice (the backward recomputes the forward bit for bit; the
+    emission-count atomics leave summation-order rounding)."""
+    rng = np.random.default_rng(12)
+    p = _mix2_params(rng, 4, 2, 2)
+    xs = [rng.integers(0, A, n) for n in (70, 5, 130)]
+    ys = [rng.integers(0, A, n) for n in (64, 9, 121)]
+    ts = [0.3, 1.1, 0.7]
+    lp0, g0 = _gpu_sum_logp(p, ts, xs, ys, grad=True)
+    lp1, g1 = _gpu_sum_logp(p, ts, xs, ys, grad=True, checkpoint=True)
+    assert lp0 == lp1
+    for k in g0:
+        np.testing.assert_allclose(g1[k], g0[k], rtol=1e-11, atol=1e-13 * max(1.0, float(np.abs(g0[k]).max())),
+                                   err_msg=k)

@@ -29,12 +29,14 @@ from . import _abi, hmm, mixdom, params, tables
 class _Spec:
     """Non-tensor inputs of one batched call."""
 
-    def __init__(self, state_types, state_group, xs, ys, device, generic=False, kernel=None):
+    def __init__(self, state_types, state_group, xs, ys, device, generic=False, kernel=None,
+                 checkpoint=False):
         self.state_types = hmm._types_list(state_types)
         self.state_group = list(state_group) if state_group is not None else [0] * len(self.state_types)
         self.pairs = _abi.PackedPairs(list(xs), list(ys), device)
         self.generic = generic
         self.kernel = kernel
+        self.checkpoint = checkpoint
 
 
 class FB2DLogProb(torch.autograd.Function):
@@ -47,7 +49,7 @@ class FB2DLogProb(torch.autograd.Function):
     def forward(ctx, log_trans, log_tab, spec):
         r = _abi.BatchRunner(log_trans.detach(), spec.state_types, spec.state_group,
                              log_tab.detach(), spec.pairs, want_emit_counts=True,
-                             generic=spec.generic, kernel=spec.kernel)
+                             generic=spec.generic, kernel=spec.kernel, checkpoint=spec.checkpoint)
         r.run(zero_outputs=False)
         ctx.save_for_backward(r.n, r.ecnt)
         ctx.shared = (log_trans.dim() == 2, log_tab.dim() == 2)
@@ -66,11 +68,13 @@ class FB2DLogProb(torch.autograd.Function):
 
 
 def log_prob_batch(log_trans, state_types, xs, ys, log_tab, state_group=None, generic=False,
-                   kernel=None):
+                   kernel=None, checkpoint=False):
     """Differentiable (B,) log-likelihoods of a ragged batch (one call).
-    ``generic`` / ``kernel`` select the kernel as in tkfmi.hmm.fb2d_batch."""
+    ``generic`` / ``kernel`` select the kernel and ``checkpoint`` turns on
+    antidiagonal checkpointing (the reference's rematerialisation in its
+    gradient path) as in tkfmi.hmm.fb2d_batch."""
     dev = log_trans.device if isinstance(log_trans, torch.Tensor) else hmm.device()
-    spec = _Spec(state_types, state_group, xs, ys, dev, generic, kernel)
+    spec = _Spec(state_types, state_group, xs, ys, dev, generic, kernel, checkpoint)
     return FB2DLogProb.apply(log_trans.to(torch.float64), log_tab.to(torch.float64), spec)
 
 
@@ -182,7 +186,7 @@ def build_class_subs(class_S_exch, class_pis, ts):
 
 def mixdom2_log_prob_batch(main_ins_rate, main_del_rate, ts, dom_ins_rates, dom_del_rates,
                            dom_weights, frag_weights, ext_rates, class_pis, class_S_exch,
-                           class_dist, xs, ys, *, generic=False, kernel=None):
+                           class_dist, xs, ys, *, generic=False, kernel=None, checkpoint=False):
     """(B,) MixDom2 log-likelihoods: per-(domain, fragment) class-mixture
     emissions (train/vjp.py:628-820; dp/hmm.py:673-783)."""
     dev = _dev()
@@ -196,7 +200,8 @@ def mixdom2_log_prob_batch(main_ins_rate, main_del_rate, ts, dom_ins_rates, dom_
     subs = build_class_subs(_as(class_S_exch, dev), _as(class_pis, dev), ts)
     tab = tables.per_class_tables(subs, _as(class_pis, dev), _as(class_dist, dev))  # (B,KF,TAB)
     groups = tables.state_groups_per_class(len(st), F)
-    return log_prob_batch(hmm.safe_log(chi), st, xs, ys, tab, groups, generic=generic, kernel=kernel)
+    return log_prob_batch(hmm.safe_log(chi), st, xs, ys, tab, groups, generic=generic, kernel=kernel,
+                          checkpoint=checkpoint)
 
 
 def mixdom2_log_prob(main_ins_rate, main_del_rate, t, dom_ins_rates, dom_del_rates, dom_weights,
