@@ -43,7 +43,7 @@ BYTES_PER_CELL_STATE = 16.0
 # write x1.00), keyed by (launch, pairs in it): profiles/fbc_kernels_c1_mixdom2_k3.md
 # (chained kernels, default class configuration) and
 # profiles/fbv2_kernel_c1_mixdom2_k3.md (one-workgroup-per-pair kernel).
-TRAFFIC = {("fbc_bwd_kernel<LMix<3,1>,3>", 823): 8.74e9, ("fbc_fwd_kernel<LMix<3,1>,3>", 823): 8.17e9,
+TRAFFIC = {("fbc_bwd_kernel<LMix<3,1>,3>", 823): 8.69e9, ("fbc_fwd_kernel<LMix<3,1>,3>", 823): 8.17e9,
            ("fbc_bwd_kernel<LMix<3,1>,2>", 189): 3.25e8, ("fbc_fwd_kernel<LMix<3,1>,2>", 189): 2.74e8,
            ("fbv2_kernel<LMix<3,1>,8>", 33): 3.13e9, ("fbv2_kernel<LMix<3,1>,4>", 269): 8.28e9,
            ("fbv2_kernel<LMix<3,1>,2>", 506): 5.14e9, ("fbv2_kernel<LMix<3,1>,1>", 216): 0.665e9}
