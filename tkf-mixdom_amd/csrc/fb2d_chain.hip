@@ -275,6 +275,16 @@ __global__ __launch_bounds__(W * 64) void fbc_fwd_kernel(CParams Pr, const doubl
         if (ok_run) ok_run = wait_ge(Pr.flagv + pd.fin, e_end, Pr.abortf, Pr.spin_max);
         fetch_chunk(s_xc, Pr.xring + pd.xin_off + (size_t)t * NX, (e_end - t) * NX);
       }
+      // ---- emissions at (i, j): issued first, so that their LDS round trip
+      // overlaps the ring read's
+      int yj = dpp_i32<DPP_WAVE_SHR1>(yj_prev);
+      if (lane == 0) yj = y0_next;
+      double emM[NG], emI[NG];
+#pragma unroll
+      for (int g = 0; g < NG; ++g) {
+        emM[g] = s_tab[g * TAB_SZ + xi * A_PAD + yj];
+        emI[g] = s_tab[g * TAB_SZ + A_PAD * A_PAD + yj];
+      }
       // ---- receive the predecessor row's h (lane-1, or a ring for lane 0)
       double rD[nD > 0 ? nD : 1], rM[nM > 0 ? nM : 1];
       int eR;
@@ -292,15 +302,6 @@ __global__ __launch_bounds__(W * 64) void fbc_fwd_kernel(CParams Pr, const doubl
 #pragma unroll
         for (int d = 0; d < nD; ++d) rD[d] = e[nM + d];
         eR = (int)e[NX - 1];
-      }
-      // ---- emissions at (i, j)
-      int yj = dpp_i32<DPP_WAVE_SHR1>(yj_prev);
-      if (lane == 0) yj = y0_next;
-      double emM[NG], emI[NG];
-#pragma unroll
-      for (int g = 0; g < NG; ++g) {
-        emM[g] = s_tab[g * TAB_SZ + xi * A_PAD + yj];
-        emI[g] = s_tab[g * TAB_SZ + A_PAD * A_PAD + yj];
       }
       // ---- new cell: raw products and per-type maxima.  No validity masks:
       // a valid cell's out-of-lattice predecessors (row -1: DPP zero fill;

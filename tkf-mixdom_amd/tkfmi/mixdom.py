@@ -66,7 +66,7 @@ def effective_trans_per_type(main_lam, main_mu, t, dom_lam, dom_mu, dom_w):
     T_nullnull = (z_flat[..., :, None] * dom_w[j_k][None, :]
                   * tau_null_MID[..., i_e[:, None], j_e[None, :]])
     eye = torch.eye(3 * K, dtype=torch.float64, device=dev)
-    closure = torch.linalg.inv(eye - T_nullnull)
+    closure = torch.linalg.inv_ex(eye - T_nullnull)[0]  # no singularity check: it would wait for the GPU
     closure_3d = closure.reshape(bt + (3 * K, K, 3))
     T_nullN_3d = T_nullN.reshape(bt + (K, 3, 5))
     closure_exit = torch.einsum('...ike,...kev->...kiv', closure_3d, T_nullN_3d)

@@ -79,12 +79,47 @@ def log_prob_batch(log_trans, state_types, xs, ys, log_tab, state_group=None, ge
 
 
 # ------------------------------------------------------------ substitution
+# Pade(13) numerator coefficients b_0..b_13 and the 1-norm bound theta_13 of
+# the scaling and squaring method (Higham, "The scaling and squaring method
+# for the matrix exponential revisited", SIAM J. Matrix Anal. Appl. 26, 2005,
+# Table 2.3), the method of scipy.linalg.expm and jax.scipy.linalg.expm.
+_PADE13 = (64764752532480000.0, 32382376266240000.0, 7771770303897600.0, 1187353796428800.0,
+           129060195264000.0, 10559470521600.0, 670442572800.0, 33522128640.0, 1323241920.0,
+           40840800.0, 960960.0, 16380.0, 182.0, 1.0)
+_THETA13 = 5.371920351148152
+_MAX_SQUARINGS = 16  # 1-norms up to theta_13 * 2**16 (~3.5e5)
+
+
+def expm(A):
+    """exp(A) for a batch of square FP64 matrices: degree-13 Pade with
+    scaling and squaring.  The scaling exponent is chosen per matrix on the
+    device and the squarings are masked, so the call never waits for the GPU
+    (torch.linalg.matrix_exp copies norms to the host to pick its degree);
+    autograd differentiates through it.  Matrices with 1-norm above
+    theta_13 * 2**16 get too few squarings."""
+    b = _PADE13
+    eye = torch.eye(A.shape[-1], dtype=A.dtype, device=A.device)
+    norm = A.abs().sum(-2).amax(-1)
+    s = torch.clamp(torch.ceil(torch.log2(norm / _THETA13)), min=0.0, max=float(_MAX_SQUARINGS))
+    s = torch.nan_to_num(s, nan=0.0).detach()
+    A = A * torch.exp2(-s)[..., None, None]
+    A2 = A @ A
+    A4 = A2 @ A2
+    A6 = A4 @ A2
+    U = A @ (A6 @ (b[13] * A6 + b[11] * A4 + b[9] * A2) + b[7] * A6 + b[5] * A4 + b[3] * A2 + b[1] * eye)
+    V = A6 @ (b[12] * A6 + b[10] * A4 + b[8] * A2) + b[6] * A6 + b[4] * A4 + b[2] * A2 + b[0] * eye
+    X = torch.linalg.solve_ex(V - U, V + U)[0]
+    for k in range(_MAX_SQUARINGS):
+        X = torch.where((s > k)[..., None, None], X @ X, X)
+    return X
+
+
 def transition_matrix(Q, t):
     """P(t) = expm(Q t) (core/ctmc.py:31-60, Pade with its canonical
     tangent); ``t`` may carry leading batch dimensions."""
     Q = torch.as_tensor(Q, dtype=torch.float64)
     t = torch.as_tensor(t, dtype=torch.float64, device=Q.device)
-    return torch.linalg.matrix_exp(t[..., None, None] * Q)
+    return expm(t[..., None, None] * Q)
 
 
 def build_Q_from_S_pi(S_exch, pi):
