@@ -164,12 +164,28 @@ __device__ inline void gemm_rows(const double* __restrict__ A, const double* __r
     d4 acc[CT];
 #pragma unroll
     for (int c = 0; c < CT; ++c) acc[c] = d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-    for (int kk = 0; kk < KL4; ++kk) {
-      const double a = va ? ap[4 * kk] : 0.0;
+    // A fragments (cell rows, 1280 B apart: one memory round trip each) are
+    // loaded one batch of 4 k-chunks ahead of the MFMAs that use them
+    static_assert(KL4 % 4 == 0, "LDS k-chunks in batches of 4");
+    double an[4];
 #pragma unroll
-      for (int c = 0; c < CT; ++c)
-        acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sbp[4 * kk * SB + 16 * c], acc[c], 0, 0, 0);
+    for (int q = 0; q < 4; ++q) an[q] = ap[4 * q];
+#pragma unroll 1
+    for (int kb = 0; kb < KL4; kb += 4) {
+      double a[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a[q] = va ? an[q] : 0.0;
+      if (kb + 4 < NBP / 4) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) an[q] = (kb + 4 + q < NBP / 4) ? ap[4 * (kb + 4 + q)] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int kk = kb + q;
+#pragma unroll
+        for (int c = 0; c < CT; ++c)
+          acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], sbp[4 * kk * SB + 16 * c], acc[c], 0, 0, 0);
+      }
     }
 #pragma unroll 2
     for (int kk = KL4; kk < NBP / 4; ++kk) {
