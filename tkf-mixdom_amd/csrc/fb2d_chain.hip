@@ -6,24 +6,29 @@
 //
 // The per-cell arithmetic is the one of the single-workgroup kernel in
 // fb2d_v2.hip (one lattice cell per lane, probability space with a binary
-// exponent per cell, DPP row shifts, P as an SGPR operand, MFMA counts).
-// What differs is how a pair is spread over the machine:
+// exponent per cell, DPP row broadcasts of P, MFMA counts).  What differs is
+// how a pair is spread over the machine:
 //  * A pair's bands of 64 rows are cut into a chain of G workgroups of at
 //    most W waves, one band per wave, so a long pair runs on G compute units
 //    at one wave per SIMD instead of on one compute unit at two.
 //  * Inside a workgroup, waves run 64 steps apart in lock-step and pass the
 //    boundary row through LDS (as in the single-workgroup kernel).  Between
-//    workgroups of a chain the boundary row goes through global memory:
-//    the producer publishes every CQ columns with a release store of a
-//    progress counter, the consumer acquires it and copies the next CQ
-//    entries into LDS.
-//  * Workgroups take tickets from a counter in launch order, and a pair's
-//    chain is ticketed in dependency order (top band first in the forward,
-//    bottom band first in the backward).  A workgroup therefore waits only
-//    on workgroups that already hold a ticket, i.e. are running: no
-//    deadlock whatever the occupancy.  Every wait is bounded; on a time-out
-//    the launch raises an abort flag, all waits end, and the affected pairs
-//    get a NaN log-likelihood.
+//    workgroups of a chain the boundary row goes through global memory as
+//    device-scope atomic stores; every CQ columns the producer waits for
+//    them to complete and stores a progress counter, the consumer acquires
+//    it and copies the next CQ entries into LDS.
+//  * Workgroups take tickets from a counter in launch order.  Tickets go by
+//    critical path (the steps from a workgroup's start to the end of its
+//    pair's chain), which falls along a chain, so a pair's chain is ticketed
+//    in dependency order (top band first in the forward, bottom band first
+//    in the backward).  A workgroup therefore waits only on workgroups that
+//    already hold a ticket, i.e. are running: no deadlock whatever the
+//    occupancy.  Every wait is bounded; on a time-out the launch raises an
+//    abort flag, all waits end, and the affected pairs get a NaN
+//    log-likelihood.
+//  * A backward workgroup has one more wave, the count wave: the recursion
+//    waves stage their cells' count operands in LDS each step and the count
+//    wave runs the expected-count MFMAs on its own SIMD.
 //  * The forward and the backward are separate launches on the class's
 //    stream; log P and its binary exponent pass through a per-pair slot.
 #include <hip/hip_runtime.h>
