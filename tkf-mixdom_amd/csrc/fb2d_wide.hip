@@ -468,22 +468,36 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
     // ---- B = g P^T, counts += Fs^T g
     gemm_rows<NT, gemm_ct(NT)>(cur1, PT, s_B, cur2, i_lo, i_hi, nullptr);
     if (want_counts) {
+      // 4 cells per step of the loop, a group of the wave's tiles at a time:
+      // the group's operand loads (two per tile, from the antidiagonal
+      // buffers) are in flight together and its MFMAs form independent
+      // chains, instead of one load round trip per MFMA
+      constexpr int TG = (NTILE + 1) / 2;  // tiles per group
       const int n = i_hi - i_lo + 1;
       const int nq = (n + 3) >> 2;
 #pragma unroll
-      for (int q = 0; q < NTILE; ++q) {
-        const int tile = w + kWaves * q;
-        if (tile >= NT * NT) break;
-        const int kt = tile / NT, k2t = tile - kt * NT;
-        d4 acc = cacc[q];
+      for (int g0 = 0; g0 < NTILE; g0 += TG) {
         for (int c4 = 0; c4 < nq; ++c4) {
           const int row = i_lo + 4 * c4 + (lane >> 4);
           const bool v = row <= i_hi;
-          const double a = v ? cur0[(size_t)row * NBP + 16 * kt + (lane & 15)] : 0.0;
-          const double bv = v ? cur1[(size_t)row * NBP + 16 * k2t + (lane & 15)] : 0.0;
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
+          const double* r0 = cur0 + (size_t)(v ? row : i_lo) * NBP + (lane & 15);
+          const double* r1 = cur1 + (size_t)(v ? row : i_lo) * NBP + (lane & 15);
+          double a[TG], bv[TG];
+#pragma unroll
+          for (int u = 0; u < TG; ++u) {
+            const int tile = min(w + kWaves * min(g0 + u, NTILE - 1), NT * NT - 1);
+            const int kt = tile / NT, k2t = tile - kt * NT;
+            const double x = r0[16 * kt], y = r1[16 * k2t];
+            a[u] = v ? x : 0.0;
+            bv[u] = v ? y : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < TG; ++u) {
+            const int q = g0 + u;
+            if (q < NTILE && w + kWaves * q < NT * NT)
+              cacc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bv[u], cacc[q], 0, 0, 0);
+          }
         }
-        cacc[q] = acc;
       }
     }
     __syncthreads();
