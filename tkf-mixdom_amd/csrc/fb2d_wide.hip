@@ -284,14 +284,29 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
   auto fstep = [&](int d, double* rg, double* rgE, double* fb, size_t fs, double* eb, size_t es, bool main) {
     const int i_lo = max(0, d - Ly), i_hi = min(d, Lx);
     const int sd = d % 3, s1 = (d + 2) % 3, s2 = (d + 1) % 3;  // slots of d, d-1, d-2
-    for (int i = i_lo + w; i <= i_hi; i += kWaves) {
+    // The residue codes and predecessor exponents of the cells this wave
+    // visits (i = i_lo + w + kWaves k) are loaded 64 cells at a time, lane l
+    // holding the l-th, and handed out with readlane: a cell's chain of
+    // dependent loads then starts at its emission-table reads.
+    int bx = 0, by = 0, beM = EXP_NONE, beI = EXP_NONE, beD = EXP_NONE;
+    int kc = 0;
+    for (int i = i_lo + w; i <= i_hi; i += kWaves, ++kc) {
+      if ((kc & 63) == 0) {
+        const int ci = i + kWaves * lane, cj = d - ci;
+        const bool in = ci <= i_hi;
+        bx = in ? xchar(ci) : 0;
+        by = in ? ychar(cj) : 0;
+        // predecessor exponents: M from (i-1, j-1) [d-2], I from (i, j-1) [d-1], D from (i-1, j) [d-1]
+        beM = (in && ci >= 1 && cj >= 1) ? (int)rgE[s2 * (Lx + 1) + (ci - 1)] : EXP_NONE;
+        beI = (in && cj >= 1) ? (int)rgE[s1 * (Lx + 1) + ci] : EXP_NONE;
+        beD = (in && ci >= 1) ? (int)rgE[s1 * (Lx + 1) + (ci - 1)] : EXP_NONE;
+      }
       const int j = d - i;
-      const int xi = xchar(i), yj = ychar(j);
-      // predecessor exponents: M from (i-1, j-1) [d-2], I from (i, j-1) [d-1], D from (i-1, j) [d-1]
+      const int xi = __builtin_amdgcn_readlane(bx, kc & 63), yj = __builtin_amdgcn_readlane(by, kc & 63);
       const bool okM = i >= 1 && j >= 1, okI = j >= 1, okD = i >= 1;
-      const int eM = okM ? (int)rgE[s2 * (Lx + 1) + (i - 1)] : EXP_NONE;
-      const int eI = okI ? (int)rgE[s1 * (Lx + 1) + i] : EXP_NONE;
-      const int eD = okD ? (int)rgE[s1 * (Lx + 1) + (i - 1)] : EXP_NONE;
+      const int eM = __builtin_amdgcn_readlane(beM, kc & 63);
+      const int eI = __builtin_amdgcn_readlane(beI, kc & 63);
+      const int eD = __builtin_amdgcn_readlane(beD, kc & 63);
       double raw[KR];
       double mM = 0.0, mI = 0.0, mD = 0.0;
 #pragma unroll
@@ -429,15 +444,26 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
     const double* eb = ck ? segE + (size_t)(d - d0) * (Lx + 1) : latE + d;
     const size_t es = ck ? 1 : (size_t)Ly;
     // ---- g (successor-pulled q), scaled forward values
-    for (int i = i_lo + w; i <= i_hi; i += kWaves) {
+    // successor and forward exponents of the wave's cells, 64 at a time
+    int beM = EXP_NONE, beI = EXP_NONE, beD = EXP_NONE, bEF = 0;
+    int kc = 0;
+    for (int i = i_lo + w; i <= i_hi; i += kWaves, ++kc) {
+      if ((kc & 63) == 0) {
+        const int ci = i + kWaves * lane, cj = d - ci;
+        const bool in = ci <= i_hi;
+        beM = (in && ci < Lx && cj < Ly) ? (int)ringE[s2 * (Lx + 1) + (ci + 1)] : EXP_NONE;
+        beI = (in && cj < Ly) ? (int)ringE[s1 * (Lx + 1) + ci] : EXP_NONE;
+        beD = (in && ci < Lx) ? (int)ringE[s1 * (Lx + 1) + (ci + 1)] : EXP_NONE;
+        bEF = in ? (int)eb[(size_t)ci * es] : 0;
+      }
       const int j = d - i;
       const bool okM = i < Lx && j < Ly, okI = j < Ly, okD = i < Lx;
-      const int eM = okM ? (int)ringE[s2 * (Lx + 1) + (i + 1)] : EXP_NONE;
-      const int eI = okI ? (int)ringE[s1 * (Lx + 1) + i] : EXP_NONE;
-      const int eD = okD ? (int)ringE[s1 * (Lx + 1) + (i + 1)] : EXP_NONE;
+      const int eM = __builtin_amdgcn_readlane(beM, kc & 63);
+      const int eI = __builtin_amdgcn_readlane(beI, kc & 63);
+      const int eD = __builtin_amdgcn_readlane(beD, kc & 63);
       int Eg = max(eM, max(eI, eD));
       if (Eg == EXP_NONE) Eg = 0;
-      const int EF = (int)eb[(size_t)i * es];
+      const int EF = __builtin_amdgcn_readlane(bEF, kc & 63);
       const double fsc = ldexp(inv_mP, EF + Eg - EP);
 #pragma unroll
       for (int r = 0; r < KR; ++r) {
@@ -502,10 +528,18 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
     }
     __syncthreads();
     // ---- normalise B, posteriors, emission counts, q = e . B for the predecessors
-    for (int i = i_lo + w; i <= i_hi; i += kWaves) {
+    int qx = 0, qy = 0;
+    int kq = 0;
+    for (int i = i_lo + w; i <= i_hi; i += kWaves, ++kq) {
+      if ((kq & 63) == 0) {  // residue codes of the wave's cells, 64 at a time
+        const int ci = i + kWaves * lane;
+        const bool in = ci <= i_hi;
+        qx = in ? xchar(ci) : 0;
+        qy = in ? ychar(d - ci) : 0;
+      }
       const int j = d - i;
       const bool corner = (i == Lx && j == Ly);
-      const int xi = xchar(i), yj = ychar(j);
+      const int xi = __builtin_amdgcn_readlane(qx, kq & 63), yj = __builtin_amdgcn_readlane(qy, kq & 63);
       double Bv[KR];
       double mx = 0.0;
 #pragma unroll
