@@ -289,6 +289,25 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
     // holding the l-th, and handed out with readlane: a cell's chain of
     // dependent loads then starts at its emission-table reads.
     int bx = 0, by = 0, beM = EXP_NONE, beI = EXP_NONE, beD = EXP_NONE;
+    // raw products (predecessor value x emission) of cell i, batch index kc;
+    // the next cell's are loaded before this cell's stores, so its loads
+    // overlap this cell's reductions
+    auto load_raw = [&](int i, int kc, double (&raw)[KR]) {
+      const int j = d - i;
+      const int xi = __builtin_amdgcn_readlane(bx, kc & 63), yj = __builtin_amdgcn_readlane(by, kc & 63);
+      const bool okM = i >= 1 && j >= 1, okI = j >= 1, okD = i >= 1;
+#pragma unroll
+      for (int r = 0; r < KR; ++r) {
+        const int k = lane + 64 * r;
+        double v = 0.0;
+        if (ty[r] == ST_M && okM) v = rg[((size_t)s2 * (Lx + 1) + i - 1) * NBP + k] * emit(tab, ST_M, gr[r], xi, yj, i, j);
+        else if (ty[r] == ST_I && okI) v = rg[((size_t)s1 * (Lx + 1) + i) * NBP + k] * emit(tab, ST_I, gr[r], xi, yj, i, j);
+        else if (ty[r] == ST_D && okD) v = rg[((size_t)s1 * (Lx + 1) + i - 1) * NBP + k] * emit(tab, ST_D, gr[r], xi, yj, i, j);
+        raw[r] = v;
+      }
+    };
+    double pre[KR];
+    bool have_pre = false;
     int kc = 0;
     for (int i = i_lo + w; i <= i_hi; i += kWaves, ++kc) {
       if ((kc & 63) == 0) {
@@ -302,21 +321,24 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
         beD = (in && ci >= 1) ? (int)rgE[s1 * (Lx + 1) + (ci - 1)] : EXP_NONE;
       }
       const int j = d - i;
-      const int xi = __builtin_amdgcn_readlane(bx, kc & 63), yj = __builtin_amdgcn_readlane(by, kc & 63);
-      const bool okM = i >= 1 && j >= 1, okI = j >= 1, okD = i >= 1;
       const int eM = __builtin_amdgcn_readlane(beM, kc & 63);
       const int eI = __builtin_amdgcn_readlane(beI, kc & 63);
       const int eD = __builtin_amdgcn_readlane(beD, kc & 63);
       double raw[KR];
+      if (have_pre) {
+#pragma unroll
+        for (int r = 0; r < KR; ++r) raw[r] = pre[r];
+      } else {
+        load_raw(i, kc, raw);
+      }
+      have_pre = i + kWaves <= i_hi && ((kc + 1) & 63) != 0;
+      if (have_pre) load_raw(i + kWaves, kc + 1, pre);
       double mM = 0.0, mI = 0.0, mD = 0.0;
 #pragma unroll
       for (int r = 0; r < KR; ++r) {
-        const int k = lane + 64 * r;
-        double v = 0.0;
-        if (ty[r] == ST_M && okM) { v = rg[((size_t)s2 * (Lx + 1) + i - 1) * NBP + k] * emit(tab, ST_M, gr[r], xi, yj, i, j); mM = fmax(mM, v); }
-        else if (ty[r] == ST_I && okI) { v = rg[((size_t)s1 * (Lx + 1) + i) * NBP + k] * emit(tab, ST_I, gr[r], xi, yj, i, j); mI = fmax(mI, v); }
-        else if (ty[r] == ST_D && okD) { v = rg[((size_t)s1 * (Lx + 1) + i - 1) * NBP + k] * emit(tab, ST_D, gr[r], xi, yj, i, j); mD = fmax(mD, v); }
-        raw[r] = v;
+        if (ty[r] == ST_M) mM = fmax(mM, raw[r]);
+        else if (ty[r] == ST_I) mI = fmax(mI, raw[r]);
+        else if (ty[r] == ST_D) mD = fmax(mD, raw[r]);
       }
       mM = wave_max(mM);
       mI = wave_max(mI);
