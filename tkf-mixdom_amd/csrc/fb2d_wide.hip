@@ -221,10 +221,61 @@ __device__ inline void stage_b(const double* __restrict__ Bm, double* __restrict
   }
 }
 
+// The NT x NT count tiles (16 x 16 state blocks: row kt holds forward-value
+// fragment kt, column k2t g fragment k2t) are dealt out to the waves, tile
+// w + kWaves q to wave w.  A wave's tiles use most fragments more than once,
+// so a 4-cell step loads each fragment the wave needs once (at NT = 10: 15
+// loads for 13 tiles instead of 26), all in flight together.
+__host__ __device__ constexpr int count_ntile(int NT) { return (NT * NT + kWaves - 1) / kWaves; }
+__host__ __device__ constexpr bool count_uses(int NT, int W0, int frag, bool row) {
+  for (int tile = W0; tile < NT * NT; tile += kWaves)
+    if ((row ? tile / NT : tile % NT) == frag) return true;
+  return false;
+}
+
+// counts += F^T g over the cells i_lo..i_hi of one antidiagonal, for the
+// tiles of wave W0.  cur0 rows hold the scaled forward values of the cells,
+// cur1 rows their g; 4 cells per MFMA (lane >> 4 picks the cell, lane & 15
+// the state within the fragment).
+template <int NT, int W0, int NTILE>
+__device__ __forceinline__ void count_antidiag(const double* __restrict__ cur0, const double* __restrict__ cur1,
+                                               int i_lo, int i_hi, int lane, d4 (&cacc)[NTILE]) {
+  constexpr int NBP = 16 * NT;
+  const int nq = (i_hi - i_lo + 4) >> 2;
+  for (int c4 = 0; c4 < nq; ++c4) {
+    const int row = i_lo + 4 * c4 + (lane >> 4);
+    const bool v = row <= i_hi;
+    const double* r0 = cur0 + (size_t)(v ? row : i_lo) * NBP + (lane & 15);
+    const double* r1 = cur1 + (size_t)(v ? row : i_lo) * NBP + (lane & 15);
+    double fa[NT], fb[NT];
+#pragma unroll
+    for (int f = 0; f < NT; ++f) {
+      if (count_uses(NT, W0, f, true)) { const double x = r0[16 * f]; fa[f] = v ? x : 0.0; }
+      if (count_uses(NT, W0, f, false)) { const double y = r1[16 * f]; fb[f] = v ? y : 0.0; }
+    }
+#pragma unroll
+    for (int q = 0; q < NTILE; ++q) {
+      const int tile = W0 + kWaves * q;
+      if (tile < NT * NT)
+        cacc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[tile / NT], fb[tile % NT], cacc[q], 0, 0, 0);
+    }
+  }
+}
+
+// count_antidiag for the runtime wave index w (one instantiation per wave,
+// so that fragment and tile indices are compile-time register indices).
+template <int NT, int NTILE, int W0 = 0>
+__device__ __forceinline__ void count_antidiag_w(int w, const double* __restrict__ cur0,
+                                                 const double* __restrict__ cur1, int i_lo, int i_hi, int lane,
+                                                 d4 (&cacc)[NTILE]) {
+  if (w == W0) count_antidiag<NT, W0, NTILE>(cur0, cur1, i_lo, i_hi, lane, cacc);
+  else if constexpr (W0 + 1 < kWaves) count_antidiag_w<NT, NTILE, W0 + 1>(w, cur0, cur1, i_lo, i_hi, lane, cacc);
+}
+
 template <int NT>
 __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
   constexpr int NBP = 16 * NT;
-  constexpr int NTILE = (NT * NT + kWaves - 1) / kWaves;  // count tiles per wave
+  constexpr int NTILE = count_ntile(NT);                   // count tiles per wave
   constexpr int KR = (NBP + 63) / 64;                      // state rounds per lane
   __shared__ double s_misc[4];
   extern __shared__ double s_B[];  // rows of P (forward) or PT (backward)
@@ -515,39 +566,8 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
     __syncthreads();
     // ---- B = g P^T, counts += Fs^T g
     gemm_rows<NT, gemm_ct(NT)>(cur1, PT, s_B, cur2, i_lo, i_hi, nullptr);
-    if (want_counts) {
-      // 4 cells per step of the loop, a group of the wave's tiles at a time:
-      // the group's operand loads (two per tile, from the antidiagonal
-      // buffers) are in flight together and its MFMAs form independent
-      // chains, instead of one load round trip per MFMA
-      constexpr int TG = (NTILE + 1) / 2;  // tiles per group
-      const int n = i_hi - i_lo + 1;
-      const int nq = (n + 3) >> 2;
-#pragma unroll
-      for (int g0 = 0; g0 < NTILE; g0 += TG) {
-        for (int c4 = 0; c4 < nq; ++c4) {
-          const int row = i_lo + 4 * c4 + (lane >> 4);
-          const bool v = row <= i_hi;
-          const double* r0 = cur0 + (size_t)(v ? row : i_lo) * NBP + (lane & 15);
-          const double* r1 = cur1 + (size_t)(v ? row : i_lo) * NBP + (lane & 15);
-          double a[TG], bv[TG];
-#pragma unroll
-          for (int u = 0; u < TG; ++u) {
-            const int tile = min(w + kWaves * min(g0 + u, NTILE - 1), NT * NT - 1);
-            const int kt = tile / NT, k2t = tile - kt * NT;
-            const double x = r0[16 * kt], y = r1[16 * k2t];
-            a[u] = v ? x : 0.0;
-            bv[u] = v ? y : 0.0;
-          }
-#pragma unroll
-          for (int u = 0; u < TG; ++u) {
-            const int q = g0 + u;
-            if (q < NTILE && w + kWaves * q < NT * NT)
-              cacc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bv[u], cacc[q], 0, 0, 0);
-          }
-        }
-      }
-    }
+    if (want_counts)
+      count_antidiag_w<NT, NTILE>(w, cur0, cur1, i_lo, i_hi, lane, cacc);
     __syncthreads();
     // ---- normalise B, posteriors, emission counts, q = e . B for the predecessors
     int qx = 0, qy = 0;
