@@ -274,6 +274,8 @@ def main():
                     help="transition counts only (no expected emission counts)")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the c0 / c2 / c3 / fb1d measurements")
+    ap.add_argument("--configs", default="fb1d,c0,c3,c2",
+                    help="which of the fb1d / c0 / c3 / c2 measurements to run (comma list)")
     ap.add_argument("--c2-pairs", type=int, default=256,
                     help="pairs of the c2 (162-state) measurement; 0 skips it")
     ap.add_argument("--kernel", default="chain", choices=["chain", "class", "generic"],
@@ -400,17 +402,23 @@ def main():
         pl = plain_per_pair(w, emit=not args.no_emit)
         out["plain_per_pair"] = pl
         out["batched_over_plain"] = value / pl["value"]
-    if not args.no_configs:
+    sel = set() if args.no_configs else {c.strip() for c in args.configs.split(",") if c.strip()}
+    if sel - {"fb1d", "c0", "c3", "c2"}:
+        raise SystemExit(f"unknown --configs entries: {sorted(sel - {'fb1d', 'c0', 'c3', 'c2'})}")
+    if sel:
         cfg = {}
-        _log("1D alignment-constrained E-step DP")
-        cfg["fb1d"] = fb1d_bench(w, args.steps)
-        _log(f"fb1d: {cfg['fb1d']['ms']:.3f} ms, {cfg['fb1d']['value']:.3e} column-states/s")
-        _log("c0 TKF91 forward-only")
-        cfg["c0"] = c0_bench(args.steps)
-        _log("c3 gradients, 200 pairs")
-        cfg["c3"] = c3_bench(max(1, args.steps // 2))
-        _log(f"c3: {cfg['c3']['ms']:.1f} ms per gradient")
-        if args.c2_pairs > 0:
+        if "fb1d" in sel:
+            _log("1D alignment-constrained E-step DP")
+            cfg["fb1d"] = fb1d_bench(w, args.steps)
+            _log(f"fb1d: {cfg['fb1d']['ms']:.3f} ms, {cfg['fb1d']['value']:.3e} column-states/s")
+        if "c0" in sel:
+            _log("c0 TKF91 forward-only")
+            cfg["c0"] = c0_bench(args.steps)
+        if "c3" in sel:
+            _log("c3 gradients, 200 pairs")
+            cfg["c3"] = c3_bench(max(1, args.steps // 2))
+            _log(f"c3: {cfg['c3']['ms']:.1f} ms per gradient")
+        if "c2" in sel and args.c2_pairs > 0:
             _log(f"c2 162-state wide kernel, {args.c2_pairs} pairs")
             cfg["c2"] = c2_bench(args.c2_pairs)
             _log(f"c2: {cfg['c2']['ms']:.0f} ms, {cfg['c2']['fp64_tflops']:.2f} TF/s")
