@@ -288,8 +288,9 @@ def main():
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE against known traffic")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(pairs shard across GPUs by cost; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path "
+              "(pairs would be dealt out to GPUs by their DP cost, one process per GPU)",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
