@@ -287,6 +287,9 @@ def main():
                     help="only run a device-to-device copy of this many MiB (x5), to calibrate "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE against known traffic")
     args = ap.parse_args()
+    sel = set() if args.no_configs else {c.strip() for c in args.configs.split(",") if c.strip()}
+    if sel - {"fb1d", "c0", "c3", "c2"}:
+        raise SystemExit(f"unknown --configs entries: {sorted(sel - {'fb1d', 'c0', 'c3', 'c2'})}")
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: there is no multi-GPU path "
               "(pairs would be dealt out to GPUs by their DP cost, one process per GPU)",
@@ -403,9 +406,6 @@ def main():
         pl = plain_per_pair(w, emit=not args.no_emit)
         out["plain_per_pair"] = pl
         out["batched_over_plain"] = value / pl["value"]
-    sel = set() if args.no_configs else {c.strip() for c in args.configs.split(",") if c.strip()}
-    if sel - {"fb1d", "c0", "c3", "c2"}:
-        raise SystemExit(f"unknown --configs entries: {sorted(sel - {'fb1d', 'c0', 'c3', 'c2'})}")
     if sel:
         cfg = {}
         if "fb1d" in sel:
