@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define TKFMI_ABI_VERSION 3
+#define TKFMI_ABI_VERSION 4
 
 /* error codes */
 #define TKFMI_OK 0
@@ -320,10 +320,9 @@ typedef struct tkfmi_builder {
   int32_t pad;
   double lam, mu, ext;                 /* TKF models                          */
   const tkfmi_mixdom_model* mix;       /* host, TKFMI_MODEL_MIXDOM            */
-  const double* eigvec;  /* device [C,A,A]: eigenvectors of (S + S^T)/2,
-                            S = D^1/2 Q D^-1/2 (Q reversible w.r.t. pi)      */
-  const double* eigval;  /* device [C,A]                                     */
-  const double* pi;      /* device [C,A]                                     */
+  const double* Q;       /* device [C,A,A] rate matrices (rows sum to zero;
+                            reversibility is not assumed)                    */
+  const double* pi;      /* device [C,A] equilibria (emission pi factors)    */
   const double* class_dist;  /* device [K,F,C] (per-class) or NULL           */
 } tkfmi_builder;
 
@@ -334,7 +333,10 @@ size_t tkfmi_build_workspace_bytes(const tkfmi_builder* b, int32_t B);
  * For every pair b at t[b]: log_trans [B,S,S] (TKF91 / TKF91-cond / TKF92,
  * core/params.py:29-108, or the collapsed MixDom chi, models/mixdom.py:252-423,
  * obtained as the reduced matrix of the exploded machine) and log_tab
- * [B,G,TKFMI_TAB_SZ] built from P_c(t) = expm(Q_c t) (core/ctmc.py:31-60).
+ * [B,G,TKFMI_TAB_SZ] built from P_c(t) = expm(Q_c t) (core/ctmc.py:31-60),
+ * evaluated on the device by the Pade scaling-and-squaring rule of
+ * jax.scipy.linalg.expm (degree 3..13 by the 1-norm of Q t, floor(log2(
+ * norm / 5.3719...)) squarings, NaN beyond 16 squarings) for any Q.
  * Replaces the per-pair builders of train_pfam.py:1613-1664, 1940-1988 and
  * train/tkf92_svi_bw.py:148-152.  G = 1 (shared), K (per-domain) or K*F
  * (per-class).  One workgroup per pair.

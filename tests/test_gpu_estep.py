@@ -150,26 +150,14 @@ def test_e_step_mixdom2_matches_oracle():
         _assert_close(ss["class_U"][c], U, rtol=1e-9, what="U")
 
 
-def test_process_pairs_batched_2d_delta_matches_oracle():
-    from oracle import bdi, dp2d, emissions, hr as ohr, mixdom, suffstats, tkf
+def _check_process_pairs_2d(cp, Qs, pis, K, F, pairs):
+    """process_pairs_batched_2d's delta dict vs the oracle chain: chi and
+    scipy expm emissions at the float32-rounded t, oracle forward/backward,
+    emission counts, Holmes-Rubin and the restored top-level statistics."""
+    from oracle import bdi, dp2d, emissions, hr as ohr, mixdom, suffstats
     from tkfmi import estep
-    rng = np.random.default_rng(4)
-    K, F = 2, 1
-    Qs, pis = [], []
-    for _ in range(K):
-        Q, pi, _ = _rand_Q(rng)
-        Qs.append(Q)
-        pis.append(pi)
-    Qs, pis = np.stack(Qs), np.stack(pis)
-    cp = dict(main_ins=0.04, main_del=0.05, dom_ins=np.array([0.03, 0.045]),
-              dom_del=np.array([0.05, 0.06]), dom_weights=np.array([0.4, 0.6]),
-              frag_weights=np.array([[1.0], [1.0]]), ext_rates=np.array([[0.7], [0.85]]))
     st = mixdom.state_types(K, F)
     N = len(st)
-    pairs = []
-    for b in range(5):
-        x, y = rng.integers(0, A, 20 + 9 * b), rng.integers(0, A, 25 + 7 * b)
-        pairs.append((x, y, None, None, None, float(rng.uniform(0.1, 1.5))))
     delta = estep.process_pairs_batched_2d(pairs, cp, st, Qs[0], pis[0], N, dom_Qs=Qs, dom_pis=pis,
                                            n_dom=K, n_frag=F)
     agg = np.zeros((N, N)); ll = 0.0
@@ -192,7 +180,7 @@ def test_process_pairs_batched_2d_delta_matches_oracle():
             w_, u_ = ohr.weighted_stats(Qs[k], pis[k], t, m[k])
             W[k] += w_; U[k] += u_
         recs.append((n, t))
-    assert delta["n_pairs"] == 5 and delta["n_files"] == 1
+    assert delta["n_pairs"] == len(pairs) and delta["n_files"] == 1
     assert delta["total_ll"] == pytest.approx(ll, rel=1e-10)
     _assert_close(delta["agg_n_chi"], agg, what="agg_n_chi")
     _assert_close(delta["dom_match_counts"], dm, what="dom_match")
@@ -202,14 +190,55 @@ def test_process_pairs_batched_2d_delta_matches_oracle():
     _assert_close(delta["dom_W"], W, rtol=1e-9, what="W")
     _assert_close(delta["dom_U"], U, rtol=1e-9, what="U")
     es = delta["exact_ss"]
-    top = sum(suffstats.exact_suffstats(n, cp["main_ins"], cp["main_del"], t, cp["dom_ins"],
-                                        cp["dom_del"], cp["dom_weights"], cp["frag_weights"],
-                                        cp["ext_rates"])["top_5x5"] for n, t in recs)
-    _assert_close(es["top_5x5"], top, rtol=1e-9, what="top_5x5")
-    EBs = []
-    for n, t in recs:
-        tt = suffstats.exact_suffstats(n, cp["main_ins"], cp["main_del"], t, cp["dom_ins"],
-                                       cp["dom_del"], cp["dom_weights"], cp["frag_weights"],
-                                       cp["ext_rates"])["top_5x5"]
-        EBs.append(bdi.bdi_stats(tt, cp["main_ins"], cp["main_del"], t, t * tt[:, 4].sum())[0])
+    sss = [suffstats.exact_suffstats(n, cp["main_ins"], cp["main_del"], t, cp["dom_ins"],
+                                     cp["dom_del"], cp["dom_weights"], cp["frag_weights"],
+                                     cp["ext_rates"])["top_5x5"] for n, t in recs]
+    _assert_close(es["top_5x5"], sum(sss), rtol=1e-9, what="top_5x5")
+    EBs = [bdi.bdi_stats(tt, cp["main_ins"], cp["main_del"], t, t * tt[:, 4].sum())[0]
+           for tt, (_, t) in zip(sss, recs)]
     assert es["top_E_B"] == pytest.approx(sum(EBs), rel=1e-8)
+
+
+def test_process_pairs_batched_2d_delta_matches_oracle():
+    rng = np.random.default_rng(4)
+    K, F = 2, 1
+    Qs, pis = [], []
+    for _ in range(K):
+        Q, pi, _ = _rand_Q(rng)
+        Qs.append(Q)
+        pis.append(pi)
+    Qs, pis = np.stack(Qs), np.stack(pis)
+    cp = dict(main_ins=0.04, main_del=0.05, dom_ins=np.array([0.03, 0.045]),
+              dom_del=np.array([0.05, 0.06]), dom_weights=np.array([0.4, 0.6]),
+              frag_weights=np.array([[1.0], [1.0]]), ext_rates=np.array([[0.7], [0.85]]))
+    pairs = []
+    for b in range(5):
+        x, y = rng.integers(0, A, 20 + 9 * b), rng.integers(0, A, 25 + 7 * b)
+        pairs.append((x, y, None, None, None, float(rng.uniform(0.1, 1.5))))
+    _check_process_pairs_2d(cp, Qs, pis, K, F, pairs)
+
+
+@pytest.mark.parametrize("name", ["bw_d8f2_fullseed_15iter", "bw_d3f2_pfam100_20iter",
+                                  "bw_d5f2_pfam100_20iter"])
+def test_process_pairs_batched_2d_on_reference_parameters(name):
+    """The E-step delta on the reference's own trained parameter sets
+    (python/params/best, via tests/golden/params_best.npz): K = 8, 3, 5
+    domains of F = 2 fragments (82, 32, 52 states).  bw_d8f2_fullseed's rate
+    matrices are not exactly reversible and are used as given."""
+    import os
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "params_best.npz"), allow_pickle=False)
+    g = lambda k: d[f"{name}/{k}"]
+    Qs, pis = g("dom_Qs"), g("dom_pis")
+    K, F = g("ext_rates").shape
+    cp = dict(main_ins=float(g("main_ins")), main_del=float(g("main_del")), dom_ins=g("dom_ins"),
+              dom_del=g("dom_del"), dom_weights=g("dom_weights"), frag_weights=g("frag_weights"),
+              ext_rates=g("ext_rates"))
+    rng = np.random.default_rng(11)
+    pairs = []
+    for b in range(4):
+        x = rng.choice(A, 18 + 11 * b, p=pis[b % K])
+        y = rng.choice(A, 15 + 13 * b, p=pis[(b + 1) % K])
+        pairs.append((x, y, None, None, None, float(rng.uniform(0.05, 2.0))))
+    pairs.append((rng.integers(0, A, 3), np.zeros(0, np.int64), None, None, None, float(g("t_rep"))))
+    _check_process_pairs_2d(cp, Qs, pis, K, F, pairs)

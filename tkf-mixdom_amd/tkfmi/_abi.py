@@ -14,7 +14,7 @@ import torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libtkfmi.so")
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 FORWARD_ONLY = 1
 WANT_POSTERIORS = 2
 NO_COUNTS = 4
@@ -111,8 +111,7 @@ class Builder(ctypes.Structure):
         ("mu", ctypes.c_double),
         ("ext", ctypes.c_double),
         ("mix", ctypes.POINTER(MixdomModel)),
-        ("eigvec", _c_p),
-        ("eigval", _c_p),
+        ("Q", _c_p),
         ("pi", _c_p),
         ("class_dist", _c_p),
     ]

@@ -14,7 +14,8 @@ def build(arch="gfx950", force=True, verbose=False):
     if force:
         subprocess.run(["make", "-C", CSRC, "clean"], check=True, env=env,
                        stdout=subprocess.DEVNULL)
-    r = subprocess.run(["make", "-C", CSRC, f"ARCH={arch}"], env=env,
+    jobs = str(max(1, min(8, os.cpu_count() or 1)))
+    r = subprocess.run(["make", "-C", CSRC, f"-j{jobs}", f"ARCH={arch}"], env=env,
                        capture_output=True, text=True)
     if verbose or r.returncode != 0:
         print(r.stdout[-4000:])

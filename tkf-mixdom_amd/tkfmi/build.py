@@ -3,9 +3,10 @@
 per-pair torch builders when no gradient is needed (E-steps, serving).
 
 A :class:`DeviceBuilder` holds everything that does not depend on t: the
-model's structural parameters, the eigendecomposition of each substitution
-rate matrix (Q must be reversible w.r.t. its pi, as every GTR / LG matrix
-is) and the class distribution.  ``build(ts)`` then returns
+model's structural parameters, the substitution rate matrices (any Q: the
+kernel exponentiates Q t by Pade scaling and squaring, as the reference's
+transition_matrix does, core/ctmc.py:31-60, so reversibility is not
+assumed) and the class distribution.  ``build(ts)`` then returns
 (log_trans (B,S,S), log_tab (B,G,TAB)) for per-pair t.
 """
 import ctypes
@@ -23,19 +24,6 @@ EMIT = {"shared": 0, "per_domain": 1, "per_class": 2}
 def _np64(x):
     return np.ascontiguousarray(np.asarray(x.detach().cpu() if isinstance(x, torch.Tensor) else x,
                                            dtype=np.float64))
-
-
-def reversible_eig(Qs, pis, tol=1e-9):
-    """Eigenvectors / eigenvalues of (S + S^T)/2, S = D^1/2 Q D^-1/2, per
-    component; raises if Q is not reversible w.r.t. pi."""
-    Qs, pis = _np64(Qs), _np64(pis)
-    sp = np.sqrt(pis)
-    S = Qs * (sp[:, :, None] / sp[:, None, :])
-    asym = np.abs(S - np.swapaxes(S, 1, 2)).max() / max(np.abs(S).max(), 1e-300)
-    if asym > tol:
-        raise ValueError(f"rate matrix not reversible w.r.t. pi (asymmetry {asym:.2e})")
-    lam, V = np.linalg.eigh((S + np.swapaxes(S, 1, 2)) / 2)
-    return np.ascontiguousarray(V), np.ascontiguousarray(lam)
 
 
 class DeviceBuilder:
@@ -56,9 +44,10 @@ class DeviceBuilder:
         if Qs.ndim == 2:
             Qs, pis = Qs[None], pis[None]
         C, A = pis.shape
-        V, lam_e = reversible_eig(Qs, pis)
+        if Qs.shape != (C, A, A):
+            raise ValueError(f"Qs must be ({C}, {A}, {A}), got {Qs.shape}")
         d = lambda a: torch.as_tensor(a, dtype=torch.float64, device=self.dev).contiguous()
-        self.V, self.ev, self.pi = d(V), d(lam_e), d(pis)
+        self.Q, self.pi = d(Qs), d(pis)
         self.cd = d(_np64(class_dist)) if class_dist is not None else None
         self.mix = None
         if model == "mixdom":
@@ -82,7 +71,7 @@ class DeviceBuilder:
         self.c = _abi.Builder(MODELS[model], EMIT[emission], int(bool(log_floor)), C, A, 0,
                               float(lam), float(mu), float(ext),
                               ctypes.pointer(self.mix.c) if self.mix else None,
-                              _abi._ptr(self.V), _abi._ptr(self.ev), _abi._ptr(self.pi),
+                              _abi._ptr(self.Q), _abi._ptr(self.pi),
                               _abi._ptr(self.cd))
 
     def build(self, ts, stream=None):
