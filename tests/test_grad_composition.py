@@ -150,3 +150,33 @@ def test_pade13_expm_matches_scipy_and_its_gradient():
     (vjp.expm(At) * torch.tensor(W)).sum().backward()
     fd = ((scipy.linalg.expm(Am + H * D) - scipy.linalg.expm(Am - H * D)) * W).sum() / (2 * H)
     assert abs(float((At.grad.numpy() * D).sum()) - fd) <= 1e-6 * max(1.0, abs(fd))
+
+
+@pytest.mark.parametrize("K,F", [(1, 1), (3, 1), (2, 2), (3, 2), (8, 2), (8, 4)])
+def test_chi_by_elimination_matches_closed_form(K, F):
+    """tkfmi.mixdom.build_nested_trans (reduced matrix of the exploded
+    machine, one block solve) against the oracle's closed-form collapse
+    (models/mixdom.py:252-423 restated) at 1e-13 relative, exact zeros
+    included, batched over t; and the exploded machine's rows sum to one
+    (End excepted, which has no out-transitions)."""
+    from tkfmi import mixdom as tm
+    rng = np.random.default_rng(100 + 10 * K + F)
+    dl, dm = rng.uniform(0.02, 0.06, K), rng.uniform(0.05, 0.08, K)
+    dw, fw = rng.dirichlet(np.ones(K)), rng.dirichlet(np.ones(F), size=K)
+    ext = rng.uniform(0.5, 0.9, (K, F, 1)) * rng.dirichlet(np.ones(F), size=(K, F))
+    ts = np.array([0.05, 0.3, 1.0, 2.0])
+    got = tm.build_nested_trans(0.04, 0.05, torch.tensor(ts), dl, dm, dw, fw, ext).numpy()
+    for i, t in enumerate(ts):
+        want = mixdom.build_nested_trans(0.04, 0.05, t, dl, dm, dw, fw, ext)
+        nz = want != 0
+        assert (got[i][~nz] == 0).all()
+        assert (np.abs(got[i][nz] - want[nz]) / np.abs(want[nz])).max() <= 1e-13
+    T = tm.exploded_trans(0.04, 0.05, 0.7, dl, dm, dw, fw, ext).numpy()
+    rows = T.sum(axis=1)
+    assert rows[1] == 0.0
+    np.testing.assert_allclose(np.delete(rows, 1), 1.0, rtol=1e-13)
+    # MixDom1 (K,F) extension rates go on the diagonal (models/mixdom.py:284-287)
+    e1 = rng.uniform(0.3, 0.8, (K, F))
+    np.testing.assert_allclose(tm.build_nested_trans(0.04, 0.05, 0.5, dl, dm, dw, fw, e1).numpy(),
+                               mixdom.build_nested_trans(0.04, 0.05, 0.5, dl, dm, dw, fw, e1),
+                               rtol=1e-13, atol=1e-300)

@@ -1,20 +1,38 @@
 """Collapsed MixDom chi in torch (FP64, differentiable, batched over t).
 
-Reference: models/mixdom.py -- state layout (:48-77), exit-type decomposed
-effective transitions with the (I - T_nullnull)^-1 closure (:104-214) and the
-nested chi (:252-423).  ``t`` may be a scalar or a (B,) tensor; the result is
-(N, N) or (B, N, N), N = 2 + 5 K F.  Index 0 = SS, 1 = EE, body state
-2 + d*5F + uv*F + f with uv in (MM, MI, MD, II, DD).
+chi is obtained as the reduced matrix of the exploded MixDom machine -- the
+route libtkfmi's device builder takes -- instead of the reference's
+closed-form collapse (models/mixdom.py:104-214, 252-423); both give the same
+matrix, and the oracle checks that identity.
+
+The exploded machine (the reference's fully exploded model,
+models/exact_suffstats_batch.py:61-253) has N0 = 8 + 8K + 15KF states: Start,
+End, the three top-level domain-type states, per domain three domain-type
+states and five fragment-type states, per (domain, fragment) five
+fragment-entry, five emitting and five fragment-end states, and three
+domain-end states.  Every transition is the product of at most two "atoms":
+an entry of the top-level TKF91 matrix or of a domain's TKF91 matrix at t,
+a domain weight, kappa or 1 - kappa of a domain, a fragment weight, a
+fragment extension probability, or the non-extension probability
+1 - sum_g ext[d,f,g].  Keeping Start, End and the 5KF emitting states and
+eliminating the null states in one block,
+
+    chi = T_KK + T_KZ (I - T_ZZ)^-1 T_ZK,
+
+gives the collapsed matrix in the reference's index order: 0 = Start (SS),
+1 = End (EE), body state 2 + d*5F + uv*F + f with uv in (MM, MI, MD, II, DD)
+(models/mixdom.py:48-77).  The solve is one batched LU per call and never
+waits for the GPU; autograd differentiates through it.
 """
+import functools
+
+import numpy as np
 import torch
 
-from .params import S, M, I, D, E, tkf91_trans, tkf_beta
+from .params import S, M, I, D, E, tkf91_trans
 
-_UV_EMIT_TYPE = [M, I, D, I, D]
-_UV_U = [M, M, M, I, D]
-_UV_X = [M, I, D, I, D]
-_IS_M_TYPE = [True, True, True, False, False]
-_MID = [M, I, D]
+# emission type of body slot uv = MM, MI, MD, II, DD (models/mixdom.py:66)
+_UV_TYPE = (M, I, D, I, D)
 
 
 def n_states(K, F):
@@ -25,72 +43,107 @@ def state_types(K, F):
     body = []
     for _d in range(K):
         for uv in range(5):
-            body += [_UV_EMIT_TYPE[uv]] * F
+            body += [_UV_TYPE[uv]] * F
     return [S, E] + body
 
 
+class _Plan:
+    """Index structure of the exploded machine of a (K, F) model: the flat
+    position of every transition and the two atoms whose product is its
+    value, plus the kept / eliminated state lists."""
+
+    def __init__(self, K, F):
+        self.K, self.F = K, F
+        N0 = 8 + 8 * K + 15 * K * F
+        self.N0 = N0
+        # atom offsets: tau_main (25) | tau_dom (K*25) | dom_w (K) | kappa (K) |
+        # 1 - kappa (K) | frag_w (K*F) | ext (K*F*F) | notext (K*F) | one
+        self.o_tau0 = 0
+        self.o_taud = 25
+        self.o_w = self.o_taud + 25 * K
+        self.o_kap = self.o_w + K
+        self.o_kap1 = self.o_kap + K
+        self.o_fw = self.o_kap1 + K
+        self.o_ext = self.o_fw + K * F
+        self.o_notext = self.o_ext + K * F * F
+        self.o_one = self.o_notext + K * F
+        self.n_atoms = self.o_one + 1
+
+        start, end = 0, 1
+        dom = lambda p: 2 + p                                           # noqa: E731
+        domtype = lambda k, p: 5 + 3 * k + p                            # noqa: E731
+        frag = lambda k, q: 5 + 3 * K + 5 * k + q                       # noqa: E731
+        fragtype = lambda k, f, q: 5 + 8 * K + (k * F + f) * 5 + q      # noqa: E731
+        emit = lambda k, q, f: 5 + 8 * K + 5 * K * F + k * 5 * F + q * F + f  # noqa: E731
+        fragend = lambda k, f, q: 5 + 8 * K + 10 * K * F + (k * F + f) * 5 + q  # noqa: E731
+        domend = lambda p: 5 + 8 * K + 15 * K * F + p                   # noqa: E731
+
+        one = self.o_one
+        tau0 = lambda r, c: self.o_tau0 + 5 * r + c                     # noqa: E731
+        taud = lambda k, r, c: self.o_taud + 25 * k + 5 * r + c         # noqa: E731
+        rows, cols, a1, a2 = [], [], [], []
+
+        def put(r, c, x=one, y=one):
+            rows.append(r); cols.append(c); a1.append(x); a2.append(y)
+
+        # top level: Start and the domain ends re-enter the top TKF91 chain
+        # (a domain end is the top-level chain's visit to its emitting state)
+        for src, row in ((start, S), (domend(0), M), (domend(1), I), (domend(2), D)):
+            for p, col in enumerate((M, I, D)):
+                put(src, dom(p), tau0(row, col))
+            put(src, end, tau0(row, E))
+        for k in range(K):
+            for p in range(3):
+                put(dom(p), domtype(k, p), self.o_w + k)
+            # a Mat-type domain runs the domain's own TKF91 chain; Ins/Del-type
+            # domains emit only their own kind, continuing with probability kappa
+            for q, col in enumerate((M, I, D)):
+                put(domtype(k, 0), frag(k, q), taud(k, S, col))
+            put(domtype(k, 0), domend(0), taud(k, S, E))
+            put(domtype(k, 1), frag(k, 3), self.o_kap + k)
+            put(domtype(k, 1), domend(1), self.o_kap1 + k)
+            put(domtype(k, 2), frag(k, 4), self.o_kap + k)
+            put(domtype(k, 2), domend(2), self.o_kap1 + k)
+            for f in range(F):
+                nx = self.o_notext + k * F + f
+                for q in range(5):
+                    put(frag(k, q), fragtype(k, f, q), self.o_fw + k * F + f)
+                    put(fragtype(k, f, q), emit(k, q, f))
+                    put(emit(k, q, f), fragend(k, f, q))
+                    for g in range(F):
+                        put(fragend(k, f, q), fragtype(k, g, q), self.o_ext + (k * F + f) * F + g)
+                for q, row in enumerate((M, I, D)):
+                    for p, col in enumerate((M, I, D)):
+                        put(fragend(k, f, q), frag(k, p), nx, taud(k, row, col))
+                    put(fragend(k, f, q), domend(0), nx, taud(k, row, E))
+                put(fragend(k, f, 3), frag(k, 3), nx, self.o_kap + k)
+                put(fragend(k, f, 3), domend(1), nx, self.o_kap1 + k)
+                put(fragend(k, f, 4), frag(k, 4), nx, self.o_kap + k)
+                put(fragend(k, f, 4), domend(2), nx, self.o_kap1 + k)
+        flat = np.asarray(rows) * N0 + np.asarray(cols)
+        if len(np.unique(flat)) != len(flat):
+            raise AssertionError("exploded machine has a repeated transition")
+        self.flat, self.a1, self.a2 = flat, np.asarray(a1), np.asarray(a2)
+        first_emit = emit(0, 0, 0)
+        self.keep = np.array([start, end] + list(range(first_emit, first_emit + 5 * K * F)))
+        mask = np.ones(N0, bool)
+        mask[self.keep] = False
+        self.elim = np.nonzero(mask)[0]
+
+
+@functools.lru_cache(maxsize=None)
+def _plan(K, F):
+    return _Plan(K, F)
+
+
 def _f64(x, dev):
-    return torch.as_tensor(x, dtype=torch.float64, device=dev) if not isinstance(x, torch.Tensor) \
-        else x.to(device=dev, dtype=torch.float64)
+    return x.to(device=dev, dtype=torch.float64) if isinstance(x, torch.Tensor) \
+        else torch.as_tensor(x, dtype=torch.float64, device=dev)
 
 
-def effective_trans_per_type(main_lam, main_mu, t, dom_lam, dom_mu, dom_w):
-    """Returns T_exit_k (...,K,5,5) and T_eff (...,5,5) (mixdom.py:104-214)."""
-    dev = t.device
-    dom_lam, dom_mu, dom_w = (_f64(x, dev) for x in (dom_lam, dom_mu, dom_w))
-    K = dom_lam.shape[0]
-    bt = t.shape
-    tau = tkf91_trans(_f64(main_lam, dev), _f64(main_mu, dev), t)     # (...,5,5)
-    kappas = dom_lam / dom_mu                                          # (K,)
-    betas = tkf_beta(dom_lam, dom_mu, t[..., None])                    # (...,K)
-    z_t = (1.0 - kappas) * (1.0 - betas)                               # (...,K)
-    z_0 = (1.0 - kappas).expand_as(z_t)
-    tau_rows = torch.stack([tau[..., S, :], tau[..., M, :], tau[..., I, :], tau[..., D, :],
-                            torch.zeros_like(tau[..., S, :])], dim=-2)  # (...,5,5)
-    tau_MID = tau_rows[..., _MID]                                      # (...,5,3)
-    w_rep = dom_w.repeat_interleave(3)                                 # (3K,)
-    T_Nnull = tau_MID.repeat(*([1] * len(bt)), 1, K) * w_rep           # (...,5,3K)
-    nonnull = torch.stack([1.0 - z_t, kappas.expand_as(z_t), kappas.expand_as(z_t)], dim=-1)  # (...,K,3)
-    null_prob = torch.stack([z_t, z_0, z_0], dim=-1)                   # (...,K,3)
-    tau_E = torch.stack([tau[..., M, E], tau[..., M, E], tau[..., D, E]], dim=-1)  # (...,3)
-    T_nullN = torch.zeros(bt + (K, 3, 5), dtype=torch.float64, device=dev)
-    for e_, col in enumerate(_MID):
-        T_nullN[..., e_, col] = nonnull[..., e_]
-        T_nullN[..., e_, E] = null_prob[..., e_] * tau_E[..., None, e_]
-    T_nullN = T_nullN.reshape(bt + (3 * K, 5))
-    tau_null_rows = torch.stack([tau[..., M, :], tau[..., M, :], tau[..., D, :]], dim=-2)
-    tau_null_MID = tau_null_rows[..., _MID]                            # (...,3,3)
-    z_flat = null_prob.reshape(bt + (3 * K,))
-    idx = torch.arange(3 * K, device=dev)
-    i_e, j_k, j_e = idx % 3, idx // 3, idx % 3
-    T_nullnull = (z_flat[..., :, None] * dom_w[j_k][None, :]
-                  * tau_null_MID[..., i_e[:, None], j_e[None, :]])
-    eye = torch.eye(3 * K, dtype=torch.float64, device=dev)
-    closure = torch.linalg.inv_ex(eye - T_nullnull)[0]  # no singularity check: it would wait for the GPU
-    closure_3d = closure.reshape(bt + (3 * K, K, 3))
-    T_nullN_3d = T_nullN.reshape(bt + (K, 3, 5))
-    closure_exit = torch.einsum('...ike,...kev->...kiv', closure_3d, T_nullN_3d)
-    T_exit_k = torch.einsum('...ui,...kiv->...kuv', T_Nnull, closure_exit)
-    T_NN = torch.zeros(bt + (5, 5), dtype=torch.float64, device=dev)
-    T_NN[..., :, E] = tau_rows[..., :, E]
-    return T_exit_k, T_NN + T_exit_k.sum(dim=-3)
-
-
-def _block_diag3(a, b, c):
-    """Block-diagonal of three (...,n,n) blocks (n may differ)."""
-    n = a.shape[-1] + b.shape[-1] + c.shape[-1]
-    out = torch.zeros(a.shape[:-2] + (n, n), dtype=a.dtype, device=a.device)
-    o = 0
-    for blk in (a, b, c):
-        q = blk.shape[-1]
-        out[..., o:o + q, o:o + q] = blk
-        o += q
-    return out
-
-
-def build_nested_trans(main_lam, main_mu, t, dom_lam, dom_mu, dom_w, frag_w, ext_rates):
-    """Collapsed MixDom chi (mixdom.py:252-423).  ext_rates (K,F) is made
-    diagonal (MixDom1 format, mixdom.py:284-287) or given as (K,F,F)."""
+def exploded_trans(main_lam, main_mu, t, dom_lam, dom_mu, dom_w, frag_w, ext_rates):
+    """The exploded transition matrix (..., N0, N0) at t (scalar or (B,));
+    ext_rates (K,F) is put on the diagonal of (K,F,F) (models/mixdom.py:284-287)."""
     if not isinstance(t, torch.Tensor):
         t = torch.as_tensor(t, dtype=torch.float64)
     t = t.to(torch.float64)
@@ -100,51 +153,41 @@ def build_nested_trans(main_lam, main_mu, t, dom_lam, dom_mu, dom_w, frag_w, ext
     if ext.dim() == 2:
         ext = torch.diag_embed(ext)
     K, F = frag_w.shape
+    pl = _plan(K, F)
     bt = t.shape
-    n_body = 5 * K * F
-    N = 2 + n_body
-    T_exit_k, T = effective_trans_per_type(main_lam, main_mu, t, dom_lam, dom_mu, dom_w)
-    tau_all = tkf91_trans(dom_lam, dom_mu, t[..., None])               # (...,K,5,5)
-    kappas = dom_lam / dom_mu
-    beta_d = torch.clamp(1.0 - tau_all[..., S, E], min=1e-30)         # (...,K)
-    isM = torch.tensor(_IS_M_TYPE, device=dev)
-    entry_M = tau_all[..., S, :][..., _UV_X] / beta_d[..., None]      # (...,K,5)
-    entry_factor = torch.where(isM, entry_M, torch.ones_like(entry_M))
-    exit_inner_M = tau_all[..., :, E][..., _UV_X]                      # (...,K,5)
-    exit_inner_ID = ((1.0 - kappas)[:, None]).expand(K, 5).expand_as(exit_inner_M)
-    exit_inner = torch.where(isM, exit_inner_M, exit_inner_ID)
-    notext = 1.0 - ext.sum(dim=-1)                                     # (K,F)
-    exit_full = notext[:, None, :] * exit_inner[..., :, :, None]       # (...,K,5,F)
-    exit_flat = exit_full.reshape(bt + (n_body,))
-    dest_flat = (frag_w[:, None, :] * entry_factor[..., :, :, None]).reshape(bt + (n_body,))
-    T_k_compound = T_exit_k[..., _UV_U, :][..., _UV_U]                 # (...,K,5,5)
-    block = 5 * F
-    d_idx = torch.arange(n_body, device=dev) // block
-    suv = torch.arange(5, device=dev).repeat_interleave(F).repeat(K)
-    T_expanded = T_k_compound[..., d_idx[None, :], suv[:, None], suv[None, :]]
-    chi_inter = exit_flat[..., :, None] * T_expanded * dest_flat[..., None, :]
-    tau_MID_all = tau_all[..., _MID, :][..., _MID]                     # (...,K,3,3)
-    non_ext_M = torch.einsum('df,...dxy,dg->...dxfyg', notext, tau_MID_all,
-                             frag_w).reshape(bt + (K, 3 * F, 3 * F))
-    zF = torch.zeros((K, F, F), dtype=torch.float64, device=dev)
-    ext_M = _block_diag3(ext, ext, ext)                                # (K,3F,3F)
-    intra_M = non_ext_M + ext_M
-    non_ext_I = notext[:, :, None] * kappas[:, None, None] * frag_w[:, None, :]
-    intra_I = (non_ext_I + ext).expand(bt + (K, F, F))
-    block_all = _block_diag3(intra_M, intra_I, intra_I)               # (...,K,5F,5F)
-    del zF
-    local = torch.arange(n_body, device=dev) % block
-    same = (d_idx[:, None] == d_idx[None, :]).to(torch.float64)
-    chi_intra = block_all[..., d_idx[:, None], local[:, None], local[None, :]] * same
-    chi_body = chi_inter + chi_intra
-    ss_T_k = T_exit_k[..., S, :][..., _UV_U]                           # (...,K,5)
-    ss_body = (ss_T_k[..., :, :, None] * frag_w[:, None, :]
-               * entry_factor[..., :, :, None]).reshape(bt + (n_body,))
-    uvu_E = T[..., _UV_U, E]                                           # (...,5)
-    ee_body = (exit_full * uvu_E[..., None, :, None]).reshape(bt + (n_body,))
-    chi = torch.zeros(bt + (N, N), dtype=torch.float64, device=dev)
-    chi[..., 0, 1] = T[..., S, E]
-    chi[..., 0, 2:] = ss_body
-    chi[..., 2:, 1] = ee_body
-    chi[..., 2:, 2:] = chi_body
-    return chi
+    tau0 = tkf91_trans(_f64(main_lam, dev), _f64(main_mu, dev), t)          # (...,5,5)
+    taud = tkf91_trans(dom_lam, dom_mu, t[..., None])                        # (...,K,5,5)
+    kap = dom_lam / dom_mu
+    notext = 1.0 - ext.sum(dim=-1)
+    fixed = torch.cat([dom_w, kap, 1.0 - kap, frag_w.reshape(-1), ext.reshape(-1),
+                       notext.reshape(-1), torch.ones(1, dtype=torch.float64, device=dev)])
+    atoms = torch.cat([tau0.reshape(bt + (25,)), taud.reshape(bt + (25 * K,)),
+                       fixed.expand(bt + fixed.shape)], dim=-1)
+    a1 = torch.as_tensor(pl.a1, device=dev)
+    a2 = torch.as_tensor(pl.a2, device=dev)
+    vals = atoms[..., a1] * atoms[..., a2]
+    T = torch.zeros(bt + (pl.N0 * pl.N0,), dtype=torch.float64, device=dev)
+    T = T.scatter(-1, torch.as_tensor(pl.flat, device=dev).expand(bt + (len(pl.flat),)), vals)
+    return T.reshape(bt + (pl.N0, pl.N0))
+
+
+def build_nested_trans(main_lam, main_mu, t, dom_lam, dom_mu, dom_w, frag_w, ext_rates):
+    """Collapsed MixDom chi (..., N, N), N = 2 + 5KF, the reference's
+    build_nested_trans (models/mixdom.py:252-423) obtained by eliminating the
+    exploded machine's null states.  ext_rates (K,F) is made diagonal
+    (MixDom1 format, models/mixdom.py:284-287) or given as (K,F,F)."""
+    T = exploded_trans(main_lam, main_mu, t, dom_lam, dom_mu, dom_w, frag_w, ext_rates)
+    K, F = torch.as_tensor(frag_w).shape
+    pl = _plan(K, F)
+    dev = T.device
+    kp = torch.as_tensor(pl.keep, device=dev)
+    ez = torch.as_tensor(pl.elim, device=dev)
+    T_kk = T[..., kp[:, None], kp[None, :]]
+    T_kz = T[..., kp[:, None], ez[None, :]]
+    T_zk = T[..., ez[:, None], kp[None, :]]
+    T_zz = T[..., ez[:, None], ez[None, :]]
+    eye = torch.eye(len(pl.elim), dtype=torch.float64, device=dev)
+    # I - T_ZZ is a nonsingular M-matrix (the null states reach End), so no
+    # error check is needed; solve_ex does not wait for the GPU
+    W = torch.linalg.solve_ex(eye - T_zz, T_zk)[0]
+    return T_kk + T_kz @ W
