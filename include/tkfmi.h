@@ -102,6 +102,12 @@ typedef struct tkfmi_outputs {
                           Same layout as log_tab: entry (g, e) is the
                           posterior-expected number of times table entry e of
                           group g is used, i.e. d logP / d log_tab[g, e].    */
+  int32_t* status;     /* device [1] or NULL, zero-filled by the caller: set
+                          to 1 (never cleared by the library) when a chained
+                          launch's bounded internal wait gave up, the case in
+                          which the affected pairs' log_prob is NaN; lets a
+                          caller tell such pairs from impossible ones (which
+                          the reference filters as NaN / -inf)              */
 } tkfmi_outputs;
 
 /* Version of this header's ABI (TKFMI_ABI_VERSION). */

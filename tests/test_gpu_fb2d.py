@@ -336,7 +336,14 @@ def test_chain_wait_timeout_aborts_cleanly(monkeypatch):
     # a chain's first wait comes ~200 steps before its producer can publish,
     # so with one spin it times out: the abort path did run
     assert np.isnan(lb[[1, 3, 5, 7]]).any()
+    # the call's status word says so, and the callers' check raises on it
+    from tkfmi import _abi
+    assert int(bad["status"].item()) == 1
+    with pytest.raises(_abi.TkfmiError):
+        _abi.check_status(bad)
     good = _run(w, want_post=False, kernel="chain")
+    assert int(good["status"].item()) == 0
+    _abi.check_status(good)
     lg = good["log_p"].cpu().numpy()
     assert np.all(np.abs(lg - lr) <= 1e-12 * np.abs(lr))
     ok, worst = _close(good["n"].cpu().numpy(), ref["n"].cpu().numpy())

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 
 #include "tkfmi.h"
@@ -233,14 +234,23 @@ inline int launch_prep(const tkfmi_problem* p, double* pmat, hipStream_t s, std:
   return TKFMI_OK;
 }
 
-// Per-device library streams (one per launch class) and their events.
+// Library streams (one per launch class) and the events that fork them from
+// and join them to a caller's stream.  A call holds `mu` from its start record
+// to its last join, so concurrent calls never pick up each other's records.
 struct Streams {
   bool init = false;
+  hipStream_t key = nullptr;  // the caller stream this set serves
   hipStream_t s[4];
   hipEvent_t start, done[4];
+  std::mutex mu;
 };
-// Returns the current device's streams, creating them on first use.
-int get_streams(Streams*& out, std::string& err);
+// At most this many sets per device; further caller streams share one (their
+// calls are then serialised on its class streams, still correctly ordered).
+constexpr int kStreamSets = 16;
+// Returns the set serving calls on `caller` on the current device, creating
+// it on first use: calls on different streams get different class streams
+// and overlap on the device.
+int get_streams(hipStream_t caller, Streams*& out, std::string& err);
 
 inline int nx_of(const Plan& pl) {
   return pl.lay == Layout::Tkf ? 3 : (3 * pl.K * pl.F + 1);  // nM + nD + 1

@@ -151,11 +151,26 @@ __device__ __forceinline__ void put_entry(double* e, bool global, const double* 
   else e[NX - 1] = (double)E;
 }
 
-__device__ __forceinline__ int take_ticket(int32_t* counter) {
+// status[0] = 1 if the call's abort flag was raised (after every class joined).
+__global__ void raise_status_kernel(const int32_t* __restrict__ abortf, int32_t* __restrict__ status) {
+  if (threadIdx.x == 0 && __hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+    status[0] = 1;
+}
+
+// The launch's next ticket, the same for the whole workgroup.  A ticket past
+// the grid means the counter was not zeroed before the launch (a caller
+// error, e.g. a workspace shared by overlapping calls): the workgroup raises
+// the abort flag and returns -1 without touching a descriptor.
+__device__ __forceinline__ int take_ticket(int32_t* counter, int32_t* abortf) {
   __shared__ int s_ticket;
   if (threadIdx.x == 0) s_ticket = atomicAdd(counter, 1);
   __syncthreads();
-  return __builtin_amdgcn_readfirstlane(s_ticket);
+  int t = __builtin_amdgcn_readfirstlane(s_ticket);
+  if (t < 0 || t >= (int)gridDim.x) {
+    if (threadIdx.x == 0) __hip_atomic_store(abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return -1;
+  }
+  return t;
 }
 
 // LDS layout (doubles): tables | start / end transitions | emission counts
@@ -216,7 +231,9 @@ __global__ __launch_bounds__(W * 64) void fbc_fwd_kernel(CParams Pr, const doubl
 
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  const CDesc pd = descs[take_ticket(Pr.ticket)];
+  const int ticket = take_ticket(Pr.ticket, Pr.abortf);
+  if (ticket < 0) return;
+  const CDesc pd = descs[ticket];
   const int Lx = pd.Lx, Ly = pd.Ly, NW = pd.nw;
   const int S = Pr.S;
   const int BSTRIDE = Ly + 64;
@@ -465,7 +482,9 @@ __global__ __launch_bounds__((W + 1) * 64, 2) void fbc_bwd_kernel(CParams Pr, co
 
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  const CDesc pd = descs[take_ticket(Pr.ticket)];
+  const int ticket = take_ticket(Pr.ticket, Pr.abortf);
+  if (ticket < 0) return;
+  const CDesc pd = descs[ticket];
   const int Lx = pd.Lx, Ly = pd.Ly, NW = pd.nw;
   const int S = Pr.S;
   const int W1 = Ly + 1;
@@ -1069,8 +1088,9 @@ int tkfmi_chain_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, si
     all.insert(all.end(), cp.bwd[c].begin(), cp.bwd[c].end());
   }
   Streams* sts = nullptr;
-  int rc = get_streams(sts, err);
+  int rc = get_streams(stream, sts, err);
   if (rc != TKFMI_OK) return rc;
+  std::lock_guard<std::mutex> lk(sts->mu);  // until the last join below
   rc = (flags & TKFMI_REUSE_DESCRIPTORS) ? TKFMI_OK : tkfmi_stage_upload(ddesc, all.data(), sizeof(CDesc) * all.size(), stream, err);
   if (rc != TKFMI_OK) return rc;
   hipLaunchKernelGGL(zero_ints_kernel, dim3(std::min(1024, (cp.n_flags + 255) / 256)), dim3(256), 0, stream,
@@ -1115,6 +1135,10 @@ int tkfmi_chain_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, si
     }
     if (hipEventRecord(sts->done[c], s) != hipSuccess) { err = "hipEventRecord failed"; return TKFMI_EHIP; }
     if (hipStreamWaitEvent(stream, sts->done[c], 0) != hipSuccess) { err = "hipStreamWaitEvent failed"; return TKFMI_EHIP; }
+  }
+  if (o->status) {
+    hipLaunchKernelGGL(raise_status_kernel, dim3(1), dim3(64), 0, stream, prm.abortf, o->status);
+    if (hipGetLastError() != hipSuccess) { err = "status launch failed"; return TKFMI_EHIP; }
   }
   return TKFMI_OK;
 }

@@ -630,22 +630,29 @@ PairGeom geom(int Lx, int Ly) {
 
 
 std::mutex g_streams_mu;
-Streams g_streams[64];
+Streams g_stream_sets[64][kStreamSets];
 
-int get_streams(Streams*& out, std::string& err) {
+int get_streams(hipStream_t caller, Streams*& out, std::string& err) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) { err = "hipGetDevice failed"; return TKFMI_EHIP; }
   std::lock_guard<std::mutex> lk(g_streams_mu);
-  Streams& st = g_streams[dev];
-  if (!st.init) {
+  Streams* row = g_stream_sets[dev];
+  for (int i = 0; i < kStreamSets; ++i)
+    if (row[i].init && row[i].key == caller) { out = &row[i]; return TKFMI_OK; }
+  for (int i = 0; i < kStreamSets; ++i) {
+    Streams& st = row[i];
+    if (st.init) continue;
     for (int c = 0; c < 4; ++c) {
       if (hipStreamCreateWithFlags(&st.s[c], hipStreamNonBlocking) != hipSuccess) { err = "hipStreamCreate failed"; return TKFMI_EHIP; }
       if (hipEventCreateWithFlags(&st.done[c], hipEventDisableTiming) != hipSuccess) { err = "hipEventCreate failed"; return TKFMI_EHIP; }
     }
     if (hipEventCreateWithFlags(&st.start, hipEventDisableTiming) != hipSuccess) { err = "hipEventCreate failed"; return TKFMI_EHIP; }
+    st.key = caller;
     st.init = true;
+    out = &st;
+    return TKFMI_OK;
   }
-  out = &st;
+  out = &row[(reinterpret_cast<uintptr_t>(caller) >> 8) % kStreamSets];  // table full: share a set
   return TKFMI_OK;
 }
 
@@ -823,8 +830,9 @@ int tkfmi_v2_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, size_
   double* lattice = reinterpret_cast<double*>(wsb + cp.desc_b + cp.pm_b);
   double* gring = reinterpret_cast<double*>(wsb + cp.desc_b + cp.pm_b + cp.lat_b);
   Streams* sts = nullptr;
-  int rc = get_streams(sts, err);
+  int rc = get_streams(stream, sts, err);
   if (rc != TKFMI_OK) return rc;
+  std::lock_guard<std::mutex> lk(sts->mu);  // until the last join below
   rc = (flags & TKFMI_REUSE_DESCRIPTORS) ? TKFMI_OK : tkfmi_stage_upload(ddesc, cp.desc.data(), sizeof(V2Desc) * (size_t)B, stream, err);
   if (rc != TKFMI_OK) return rc;
   switch (pl.lay) {

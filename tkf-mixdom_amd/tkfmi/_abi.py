@@ -79,6 +79,7 @@ class Outputs(ctypes.Structure):
         ("n_trans", _c_p),
         ("posteriors", _c_p),
         ("emit_counts", _c_p),
+        ("status", _c_p),
     ]
 
 
@@ -297,6 +298,18 @@ def fb2d_batched(log_trans, state_types, state_group, log_tab, pairs, *,
     return r.results()
 
 
+def check_status(status):
+    """Raise TkfmiError if a call's device status word (tkfmi_outputs.status,
+    or the 'status' entry of a results dict) is set: a chained launch gave
+    up on a bounded internal wait, and the affected pairs' results are NaN.
+    Waits for the call."""
+    if isinstance(status, dict):
+        status = status["status"]
+    if int(status.item()) != 0:
+        raise TkfmiError("a chained launch timed out on an internal wait: the affected pairs' "
+                         "results are NaN")
+
+
 def split_posteriors(post_flat, pairs, S):
     """Views (Lx+1, Ly+1, S) of each pair in a flat posterior buffer."""
     out = []
@@ -365,16 +378,21 @@ class BatchRunner:
                      if (self.flags & WANT_POSTERIORS) else None)
         self.ecnt = (torch.zeros((B, G, TAB_SZ), dtype=torch.float64, device=dev)
                      if (self.flags & WANT_EMIT_COUNTS) else None)
-        self.out = Outputs(_ptr(self.log_p), _ptr(self.n), _ptr(self.post), _ptr(self.ecnt))
+        self.status = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.out = Outputs(_ptr(self.log_p), _ptr(self.n), _ptr(self.post), _ptr(self.ecnt),
+                           _ptr(self.status))
 
     def run(self, zero_outputs=True, stream=None):
-        """Enqueue one call on ``stream`` (default: torch's current stream)."""
-        if zero_outputs:
-            for t in (self.n, self.post, self.ecnt):
-                if t is not None:
-                    t.zero_()
+        """Enqueue one call on ``stream`` (default: torch's current stream);
+        the output zeroing goes on the same stream."""
         if stream is None:
             stream = torch.cuda.current_stream(self.dev)
+        with torch.cuda.stream(stream):
+            self.status.zero_()
+            if zero_outputs:
+                for t in (self.n, self.post, self.ecnt):
+                    if t is not None:
+                        t.zero_()
         rc = self.L.tkfmi_fb2d_batched(ctypes.byref(self.prob), ctypes.byref(self.out),
                                        _ptr(self.ws), ctypes.c_size_t(self.ws_bytes),
                                        ctypes.c_void_p(stream.cuda_stream), self.flags)
@@ -399,6 +417,7 @@ class BatchRunner:
         side = torch.cuda.Stream(self.dev)
         flags = self.flags | REUSE_DESCRIPTORS
         with torch.cuda.graph(graph, stream=side, capture_error_mode="relaxed"):
+            self.status.zero_()
             if zero_outputs:
                 for t in (self.n, self.post, self.ecnt):
                     if t is not None:
@@ -410,8 +429,14 @@ class BatchRunner:
             raise TkfmiError(self.L.tkfmi_last_error().decode() or f"tkfmi error {rc}")
         return graph
 
+    def check_status(self):
+        """Raise if the last call's chained launches gave up on an internal
+        wait (their pairs' log_prob is then NaN, which must not be mistaken
+        for an impossible pair).  Waits for the call."""
+        check_status(self.status)
+
     def results(self):
-        out = {"log_p": self.log_p}
+        out = {"log_p": self.log_p, "status": self.status}
         if self.n is not None:
             out["n"] = self.n
         if self.post is not None:

@@ -65,6 +65,7 @@ def _tkf92_fb(xs, ys, ts, lam, mu, ext, Q, pi, forward_only):
     r = _abi.BatchRunner(lt, params.TKF_STATE_TYPES, [0] * 5, tab, pairs,
                          forward_only=forward_only)
     r.run(zero_outputs=False)
+    r.check_status()
     return r, ts
 
 
@@ -121,6 +122,7 @@ def e_step(model, p, x, y):
     tab = tables.shared_tables(sub, _dev64(p["pi"], dev))
     res = hmm.fb2d_batch(hmm.safe_log(tau), st, [x], [y], tab, want_posteriors=True,
                          want_emit_counts=True)
+    _abi.check_status(res)
     post = res["posteriors"][0]
     stt = torch.as_tensor(st, device=dev)
     m_post = (post[1:, 1:, :] * (stt == params.M)).sum(-1).cpu().numpy()   # (Lx, Ly)
@@ -179,6 +181,7 @@ def e_step_mixdom2(p, x, y):
     tab = tables.per_class_tables(subs, cpis, cd)                     # (1, KF, TAB)
     groups = tables.state_groups_per_class(len(st), F)
     res = hmm.fb2d_batch(hmm.safe_log(chi), st, [x], [y], tab, groups, want_emit_counts=True)
+    _abi.check_status(res)
     cm, ci, cdl, cdc = _class_counts(res["emit_counts"], subs, cpis, cd, K, F)
     Qc = vjp.build_Q_from_S_pi(S_exch, cpis)
     W, U = hr.per_pair_hr_stats(Qc, cpis, t, cm)
@@ -221,6 +224,7 @@ def process_pairs_batched_2d(batch_pairs, chi_params, st, Q_lg, pi_lg, N, dom_Qs
     pairs = _abi.PackedPairs(xs, ys, dev)
     r = _abi.BatchRunner(lt, b.state_types, b.state_group, tab, pairs, want_emit_counts=True)
     r.run(zero_outputs=False)
+    r.check_status()  # a timed-out wait is an error, not an impossible pair
     ok = torch.isfinite(r.log_p)
     dm, di, dd = _split_ecnt(r.ecnt)                                  # (B, K, 20, 20) ...
     okf = ok.to(torch.float64)
