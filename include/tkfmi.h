@@ -89,6 +89,18 @@ typedef struct tkfmi_problem {
   const int64_t* h_y_off;      /* host [B] offsets into y_cat               */
   const int32_t* h_x_len;      /* host [B] Lx                               */
   const int32_t* h_y_len;      /* host [B] Ly                               */
+  const double* log_emit;      /* device, or NULL (table emissions): dense
+                                  log emissions, the reference's
+                                  log_emit_table (dp/hmm.py:1834-1873), ragged
+                                  like the posteriors -- pair b's
+                                  [Lx+1][Ly+1][S] block at offset
+                                  sum_{b'<b} (Lx+1)(Ly+1) S -- already cut to
+                                  the real (Lx, Ly) region; -1e30 = impossible.
+                                  Replaces log_tab / x_cat / y_cat (which may
+                                  then be NULL); TKFMI_WANT_EMIT_COUNTS is
+                                  refused (d logP / d log_emit is the posterior
+                                  lattice).  Runs on the generic kernel (<= 15
+                                  emitting states) or the 16..160-state kernel */
 } tkfmi_problem;
 
 typedef struct tkfmi_outputs {
@@ -141,17 +153,24 @@ size_t tkfmi_fb2d_workspace_bytes(const tkfmi_problem* prob, uint32_t flags);
  *   (train/likelihood.py:296-319).  Accumulated in LDS with FP64 atomics,
  *   so the summation order (not the value beyond rounding) may vary from
  *   run to run.
- * Requirements: state 0 has type S, some state has type E, at most 15
+ * Requirements: state 0 has type S, some state has type E, at most 160
  * emitting (M/I/D) states, 0 <= Lx, Ly, Lx + Ly <= 32000.
- * The TKF (S,M,I,D,E) layout and the MixDom layout (core/params.py:22,
- * models/mixdom.py:69-77) with K*F <= 3 run on the cell-per-lane kernels:
- * by default as chains of workgroups per pair (a pair's 64-row bands spread
- * over several compute units, one forward and one backward launch per
- * wave-count class, workgroups ticketed in dependency order), or with
- * TKFMI_CLASS_KERNEL one workgroup per pair.  Other layouts (or
- * TKFMI_GENERIC_KERNEL) run on the generic kernel.  All compute the same
- * quantities.  If a chained launch ever times out on an internal wait, the
- * affected pairs' log_prob is NaN.
+ * Kernel choice (all compute the same quantities):
+ *   - the TKF (S,M,I,D,E) layout and the MixDom layout (core/params.py:22,
+ *     models/mixdom.py:69-77) with K*F <= 3, table emissions: the
+ *     cell-per-lane kernels, by default as chains of workgroups per pair (a
+ *     pair's 64-row bands spread over several compute units, one forward and
+ *     one backward launch per wave-count class, workgroups ticketed in
+ *     dependency order), or with TKFMI_CLASS_KERNEL one workgroup per pair;
+ *   - other layouts with at most 15 emitting states, dense emissions, or
+ *     TKFMI_GENERIC_KERNEL: the generic kernel;
+ *   - 16..160 emitting states: the wide kernel (MFMA transition and count
+ *     contractions, optional TKFMI_CHECKPOINT).
+ * If a chained launch ever times out on an internal wait, the affected
+ * pairs' log_prob is NaN and outputs.status (if given) is set.
+ * Thread safety: calls from several host threads are safe as long as each
+ * call has its own workspace and outputs; calls on different streams
+ * overlap on the device (each caller stream gets its own library streams).
  */
 int tkfmi_fb2d_batched(const tkfmi_problem* prob, const tkfmi_outputs* out,
                        void* workspace, size_t workspace_bytes,

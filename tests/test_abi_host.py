@@ -121,6 +121,31 @@ def test_invalid_problems_rejected_with_message():
     p, k = _problem([0, 1, 2, 3, 4], [0] * 5, [-1], [4])
     rc = L.tkfmi_fb2d_batched(ctypes.byref(p), ctypes.byref(_abi.Outputs()), None, 0, None, 0)
     assert rc == -1
+    # dense log emissions with emission counts: refused before any device work
+    p, k = _problem([0, 1, 2, 3, 4], [0] * 5, [3], [4])
+    p.log_emit = ctypes.c_void_p(8)  # never dereferenced: the call fails first
+    rc = L.tkfmi_fb2d_batched(ctypes.byref(p), ctypes.byref(_abi.Outputs()), None, 0, None,
+                              _abi.WANT_EMIT_COUNTS)
+    assert rc == -1 and "TKFMI_WANT_EMIT_COUNTS" in L.tkfmi_last_error().decode()
+
+
+def test_dense_problem_workspace_takes_generic_or_wide_kernel():
+    """With dense emissions the layouts the cell-per-lane kernels would take
+    (TKF, MixDom K*F <= 3) get the generic kernel's workspace, and the
+    16..160-state layouts no emission-table staging."""
+    from tkfmi import _abi, mixdom
+    L = _abi.lib()
+    st = list(mixdom.state_types(3, 1))
+    p, k = _problem(st, [0] * 17, [100, 40], [90, 70], G=1)
+    chained = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), 0)
+    generic = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), _abi.GENERIC_KERNEL)
+    p.log_emit = ctypes.c_void_p(8)
+    assert L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), 0) == generic != chained
+    st82 = list(mixdom.state_types(8, 2))
+    p, k = _problem(st82, [0] * 82, [100, 40], [90, 70], G=8)
+    table = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), 0)
+    p.log_emit = ctypes.c_void_p(8)
+    assert 0 < L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), 0) < table
 
 
 def _problem1d(S, lens, s_idx=0, e_idx=1, constrained=True, G=1, sg=None):

@@ -67,6 +67,7 @@ struct DevParams {
   int32_t bgrp[16];   // emission group of lane k
   const double* log_trans;
   const double* log_tab;
+  const double* log_emit;  // dense mode: pair b's [i][j][state] table at desc.post_off
   const uint8_t* x_cat;
   const uint8_t* y_cat;
   const PairDesc* desc;
@@ -198,7 +199,10 @@ __device__ __forceinline__ double group_normalise(double v, int e, int& E) {
 }
 
 // ------------------------------------------------------------------- kernel
-template <int GL, int NB, int W>
+// DENSE: emissions come from the caller's dense log table (the reference's
+// log_emit_table) instead of the per-group tables; they are prefetched as
+// log values PF steps ahead and exponentiated where used.
+template <int GL, int NB, int W, bool DENSE>
 __global__ __launch_bounds__(W * 64) void fb2d_kernel(DevParams P) {
   static_assert(NB <= GL - 1, "exponent lane must be free");
   constexpr int CPW = 64 / GL;
@@ -226,16 +230,19 @@ __global__ __launch_bounds__(W * 64) void fb2d_kernel(DevParams P) {
   uint8_t* s_y = s_x + ((Lx + 8) & ~7);
 
   // ---- stage emission tables (probability space) and sequences in LDS
-  const double* ltab = P.log_tab + (P.n_tab > 1 ? (size_t)pd.pair * G * TAB_SZ : 0);
-  for (int q = tid; q < G * TAB_SZ; q += W * 64) s_tab[q] = exp(ltab[q]);
-  for (int q = tid; q < Lx; q += W * 64) s_x[q] = min((int)P.x_cat[pd.x_off + q], A_PAD - 1);
-  for (int q = tid; q < Ly; q += W * 64) s_y[q] = min((int)P.y_cat[pd.y_off + q], A_PAD - 1);
-  if (tid == 0) {  // dummy characters for empty sequences (never used)
-    if (Lx == 0) s_x[0] = 0;
-    if (Ly == 0) s_y[0] = 0;
+  if constexpr (!DENSE) {
+    const double* ltab = P.log_tab + (P.n_tab > 1 ? (size_t)pd.pair * G * TAB_SZ : 0);
+    for (int q = tid; q < G * TAB_SZ; q += W * 64) s_tab[q] = exp(ltab[q]);
+    for (int q = tid; q < Lx; q += W * 64) s_x[q] = min((int)P.x_cat[pd.x_off + q], A_PAD - 1);
+    for (int q = tid; q < Ly; q += W * 64) s_y[q] = min((int)P.y_cat[pd.y_off + q], A_PAD - 1);
+    if (tid == 0) {  // dummy characters for empty sequences (never used)
+      if (Lx == 0) s_x[0] = 0;
+      if (Ly == 0) s_y[0] = 0;
+    }
   }
-  const bool want_ecnt = !(P.flags & TKFMI_FORWARD_ONLY) && (P.flags & TKFMI_WANT_EMIT_COUNTS) &&
+  const bool want_ecnt = !DENSE && !(P.flags & TKFMI_FORWARD_ONLY) && (P.flags & TKFMI_WANT_EMIT_COUNTS) &&
                          P.out_ecnt != nullptr;
+  const double* dem = DENSE ? P.log_emit + pd.post_off : nullptr;  // [i][j][state]
   double* s_ecnt = smem + P.smem_ecnt_off;  // [G][TAB_SZ] expected emission counts
   if (want_ecnt)
     for (int q = tid; q < G * TAB_SZ; q += W * 64) s_ecnt[q] = 0.0;
@@ -288,15 +295,22 @@ __global__ __launch_bounds__(W * 64) void fb2d_kernel(DevParams P) {
     int qm = 0, qc = -R;
     int m = 0, c = -R;
     double rb[PF];  // row-buffer values of band-row 0, PF steps ahead
+    double de[PF];  // dense log emissions of this lane's state, PF steps ahead
     auto rb_load = [&](int m_, int c_) -> double {
       const bool need = (R == 0) && (m_ >= 1) && (m_ < nbands) && (c_ >= 0) && (c_ <= Ly) &&
                         (m_ * H <= Lx);
       if (!need) return 0.0;
       return rb_lds ? s_rowb[c_ * GL + k] : rowb[(size_t)c_ * GL + k];
     };
+    auto e_load = [&](int m_, int c_) -> double {
+      const int i_ = m_ * H + R;
+      const bool v_ = (c_ >= 0) && (m_ < nbands) && (c_ <= Ly) && (i_ <= Lx) && my_state >= 0;
+      return v_ ? dem[((size_t)i_ * W1 + c_) * S + my_state] : -1e30;
+    };
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
       rb[u] = rb_load(qm, qc);
+      if constexpr (DENSE) de[u] = e_load(qm, qc);
       if (++qc == Pper) { qc = 0; ++qm; }
     }
     double hI = 0.0; int eI = EXP_NONE;  // own h of the previous step
@@ -308,6 +322,11 @@ __global__ __launch_bounds__(W * 64) void fb2d_kernel(DevParams P) {
         if (t < T_steps) {
           const double rbv = rb[u];
           rb[u] = rb_load(qm, qc);
+          double dlog = 0.0;
+          if constexpr (DENSE) {
+            dlog = de[u];
+            de[u] = e_load(qm, qc);
+          }
           if (++qc == Pper) { qc = 0; ++qm; }
           const int j = c;
           const int i = m * H + R;
@@ -324,9 +343,14 @@ __global__ __launch_bounds__(W * 64) void fb2d_kernel(DevParams P) {
           if (my_type == ST_M) { sV = hM; sE = eM; ok = valid && i >= 1 && j >= 1; }
           else if (my_type == ST_I) { sV = hI; sE = eI; ok = valid && j >= 1; }
           else { sV = dV; sE = dE; ok = has_d && my_type == ST_D; }
-          const int xi = valid ? s_x[i >= 1 ? i - 1 : 0] : 0;
-          const int yj = valid ? s_y[j >= 1 ? j - 1 : 0] : 0;
-          const double em = s_tab[ebase + xi * emx + yj * emy];
+          double em;
+          if constexpr (DENSE) {
+            em = exp(dlog);
+          } else {
+            const int xi = valid ? s_x[i >= 1 ? i - 1 : 0] : 0;
+            const int yj = valid ? s_y[j >= 1 ? j - 1 : 0] : 0;
+            em = s_tab[ebase + xi * emx + yj * emy];
+          }
           const double v = ok ? sV * em : 0.0;
           int En;
           double mv = group_normalise<GL>(v, sE, En);
@@ -398,10 +422,16 @@ __global__ __launch_bounds__(W * 64) void fb2d_kernel(DevParams P) {
     int mb = 0, c = -R;
     double fr[PF];  // forward-lattice values, PF steps ahead
     double rb[PF];  // row-buffer values of band-row 0, PF steps ahead
+    double de[PF];  // dense log emissions, PF steps ahead
     auto f_load = [&](int mb_, int c_) -> double {
       const int i_ = Lx - mb_ * H - R;
       const bool v_ = (c_ >= 0) && (mb_ < nbands) && (c_ <= Ly) && (i_ >= 0);
       return v_ ? lat[((size_t)i_ * W1 + (Ly - c_)) * GL + k] : 0.0;
+    };
+    auto e_load = [&](int mb_, int c_) -> double {
+      const int i_ = Lx - mb_ * H - R;
+      const bool v_ = (c_ >= 0) && (mb_ < nbands) && (c_ <= Ly) && (i_ >= 0) && my_state >= 0;
+      return v_ ? dem[((size_t)i_ * W1 + (Ly - c_)) * S + my_state] : -1e30;
     };
     auto rb_load = [&](int mb_, int c_) -> double {
       const bool need = (R == 0) && (mb_ >= 1) && (mb_ < nbands) && (c_ >= 0) && (c_ <= Ly) &&
@@ -413,6 +443,7 @@ __global__ __launch_bounds__(W * 64) void fb2d_kernel(DevParams P) {
     for (int u = 0; u < PF; ++u) {
       fr[u] = f_load(qm, qc);
       rb[u] = rb_load(qm, qc);
+      if constexpr (DENSE) de[u] = e_load(qm, qc);
       if (++qc == Pper) { qc = 0; ++qm; }
     }
     double qI = 0.0; int eI = EXP_NONE;  // own q of the previous step
@@ -426,6 +457,11 @@ __global__ __launch_bounds__(W * 64) void fb2d_kernel(DevParams P) {
           const double rbv = rb[u];
           fr[u] = f_load(qm, qc);
           rb[u] = rb_load(qm, qc);
+          double dlog = 0.0;
+          if constexpr (DENSE) {
+            dlog = de[u];
+            de[u] = e_load(qm, qc);
+          }
           if (++qc == Pper) { qc = 0; ++qm; }
           const int j = Ly - c;
           const int i = Lx - mb * H - R;
@@ -462,14 +498,19 @@ __global__ __launch_bounds__(W * 64) void fb2d_kernel(DevParams P) {
           if (want_post && valid && my_state >= 0)
             post[((size_t)i * W1 + j) * S + my_state] = fv * ldexp(mB * inv_mP, EF + EB - EP);
           // q = e(i,j,k) * B(i,j,k) for the predecessors
-          const int xi = valid ? s_x[i >= 1 ? i - 1 : 0] : 0;
-          const int yj = valid ? s_y[j >= 1 ? j - 1 : 0] : 0;
           const bool eok = (my_type == ST_M) ? (i >= 1 && j >= 1)
                          : (my_type == ST_I) ? (j >= 1)
                          : (my_type == ST_D) ? (i >= 1) : false;
-          const double q = eok ? s_tab[ebase + xi * emx + yj * emy] * mB : 0.0;
-          if (want_ecnt && valid && eok && my_state >= 0)
-            atomicAdd(s_ecnt + ebase + xi * emx + yj * emy, fv * ldexp(mB * inv_mP, EF + EB - EP));
+          double q;
+          if constexpr (DENSE) {
+            q = eok ? exp(dlog) * mB : 0.0;
+          } else {
+            const int xi = valid ? s_x[i >= 1 ? i - 1 : 0] : 0;
+            const int yj = valid ? s_y[j >= 1 ? j - 1 : 0] : 0;
+            q = eok ? s_tab[ebase + xi * emx + yj * emy] * mB : 0.0;
+            if (want_ecnt && valid && eok && my_state >= 0)
+              atomicAdd(s_ecnt + ebase + xi * emx + yj * emy, fv * ldexp(mB * inv_mP, EF + EB - EP));
+          }
           const double qout = (k == GL - 1) ? (double)EB : q;
           s_hbuf[((t & 1) * H + R) * GL + k] = qout;
           if (valid && R == H - 1) {
@@ -564,7 +605,7 @@ int make_plan(const tkfmi_problem* p, uint32_t flags, Plan& pl) {
   pl.H = pl.W * 64 / pl.GL;
   for (int s = pl.nb; s < 16; ++s) { pl.bidx[s] = -1; pl.btype[s] = 0; pl.bgrp[s] = 0; }
   if (p->n_trans != 1 && p->n_trans != p->B) return fail(TKFMI_EINVAL, "n_trans must be 1 or B");
-  if (p->n_tab != 1 && p->n_tab != p->B) return fail(TKFMI_EINVAL, "n_tab must be 1 or B");
+  if (!p->log_emit && p->n_tab != 1 && p->n_tab != p->B) return fail(TKFMI_EINVAL, "n_tab must be 1 or B");
   if (p->B > 0 && (!p->h_x_len || !p->h_y_len || !p->h_x_off || !p->h_y_off))
     return fail(TKFMI_EINVAL, "host length/offset arrays missing");
   pl.lat_doubles = 0;
@@ -600,9 +641,9 @@ struct Stage {
 constexpr int kMaxStages = 64;
 thread_local std::vector<Stage> g_stages;
 
-template <int GL, int NB, int W>
+template <int GL, int NB, int W, bool DENSE>
 int launch(const Plan& pl, DevParams dp, int B, size_t smem, hipStream_t stream) {
-  auto kern = fb2d_kernel<GL, NB, W>;
+  auto kern = fb2d_kernel<GL, NB, W, DENSE>;
   static std::once_flag once;
   static hipError_t attr_err = hipSuccess;
   std::call_once(once, [&] {
@@ -675,9 +716,11 @@ const char* tkfmi_last_error(void) { return g_err.c_str(); }
 
 size_t tkfmi_fb2d_workspace_bytes(const tkfmi_problem* prob, uint32_t flags) {
   size_t v2b = 0;
-  if (!(flags & (TKFMI_GENERIC_KERNEL | TKFMI_CLASS_KERNEL)) && tkfmi_chain_workspace(prob, flags, &v2b))
+  // dense emissions run on the generic or the wide kernel only
+  const bool dense = prob && prob->log_emit;
+  if (!dense && !(flags & (TKFMI_GENERIC_KERNEL | TKFMI_CLASS_KERNEL)) && tkfmi_chain_workspace(prob, flags, &v2b))
     return v2b > 0 ? v2b : 1;
-  if (!(flags & TKFMI_GENERIC_KERNEL) && tkfmi_v2_workspace(prob, flags, &v2b))
+  if (!dense && !(flags & TKFMI_GENERIC_KERNEL) && tkfmi_v2_workspace(prob, flags, &v2b))
     return v2b > 0 ? v2b : 1;
   if (prob && tkfmi_wide_applies(prob)) return tkfmi_wide_workspace(prob, flags);
   Plan pl;
@@ -689,13 +732,17 @@ int tkfmi_fb2d_batched(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws,
                        size_t ws_bytes, void* stream_v, uint32_t flags) {
   g_err.clear();
   tkfmi_ktime_begin_call((flags & TKFMI_TIME_KERNELS) != 0);
+  const bool dense = p && p->log_emit;
+  if (dense && (flags & TKFMI_WANT_EMIT_COUNTS) && !(flags & TKFMI_FORWARD_ONLY))
+    return fail(TKFMI_EINVAL, "TKFMI_WANT_EMIT_COUNTS needs table emissions (with dense log_emit, "
+                              "d logP / d log_emit is the posterior lattice)");
   if (p && tkfmi_wide_applies(p)) {
     if (!o || !o->log_prob) return fail(TKFMI_EINVAL, "outputs.log_prob missing");
     if ((flags & TKFMI_WANT_POSTERIORS) && !(flags & TKFMI_FORWARD_ONLY) && !o->posteriors)
       return fail(TKFMI_EINVAL, "TKFMI_WANT_POSTERIORS without a posterior buffer");
     if ((flags & TKFMI_WANT_EMIT_COUNTS) && !(flags & TKFMI_FORWARD_ONLY) && !o->emit_counts)
       return fail(TKFMI_EINVAL, "TKFMI_WANT_EMIT_COUNTS without an emission-count buffer");
-    if (p->B > 0 && (!p->log_trans || !p->log_tab || !p->x_cat || !p->y_cat))
+    if (p->B > 0 && (!p->log_trans || (!dense && (!p->log_tab || !p->x_cat || !p->y_cat))))
       return fail(TKFMI_EINVAL, "device input pointer missing");
     std::string err;
     const int rcw = tkfmi_wide_run(p, o, ws, ws_bytes, reinterpret_cast<hipStream_t>(stream_v), flags, err);
@@ -710,15 +757,15 @@ int tkfmi_fb2d_batched(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws,
   if ((flags & TKFMI_WANT_EMIT_COUNTS) && !(flags & TKFMI_FORWARD_ONLY) && !o->emit_counts)
     return fail(TKFMI_EINVAL, "TKFMI_WANT_EMIT_COUNTS without an emission-count buffer");
   if (p->B == 0) return TKFMI_OK;
-  if (!p->log_trans || !p->log_tab || !p->x_cat || !p->y_cat)
+  if (!p->log_trans || (!dense && (!p->log_tab || !p->x_cat || !p->y_cat)))
     return fail(TKFMI_EINVAL, "device input pointer missing");
   size_t v2b = 0;
-  if (!(flags & (TKFMI_GENERIC_KERNEL | TKFMI_CLASS_KERNEL)) && tkfmi_chain_workspace(p, flags, &v2b)) {
+  if (!dense && !(flags & (TKFMI_GENERIC_KERNEL | TKFMI_CLASS_KERNEL)) && tkfmi_chain_workspace(p, flags, &v2b)) {
     std::string err;
     const int rc2 = tkfmi_chain_run(p, o, ws, ws_bytes, reinterpret_cast<hipStream_t>(stream_v), flags, err);
     return rc2 == TKFMI_OK ? TKFMI_OK : fail(rc2, err);
   }
-  if (!(flags & TKFMI_GENERIC_KERNEL) && tkfmi_v2_workspace(p, flags, &v2b)) {
+  if (!dense && !(flags & TKFMI_GENERIC_KERNEL) && tkfmi_v2_workspace(p, flags, &v2b)) {
     std::string err;
     const int rc2 = tkfmi_v2_run(p, o, ws, ws_bytes, reinterpret_cast<hipStream_t>(stream_v), flags, err);
     return rc2 == TKFMI_OK ? TKFMI_OK : fail(rc2, err);
@@ -784,6 +831,7 @@ int tkfmi_fb2d_batched(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws,
   for (int s = 0; s < 16; ++s) { dp.bidx[s] = pl.bidx[s]; dp.btype[s] = pl.btype[s]; dp.bgrp[s] = pl.bgrp[s]; }
   dp.log_trans = p->log_trans;
   dp.log_tab = p->log_tab;
+  dp.log_emit = p->log_emit;
   dp.x_cat = p->x_cat;
   dp.y_cat = p->y_cat;
   dp.desc = ddesc;
@@ -800,7 +848,7 @@ int tkfmi_fb2d_batched(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws,
   const bool ecnt = (flags & TKFMI_WANT_EMIT_COUNTS) && !(flags & TKFMI_FORWARD_ONLY);
   const size_t fixed_d = (size_t)p->n_groups * TAB_SZ * (ecnt ? 2 : 1) + HB + 8;
   dp.smem_ecnt_off = (int32_t)((size_t)p->n_groups * TAB_SZ + HB + 8);
-  const size_t seq_bytes = (size_t)((pl.max_lx + 8) & ~7) + (size_t)((pl.max_ly + 8) & ~7) + 16;
+  const size_t seq_bytes = dense ? 16 : (size_t)((pl.max_lx + 8) & ~7) + (size_t)((pl.max_ly + 8) & ~7) + 16;
   if (fixed_d * 8 + seq_bytes > kLdsBudget)
     return fail(TKFMI_EINVAL, "LDS budget exceeded (sequences too long for the emission groups)");
   const size_t rb_want = (size_t)(pl.max_ly + 1) * pl.GL;
@@ -817,7 +865,10 @@ int tkfmi_fb2d_batched(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws,
   const int slot = tkfmi_ktime_start(pl.GL == 4 ? "fb2d_kernel<4,3,4>" : "fb2d_kernel<16,15,8>", B,
                                      cells, stream, err);
   if (slot == -2) return fail(TKFMI_EHIP, err);
-  rc = (pl.GL == 4) ? launch<4, 3, 4>(pl, dp, B, smem, stream) : launch<16, 15, 8>(pl, dp, B, smem, stream);
+  if (dense)
+    rc = (pl.GL == 4) ? launch<4, 3, 4, true>(pl, dp, B, smem, stream) : launch<16, 15, 8, true>(pl, dp, B, smem, stream);
+  else
+    rc = (pl.GL == 4) ? launch<4, 3, 4, false>(pl, dp, B, smem, stream) : launch<16, 15, 8, false>(pl, dp, B, smem, stream);
   if (rc != TKFMI_OK) return rc;
   if (tkfmi_ktime_stop(slot, stream, err) != TKFMI_OK) return fail(TKFMI_EHIP, err);
   return TKFMI_OK;

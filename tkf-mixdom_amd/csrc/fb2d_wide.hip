@@ -66,6 +66,7 @@ struct WParams {
   const int32_t* bgrp;   // [NBP] emission group
   const uint8_t* x_cat;
   const uint8_t* y_cat;
+  const double* log_emit;  // dense mode: pair b's [i][j][state] log table at desc.post_off
   const WDesc* desc;
   double* ws;
   double* out_logP;
@@ -272,7 +273,9 @@ __device__ __forceinline__ void count_antidiag_w(int w, const double* __restrict
   else if constexpr (W0 + 1 < kWaves) count_antidiag_w<NT, NTILE, W0 + 1>(w, cur0, cur1, i_lo, i_hi, lane, cacc);
 }
 
-template <int NT>
+// DENSE: emissions from the caller's dense log table (the reference's
+// log_emit_table) instead of the per-group tables.
+template <int NT, bool DENSE>
 __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
   constexpr int NBP = 16 * NT;
   constexpr int NTILE = count_ntile(NT);                   // count tiles per wave
@@ -289,8 +292,9 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
   const double* p_end = p_start + NBP;
   const double p_se = p_start[2 * NBP];
   const double* tab = Pr.ws + pd.tab_off;
-  const uint8_t* xs = Pr.x_cat + pd.x_off;
-  const uint8_t* ys = Pr.y_cat + pd.y_off;
+  const uint8_t* xs = DENSE ? nullptr : Pr.x_cat + pd.x_off;
+  const uint8_t* ys = DENSE ? nullptr : Pr.y_cat + pd.y_off;
+  const double* dem = DENSE ? Pr.log_emit + pd.post_off : nullptr;
   double* lat = Pr.ws + pd.lat_off;
   double* latE = lat + (size_t)(Lx + 1) * W1 * NBP;
   double* ring = Pr.ws + pd.ring_off;
@@ -304,15 +308,25 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
   const bool want_ecnt = want_lat && (Pr.flags & TKFMI_WANT_EMIT_COUNTS) && Pr.out_ecnt != nullptr;
   double* out_n = want_counts ? Pr.out_n + (size_t)pd.pair * S * S : nullptr;
   // slot metadata of this lane's states
-  int ty[KR], gr[KR];
+  int ty[KR], gr[KR], sx[KR];
 #pragma unroll
   for (int r = 0; r < KR; ++r) {
     const int k = lane + 64 * r;
     ty[r] = (k < NB) ? Pr.btype[k] : 0;
     gr[r] = (k < NB) ? Pr.bgrp[k] : 0;
+    sx[r] = (k < NB) ? Pr.bidx[k] : 0;
   }
-  auto xchar = [&](int i) { return (i >= 1) ? (int)min((int)xs[i - 1], A_PAD - 1) : 0; };
-  auto ychar = [&](int j) { return (j >= 1) ? (int)min((int)ys[j - 1], A_PAD - 1) : 0; };
+  auto xchar = [&](int i) { return (!DENSE && i >= 1) ? (int)min((int)xs[i - 1], A_PAD - 1) : 0; };
+  auto ychar = [&](int j) { return (!DENSE && j >= 1) ? (int)min((int)ys[j - 1], A_PAD - 1) : 0; };
+  // emission probability of slot round r (type t) at cell (i, j)
+  auto em = [&](int r, int t, int xi, int yj, int i, int j) -> double {
+    if constexpr (DENSE) {
+      const bool ok = (t == ST_M) ? (i >= 1 && j >= 1) : (t == ST_I) ? (j >= 1) : (t == ST_D) ? (i >= 1) : false;
+      return ok ? exp(dem[((size_t)i * W1 + j) * S + sx[r]]) : 0.0;
+    } else {
+      return emit(tab, t, gr[r], xi, yj, i, j);
+    }
+  };
 
   // ======================================================== forward sweep
   // One antidiagonal d of the forward: F from the ring slots of d-1 and d-2
@@ -351,9 +365,9 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
       for (int r = 0; r < KR; ++r) {
         const int k = lane + 64 * r;
         double v = 0.0;
-        if (ty[r] == ST_M && okM) v = rg[((size_t)s2 * (Lx + 1) + i - 1) * NBP + k] * emit(tab, ST_M, gr[r], xi, yj, i, j);
-        else if (ty[r] == ST_I && okI) v = rg[((size_t)s1 * (Lx + 1) + i) * NBP + k] * emit(tab, ST_I, gr[r], xi, yj, i, j);
-        else if (ty[r] == ST_D && okD) v = rg[((size_t)s1 * (Lx + 1) + i - 1) * NBP + k] * emit(tab, ST_D, gr[r], xi, yj, i, j);
+        if (ty[r] == ST_M && okM) v = rg[((size_t)s2 * (Lx + 1) + i - 1) * NBP + k] * em(r, ST_M, xi, yj, i, j);
+        else if (ty[r] == ST_I && okI) v = rg[((size_t)s1 * (Lx + 1) + i) * NBP + k] * em(r, ST_I, xi, yj, i, j);
+        else if (ty[r] == ST_D && okD) v = rg[((size_t)s1 * (Lx + 1) + i - 1) * NBP + k] * em(r, ST_D, xi, yj, i, j);
         raw[r] = v;
       }
     };
@@ -601,7 +615,7 @@ __global__ __launch_bounds__(kWaves * 64) void wide_kernel(WParams Pr) {
         const int k = lane + 64 * r;
         if (k >= NBP) continue;
         const double b = ldexp(Bv[r], -sh);
-        const double e = (k < NB) ? emit(tab, ty[r], gr[r], xi, yj, i, j) : 0.0;
+        const double e = (k < NB) ? em(r, ty[r], xi, yj, i, j) : 0.0;
         ring[((size_t)sd * (Lx + 1) + i) * NBP + k] = e * b;
         if (k < NB && (want_post || want_ecnt)) {
           const double pv = fb[(size_t)i * fs + k] * b * psc;
@@ -674,13 +688,14 @@ bool applies(const tkfmi_problem* p, WPlan& pl, std::string& why) {
 bool plan(const tkfmi_problem* p, uint32_t flags, WPlan& pl, std::string& why) {
   if (!applies(p, pl, why)) return false;
   if (p->n_trans != 1 && p->n_trans != p->B) { why = "n_trans must be 1 or B"; return false; }
-  if (p->n_tab != 1 && p->n_tab != p->B) { why = "n_tab must be 1 or B"; return false; }
+  const bool dense = p->log_emit != nullptr;
+  if (!dense && p->n_tab != 1 && p->n_tab != p->B) { why = "n_tab must be 1 or B"; return false; }
   const int B = p->B;
   if (B > 0 && (!p->h_x_len || !p->h_y_len || !p->h_x_off || !p->h_y_off)) { why = "host arrays missing"; return false; }
   pl.idx_b = align256(3 * (size_t)pl.NBP * 4);
   pl.desc_b = align256(sizeof(WDesc) * (size_t)std::max(B, 1));
   pl.pm_b = align256((size_t)pm_stride(pl.NBP) * 8 * p->n_trans);
-  pl.tab_b = align256((size_t)p->n_tab * pl.G * TAB_SZ * 8);
+  pl.tab_b = dense ? 0 : align256((size_t)p->n_tab * pl.G * TAB_SZ * 8);
   // pairs in decreasing cost order
   std::vector<int> order(B);
   for (int b = 0; b < B; ++b) order[b] = b;
@@ -705,7 +720,8 @@ bool plan(const tkfmi_problem* p, uint32_t flags, WPlan& pl, std::string& why) {
     memset(&d, 0, sizeof(d));
     d.pair = b; d.Lx = lx; d.Ly = ly;
     d.pm_off = (int64_t)(pl.idx_b + pl.desc_b) / 8 + (p->n_trans > 1 ? (int64_t)b : 0) * pm_stride(pl.NBP);
-    d.tab_off = (int64_t)(pl.idx_b + pl.desc_b + pl.pm_b) / 8 + (p->n_tab > 1 ? (int64_t)b : 0) * pl.G * TAB_SZ;
+    d.tab_off = (int64_t)(pl.idx_b + pl.desc_b + pl.pm_b) / 8 +
+                (!dense && p->n_tab > 1 ? (int64_t)b : 0) * pl.G * TAB_SZ;
     d.x_off = p->h_x_off[b]; d.y_off = p->h_y_off[b];
     d.post_off = post_off[b];
     const int64_t cells = (int64_t)(lx + 1) * (ly + 1);
@@ -732,25 +748,30 @@ bool plan(const tkfmi_problem* p, uint32_t flags, WPlan& pl, std::string& why) {
   return true;
 }
 
-template <int NT>
+template <int NT, bool DENSE>
 int launch_nt(const WParams& prm, int B, hipStream_t s, std::string& err) {
   const size_t smem = (size_t)lds_b_rows(16 * NT) * lds_b_stride(16 * NT) * 8;
-  hipError_t e = hipFuncSetAttribute((const void*)wide_kernel<NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  hipError_t e = hipFuncSetAttribute((const void*)wide_kernel<NT, DENSE>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
   if (e != hipSuccess) { err = std::string("hipFuncSetAttribute: ") + hipGetErrorString(e); return TKFMI_EHIP; }
-  hipLaunchKernelGGL((wide_kernel<NT>), dim3(B), dim3(kWaves * 64), smem, s, prm);
+  hipLaunchKernelGGL((wide_kernel<NT, DENSE>), dim3(B), dim3(kWaves * 64), smem, s, prm);
   e = hipGetLastError();
   if (e != hipSuccess) { err = std::string("wide launch: ") + hipGetErrorString(e); return TKFMI_EHIP; }
   return TKFMI_OK;
 }
 
-int launch(int NT, const WParams& prm, int B, hipStream_t s, std::string& err) {
+template <bool DENSE>
+int launch_d(int NT, const WParams& prm, int B, hipStream_t s, std::string& err) {
   switch (NT) {
-    case 2: return launch_nt<2>(prm, B, s, err);
-    case 4: return launch_nt<4>(prm, B, s, err);
-    case 6: return launch_nt<6>(prm, B, s, err);
-    case 8: return launch_nt<8>(prm, B, s, err);
-    default: return launch_nt<10>(prm, B, s, err);
+    case 2: return launch_nt<2, DENSE>(prm, B, s, err);
+    case 4: return launch_nt<4, DENSE>(prm, B, s, err);
+    case 6: return launch_nt<6, DENSE>(prm, B, s, err);
+    case 8: return launch_nt<8, DENSE>(prm, B, s, err);
+    default: return launch_nt<10, DENSE>(prm, B, s, err);
   }
+}
+
+int launch(int NT, const WParams& prm, int B, hipStream_t s, std::string& err) {
+  return prm.log_emit ? launch_d<true>(NT, prm, B, s, err) : launch_d<false>(NT, prm, B, s, err);
 }
 
 }  // namespace tkfmi_wide
@@ -790,9 +811,11 @@ int tkfmi_wide_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
   double* wsd = reinterpret_cast<double*>(wsb);
   hipLaunchKernelGGL(wide_prep_kernel, dim3(p->n_trans), dim3(256), 0, stream, p->log_trans, p->S, pl.NB,
                      pl.NBP, pl.e_idx, d_idx, wsd + (pl.idx_b + pl.desc_b) / 8);
-  const int ntab = p->n_tab * pl.G * TAB_SZ;
-  hipLaunchKernelGGL(wide_tab_kernel, dim3(std::min(1024, (ntab + 255) / 256)), dim3(256), 0, stream,
-                     p->log_tab, ntab, wsd + (pl.idx_b + pl.desc_b + pl.pm_b) / 8);
+  if (!p->log_emit) {
+    const int ntab = p->n_tab * pl.G * TAB_SZ;
+    hipLaunchKernelGGL(wide_tab_kernel, dim3(std::min(1024, (ntab + 255) / 256)), dim3(256), 0, stream,
+                       p->log_tab, ntab, wsd + (pl.idx_b + pl.desc_b + pl.pm_b) / 8);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) { err = std::string("wide prep launch: ") + hipGetErrorString(e); return TKFMI_EHIP; }
   WParams prm;
@@ -800,7 +823,7 @@ int tkfmi_wide_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
   prm.S = p->S; prm.NB = pl.NB; prm.NBP = pl.NBP; prm.flags = (int32_t)flags;
   prm.e_idx = pl.e_idx; prm.G = pl.G;
   prm.bidx = d_idx; prm.btype = d_idx + pl.NBP; prm.bgrp = d_idx + 2 * pl.NBP;
-  prm.x_cat = p->x_cat; prm.y_cat = p->y_cat;
+  prm.x_cat = p->x_cat; prm.y_cat = p->y_cat; prm.log_emit = p->log_emit;
   prm.desc = reinterpret_cast<const WDesc*>(wsb + pl.idx_b);
   prm.ws = wsd;
   prm.out_logP = o->log_prob;

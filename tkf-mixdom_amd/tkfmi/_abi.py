@@ -47,6 +47,7 @@ class Problem(ctypes.Structure):
         ("h_y_off", _c_p),
         ("h_x_len", _c_p),
         ("h_y_len", _c_p),
+        ("log_emit", _c_p),
     ]
 
 
@@ -277,7 +278,8 @@ def kernel_flags(kernel=None, generic=False):
 
 def fb2d_batched(log_trans, state_types, state_group, log_tab, pairs, *,
                  forward_only=False, want_posteriors=False, want_counts=True,
-                 want_emit_counts=False, generic=False, kernel=None, checkpoint=False):
+                 want_emit_counts=False, generic=False, kernel=None, checkpoint=False,
+                 log_emit=None):
     """Run the batched forward/backward kernel once.
 
     log_trans: float64 device tensor (S,S) or (B,S,S) of log transitions.
@@ -288,12 +290,13 @@ def fb2d_batched(log_trans, state_types, state_group, log_tab, pairs, *,
     (want_counts), 'posteriors' (flat, want_posteriors), 'emit_counts'
     (B,G,TAB_SZ, want_emit_counts).  checkpoint: antidiagonal checkpointing
     (TKFMI_CHECKPOINT: less lattice memory, one extra forward sweep; used by
-    the 17..162-state kernel).
+    the 17..162-state kernel).  log_emit: flat dense log emissions in the
+    posterior layout instead of the tables (see BatchRunner).
     """
     r = BatchRunner(log_trans, state_types, state_group, log_tab, pairs,
                     forward_only=forward_only, want_posteriors=want_posteriors,
                     want_counts=want_counts, want_emit_counts=want_emit_counts,
-                    generic=generic, kernel=kernel, checkpoint=checkpoint)
+                    generic=generic, kernel=kernel, checkpoint=checkpoint, log_emit=log_emit)
     r.run(zero_outputs=False)  # outputs are freshly zeroed
     return r.results()
 
@@ -329,7 +332,10 @@ class BatchRunner:
     def __init__(self, log_trans, state_types, state_group, log_tab, pairs, *,
                  forward_only=False, want_posteriors=False, want_counts=True,
                  want_emit_counts=False, generic=False, time_kernels=False, kernel=None,
-                 checkpoint=False):
+                 checkpoint=False, log_emit=None):
+        """log_emit: optional flat FP64 device tensor of dense log emissions,
+        pair b's (Lx+1, Ly+1, S) block at offset sum_{b'<b} (Lx+1)(Ly+1) S
+        (the posterior layout); log_tab is then ignored and may be None."""
         self.L = lib()
         dev = log_trans.device
         if dev.type != "cuda":
@@ -342,6 +348,19 @@ class BatchRunner:
         if self.st.shape != (S,) or self.sg.shape != (S,):
             raise ValueError("state_types/state_group must have length S")
         self.log_trans = log_trans.to(torch.float64).contiguous()
+        self.log_emit = None
+        if log_emit is not None:
+            self.log_emit = log_emit.to(torch.float64).contiguous().reshape(-1)
+            need = int(pairs.cells.sum()) * S
+            if self.log_emit.numel() != need:
+                raise ValueError(f"log_emit must hold sum (Lx+1)(Ly+1)S = {need} values, "
+                                 f"got {self.log_emit.numel()}")
+            if want_emit_counts:
+                raise ValueError("want_emit_counts needs table emissions; with log_emit, "
+                                 "d logP / d log_emit is the posterior lattice")
+            if log_tab is None:  # never read: a placeholder covering the state groups
+                log_tab = torch.zeros((int(self.sg.max(initial=0)) + 1, TAB_SZ), dtype=torch.float64,
+                                      device=dev)
         self.log_tab = log_tab.to(torch.float64).contiguous()
         if self.log_tab.shape[-1] != TAB_SZ:
             raise ValueError(f"log_tab last dim must be {TAB_SZ}")
@@ -363,7 +382,8 @@ class BatchRunner:
         self.prob = Problem(B, S, _np_ptr(self.st), _np_ptr(self.sg), G, n_trans,
                             _ptr(self.log_trans), n_tab, _ptr(self.log_tab),
                             _ptr(pairs.x_cat), _ptr(pairs.y_cat), _np_ptr(pairs.x_off),
-                            _np_ptr(pairs.y_off), _np_ptr(pairs.x_len), _np_ptr(pairs.y_len))
+                            _np_ptr(pairs.y_off), _np_ptr(pairs.x_len), _np_ptr(pairs.y_len),
+                            _ptr(self.log_emit))
         self.ws_bytes = int(self.L.tkfmi_fb2d_workspace_bytes(ctypes.byref(self.prob), self.flags))
         if self.ws_bytes == 0 and B > 0:
             # size query failed: run the call anyway to obtain the message

@@ -2,8 +2,8 @@
 
 Same call surfaces as the reference's 2D functions:
 
-* ``forward_backward_2d`` (dp/hmm.py:1834-1973)
-* ``forward_2d`` (dp/hmm.py:1775-1801) and ``backward_2d`` (dp/hmm.py:1804-1824)
+* ``forward_backward_2d`` (dp/hmm.py:1834-1973), with table emissions or a
+  dense ``log_emit_table`` and ``real_Lx`` / ``real_Ly``
 * ``_pad_to_bin`` (dp/hmm.py:66-91), ``safe_log`` (dp/hmm.py:111-122)
 
 plus the batched entry ``fb2d_batch`` that replaces the vmapped, bin-bucketed
@@ -84,7 +84,8 @@ def _types_list(state_types):
 
 def fb2d_batch(log_trans, state_types, xs, ys, log_tab, state_group=None, *,
                forward_only=False, want_posteriors=False, want_counts=True,
-               want_emit_counts=False, generic=False, kernel=None, checkpoint=False):
+               want_emit_counts=False, generic=False, kernel=None, checkpoint=False,
+               log_emit=None):
     """One launch over a ragged batch of pairs.
 
     log_trans: (S,S) or (B,S,S) log transitions; log_tab: (G,TAB) or
@@ -100,19 +101,35 @@ def fb2d_batch(log_trans, state_types, xs, ys, log_tab, state_group=None, *,
     kernel="class".  ``checkpoint=True`` keeps only every K-th antidiagonal's
     forward state and recomputes the rest in the backward (the 17..162-state
     kernel; same results, less lattice memory).
+    ``log_emit``: a list of dense (Lx+1, Ly+1, S) log-emission tables, one
+    per pair (the reference's log_emit_table), used instead of ``log_tab``
+    (then None); xs / ys give only the lengths.  Runs on the generic kernel
+    (<= 15 emitting states) or the 16..160-state kernel.
     """
     dev = device()
     lt = _dev64(log_trans, dev)
-    tab = _dev64(log_tab, dev)
     st = _types_list(state_types)
     S = len(st)
     if state_group is None:
         state_group = [0] * S
     pairs = _abi.PackedPairs(list(xs), list(ys), dev)
+    dense = None
+    if log_emit is not None:
+        if len(log_emit) != pairs.B:
+            raise ValueError("log_emit needs one table per pair")
+        blocks = []
+        for b, e in enumerate(log_emit):
+            e = _dev64(e, dev)
+            shp = (int(pairs.x_len[b]) + 1, int(pairs.y_len[b]) + 1, S)
+            if tuple(e.shape) != shp:
+                raise ValueError(f"log_emit[{b}] must be {shp}, got {tuple(e.shape)}")
+            blocks.append(e.reshape(-1))
+        dense = torch.cat(blocks) if blocks else torch.zeros(0, dtype=torch.float64, device=dev)
+    tab = _dev64(log_tab, dev) if log_tab is not None else None
     out = _abi.fb2d_batched(lt, st, state_group, tab, pairs, forward_only=forward_only,
                             want_posteriors=want_posteriors, want_counts=want_counts,
                             want_emit_counts=want_emit_counts, generic=generic,
-                            kernel=kernel, checkpoint=checkpoint)
+                            kernel=kernel, checkpoint=checkpoint, log_emit=dense)
     if "posteriors" in out:
         out["posteriors"] = _abi.split_posteriors(out["posteriors"], pairs, S)
     return out
@@ -126,31 +143,44 @@ def _real_len(seq, real):
 def forward_backward_2d(log_trans, state_types, x_seq, y_seq, sub_matrix, pi,
                         log_emit_table=None, real_Lx=None, real_Ly=None,
                         forward_only=False):
-    """Reference surface of dp/hmm.py:1834-1873.
+    """Reference surface of dp/hmm.py:1834-1973.
 
     Returns log_prob, or (log_prob, posteriors (Lx_pad+1, Ly_pad+1, S),
-    expected_trans (S,S)); posteriors are zero outside the real region.
-    The emission model is the shared one (pair_hmm_emissions); a precomputed
-    dense ``log_emit_table`` is not supported by the table-lookup kernel.
+    expected_trans (S,S)) with (Lx_pad, Ly_pad) the _pad_to_bin sizes of
+    the array shapes; posteriors are zero outside the real region.
+    Emissions are the shared model (pair_hmm_emissions) of sub_matrix / pi,
+    or the caller's dense ``log_emit_table`` ((Lx_arr+1, Ly_arr+1, S)),
+    whose region beyond the real lengths is NEG_INF as in the reference
+    (dp/hmm.py:1878-1889).  ``real_Lx`` / ``real_Ly`` are the real lengths
+    of sequences already padded to their arrays' shapes.  The DP runs at
+    the real lengths only: cells beyond them are unreachable.
     """
-    if log_emit_table is not None:
-        raise NotImplementedError(
-            "log_emit_table: pass per-group tables to tkfmi.hmm.fb2d_batch instead "
-            "(tkfmi.tables builds the per-domain / per-class tables)")
     Lx_arr = int(np.asarray(x_seq.shape[0] if hasattr(x_seq, "shape") else len(x_seq)))
     Ly_arr = int(np.asarray(y_seq.shape[0] if hasattr(y_seq, "shape") else len(y_seq)))
     Lx = _real_len(x_seq, real_Lx)
     Ly = _real_len(y_seq, real_Ly)
+    if not (0 <= Lx <= Lx_arr and 0 <= Ly <= Ly_arr):
+        raise ValueError("real lengths must lie within the sequence arrays")
     x = np.asarray(x_seq.detach().cpu() if isinstance(x_seq, torch.Tensor) else x_seq)[:Lx]
     y = np.asarray(y_seq.detach().cpu() if isinstance(y_seq, torch.Tensor) else y_seq)[:Ly]
     dev = device()
-    tab = tables.shared_tables(_dev64(sub_matrix, dev), _dev64(pi, dev))
-    res = fb2d_batch(log_trans, state_types, [x], [y], tab, forward_only=forward_only,
-                     want_posteriors=not forward_only)
+    S = len(_types_list(state_types))
+    if log_emit_table is not None:
+        le = _dev64(log_emit_table, dev)
+        if tuple(le.shape) != (Lx_arr + 1, Ly_arr + 1, S):
+            raise ValueError(f"log_emit_table must be {(Lx_arr + 1, Ly_arr + 1, S)}, "
+                             f"got {tuple(le.shape)}")
+        res = fb2d_batch(log_trans, state_types, [x], [y], None, forward_only=forward_only,
+                         want_posteriors=not forward_only,
+                         log_emit=[le[:Lx + 1, :Ly + 1].contiguous()])
+    else:
+        tab = tables.shared_tables(_dev64(sub_matrix, dev), _dev64(pi, dev))
+        res = fb2d_batch(log_trans, state_types, [x], [y], tab, forward_only=forward_only,
+                         want_posteriors=not forward_only)
+    _abi.check_status(res)
     logp = res["log_p"][0]
     if forward_only:
         return logp
-    S = len(_types_list(state_types))
     Lxp, Lyp = _pad_to_bin(Lx_arr), _pad_to_bin(Ly_arr)
     post = torch.zeros((Lxp + 1, Lyp + 1, S), dtype=torch.float64, device=dev)
     post[:Lx + 1, :Ly + 1] = res["posteriors"][0]
