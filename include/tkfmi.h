@@ -176,6 +176,30 @@ int tkfmi_fb2d_batched(const tkfmi_problem* prob, const tkfmi_outputs* out,
                        void* workspace, size_t workspace_bytes,
                        void* stream /* hipStream_t */, uint32_t flags);
 
+/* Bytes of device scratch behind tkfmi_fb2d_batched_noalloc (576 MiB): one
+ * full forward/backward of a 2000 x 2000 pair at 17 states (518 MiB), a
+ * minibatch of ~50 pairs of 250 x 250 at 17 states, or forward-only scoring
+ * of pairs up to 2000 x 2000 at 162 states. */
+#define TKFMI_NOALLOC_SCRATCH_BYTES ((size_t)576 << 20)
+
+/*
+ * tkfmi_fb2d_batched without a workspace and without any allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  The scratch the kernels need comes from
+ * a library-owned device array of TKFMI_NOALLOC_SCRATCH_BYTES, one per GPU,
+ * held until the process ends; a batch needing more
+ * (tkfmi_fb2d_workspace_bytes) is refused with TKFMI_EINVAL and a message.
+ * Descriptors are copied from host memory on `stream` and every kernel runs
+ * on `stream` (no library streams, events or pinned buffers are created).
+ * TKFMI_TIME_KERNELS and TKFMI_REUSE_DESCRIPTORS are refused.  Results are
+ * those of tkfmi_fb2d_batched.  All calls on one GPU share its scratch:
+ * calls that overlap on one GPU (several streams or threads) race, and the
+ * caller must serialise them.
+ */
+int tkfmi_fb2d_batched_noalloc(const tkfmi_problem* prob, const tkfmi_outputs* out,
+                               void* stream /* hipStream_t */, uint32_t flags);
+
 /* ------------------------------------------------------------------------
  * 1D alignment-constrained forward/backward (the MSA-guided DP).
  * ---------------------------------------------------------------------- */

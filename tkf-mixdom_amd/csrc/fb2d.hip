@@ -665,6 +665,13 @@ void tkfmi_clear_error() { g_err.clear(); }
 
 int tkfmi_stage_upload(void* dst, const void* src, size_t bytes, hipStream_t stream,
                        std::string& err) {
+  if (tkfmi_noalloc_active()) {
+    // from pageable host memory: the runtime stages the bytes before the call
+    // returns, so `src` may be released afterwards; nothing is allocated here
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) { err = std::string("hipMemcpyAsync: ") + hipGetErrorString(e); return TKFMI_EHIP; }
+    return TKFMI_OK;
+  }
   Stage* st = nullptr;
   for (Stage& c : g_stages) {  // a finished buffer, large enough if possible
     if (c.pending) {

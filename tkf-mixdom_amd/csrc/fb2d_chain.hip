@@ -1087,10 +1087,16 @@ int tkfmi_chain_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, si
     base[c][1] = all.size();
     all.insert(all.end(), cp.bwd[c].begin(), cp.bwd[c].end());
   }
+  // library class streams, or (allocation-free calls) the caller's stream only
+  const bool one_stream = tkfmi_noalloc_active();
   Streams* sts = nullptr;
-  int rc = get_streams(stream, sts, err);
-  if (rc != TKFMI_OK) return rc;
-  std::lock_guard<std::mutex> lk(sts->mu);  // until the last join below
+  int rc = TKFMI_OK;
+  std::unique_lock<std::mutex> lk;
+  if (!one_stream) {
+    rc = get_streams(stream, sts, err);
+    if (rc != TKFMI_OK) return rc;
+    lk = std::unique_lock<std::mutex>(sts->mu);  // until the last join below
+  }
   rc = (flags & TKFMI_REUSE_DESCRIPTORS) ? TKFMI_OK : tkfmi_stage_upload(ddesc, all.data(), sizeof(CDesc) * all.size(), stream, err);
   if (rc != TKFMI_OK) return rc;
   hipLaunchKernelGGL(zero_ints_kernel, dim3(std::min(1024, (cp.n_flags + 255) / 256)), dim3(256), 0, stream,
@@ -1098,7 +1104,7 @@ int tkfmi_chain_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, si
   if (hipGetLastError() != hipSuccess) { err = "zero launch failed"; return TKFMI_EHIP; }
   rc = prep_dispatch(pl, p, pmat, stream, err);
   if (rc != TKFMI_OK) return rc;
-  if (hipEventRecord(sts->start, stream) != hipSuccess) { err = "hipEventRecord failed"; return TKFMI_EHIP; }
+  if (!one_stream && hipEventRecord(sts->start, stream) != hipSuccess) { err = "hipEventRecord failed"; return TKFMI_EHIP; }
   CParams prm;
   memset(&prm, 0, sizeof(prm));
   prm.S = p->S; prm.flags = (int32_t)flags; prm.n_tab = p->n_tab; prm.G_tab = p->n_groups;
@@ -1119,8 +1125,8 @@ int tkfmi_chain_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, si
   const bool fwd_only = (flags & TKFMI_FORWARD_ONLY) != 0;
   for (int c = 0; c < NCLS; ++c) {
     if (!cp.cls_pairs[c]) continue;
-    hipStream_t s = sts->s[c];
-    if (hipStreamWaitEvent(s, sts->start, 0) != hipSuccess) { err = "hipStreamWaitEvent failed"; return TKFMI_EHIP; }
+    hipStream_t s = one_stream ? stream : sts->s[c];
+    if (!one_stream && hipStreamWaitEvent(s, sts->start, 0) != hipSuccess) { err = "hipStreamWaitEvent failed"; return TKFMI_EHIP; }
     for (int dir = 0; dir < (fwd_only ? 1 : 2); ++dir) {
       CParams pc = prm;
       pc.ticket = flagv + 2 * c + dir;
@@ -1133,6 +1139,7 @@ int tkfmi_chain_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, si
       if (rc != TKFMI_OK) return rc;
       if (tkfmi_ktime_stop(slot, s, err) != TKFMI_OK) return TKFMI_EHIP;
     }
+    if (one_stream) continue;
     if (hipEventRecord(sts->done[c], s) != hipSuccess) { err = "hipEventRecord failed"; return TKFMI_EHIP; }
     if (hipStreamWaitEvent(stream, sts->done[c], 0) != hipSuccess) { err = "hipStreamWaitEvent failed"; return TKFMI_EHIP; }
   }

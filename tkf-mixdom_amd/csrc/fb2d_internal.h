@@ -34,6 +34,11 @@ size_t tkfmi_wide_workspace(const tkfmi_problem* p, uint32_t flags);
 int tkfmi_wide_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, size_t ws_bytes,
                    hipStream_t stream, uint32_t flags, std::string& err);
 
+// True while this thread runs tkfmi_fb2d_batched_noalloc: uploads then copy
+// from the caller's host memory (no pinned staging buffer is created) and
+// every launch goes on the caller's stream (no library streams or events).
+bool tkfmi_noalloc_active();
+
 // Records `msg` as this thread's tkfmi_last_error() and returns `code`.
 int tkfmi_fail(int code, const std::string& msg);
 // Clears this thread's error message.

@@ -829,10 +829,16 @@ int tkfmi_v2_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, size_
   double* pmat = reinterpret_cast<double*>(wsb + cp.desc_b);
   double* lattice = reinterpret_cast<double*>(wsb + cp.desc_b + cp.pm_b);
   double* gring = reinterpret_cast<double*>(wsb + cp.desc_b + cp.pm_b + cp.lat_b);
+  // library class streams, or (allocation-free calls) the caller's stream only
+  const bool one_stream = tkfmi_noalloc_active();
   Streams* sts = nullptr;
-  int rc = get_streams(stream, sts, err);
-  if (rc != TKFMI_OK) return rc;
-  std::lock_guard<std::mutex> lk(sts->mu);  // until the last join below
+  int rc = TKFMI_OK;
+  std::unique_lock<std::mutex> lk;
+  if (!one_stream) {
+    rc = get_streams(stream, sts, err);
+    if (rc != TKFMI_OK) return rc;
+    lk = std::unique_lock<std::mutex>(sts->mu);  // until the last join below
+  }
   rc = (flags & TKFMI_REUSE_DESCRIPTORS) ? TKFMI_OK : tkfmi_stage_upload(ddesc, cp.desc.data(), sizeof(V2Desc) * (size_t)B, stream, err);
   if (rc != TKFMI_OK) return rc;
   switch (pl.lay) {
@@ -845,7 +851,7 @@ int tkfmi_v2_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, size_
     default: rc = TKFMI_EINVAL;
   }
   if (rc != TKFMI_OK) return rc;
-  if (hipEventRecord(sts->start, stream) != hipSuccess) { err = "hipEventRecord failed"; return TKFMI_EHIP; }
+  if (!one_stream && hipEventRecord(sts->start, stream) != hipSuccess) { err = "hipEventRecord failed"; return TKFMI_EHIP; }
   V2Params prm;
   memset(&prm, 0, sizeof(prm));
   prm.S = p->S; prm.flags = (int32_t)flags; prm.n_tab = p->n_tab; prm.G_tab = p->n_groups;
@@ -858,8 +864,8 @@ int tkfmi_v2_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, size_
   prm.out_ecnt = (flags & TKFMI_WANT_EMIT_COUNTS) ? o->emit_counts : nullptr;
   for (int c = 0; c < 4; ++c) {
     if (!cp.cls_count[c]) continue;
-    hipStream_t s = sts->s[c];
-    if (hipStreamWaitEvent(s, sts->start, 0) != hipSuccess) { err = "hipStreamWaitEvent failed"; return TKFMI_EHIP; }
+    hipStream_t s = one_stream ? stream : sts->s[c];
+    if (!one_stream && hipStreamWaitEvent(s, sts->start, 0) != hipSuccess) { err = "hipStreamWaitEvent failed"; return TKFMI_EHIP; }
     V2Params pc = prm;
     pc.desc = ddesc + cp.cls_begin[c];
     pc.smem_ring_off = cp.ring_off[c]; pc.smem_stage_off = cp.stage_off[c]; pc.smem_seq_off = cp.seq_off[c];
@@ -878,6 +884,7 @@ int tkfmi_v2_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, size_
     }
     if (rc != TKFMI_OK) return rc;
     if (tkfmi_ktime_stop(slot, s, err) != TKFMI_OK) return TKFMI_EHIP;
+    if (one_stream) continue;
     if (hipEventRecord(sts->done[c], s) != hipSuccess) { err = "hipEventRecord failed"; return TKFMI_EHIP; }
     if (hipStreamWaitEvent(stream, sts->done[c], 0) != hipSuccess) { err = "hipStreamWaitEvent failed"; return TKFMI_EHIP; }
   }

@@ -139,7 +139,9 @@ EXPORTED = ("tkfmi_abi_version", "tkfmi_last_error",
             "tkfmi_last_kernel_times", "tkfmi_chain_config", "tkfmi_suffstats_size",
             "tkfmi_restore_workspace_bytes", "tkfmi_restore_counts_batched",
             "tkfmi_bdi_stats_batched", "tkfmi_tkf92_estep_stats", "tkfmi_hr_stats_batched",
-            "tkfmi_build_workspace_bytes", "tkfmi_build_params_batched")
+            "tkfmi_build_workspace_bytes", "tkfmi_build_params_batched",
+            "tkfmi_fb2d_batched_noalloc")
+NOALLOC_SCRATCH_BYTES = 576 << 20  # TKFMI_NOALLOC_SCRATCH_BYTES
 
 
 class TkfmiError(RuntimeError):
@@ -163,6 +165,9 @@ def lib():
             L.tkfmi_fb2d_batched.restype = ctypes.c_int
             L.tkfmi_fb2d_batched.argtypes = [ctypes.POINTER(Problem), ctypes.POINTER(Outputs),
                                              _c_p, ctypes.c_size_t, _c_p, ctypes.c_uint32]
+            L.tkfmi_fb2d_batched_noalloc.restype = ctypes.c_int
+            L.tkfmi_fb2d_batched_noalloc.argtypes = [ctypes.POINTER(Problem), ctypes.POINTER(Outputs),
+                                                     _c_p, ctypes.c_uint32]
             L.tkfmi_fb1d_workspace_bytes.restype = ctypes.c_size_t
             L.tkfmi_fb1d_workspace_bytes.argtypes = [ctypes.POINTER(Problem1D), ctypes.c_uint32]
             L.tkfmi_fb1d_batched.restype = ctypes.c_int
