@@ -70,3 +70,50 @@ def fb2d_batch(log_trans, state_types, state_group, log_tab, xs, ys, forward_onl
     if post is not None:
         posts = [post[poff[b]:poff[b] + cells[b]].reshape(xl[b] + 1, yl[b] + 1, S) for b in range(B)]
     return logp, n, posts
+
+
+_LIBS_SCALED = {}
+
+
+def _lib_scaled(long_double):
+    name = "liboracle_fb2d_ld.so" if long_double else "liboracle_fb2d_sc.so"
+    if name not in _LIBS_SCALED:
+        path = os.path.join(_HERE, "build", name)
+        if not os.path.exists(path):
+            subprocess.run(["make", "-C", _HERE], check=True, stdout=subprocess.DEVNULL)
+        L = ctypes.CDLL(path)
+        fn = L.oracle_fb2d_scaled_ld if long_double else L.oracle_fb2d_scaled
+        fn.restype = ctypes.c_int
+        _LIBS_SCALED[name] = fn
+    return _LIBS_SCALED[name]
+
+
+def fb2d_scaled(log_trans, state_types, state_group, log_tab, x, y, samples=None,
+                long_double=True, nthreads=8):
+    """One pair through oracle/c/fb2d_scaled.c (max-shifted product form of
+    the log-space recursion, in 80-bit long double by default): returns
+    (logP, n (S,S), emission counts (G,483), posteriors at the sampled cells
+    (n_samp,S)).  samples: (n_samp, 2) int array of (i, j) cells.  The FP64
+    build needs |logP| < ~700 (the count scale factor is exp(-logP))."""
+    lt = np.ascontiguousarray(log_trans, dtype=np.float64)
+    S = lt.shape[-1]
+    tab = np.ascontiguousarray(log_tab, dtype=np.float64).reshape(-1, 483)
+    G = tab.shape[0]
+    st = np.ascontiguousarray(state_types, dtype=np.int32)
+    sg = np.ascontiguousarray(state_group, dtype=np.int32)
+    xc = np.ascontiguousarray(np.concatenate([np.minimum(np.asarray(x, np.int64), 20), [0]]).astype(np.uint8))
+    yc = np.ascontiguousarray(np.concatenate([np.minimum(np.asarray(y, np.int64), 20), [0]]).astype(np.uint8))
+    samples = np.zeros((0, 2), np.int32) if samples is None else np.asarray(samples, np.int32)
+    si = np.ascontiguousarray(samples[:, 0])
+    sj = np.ascontiguousarray(samples[:, 1])
+    logp = np.zeros(1)
+    n = np.zeros((S, S))
+    ec = np.zeros((G, 483))
+    post = np.zeros((max(len(samples), 1), S))
+    rc = _lib_scaled(long_double)(
+        ctypes.c_int(S), _p(st), _p(sg), ctypes.c_int(G), _p(lt), _p(tab), _p(xc),
+        ctypes.c_int(len(x)), _p(yc), ctypes.c_int(len(y)), ctypes.c_int(nthreads), _p(logp),
+        _p(n), _p(ec), ctypes.c_int(len(samples)), _p(si), _p(sj), _p(post))
+    if rc != 0:
+        raise RuntimeError(f"oracle_fb2d_scaled failed: {rc}")
+    return float(logp[0]), n, ec, post[:len(samples)]

@@ -24,7 +24,7 @@ def _oracle(w, b, x, y):
     return dp2d.forward_backward(w.log_trans[b].numpy(), np.asarray(w.state_types), em)
 
 
-@pytest.mark.parametrize("K,F,C", [(2, 2, 2), (4, 2, 1), (5, 4, 2), (8, 4, 4)])
+@pytest.mark.parametrize("K,F,C", [(2, 2, 2), (4, 2, 1), (5, 4, 2), (8, 2, 2), (8, 4, 4)])
 def test_wide_matches_oracle(K, F, C):
     from oracle import emissions
     from tkfmi import hmm, synth
