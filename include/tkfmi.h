@@ -394,6 +394,62 @@ int tkfmi_build_params_batched(const tkfmi_builder* b, int32_t B, const double* 
                                double* log_trans, double* log_tab, void* workspace,
                                size_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Precompiled X/A/Y pairs decoded on the device.
+ * ---------------------------------------------------------------------- */
+
+/* status bits of a decoded pair (0 = valid, util/pair_format.py:193-223) */
+#define TKFMI_PAIR_BAD_X 1      /* a character of X outside "ACDEFGHIKLMNPQRSTVWY" */
+#define TKFMI_PAIR_BAD_Y 2      /* a character of Y outside the 20 amino acids     */
+#define TKFMI_PAIR_BAD_LEN_X 4  /* the M and D run lengths do not sum to |X|       */
+#define TKFMI_PAIR_BAD_LEN_Y 8  /* the M and I run lengths do not sum to |Y|       */
+#define TKFMI_PAIR_BAD_RUN 16   /* a run length above 2^24                         */
+
+/* The text of B records (util/pair_format.py:93-137: 'x', 'a', 'y'), as
+ * uploaded: the X strings, the Y strings and the run-length alignment
+ * strings ("M5I1M3D2") each concatenated into one device byte buffer. */
+typedef struct tkfmi_pair_text {
+  int32_t B;
+  int32_t pad;
+  const uint8_t* x_text;    /* device ASCII                                   */
+  const uint8_t* y_text;    /* device ASCII                                   */
+  const uint8_t* a_text;    /* device ASCII                                   */
+  const int64_t* h_x_off;   /* host [B]: pair b's X at x_text + h_x_off[b]     */
+  const int32_t* h_x_len;   /* host [B]: |X| = Lx                              */
+  const int64_t* h_y_off;   /* host [B]                                        */
+  const int32_t* h_y_len;   /* host [B]: |Y| = Ly                              */
+  const int64_t* h_a_off;   /* host [B]                                        */
+  const int32_t* h_a_len;   /* host [B]: bytes of the alignment string         */
+} tkfmi_pair_text;
+
+/* Decoded pairs.  x_codes / y_codes have the offsets of x_text / y_text and
+ * are the x_cat / y_cat of a tkfmi_problem.  Pair b's alignment columns start
+ * at sum_{b'<b} (Lx + Ly) (its capacity; it has n_cols[b] <= Lx + Ly) and are
+ * the col_type / anc / desc of a tkfmi_problem1d. */
+typedef struct tkfmi_pair_columns {
+  uint8_t* x_codes;   /* device: codes 0..19 (20 for an invalid character)    */
+  uint8_t* y_codes;   /* device                                               */
+  uint8_t* col_type;  /* device or NULL (then no columns): 1=M 2=I 3=D          */
+  uint8_t* anc;       /* device or NULL: ancestor code at M/D columns, else 0   */
+  uint8_t* desc;      /* device or NULL: descendant code at M/I columns, else 0 */
+  int32_t* n_cols;    /* device [B]: alignment columns (0 if the lengths fail)  */
+  int32_t* status;    /* device [B]: TKFMI_PAIR_* bits, 0 for a valid pair      */
+} tkfmi_pair_columns;
+
+/* Bytes of device workspace tkfmi_pairs_decode needs. */
+size_t tkfmi_pairs_decode_workspace_bytes(const tkfmi_pair_text* in);
+
+/*
+ * Decodes B precompiled pairs in one launch, one workgroup per pair:
+ * decode_pair (util/pair_format.py:140-190) with rle_decode (:69-90; a run
+ * is an M, I or D followed by digits, other text is skipped as the
+ * reference's regex skips it) and the checks of validate_record (:193-223),
+ * for the pairs PrecompiledPairSource streams (train_pfam.py:984-1075).
+ * Stream-ordered; status and n_cols are written for every pair.
+ */
+int tkfmi_pairs_decode(const tkfmi_pair_text* in, const tkfmi_pair_columns* out, void* workspace,
+                       size_t workspace_bytes, void* stream /* hipStream_t */);
+
 /* One kernel launch of a timed call. */
 typedef struct tkfmi_kernel_time {
   char name[64];    /* kernel, e.g. "fbv2_kernel<LMix<3,1>,8>"            */

@@ -119,6 +119,34 @@ class Builder(ctypes.Structure):
     ]
 
 
+class PairText(ctypes.Structure):
+    _fields_ = [
+        ("B", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("x_text", _c_p),
+        ("y_text", _c_p),
+        ("a_text", _c_p),
+        ("h_x_off", _c_p),
+        ("h_x_len", _c_p),
+        ("h_y_off", _c_p),
+        ("h_y_len", _c_p),
+        ("h_a_off", _c_p),
+        ("h_a_len", _c_p),
+    ]
+
+
+class PairColumns(ctypes.Structure):
+    _fields_ = [
+        ("x_codes", _c_p),
+        ("y_codes", _c_p),
+        ("col_type", _c_p),
+        ("anc", _c_p),
+        ("desc", _c_p),
+        ("n_cols", _c_p),
+        ("status", _c_p),
+    ]
+
+
 class KernelTime(ctypes.Structure):
     _fields_ = [
         ("name", ctypes.c_char * 64),
@@ -140,7 +168,8 @@ EXPORTED = ("tkfmi_abi_version", "tkfmi_last_error",
             "tkfmi_restore_workspace_bytes", "tkfmi_restore_counts_batched",
             "tkfmi_bdi_stats_batched", "tkfmi_tkf92_estep_stats", "tkfmi_hr_stats_batched",
             "tkfmi_build_workspace_bytes", "tkfmi_build_params_batched",
-            "tkfmi_fb2d_batched_noalloc")
+            "tkfmi_fb2d_batched_noalloc", "tkfmi_pairs_decode_workspace_bytes",
+            "tkfmi_pairs_decode")
 NOALLOC_SCRATCH_BYTES = 576 << 20  # TKFMI_NOALLOC_SCRATCH_BYTES
 
 
@@ -168,6 +197,11 @@ def lib():
             L.tkfmi_fb2d_batched_noalloc.restype = ctypes.c_int
             L.tkfmi_fb2d_batched_noalloc.argtypes = [ctypes.POINTER(Problem), ctypes.POINTER(Outputs),
                                                      _c_p, ctypes.c_uint32]
+            L.tkfmi_pairs_decode_workspace_bytes.restype = ctypes.c_size_t
+            L.tkfmi_pairs_decode_workspace_bytes.argtypes = [ctypes.POINTER(PairText)]
+            L.tkfmi_pairs_decode.restype = ctypes.c_int
+            L.tkfmi_pairs_decode.argtypes = [ctypes.POINTER(PairText), ctypes.POINTER(PairColumns),
+                                             _c_p, ctypes.c_size_t, _c_p]
             L.tkfmi_fb1d_workspace_bytes.restype = ctypes.c_size_t
             L.tkfmi_fb1d_workspace_bytes.argtypes = [ctypes.POINTER(Problem1D), ctypes.c_uint32]
             L.tkfmi_fb1d_batched.restype = ctypes.c_int

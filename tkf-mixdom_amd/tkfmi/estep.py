@@ -197,18 +197,25 @@ def process_pairs_batched_2d(batch_pairs, chi_params, st, Q_lg, pi_lg, N, dom_Qs
     """Sufficient-statistics delta of a minibatch (train_pfam.py:2672-2903).
 
     batch_pairs: (x, y, states, anc_chars, desc_chars, t_est) tuples (only
-    x, y, t_est used).  Per-pair chi at t_est (log floor 1e-30,
+    x, y, t_est used), or a tkfmi.pairstore.PairStore, whose device-decoded
+    residue codes are used as they are (no host packing).  Per-pair chi at t_est (log floor 1e-30,
     train_pfam.py:1632-1640), per-domain substitution matrices expm(Q_d t)
     with per-domain emissions (LG Q/pi tiled when dom_Qs is None), t_est
     rounded to float32 first as the reference does (:2828).  Pairs with a
     non-finite log-likelihood are skipped (:2859-2860).  Returns the delta
     dict or None for an empty batch."""
-    if not batch_pairs:
+    if not len(batch_pairs):
         return None
     dev = hmm.device()
-    xs = [np.asarray(it[0], dtype=np.int64) for it in batch_pairs]
-    ys = [np.asarray(it[1], dtype=np.int64) for it in batch_pairs]
-    t32 = np.array([it[5] for it in batch_pairs], dtype=np.float32).astype(np.float64)
+    from .pairstore import PairStore
+    if isinstance(batch_pairs, PairStore):
+        store = batch_pairs
+        t32 = store.t.astype(np.float32).astype(np.float64)
+    else:
+        store = None
+        xs = [np.asarray(it[0], dtype=np.int64) for it in batch_pairs]
+        ys = [np.asarray(it[1], dtype=np.int64) for it in batch_pairs]
+        t32 = np.array([it[5] for it in batch_pairs], dtype=np.float32).astype(np.float64)
     cp = chi_params
     if dom_Qs is not None and dom_pis is not None:
         Qs_n, pis_n = _np(dom_Qs).astype(np.float64), _np(dom_pis).astype(np.float64)
@@ -221,7 +228,7 @@ def process_pairs_batched_2d(batch_pairs, chi_params, st, Q_lg, pi_lg, N, dom_Qs
     b = dbuild.DeviceBuilder("mixdom", "per_domain", Qs_n, pis_n, mix=mix, log_floor=True, device=dev)
     lt, tab, t = b.build(t32)                                         # chi and (B, K, TAB) tables
     Qs, pis = _dev64(Qs_n, dev), _dev64(pis_n, dev)
-    pairs = _abi.PackedPairs(xs, ys, dev)
+    pairs = store if store is not None else _abi.PackedPairs(xs, ys, dev)
     r = _abi.BatchRunner(lt, b.state_types, b.state_group, tab, pairs, want_emit_counts=True)
     r.run(zero_outputs=False)
     r.check_status()  # a timed-out wait is an error, not an impossible pair
