@@ -484,6 +484,42 @@ int tkfmi_last_kernel_times(tkfmi_kernel_time* out, int32_t max);
  */
 int tkfmi_chain_config(int32_t long_w, int32_t main_w, double long_frac);
 
+/* Result of tkfmi_autotune: the candidates timed, in order (the first is
+ * the configuration in force before the call), their ms per call (-1: not
+ * run because its workspace exceeded the one given) and the chosen one. */
+typedef struct tkfmi_tune_result {
+  int32_t layout;      /* internal layout id of the key, -1 if not tunable     */
+  int32_t length_bin;  /* longest sequence < 128, 256, 512, 1024, 2048, more   */
+  int32_t n;           /* candidates timed (0: one configuration only)         */
+  int32_t best;        /* index of the chosen candidate                        */
+  int32_t long_w[16];
+  int32_t main_w[16];
+  double long_frac[16];
+  double ms[16];
+} tkfmi_tune_result;
+
+/* Workspace tkfmi_autotune needs for `prob` (the largest over candidates). */
+size_t tkfmi_autotune_workspace_bytes(const tkfmi_problem* prob, uint32_t flags);
+
+/*
+ * Autotuner of the chained 2D kernels; no counterpart in the reference
+ * (its 2D path is one vmapped JAX call per padded shape).  Times the
+ * candidate class configurations (waves per workgroup of the long and main
+ * classes, the long-class cell fraction) on the caller's own batch -- one
+ * warm-up and `reps` timed calls each, HIP events on `stream`, synchronous
+ * -- and keeps the fastest (the configuration already in force unless
+ * another is more than 3 % faster) for the batch's key: its state layout
+ * and the length bin of its longest sequence.  Later calls whose batch has
+ * that key use it (query their workspace size after tuning: it depends on
+ * the configuration); other keys keep tkfmi_chain_config's setting (which
+ * clears every tuned key).  `out` receives the batch's results.  Layouts
+ * without a tunable configuration (the generic and 16..160-state kernels,
+ * dense emissions) return TKFMI_OK with res->n = 0.
+ */
+int tkfmi_autotune(const tkfmi_problem* prob, const tkfmi_outputs* out, void* workspace,
+                   size_t workspace_bytes, void* stream /* hipStream_t */, uint32_t flags,
+                   int32_t reps, tkfmi_tune_result* res);
+
 #ifdef __cplusplus
 }
 #endif

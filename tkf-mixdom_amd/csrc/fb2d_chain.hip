@@ -38,6 +38,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -831,9 +832,24 @@ ChainConfig g_cfg = [] {
   if (const char* e = getenv("TKFMI_CHAIN_LONG")) c.long_frac = atof(e);
   return c;
 }();
-ChainConfig chain_config() {
+// Configurations chosen by tkfmi_autotune, per (layout, length bin of the
+// batch's longest sequence); a batch with no entry uses g_cfg.
+constexpr int kLenBins = 6;  // longest sequence < 128, 256, 512, 1024, 2048, more
+std::map<std::pair<int, int>, ChainConfig> g_tuned;
+int length_bin(int max_len) {
+  int b = 0;
+  for (int lim = 128; b < kLenBins - 1 && max_len >= lim; lim *= 2) ++b;
+  return b;
+}
+int max_seq_len(const tkfmi_problem* p) {
+  int m = 0;
+  for (int b = 0; b < p->B; ++b) m = std::max(m, std::max(p->h_x_len[b], p->h_y_len[b]));
+  return m;
+}
+ChainConfig chain_config(int layout, int bin) {
   std::lock_guard<std::mutex> lk(g_cfg_mu);
-  return g_cfg;
+  auto it = g_tuned.find({layout, bin});
+  return it != g_tuned.end() ? it->second : g_cfg;
 }
 // Class of a pair with nb bands and `cells` lattice cells.
 int ccls_of(const ChainConfig& cf, int nb, int64_t cells, int64_t cells_max) {
@@ -868,7 +884,9 @@ struct ChainPlan {
   size_t total() const { return desc_b + pm_b + lat_b + ring_b + flag_b + pinfo_b; }
 };
 
-ChainPlan plan_chain(const tkfmi_problem* p, const Plan& pl, uint32_t flags) {
+// cfg: the class configuration to plan with (null: the tuned or default one
+// for the batch's layout and length bin).
+ChainPlan plan_chain(const tkfmi_problem* p, const Plan& pl, uint32_t flags, const ChainConfig* cfg = nullptr) {
   ChainPlan cp;
   const int B = p->B;
   const int NX = nx_of(pl);
@@ -895,7 +913,7 @@ ChainPlan plan_chain(const tkfmi_problem* p, const Plan& pl, uint32_t flags) {
   int64_t aring = 0;
   int nfl = 2 * NCLS + 8;
   const int64_t cells_max = B > 0 ? (int64_t)(p->h_x_len[order[0]] + 1) * (p->h_y_len[order[0]] + 1) : 0;
-  const ChainConfig cf = chain_config();
+  const ChainConfig cf = cfg ? *cfg : chain_config((int)pl.lay, length_bin(max_seq_len(p)));
   for (int c = 0; c < NCLS; ++c) cp.W[c] = cf.w[c];
   for (int q = 0; q < B; ++q) {
     const int b = order[q];
@@ -1060,11 +1078,22 @@ bool tkfmi_chain_workspace(const tkfmi_problem* p, uint32_t flags, size_t* bytes
   return true;
 }
 
+namespace {
+int chain_run_impl(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, size_t ws_bytes,
+                   hipStream_t stream, uint32_t flags, std::string& err, const ChainConfig* cfg);
+}
+
 int tkfmi_chain_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, size_t ws_bytes,
                     hipStream_t stream, uint32_t flags, std::string& err) {
+  return chain_run_impl(p, o, ws, ws_bytes, stream, flags, err, nullptr);
+}
+
+namespace {
+int chain_run_impl(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, size_t ws_bytes,
+                   hipStream_t stream, uint32_t flags, std::string& err, const ChainConfig* cfg) {
   Plan pl;
   if (!detect(p, pl)) { err = "layout not handled"; return TKFMI_EINVAL; }
-  const ChainPlan cp = plan_chain(p, pl, flags);
+  const ChainPlan cp = plan_chain(p, pl, flags, cfg);
   if (!cp.ok) { err = cp.why; return TKFMI_EINVAL; }
   if (!ws || ws_bytes < cp.total()) {
     err = "workspace too small: need " + std::to_string(cp.total());
@@ -1150,6 +1179,8 @@ int tkfmi_chain_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, si
   return TKFMI_OK;
 }
 
+}  // namespace
+
 extern "C" int tkfmi_chain_config(int32_t long_w, int32_t main_w, double long_frac) {
   if (long_w < 1 || long_w > 4 || main_w < 2 || main_w > 4 || !(long_frac >= 0.0))
     return tkfmi_fail(TKFMI_EINVAL, "tkfmi_chain_config: waves must be 1..4 (main 2..4), long_frac >= 0");
@@ -1157,5 +1188,95 @@ extern "C" int tkfmi_chain_config(int32_t long_w, int32_t main_w, double long_fr
   g_cfg.w[0] = long_w;
   g_cfg.w[1] = main_w;
   g_cfg.long_frac = long_frac;
+  g_tuned.clear();
+  return TKFMI_OK;
+}
+
+namespace {
+// Candidate (long_w, main_w, long_frac) settings timed by tkfmi_autotune; the
+// current default for the batch's key is always timed first.
+const ChainConfig kCandidates[] = {
+    {{3, 3, 2, 1}, 0.0}, {{3, 3, 2, 1}, 0.15}, {{3, 3, 2, 1}, 0.3}, {{4, 3, 2, 1}, 0.3},
+    {{4, 4, 2, 1}, 0.3}, {{4, 4, 2, 1}, 0.0},  {{3, 2, 2, 1}, 0.0}, {{2, 2, 2, 1}, 0.0}};
+constexpr int kNCand = (int)(sizeof(kCandidates) / sizeof(kCandidates[0]));
+
+bool same_cfg(const ChainConfig& a, const ChainConfig& b) {
+  return a.w[0] == b.w[0] && a.w[1] == b.w[1] && a.long_frac == b.long_frac;
+}
+}  // namespace
+
+extern "C" size_t tkfmi_autotune_workspace_bytes(const tkfmi_problem* p, uint32_t flags) {
+  Plan pl;
+  if (!p || p->B < 0 || !p->state_types || !p->state_group || p->log_emit || !detect(p, pl))
+    return tkfmi_fb2d_workspace_bytes(p, flags);  // a single configuration
+  if (p->B > 0 && (!p->h_x_len || !p->h_y_len || !p->h_x_off || !p->h_y_off)) return 0;
+  size_t need = 0;
+  for (int c = -1; c < kNCand; ++c) {
+    const ChainPlan cp = plan_chain(p, pl, flags, c >= 0 ? &kCandidates[c] : nullptr);
+    if (cp.ok) need = std::max(need, cp.total());
+  }
+  return need;
+}
+
+extern "C" int tkfmi_autotune(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, size_t ws_bytes,
+                              void* stream_v, uint32_t flags, int32_t reps, tkfmi_tune_result* res) {
+  tkfmi_clear_error();
+  if (!res) return tkfmi_fail(TKFMI_EINVAL, "tkfmi_autotune: result pointer missing");
+  memset(res, 0, sizeof(*res));
+  res->layout = -1;
+  if (flags & (TKFMI_TIME_KERNELS | TKFMI_REUSE_DESCRIPTORS | TKFMI_GENERIC_KERNEL | TKFMI_CLASS_KERNEL))
+    return tkfmi_fail(TKFMI_EINVAL, "tkfmi_autotune: tunes the default (chained) path; drop the kernel, "
+                                    "timing and reuse flags");
+  Plan pl;
+  if (!p || !p->state_types || !p->state_group || p->log_emit || !detect(p, pl))
+    return TKFMI_OK;  // no tunable configuration for this layout (res->n = 0)
+  if (p->B > 0 && (!p->h_x_len || !p->h_y_len || !p->h_x_off || !p->h_y_off))
+    return tkfmi_fail(TKFMI_EINVAL, "tkfmi_autotune: host length/offset arrays missing");
+  if (reps < 1) reps = 3;
+  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_v);
+  const int key_l = (int)pl.lay, key_b = length_bin(max_seq_len(p));
+  res->layout = key_l;
+  res->length_bin = key_b;
+  std::vector<ChainConfig> cand;
+  cand.push_back(chain_config(key_l, key_b));  // the current choice first
+  for (const ChainConfig& c : kCandidates)
+    if (!same_cfg(c, cand[0])) cand.push_back(c);
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return tkfmi_fail(TKFMI_EHIP, "tkfmi_autotune: hipEventCreate failed");
+  int rc = TKFMI_OK;
+  std::string err;
+  for (size_t c = 0; c < cand.size() && c < 16 && rc == TKFMI_OK; ++c) {
+    double ms = -1.0;
+    const ChainPlan cp = plan_chain(p, pl, flags, &cand[c]);
+    if (cp.ok && ws && ws_bytes >= cp.total()) {
+      rc = chain_run_impl(p, o, ws, ws_bytes, stream, flags, err, &cand[c]);  // warm-up
+      if (rc == TKFMI_OK && hipEventRecord(e0, stream) != hipSuccess) rc = TKFMI_EHIP;
+      for (int r = 0; r < reps && rc == TKFMI_OK; ++r)
+        rc = chain_run_impl(p, o, ws, ws_bytes, stream, flags, err, &cand[c]);
+      if (rc == TKFMI_OK && (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+        rc = TKFMI_EHIP;
+      float t = 0.f;
+      if (rc == TKFMI_OK && hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms = (double)t / reps;
+    }
+    res->long_w[c] = cand[c].w[0];
+    res->main_w[c] = cand[c].w[1];
+    res->long_frac[c] = cand[c].long_frac;
+    res->ms[c] = ms;  // -1: not run (its workspace exceeds ws_bytes)
+    res->n = (int32_t)c + 1;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // keep the first candidate unless another is > 3 % faster
+  int best = 0;
+  for (int c = 1; c < res->n; ++c)
+    if (res->ms[c] > 0.0 && (res->ms[best] < 0.0 || res->ms[c] < res->ms[best])) best = c;
+  if (res->ms[0] > 0.0 && !(res->ms[best] < 0.97 * res->ms[0])) best = 0;
+  res->best = best;
+  {
+    std::lock_guard<std::mutex> lk(g_cfg_mu);
+    g_tuned[{key_l, key_b}] = cand[best];
+  }
+  if (rc != TKFMI_OK) return tkfmi_fail(rc, "tkfmi_autotune: " + (err.empty() ? std::string("HIP error") : err));
   return TKFMI_OK;
 }

@@ -147,6 +147,19 @@ class PairColumns(ctypes.Structure):
     ]
 
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [
+        ("layout", ctypes.c_int32),
+        ("length_bin", ctypes.c_int32),
+        ("n", ctypes.c_int32),
+        ("best", ctypes.c_int32),
+        ("long_w", ctypes.c_int32 * 16),
+        ("main_w", ctypes.c_int32 * 16),
+        ("long_frac", ctypes.c_double * 16),
+        ("ms", ctypes.c_double * 16),
+    ]
+
+
 class KernelTime(ctypes.Structure):
     _fields_ = [
         ("name", ctypes.c_char * 64),
@@ -169,7 +182,7 @@ EXPORTED = ("tkfmi_abi_version", "tkfmi_last_error",
             "tkfmi_bdi_stats_batched", "tkfmi_tkf92_estep_stats", "tkfmi_hr_stats_batched",
             "tkfmi_build_workspace_bytes", "tkfmi_build_params_batched",
             "tkfmi_fb2d_batched_noalloc", "tkfmi_pairs_decode_workspace_bytes",
-            "tkfmi_pairs_decode")
+            "tkfmi_pairs_decode", "tkfmi_autotune_workspace_bytes", "tkfmi_autotune")
 NOALLOC_SCRATCH_BYTES = 576 << 20  # TKFMI_NOALLOC_SCRATCH_BYTES
 
 
@@ -197,6 +210,12 @@ def lib():
             L.tkfmi_fb2d_batched_noalloc.restype = ctypes.c_int
             L.tkfmi_fb2d_batched_noalloc.argtypes = [ctypes.POINTER(Problem), ctypes.POINTER(Outputs),
                                                      _c_p, ctypes.c_uint32]
+            L.tkfmi_autotune_workspace_bytes.restype = ctypes.c_size_t
+            L.tkfmi_autotune_workspace_bytes.argtypes = [ctypes.POINTER(Problem), ctypes.c_uint32]
+            L.tkfmi_autotune.restype = ctypes.c_int
+            L.tkfmi_autotune.argtypes = [ctypes.POINTER(Problem), ctypes.POINTER(Outputs), _c_p,
+                                         ctypes.c_size_t, _c_p, ctypes.c_uint32, ctypes.c_int32,
+                                         ctypes.POINTER(TuneResult)]
             L.tkfmi_pairs_decode_workspace_bytes.restype = ctypes.c_size_t
             L.tkfmi_pairs_decode_workspace_bytes.argtypes = [ctypes.POINTER(PairText)]
             L.tkfmi_pairs_decode.restype = ctypes.c_int
@@ -487,6 +506,15 @@ class BatchRunner:
         if rc != 0:
             raise TkfmiError(self.L.tkfmi_last_error().decode() or f"tkfmi error {rc}")
         return graph
+
+    def resize_workspace(self):
+        """Query the workspace again (after tkfmi_chain_config or
+        tkfmi_autotune changed the configuration of this batch's key) and
+        grow it if needed."""
+        need = int(self.L.tkfmi_fb2d_workspace_bytes(ctypes.byref(self.prob), self.flags))
+        if need > self.ws.numel():
+            self.ws = torch.empty(need, dtype=torch.uint8, device=self.dev)
+        self.ws_bytes = max(need, 1)
 
     def check_status(self):
         """Raise if the last call's chained launches gave up on an internal

@@ -5,10 +5,12 @@ the other pairs of several bands, 2-band and 1-band pairs) and run each
 class as chains of workgroups.  How many waves a workgroup of the two long
 classes has trades per-pair parallelism against occupancy (a 4-wave backward
 workgroup needs ~92 KB of LDS, so one fits a compute unit; a 3-wave one needs
-~76 KB and a 2-wave one ~58 KB, so two fit).  The best choice depends on the batch's length mix, so
-:func:`tune_chain` times a few settings on the caller's own batch (one
-representative call each) and keeps the fastest, through the library's
-process-wide ``tkfmi_chain_config``.
+~76 KB and a 2-wave one ~58 KB, so two fit).  The best choice depends on the batch's length mix, so :func:`tune`
+(libtkfmi's tkfmi_autotune) times a few settings on the caller's own batch
+and keeps the fastest for that batch's key -- its state layout and the
+length bin of its longest sequence -- which later calls with the same key
+use.  :func:`tune_chain` is the process-wide variant through
+``tkfmi_chain_config``.
 """
 import time
 
@@ -22,6 +24,34 @@ from . import _abi
 DEFAULT = (3, 3, 0.0)  # the library's own defaults
 CANDIDATES = ((3, 3, 0.0), (3, 3, 0.3), (4, 3, 0.3), (4, 4, 0.3), (3, 3, 0.15), (3, 3, 0.5), (4, 4, 0.0),
               (3, 2, 0.0), (2, 2, 0.0))
+
+
+def tune(runner, reps=3):
+    """libtkfmi's own autotuner (tkfmi_autotune) on a BatchRunner's batch:
+    times the candidate configurations of the chained kernels on that batch
+    and keeps the fastest for the batch's key (state layout, length bin of
+    its longest sequence), which later calls with that key then use.
+    Resizes the runner's workspace for the chosen configuration.  Returns
+    {"key": (layout, length_bin), "best": (long_w, main_w, long_frac),
+    "ms": {config: ms per call}} (empty "ms" when the layout has a single
+    configuration)."""
+    L = _abi.lib()
+    flags = runner.flags & ~(_abi.TIME_KERNELS | _abi.REUSE_DESCRIPTORS)
+    wsb = int(L.tkfmi_autotune_workspace_bytes(_abi.ctypes.byref(runner.prob), flags))
+    ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=runner.dev)
+    res = _abi.TuneResult()
+    stream = torch.cuda.current_stream(runner.dev)
+    with torch.cuda.stream(stream):
+        runner.status.zero_()
+    _abi.check(L.tkfmi_autotune(_abi.ctypes.byref(runner.prob), _abi.ctypes.byref(runner.out),
+                                _abi._ptr(ws), _abi.ctypes.c_size_t(wsb),
+                                _abi.ctypes.c_void_p(stream.cuda_stream), flags, int(reps),
+                                _abi.ctypes.byref(res)))
+    runner.resize_workspace()
+    cfgs = [(res.long_w[c], res.main_w[c], res.long_frac[c]) for c in range(res.n)]
+    return {"key": (res.layout, res.length_bin),
+            "best": cfgs[res.best] if res.n else None,
+            "ms": {c: res.ms[i] for i, c in enumerate(cfgs)}}
 
 
 def set_chain_config(long_w, main_w, long_frac):
