@@ -53,9 +53,10 @@ extern "C" {
                                     workspace: skip their host-to-device
                                     upload, so the call is only device
                                     work (e.g. captured into a HIP graph)  */
-#define TKFMI_CHECKPOINT 256u    /* antidiagonal checkpointing (the
-                                    reference's gradient path recomputes
-                                    the lattice under jax.checkpoint,
+#define TKFMI_CHECKPOINT 256u    /* antidiagonal checkpointing for every
+                                    pair (the reference's gradient path
+                                    recomputes the lattice under
+                                    jax.checkpoint,
                                     python/tkfmixdom/jax/train/vjp.py
                                     remat): the forward keeps the restart
                                     state of every K-th antidiagonal, K ~
@@ -63,10 +64,16 @@ extern "C" {
                                     recomputes one K-antidiagonal segment
                                     at a time; identical results, one extra
                                     forward sweep, lattice memory about
-                                    3 K / (Ly+1) of the full one.  Applies
-                                    to the 17..162-state kernel; the other
-                                    kernels keep their 128-B-per-cell
-                                    lattice and ignore it                  */
+                                    3 K / (Ly+1) of the full one.  Without
+                                    the flag, pairs whose lattice exceeds
+                                    the budget (tkfmi_checkpoint_budget,
+                                    256 MiB by default) are checkpointed.
+                                    Applies to the 16..160-state kernel;
+                                    the cell-per-lane and generic kernels
+                                    keep their 128-B-per-cell lattice (at
+                                    most 33 GB for the longest pair they
+                                    accept, Lx + Ly <= 32000, which fits
+                                    HBM) and ignore it                     */
 
 /* emission tables: per group g, 21x21 match block then 21 insert then 21
  * delete entries (index 20 = wildcard), all natural-log values. */
@@ -121,6 +128,13 @@ typedef struct tkfmi_outputs {
                           caller tell such pairs from impossible ones (which
                           the reference filters as NaN / -inf)              */
 } tkfmi_outputs;
+
+/* Lattice bytes of one pair above which the 16..160-state kernel
+ * checkpoints that pair without TKFMI_CHECKPOINT (default 256 MiB, the
+ * budget of the reference's gradient configuration; 0 = never without the
+ * flag).  Process-wide; query workspace sizes after changing it.  No
+ * counterpart in the reference (its remat is always on in the VJP). */
+int tkfmi_checkpoint_budget(size_t bytes);
 
 /* Version of this header's ABI (TKFMI_ABI_VERSION). */
 int tkfmi_abi_version(void);

@@ -94,7 +94,9 @@ def test_checkpoint_shrinks_the_wide_workspace():
     L = _abi.lib()
     st = list(mixdom.state_types(8, 4))
     p, k = _problem(st, [0] * len(st), [500, 800], [600, 700], G=1)
+    _abi.set_checkpoint_budget(0)  # no automatic checkpointing
     full = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), 0)
+    _abi.set_checkpoint_budget(_abi.CHECKPOINT_BUDGET_DEFAULT)
     ck = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), _abi.CHECKPOINT)
     fo = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), _abi.FORWARD_ONLY)
     lat_full = sum((lx + 1) * (ly + 1) * 161 * 8 for lx, ly in ((500, 600), (800, 700)))
@@ -107,6 +109,29 @@ def test_checkpoint_shrinks_the_wide_workspace():
     p17, k17 = _problem(st17, [0] * 17, [500, 800], [600, 700], G=1)
     assert L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p17), _abi.CHECKPOINT) == \
         L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p17), 0)
+
+
+def test_checkpoint_budget_applies_per_pair():
+    """Without TKFMI_CHECKPOINT, a 162-state pair whose full lattice
+    ((Lx+1)(Ly+1)(NBP+1) x 8 B) exceeds the budget is checkpointed and a
+    smaller one is not: with the default 256 MiB budget the 800 x 700 pair
+    (0.9 GB) is, the 200 x 150 pair (39 MB) is not; budget 0 turns it off."""
+    from tkfmi import _abi, mixdom
+    L = _abi.lib()
+    st = list(mixdom.state_types(8, 4))
+    ws = lambda xl, yl, f=0: L.tkfmi_fb2d_workspace_bytes(ctypes.byref(_problem(st, [0] * len(st), xl, yl)[0]), f)
+    try:
+        _abi.set_checkpoint_budget(_abi.CHECKPOINT_BUDGET_DEFAULT)
+        auto_big, auto_small = ws([800], [700]), ws([200], [150])
+        assert auto_big == ws([800], [700], _abi.CHECKPOINT)
+        assert auto_small < ws([200], [150], _abi.CHECKPOINT) * 10 and auto_small > ws([200], [150], _abi.CHECKPOINT)
+        both = ws([800, 200], [700, 150])
+        _abi.set_checkpoint_budget(0)
+        assert ws([800], [700]) > 4 * auto_big
+        assert ws([200], [150]) == auto_small
+        assert ws([800, 200], [700, 150]) > both
+    finally:
+        _abi.set_checkpoint_budget(_abi.CHECKPOINT_BUDGET_DEFAULT)
 
 
 def test_invalid_problems_rejected_with_message():

@@ -112,3 +112,25 @@ def test_wide_checkpoint_matches_full_lattice():
         assert _close(res["posteriors"][b].cpu().numpy(), post_o)[0], b
         ec_o = emissions.emission_counts(post_o, w.state_types, w.state_group, x, y, G)
         assert _close(res["emit_counts"][b].cpu().numpy(), ec_o)[0], b
+
+
+def test_budget_checkpointing_is_bit_identical():
+    """Pairs over the checkpoint budget are checkpointed automatically (here
+    a 2 MB budget catches the three longer pairs of the batch but not the
+    shortest); every result equals the run without checkpointing bit for
+    bit (the recomputation repeats the forward's arithmetic in its order)."""
+    from tkfmi import _abi, hmm, synth
+    w = synth.make_mixdom2(seed=44, B=4, K=4, F=2, C=2, lengths=[12, 90, 150, 60])
+    run = lambda: hmm.fb2d_batch(w.log_trans, w.state_types, w.xs, w.ys, w.log_tab, w.state_group,
+                                 want_posteriors=True, want_emit_counts=True)
+    try:
+        _abi.set_checkpoint_budget(0)
+        full = run()
+        _abi.set_checkpoint_budget(2 << 20)
+        auto = run()
+    finally:
+        _abi.set_checkpoint_budget(_abi.CHECKPOINT_BUDGET_DEFAULT)
+    assert np.array_equal(full["log_p"].cpu().numpy(), auto["log_p"].cpu().numpy())
+    assert np.array_equal(full["n"].cpu().numpy(), auto["n"].cpu().numpy())
+    for a, b in zip(full["posteriors"], auto["posteriors"]):
+        assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())

@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -715,7 +716,18 @@ int tkfmi_stage_upload(void* dst, const void* src, size_t bytes, hipStream_t str
   return TKFMI_OK;
 }
 
+namespace {
+std::atomic<size_t> g_ckpt_budget{(size_t)256 << 20};
+}
+
+size_t tkfmi_checkpoint_budget_bytes() { return g_ckpt_budget.load(); }
+
 extern "C" {
+
+int tkfmi_checkpoint_budget(size_t bytes) {
+  g_ckpt_budget.store(bytes);
+  return TKFMI_OK;
+}
 
 int tkfmi_abi_version(void) { return TKFMI_ABI_VERSION; }
 

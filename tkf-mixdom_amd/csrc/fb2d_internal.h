@@ -39,6 +39,10 @@ int tkfmi_wide_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
 // every launch goes on the caller's stream (no library streams or events).
 bool tkfmi_noalloc_active();
 
+// Lattice bytes of one pair above which the 16..160-state kernel keeps only
+// checkpoints (tkfmi_checkpoint_budget; 0 = only with TKFMI_CHECKPOINT).
+size_t tkfmi_checkpoint_budget_bytes();
+
 // Records `msg` as this thread's tkfmi_last_error() and returns `code`.
 int tkfmi_fail(int code, const std::string& msg);
 // Clears this thread's error message.

@@ -727,7 +727,9 @@ bool plan(const tkfmi_problem* p, uint32_t flags, WPlan& pl, std::string& why) {
     const int64_t cells = (int64_t)(lx + 1) * (ly + 1);
     d.lat_off = off;
     if (!(flags & TKFMI_FORWARD_ONLY)) {
-      if (flags & TKFMI_CHECKPOINT) {
+      const int64_t full_bytes = (cells * pl.NBP + cells) * 8;
+      const size_t budget = tkfmi_checkpoint_budget_bytes();
+      if ((flags & TKFMI_CHECKPOINT) || (budget > 0 && (size_t)full_bytes > budget)) {
         // K ~ sqrt(2 (Lx + Ly + 1)) minimises checkpoints (2 rows each) + segment
         const int K = std::max(4, (int)std::ceil(std::sqrt(2.0 * (lx + ly + 1))));
         const int64_t nck = (lx + ly) / K + 1, row = (int64_t)(lx + 1) * (pl.NBP + 1);

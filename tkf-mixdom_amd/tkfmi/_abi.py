@@ -182,7 +182,8 @@ EXPORTED = ("tkfmi_abi_version", "tkfmi_last_error",
             "tkfmi_bdi_stats_batched", "tkfmi_tkf92_estep_stats", "tkfmi_hr_stats_batched",
             "tkfmi_build_workspace_bytes", "tkfmi_build_params_batched",
             "tkfmi_fb2d_batched_noalloc", "tkfmi_pairs_decode_workspace_bytes",
-            "tkfmi_pairs_decode", "tkfmi_autotune_workspace_bytes", "tkfmi_autotune")
+            "tkfmi_pairs_decode", "tkfmi_autotune_workspace_bytes", "tkfmi_autotune",
+            "tkfmi_checkpoint_budget")
 NOALLOC_SCRATCH_BYTES = 576 << 20  # TKFMI_NOALLOC_SCRATCH_BYTES
 
 
@@ -210,6 +211,8 @@ def lib():
             L.tkfmi_fb2d_batched_noalloc.restype = ctypes.c_int
             L.tkfmi_fb2d_batched_noalloc.argtypes = [ctypes.POINTER(Problem), ctypes.POINTER(Outputs),
                                                      _c_p, ctypes.c_uint32]
+            L.tkfmi_checkpoint_budget.restype = ctypes.c_int
+            L.tkfmi_checkpoint_budget.argtypes = [ctypes.c_size_t]
             L.tkfmi_autotune_workspace_bytes.restype = ctypes.c_size_t
             L.tkfmi_autotune_workspace_bytes.argtypes = [ctypes.POINTER(Problem), ctypes.c_uint32]
             L.tkfmi_autotune.restype = ctypes.c_int
@@ -357,6 +360,15 @@ def fb2d_batched(log_trans, state_types, state_group, log_tab, pairs, *,
                     generic=generic, kernel=kernel, checkpoint=checkpoint, log_emit=log_emit)
     r.run(zero_outputs=False)  # outputs are freshly zeroed
     return r.results()
+
+
+CHECKPOINT_BUDGET_DEFAULT = 256 << 20
+
+
+def set_checkpoint_budget(nbytes):
+    """Lattice bytes of one pair above which the 16..160-state kernel
+    checkpoints it automatically (0: only with checkpoint=True)."""
+    check(lib().tkfmi_checkpoint_budget(int(nbytes)))
 
 
 def check_status(status):
