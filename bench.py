@@ -14,12 +14,13 @@ inputs already resident in HBM.  Prints one JSON line.
                 one-workgroup-per-pair launch doing both), divided by that
                 launch's HIP-event duration on its own stream, against 8 TB/s
   cpu_baseline= the C restatement of the reference recursion (log space,
-                OpenMP over pairs) on a bounded sample of the same pairs
+                OpenMP over pairs) on bounded samples of the same pairs, on
+                all the host cores the job may use and on one core
   plain       = the per-pair drop-in (one launch per pair, as the reference's
                 per-pair forward_backward_2d call) on a sample of pairs
   configs     = the other workloads, each timed on its own: c0 (TKF91
                 forward-only, 64 pairs), c2 (MixDom2 K=8 F=4 C=4, 162 states,
-                wide kernel), c3 (gradients of the summed MixDom2 K=3 F=1 C=2
+                wide kernel), mid (the c1 shape at K=3 F=2, 32 states), c3 (gradients of the summed MixDom2 K=3 F=1 C=2
                 log-likelihood over 200 pairs, forward + adjoint), fb1d (the
                 1D alignment-constrained forward/backward + counts over the
                 c1 pairs' generating alignments)
@@ -62,11 +63,43 @@ def _log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, seconds=15.0, threads=None):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _host_cores():
+    """Cores this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS when that is set (the share a GPU box gives a job)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cpu_baseline(w, seconds=15.0, single_seconds=8.0):
+    """The C oracle on bounded samples of the workload's pairs: once on all
+    the host cores this process may use, once on one core."""
+    cores = _host_cores()
+    allc = _cpu_sample(w, seconds, cores)
+    one = _cpu_sample(w, single_seconds, 1)
+    allc.update(cpu_model=_cpu_model(), nproc=os.cpu_count(),
+                single_core={"value": one["value"], "unit": one["unit"], "cores": 1,
+                             "sample": one["sample"]})
+    return allc
+
+
+def _cpu_sample(w, seconds, threads):
     """Time the C oracle on a bounded sample of the workload's pairs."""
     import numpy as np
     from oracle import cfb2d
-    threads = threads or min(16, os.cpu_count() or 1)
     rng = np.random.default_rng(123)
     order = rng.permutation(w.B)
     lt = w.log_trans.numpy()
@@ -116,12 +149,14 @@ def plain_per_pair(w, n_pairs=48, emit=True):
             "ms_per_pair": 1e3 * el / len(idx)}
 
 
-def serve_bench(loads, n_requests):
+def serve_bench(loads, n_requests, max_seconds=10.0):
     """Serving: a Poisson stream of ``n_requests`` scoring / E-step requests,
     50/50 TKF92 (5 states) / MixDom2 K=3 (17 states), lengths
     round(LogNormal(ln 250, 0.5)) clipped to [20, 2000], at each offered
     load (requests/s), served in real time by the continuous-batching loop
-    after a warm-up stream.  Reports p50 / p99 latency per load."""
+    after a warm-up stream.  At low loads the stream is cut to about
+    ``max_seconds`` of arrivals (e.g. 1000 requests at 100 req/s).  Reports
+    p50 / p99 latency per load."""
     import numpy as np
     from tkfmi import serve, synth
     rng = np.random.default_rng(4)
@@ -132,16 +167,19 @@ def serve_bench(loads, n_requests):
     S_of = {"tkf92": 5, "mixdom2": 17}
     out = []
     for load in loads:
-        reqs = serve.poisson_requests(4, n_requests, rate=load)
+        n = int(min(n_requests, max(200, load * max_seconds)))
+        reqs = serve.poisson_requests(4, n, rate=load)
         t0 = time.perf_counter()
         res = eng.serve(reqs, realtime=True)
         wall = time.perf_counter() - t0
         s = serve.summarize(reqs, res, wall, S_of)
         s["offered_req_per_s"] = load
+        s["requests"] = n
         _log(f"serving at {load} req/s: p50 {s['p50_ms']:.1f} ms p99 {s['p99_ms']:.1f} ms, "
              f"{s['achieved_req_per_s']:.0f} req/s, {s['cell_states_per_s']:.3e} cell-states/s")
         out.append(s)
-    return {"workload": f"{n_requests} Poisson requests, 50/50 TKF92 (S=5) / MixDom2 K=3 (S=17), "
+    return {"workload": f"up to {n_requests} Poisson requests per load (about {max_seconds:.0f} s of "
+                        "arrivals at low loads), 50/50 TKF92 (S=5) / MixDom2 K=3 (S=17), "
                         "50/50 scoring / E-step, lengths LogNormal(ln250,0.5) in [20,2000]",
             "loads": out}
 
@@ -227,6 +265,52 @@ def c2_bench(pairs):
     return out
 
 
+def mid_bench(steps, K=3, F=2):
+    """The c1 workload's shape (1024 pairs, LogNormal(ln200,0.5) lengths in
+    [30,800]) on a mid-size MixDom2 layout, K=3 F=2 (32 states, the size of
+    the reference's trained d3f2 models): fwd+bwd+transition and emission
+    counts in one call."""
+    import torch
+    from tkfmi import _abi, synth
+    w = synth.make_mixdom2(seed=1, B=1024, K=K, F=F, C=1, name=f"c1_k{K}f{F}")
+    dev = torch.device("cuda", 0)
+    r = _abi.BatchRunner(w.log_trans.to(dev), w.state_types, w.state_group, w.log_tab.to(dev),
+                         _abi.PackedPairs(w.xs, w.ys, dev), want_emit_counts=True)
+    ms = _time_calls(r.run, steps)
+    r.flags |= _abi.TIME_KERNELS
+    kern = _kernel_ms(r.run)
+    cs = w.cell_states()
+    out = {"value": cs / (ms / 1e3), "unit": "cell-states/s", "ms": ms, "pairs": w.B,
+           "states": w.S, "cell_states": cs, "kernels_ms": kern}
+    del r
+    torch.cuda.empty_cache()
+    return out
+
+
+def critical_path(w, emit):
+    """The longest pair of the batch alone, forward + backward launch times
+    (HIP events): a call cannot end before its longest pair's chain of
+    antidiagonal steps does, whatever the batch around it."""
+    import numpy as np
+    import torch
+    from tkfmi import _abi
+    dev = torch.device("cuda", 0)
+    b = int(np.argmax([(len(x) + 1) * (len(y) + 1) for x, y in zip(w.xs, w.ys)]))
+    r = _abi.BatchRunner(w.log_trans[b].to(dev), w.state_types, w.state_group, w.log_tab[b].to(dev),
+                         _abi.PackedPairs([w.xs[b]], [w.ys[b]], dev), want_emit_counts=emit,
+                         time_kernels=True)
+    for _ in range(3):
+        r.run()
+    ks = []
+    for _ in range(5):
+        r.run()
+        ks.append(sum(k["ms"] for k in _abi.last_kernel_times()))
+    torch.cuda.synchronize()
+    Lx, Ly = len(w.xs[b]), len(w.ys[b])
+    return {"pair": [Lx, Ly], "ms": float(np.median(ks)),
+            "antidiagonal_steps": Lx + Ly + 1}
+
+
 def c3_bench(steps):
     """c3: gradient of the summed MixDom2 K=3 F=1 C=2 log-likelihood of 200
     pairs (LogNormal median 200) w.r.t. every model parameter and t:
@@ -267,15 +351,15 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--serve-loads", default="1000,3000,10000",
+    ap.add_argument("--serve-loads", default="100,300,1000,3000,10000",
                     help="offered loads (requests/s) for the serving measurement; empty to skip")
     ap.add_argument("--serve-requests", type=int, default=5000)
     ap.add_argument("--no-emit", action="store_true",
                     help="transition counts only (no expected emission counts)")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the c0 / c2 / c3 / fb1d measurements")
-    ap.add_argument("--configs", default="fb1d,c0,c3,c2",
-                    help="which of the fb1d / c0 / c3 / c2 measurements to run (comma list)")
+    ap.add_argument("--configs", default="fb1d,c0,c3,mid,c2",
+                    help="which of the fb1d / c0 / c3 / mid / c2 measurements to run (comma list)")
     ap.add_argument("--c2-pairs", type=int, default=256,
                     help="pairs of the c2 (162-state) measurement; 0 skips it")
     ap.add_argument("--kernel", default="chain", choices=["chain", "class", "generic"],
@@ -288,8 +372,9 @@ def main():
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE against known traffic")
     args = ap.parse_args()
     sel = set() if args.no_configs else {c.strip() for c in args.configs.split(",") if c.strip()}
-    if sel - {"fb1d", "c0", "c3", "c2"}:
-        raise SystemExit(f"unknown --configs entries: {sorted(sel - {'fb1d', 'c0', 'c3', 'c2'})}")
+    known = {"fb1d", "c0", "c3", "mid", "c2"}
+    if sel - known:
+        raise SystemExit(f"unknown --configs entries: {sorted(sel - known)}")
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: there is no multi-GPU path "
               "(pairs would be dealt out to GPUs by their DP cost, one process per GPU)",
@@ -322,15 +407,15 @@ def main():
                                 want_emit_counts=not args.no_emit, kernel=args.kernel)
 
     tuned = None
+    runner = make_runner()
     if args.kernel == "chain" and not args.no_tune:
-        # untimed set-up: the chained kernels' workgroup widths per length
-        # class, chosen on this batch
+        # untimed set-up: libtkfmi's autotuner picks the chained kernels'
+        # workgroup widths per length class for this batch's key
         from tkfmi import autotune
         _log("autotuning the chained kernels on this batch")
-        tuned = autotune.tune_chain(make_runner)
-        _log(f"autotune: best {tuned['best']}: " +
+        tuned = autotune.tune(runner)
+        _log(f"autotune: key {tuned['key']} best {tuned['best']}: " +
              ", ".join(f"{k}: {v:.2f} ms" for k, v in tuned["ms"].items()))
-    runner = make_runner()
     cell_states = w.cell_states()
     _log(f"pairs={w.B} S={w.S} cell-states={cell_states:.4e} workspace={runner.ws_bytes/2**30:.2f} GiB")
     for _ in range(args.warmup):
@@ -376,6 +461,7 @@ def main():
                         "achieved_GBs": bpc * cs / (ms / 1e3) / 1e9})
     kernels.sort(key=lambda k: -k["ms"])
     dom = kernels[0]
+    crit = critical_path(w, not args.no_emit) if args.kernel == "chain" else None
     _log(f"step {step_ms:.3f} ms; kernels: " +
          ", ".join(f"{k['name']} {k['ms']:.3f} ms ({k['pairs']} pairs)" for k in kernels))
 
@@ -401,6 +487,14 @@ def main():
                      "kernel": dom["name"], "kernel_ms": dom["ms"],
                      "bytes_per_cell_state": dom["bytes_per_cell_state"], "kernels": kernels},
     }
+    if crit is not None:
+        # the step's floor: its longest pair alone (forward + backward)
+        out["roofline"]["critical_path"] = {
+            "longest_pair": crit["pair"], "antidiagonal_steps": crit["antidiagonal_steps"],
+            "ms": crit["ms"], "us_per_step": 1e3 * crit["ms"] / crit["antidiagonal_steps"],
+            "frac_of_step": crit["ms"] / step_ms}
+        _log(f"critical path: longest pair {crit['pair']} alone {crit['ms']:.3f} ms "
+             f"({crit['ms'] / step_ms:.2f} of the step)")
     if not args.no_plain:
         _log("timing the per-pair drop-in")
         pl = plain_per_pair(w, emit=not args.no_emit)
@@ -419,6 +513,10 @@ def main():
             _log("c3 gradients, 200 pairs")
             cfg["c3"] = c3_bench(max(1, args.steps // 2))
             _log(f"c3: {cfg['c3']['ms']:.1f} ms per gradient")
+        if "mid" in sel:
+            _log("c1-shaped batch at K=3 F=2 (32 states)")
+            cfg["mid_k3f2"] = mid_bench(max(1, args.steps // 2))
+            _log(f"mid K=3 F=2: {cfg['mid_k3f2']['ms']:.1f} ms, {cfg['mid_k3f2']['value']:.3e} cell-states/s")
         if "c2" in sel and args.c2_pairs > 0:
             _log(f"c2 162-state wide kernel, {args.c2_pairs} pairs")
             cfg["c2"] = c2_bench(args.c2_pairs)
@@ -431,6 +529,9 @@ def main():
     if not args.no_cpu:
         _log("timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(w, seconds=args.cpu_seconds)
+        _log(f"cpu baseline: {out['cpu_baseline']['value']:.3e} cell-states/s on "
+             f"{out['cpu_baseline']['cores']} cores, {out['cpu_baseline']['single_core']['value']:.3e} on one "
+             f"({out['cpu_baseline']['cpu_model']})")
     print(json.dumps(out), flush=True)
 
 

@@ -519,9 +519,10 @@ size_t tkfmi_autotune_workspace_bytes(const tkfmi_problem* prob, uint32_t flags)
  * Autotuner of the chained 2D kernels; no counterpart in the reference
  * (its 2D path is one vmapped JAX call per padded shape).  Times the
  * candidate class configurations (waves per workgroup of the long and main
- * classes, the long-class cell fraction) on the caller's own batch -- one
- * warm-up and `reps` timed calls each, HIP events on `stream`, synchronous
- * -- and keeps the fastest (the configuration already in force unless
+ * classes, the long-class cell fraction) on the caller's own batch -- an
+ * untimed warm-up, then two round-robin passes of `reps` timed calls per
+ * candidate (each keeps its faster pass), HIP events on `stream`,
+ * synchronous -- and keeps the fastest (the configuration already in force unless
  * another is more than 3 % faster) for the batch's key: its state layout
  * and the length bin of its longest sequence.  Later calls whose batch has
  * that key use it (query their workspace size after tuning: it depends on

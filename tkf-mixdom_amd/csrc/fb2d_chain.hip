@@ -161,15 +161,17 @@ __global__ void raise_status_kernel(const int32_t* __restrict__ abortf, int32_t*
 // The launch's next ticket, the same for the whole workgroup.  A ticket past
 // the grid means the counter was not zeroed before the launch (a caller
 // error, e.g. a workspace shared by overlapping calls): the workgroup raises
-// the abort flag and returns -1 without touching a descriptor.
+// the abort flag and its waves end here (every wave sees the same ticket),
+// without touching a descriptor.
 __device__ __forceinline__ int take_ticket(int32_t* counter, int32_t* abortf) {
   __shared__ int s_ticket;
   if (threadIdx.x == 0) s_ticket = atomicAdd(counter, 1);
   __syncthreads();
-  int t = __builtin_amdgcn_readfirstlane(s_ticket);
+  const int t = __builtin_amdgcn_readfirstlane(s_ticket);
   if (t < 0 || t >= (int)gridDim.x) {
     if (threadIdx.x == 0) __hip_atomic_store(abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return -1;
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_endpgm();
   }
   return t;
 }
@@ -232,9 +234,7 @@ __global__ __launch_bounds__(W * 64) void fbc_fwd_kernel(CParams Pr, const doubl
 
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  const int ticket = take_ticket(Pr.ticket, Pr.abortf);
-  if (ticket < 0) return;
-  const CDesc pd = descs[ticket];
+  const CDesc pd = descs[take_ticket(Pr.ticket, Pr.abortf)];
   const int Lx = pd.Lx, Ly = pd.Ly, NW = pd.nw;
   const int S = Pr.S;
   const int BSTRIDE = Ly + 64;
@@ -483,9 +483,7 @@ __global__ __launch_bounds__((W + 1) * 64, 2) void fbc_bwd_kernel(CParams Pr, co
 
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  const int ticket = take_ticket(Pr.ticket, Pr.abortf);
-  if (ticket < 0) return;
-  const CDesc pd = descs[ticket];
+  const CDesc pd = descs[take_ticket(Pr.ticket, Pr.abortf)];
   const int Lx = pd.Lx, Ly = pd.Ly, NW = pd.nw;
   const int S = Pr.S;
   const int W1 = Ly + 1;
@@ -1246,10 +1244,24 @@ extern "C" int tkfmi_autotune(const tkfmi_problem* p, const tkfmi_outputs* o, vo
     return tkfmi_fail(TKFMI_EHIP, "tkfmi_autotune: hipEventCreate failed");
   int rc = TKFMI_OK;
   std::string err;
-  for (size_t c = 0; c < cand.size() && c < 16 && rc == TKFMI_OK; ++c) {
-    double ms = -1.0;
-    const ChainPlan cp = plan_chain(p, pl, flags, &cand[c]);
-    if (cp.ok && ws && ws_bytes >= cp.total()) {
+  const int nc = (int)std::min<size_t>(cand.size(), 16);
+  for (int c = 0; c < nc; ++c) {
+    res->long_w[c] = cand[c].w[0];
+    res->main_w[c] = cand[c].w[1];
+    res->long_frac[c] = cand[c].long_frac;
+    res->ms[c] = -1.0;  // -1: not run (its workspace exceeds ws_bytes)
+  }
+  res->n = nc;
+  // untimed warm-up on the first candidate (clocks, caches, first-use
+  // set-up), then two round-robin passes; each candidate keeps its faster
+  // pass, so no candidate is timed only on a cold GPU
+  for (int r = 0; r < 2 * reps && rc == TKFMI_OK; ++r)
+    if (ws && ws_bytes >= plan_chain(p, pl, flags, &cand[0]).total())
+      rc = chain_run_impl(p, o, ws, ws_bytes, stream, flags, err, &cand[0]);
+  for (int pass = 0; pass < 2 && rc == TKFMI_OK; ++pass) {
+    for (int c = 0; c < nc && rc == TKFMI_OK; ++c) {
+      const ChainPlan cp = plan_chain(p, pl, flags, &cand[c]);
+      if (!cp.ok || !ws || ws_bytes < cp.total()) continue;
       rc = chain_run_impl(p, o, ws, ws_bytes, stream, flags, err, &cand[c]);  // warm-up
       if (rc == TKFMI_OK && hipEventRecord(e0, stream) != hipSuccess) rc = TKFMI_EHIP;
       for (int r = 0; r < reps && rc == TKFMI_OK; ++r)
@@ -1257,13 +1269,11 @@ extern "C" int tkfmi_autotune(const tkfmi_problem* p, const tkfmi_outputs* o, vo
       if (rc == TKFMI_OK && (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
         rc = TKFMI_EHIP;
       float t = 0.f;
-      if (rc == TKFMI_OK && hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms = (double)t / reps;
+      if (rc == TKFMI_OK && hipEventElapsedTime(&t, e0, e1) == hipSuccess) {
+        const double ms = (double)t / reps;
+        if (res->ms[c] < 0.0 || ms < res->ms[c]) res->ms[c] = ms;
+      }
     }
-    res->long_w[c] = cand[c].w[0];
-    res->main_w[c] = cand[c].w[1];
-    res->long_frac[c] = cand[c].long_frac;
-    res->ms[c] = ms;  // -1: not run (its workspace exceeds ws_bytes)
-    res->n = (int32_t)c + 1;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
