@@ -46,7 +46,7 @@ extern "C" {
 #define TKFMI_WANT_EMIT_COUNTS 16u /* expected emission counts (see below)  */
 #define TKFMI_TIME_KERNELS 32u   /* record per-kernel HIP-event timings    */
 #define TKFMI_CLASS_KERNEL 64u   /* force the one-workgroup-per-pair
-                                    cell-per-lane kernel (see below)        */
+                                    kernels (see below)                      */
 #define TKFMI_REUSE_DESCRIPTORS 128u /* the workspace already holds this
                                     problem's descriptors from an earlier
                                     call with the same problem, flags and
@@ -178,8 +178,12 @@ size_t tkfmi_fb2d_workspace_bytes(const tkfmi_problem* prob, uint32_t flags);
  *     dependency order), or with TKFMI_CLASS_KERNEL one workgroup per pair;
  *   - other layouts with at most 15 emitting states, dense emissions, or
  *     TKFMI_GENERIC_KERNEL: the generic kernel;
- *   - 16..160 emitting states: the wide kernel (MFMA transition and count
- *     contractions, optional TKFMI_CHECKPOINT).
+ *   - 16..160 emitting states: MFMA transition and count contractions, by
+ *     default as chains of workgroups per pair (a pair's 16-row bands on
+ *     separate compute units, ticketed in dependency order; full lattice),
+ *     or one workgroup per pair sweeping antidiagonals, which is what runs
+ *     with TKFMI_CLASS_KERNEL and whenever a pair is checkpointed
+ *     (TKFMI_CHECKPOINT, or a lattice above tkfmi_checkpoint_budget).
  * If a chained launch ever times out on an internal wait, the affected
  * pairs' log_prob is NaN and outputs.status (if given) is set.
  * Thread safety: calls from several host threads are safe as long as each

@@ -95,10 +95,10 @@ def test_checkpoint_shrinks_the_wide_workspace():
     st = list(mixdom.state_types(8, 4))
     p, k = _problem(st, [0] * len(st), [500, 800], [600, 700], G=1)
     _abi.set_checkpoint_budget(0)  # no automatic checkpointing
-    full = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), 0)
+    full = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), _abi.CLASS_KERNEL)
     _abi.set_checkpoint_budget(_abi.CHECKPOINT_BUDGET_DEFAULT)
     ck = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), _abi.CHECKPOINT)
-    fo = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), _abi.FORWARD_ONLY)
+    fo = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), _abi.FORWARD_ONLY | _abi.CLASS_KERNEL)
     lat_full = sum((lx + 1) * (ly + 1) * 161 * 8 for lx, ly in ((500, 600), (800, 700)))
     assert abs((full - fo) - lat_full) < 4096
     for lx, ly in ((500, 600), (800, 700)):
@@ -109,6 +109,28 @@ def test_checkpoint_shrinks_the_wide_workspace():
     p17, k17 = _problem(st17, [0] * 17, [500, 800], [600, 700], G=1)
     assert L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p17), _abi.CHECKPOINT) == \
         L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p17), 0)
+
+
+def test_banded_workspace_holds_lattice_boundaries_and_accumulators():
+    """The banded 16..160-state kernel (the default when no pair is
+    checkpointed): full minus forward-only = every pair's lattice
+    ((Lx+1)(Ly+1)(NBP+1) doubles) plus one NBP x NBP count accumulator per
+    pair of more than one 16-row band; forward-only still holds the band
+    boundaries ((ceil((Lx+1)/16) - 1)(Ly+1)(NBP+1) doubles per pair)."""
+    from tkfmi import _abi, mixdom
+    L = _abi.lib()
+    st = list(mixdom.state_types(8, 4))  # 160 emitting states: NBP = 160
+    xl, yl = [15, 16, 200, 0], [30, 40, 150, 9]
+    p, k = _problem(st, [0] * len(st), xl, yl, G=1)
+    full = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), 0)
+    fo = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), _abi.FORWARD_ONLY)
+    lat = sum((x + 1) * (y + 1) * 161 * 8 for x, y in zip(xl, yl))
+    acc = sum(160 * 160 * 8 for x in xl if x + 1 > 16)
+    assert abs((full - fo) - lat - acc) < 4096
+    bnd = sum(((x + 16) // 16 - 1) * (y + 1) * 161 * 8 for x, y in zip(xl, yl))
+    assert fo > bnd and fo - bnd < 1 << 20
+    # one workgroup per pair: the antidiagonal kernel's own layout
+    assert L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), _abi.CLASS_KERNEL) != full
 
 
 def test_checkpoint_budget_applies_per_pair():

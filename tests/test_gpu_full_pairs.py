@@ -10,9 +10,11 @@ its kind, on every kernel that accepts the layout:
   c2_300, c2_1000 (162 states, up to 1000 x 1055), best_d3f2 (32 states,
              the reference's trained bw_d3f2_fullseed set, 700 x 695),
              best_d8f2 (82 states, bw_d8f2_fullseed, 620 x 632): the
-             16..160-state kernel, with and without antidiagonal
-             checkpointing.  Rows this long also take the kernel's 64-cell
-             residue-code reload path (antidiagonals longer than 512 cells).
+             16..160-state kernels -- banded (chained 16-row bands, full
+             lattice), one workgroup per pair sweeping antidiagonals, and
+             the latter with antidiagonal checkpointing.  Rows this long
+             also take the antidiagonal kernel's 64-cell residue-code reload
+             path (antidiagonals longer than 512 cells).
 """
 import os
 
@@ -41,9 +43,13 @@ def _check(name, kernel=None, checkpoint=False):
     from tkfmi import _abi, hmm
     p = _load(name)
     G = int(p["log_tab"].shape[0])
-    res = hmm.fb2d_batch(torch.as_tensor(p["log_trans"]), p["state_types"], [p["x"]], [p["y"]],
-                         torch.as_tensor(p["log_tab"]), p["state_group"], want_posteriors=True,
-                         want_emit_counts=True, kernel=kernel, checkpoint=checkpoint)
+    try:
+        _abi.set_checkpoint_budget(0)  # full lattices unless checkpoint=True
+        res = hmm.fb2d_batch(torch.as_tensor(p["log_trans"]), p["state_types"], [p["x"]], [p["y"]],
+                             torch.as_tensor(p["log_tab"]), p["state_group"], want_posteriors=True,
+                             want_emit_counts=True, kernel=kernel, checkpoint=checkpoint)
+    finally:
+        _abi.set_checkpoint_budget(_abi.CHECKPOINT_BUDGET_DEFAULT)
     _abi.check_status(res)
     lp = float(res["log_p"][0])
     print(f"{name} kernel={kernel} ckpt={checkpoint}: logP {lp!r} vs {float(p['logp'])!r}", flush=True)
@@ -66,7 +72,7 @@ def test_c1_max_pairs_against_long_double(name, kernel):
     _check(name, kernel=kernel)
 
 
-@pytest.mark.parametrize("checkpoint", [False, True])
+@pytest.mark.parametrize("mode", ["band", "antidiagonal", "checkpoint"])
 @pytest.mark.parametrize("name", ["c2_300", "c2_1000", "best_d3f2", "best_d8f2"])
-def test_wide_pairs_against_long_double(name, checkpoint):
-    _check(name, checkpoint=checkpoint)
+def test_wide_pairs_against_long_double(name, mode):
+    _check(name, kernel="class" if mode == "antidiagonal" else None, checkpoint=mode == "checkpoint")

@@ -1,7 +1,9 @@
-"""Wide state spaces (16..160 emitting states, the MFMA antidiagonal kernel):
-MixDom2 layouts up to K=8 F=4 C=4 (S = 162) against the FP64 oracle
-(log-likelihood, transition counts, posteriors, emission counts: 1e-10
-relative), and the exact count identities at lengths up to 1000."""
+"""Wide state spaces (16..160 emitting states; the banded MFMA kernel, the
+default, and the one-workgroup-per-pair antidiagonal kernel, kernel="class",
+which also runs every checkpointed call): MixDom2 layouts up to K=8 F=4 C=4
+(S = 162) against the FP64 oracle (log-likelihood, transition counts,
+posteriors, emission counts: 1e-10 relative), and the exact count
+identities at lengths up to 1000."""
 import numpy as np
 import pytest
 
@@ -24,8 +26,9 @@ def _oracle(w, b, x, y):
     return dp2d.forward_backward(w.log_trans[b].numpy(), np.asarray(w.state_types), em)
 
 
+@pytest.mark.parametrize("kernel", [None, "class"])
 @pytest.mark.parametrize("K,F,C", [(2, 2, 2), (4, 2, 1), (5, 4, 2), (8, 2, 2), (8, 4, 4)])
-def test_wide_matches_oracle(K, F, C):
+def test_wide_matches_oracle(K, F, C, kernel):
     from oracle import emissions
     from tkfmi import hmm, synth
     lengths = [0, 1, 5, 17, 30, 40]
@@ -34,10 +37,10 @@ def test_wide_matches_oracle(K, F, C):
     w.ys[0] = w.ys[0][:0]
     w.ys[2] = rng.integers(0, 20, 33)
     res = hmm.fb2d_batch(w.log_trans, w.state_types, w.xs, w.ys, w.log_tab, w.state_group,
-                         want_posteriors=True, want_emit_counts=True)
+                         want_posteriors=True, want_emit_counts=True, kernel=kernel)
     G = res["emit_counts"].shape[1]
     for b, (x, y) in enumerate(zip(w.xs, w.ys)):
-        print(f"K{K} F{F} C{C} pair {b} Lx={len(x)} Ly={len(y)}", flush=True)
+        print(f"K{K} F{F} C{C} kernel={kernel} pair {b} Lx={len(x)} Ly={len(y)}", flush=True)
         lp_o, post_o, n_o, _ = _oracle(w, b, x, y)
         assert abs(float(res["log_p"][b]) - lp_o) <= RTOL * abs(lp_o), (b, float(res["log_p"][b]), lp_o)
         ok, worst = _close(res["n"][b].cpu().numpy(), n_o)
@@ -86,14 +89,14 @@ def test_wide_count_identities_long_pairs():
 def test_wide_checkpoint_matches_full_lattice():
     """TKFMI_CHECKPOINT: the backward recomputes each K-antidiagonal segment
     from its checkpoint with the forward's own arithmetic, so log P, counts
-    and posteriors equal the full-lattice run bit for bit (emission counts to
+    and posteriors equal the antidiagonal kernel's full-lattice run bit for bit (emission counts to
     the FP64 atomics' summation order); a small layout also against the
     oracle."""
     from oracle import emissions
     from tkfmi import hmm, synth
     w = synth.make_mixdom2(seed=21, B=5, K=8, F=4, C=4, lengths=[60, 3, 90, 0, 150])
     kw = dict(want_posteriors=True, want_emit_counts=True)
-    full = hmm.fb2d_batch(w.log_trans, w.state_types, w.xs, w.ys, w.log_tab, w.state_group, **kw)
+    full = hmm.fb2d_batch(w.log_trans, w.state_types, w.xs, w.ys, w.log_tab, w.state_group, kernel="class", **kw)
     ck = hmm.fb2d_batch(w.log_trans, w.state_types, w.xs, w.ys, w.log_tab, w.state_group, checkpoint=True, **kw)
     assert np.array_equal(full["log_p"].cpu().numpy(), ck["log_p"].cpu().numpy())
     assert np.array_equal(full["n"].cpu().numpy(), ck["n"].cpu().numpy())
@@ -122,7 +125,7 @@ def test_budget_checkpointing_is_bit_identical():
     from tkfmi import _abi, hmm, synth
     w = synth.make_mixdom2(seed=44, B=4, K=4, F=2, C=2, lengths=[12, 90, 150, 60])
     run = lambda: hmm.fb2d_batch(w.log_trans, w.state_types, w.xs, w.ys, w.log_tab, w.state_group,
-                                 want_posteriors=True, want_emit_counts=True)
+                                 want_posteriors=True, want_emit_counts=True, kernel="class")
     try:
         _abi.set_checkpoint_budget(0)
         full = run()

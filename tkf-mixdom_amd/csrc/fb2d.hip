@@ -741,7 +741,8 @@ size_t tkfmi_fb2d_workspace_bytes(const tkfmi_problem* prob, uint32_t flags) {
     return v2b > 0 ? v2b : 1;
   if (!dense && !(flags & TKFMI_GENERIC_KERNEL) && tkfmi_v2_workspace(prob, flags, &v2b))
     return v2b > 0 ? v2b : 1;
-  if (prob && tkfmi_wide_applies(prob)) return tkfmi_wide_workspace(prob, flags);
+  if (prob && tkfmi_wide_applies(prob))
+    return tkfmi_band_applies(prob, flags) ? tkfmi_band_workspace(prob, flags) : tkfmi_wide_workspace(prob, flags);
   Plan pl;
   if (make_plan(prob, flags, pl) != TKFMI_OK) return 0;
   return plan_ws_bytes(pl);
@@ -764,7 +765,9 @@ int tkfmi_fb2d_batched(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws,
     if (p->B > 0 && (!p->log_trans || (!dense && (!p->log_tab || !p->x_cat || !p->y_cat))))
       return fail(TKFMI_EINVAL, "device input pointer missing");
     std::string err;
-    const int rcw = tkfmi_wide_run(p, o, ws, ws_bytes, reinterpret_cast<hipStream_t>(stream_v), flags, err);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream_v);
+    const int rcw = tkfmi_band_applies(p, flags) ? tkfmi_band_run(p, o, ws, ws_bytes, st, flags, err)
+                                                 : tkfmi_wide_run(p, o, ws, ws_bytes, st, flags, err);
     return rcw == TKFMI_OK ? TKFMI_OK : fail(rcw, err);
   }
   Plan pl;

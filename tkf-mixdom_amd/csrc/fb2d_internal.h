@@ -39,6 +39,20 @@ int tkfmi_wide_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
 // every launch goes on the caller's stream (no library streams or events).
 bool tkfmi_noalloc_active();
 
+// The 16..160-state kernels' per-pair blocks: P [NBP][NBP], PT, p_start,
+// p_end, p(S->E) from the log transitions (pm, n_trans blocks of
+// 2 NBP^2 + 2 NBP + 8 doubles) and the probability-space emission tables
+// (tab, n_tab x G x TKFMI_TAB_SZ; skipped when tab is null).
+int tkfmi_wide_prep(const tkfmi_problem* p, int NB, int NBP, int e_idx, const int32_t* d_bidx, double* pm,
+                    int G, double* tab, hipStream_t stream, std::string& err);
+
+// Banded path for 16..160 emitting states (a pair's rows in bands of 16,
+// one workgroup per band): applies when no pair needs checkpointing.
+bool tkfmi_band_applies(const tkfmi_problem* p, uint32_t flags);
+size_t tkfmi_band_workspace(const tkfmi_problem* p, uint32_t flags);
+int tkfmi_band_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, size_t ws_bytes,
+                   hipStream_t stream, uint32_t flags, std::string& err);
+
 // Lattice bytes of one pair above which the 16..160-state kernel keeps only
 // checkpoints (tkfmi_checkpoint_budget; 0 = only with TKFMI_CHECKPOINT).
 size_t tkfmi_checkpoint_budget_bytes();

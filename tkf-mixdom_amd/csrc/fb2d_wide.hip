@@ -780,6 +780,20 @@ int launch(int NT, const WParams& prm, int B, hipStream_t s, std::string& err) {
 
 using namespace tkfmi_wide;
 
+int tkfmi_wide_prep(const tkfmi_problem* p, int NB, int NBP, int e_idx, const int32_t* d_bidx, double* pm,
+                    int G, double* tab, hipStream_t stream, std::string& err) {
+  hipLaunchKernelGGL(wide_prep_kernel, dim3(p->n_trans), dim3(256), 0, stream, p->log_trans, p->S, NB, NBP,
+                     e_idx, d_bidx, pm);
+  if (tab && p->log_tab) {
+    const int ntab = p->n_tab * G * TAB_SZ;
+    hipLaunchKernelGGL(wide_tab_kernel, dim3(std::min(1024, (ntab + 255) / 256)), dim3(256), 0, stream,
+                       p->log_tab, ntab, tab);
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { err = std::string("wide prep launch: ") + hipGetErrorString(e); return TKFMI_EHIP; }
+  return TKFMI_OK;
+}
+
 bool tkfmi_wide_applies(const tkfmi_problem* p) {
   WPlan pl;
   std::string why;

@@ -323,10 +323,12 @@ def normalise_codes(seq):
 def kernel_flags(kernel=None, generic=False):
     """Flag bits that select the 2D forward/backward kernel.
 
-    kernel: None or "chain" (default: chained workgroups per pair on the
-    cell-per-lane path where the state layout allows, else the generic
-    kernel), "class" (one workgroup per pair, cell-per-lane) or "generic"
-    (any layout).  ``generic=True`` means kernel="generic".
+    kernel: None or "chain" (default: chained workgroups per pair -- the
+    cell-per-lane path where the state layout allows, the banded MFMA kernel
+    for 16..160 emitting states -- else the generic kernel), "class" (one
+    workgroup per pair: cell-per-lane, or the antidiagonal MFMA kernel for
+    16..160 states) or "generic" (any layout with at most 15 emitting
+    states).  ``generic=True`` means kernel="generic".
     """
     if generic or kernel == "generic":
         return GENERIC_KERNEL
