@@ -113,10 +113,11 @@ def test_checkpoint_shrinks_the_wide_workspace():
 
 def test_banded_workspace_holds_lattice_boundaries_and_accumulators():
     """The banded 16..160-state kernel (the default when no pair is
-    checkpointed): full minus forward-only = every pair's lattice
-    ((Lx+1)(Ly+1)(NBP+1) doubles) plus one NBP x NBP count accumulator per
-    pair of more than one 16-row band; forward-only still holds the band
-    boundaries ((ceil((Lx+1)/16) - 1)(Ly+1)(NBP+1) doubles per pair)."""
+    checkpointed): full minus forward-only = every band's lattice (stored
+    by step: Ly + rows steps of 16 rows, NBP + 1 doubles per cell) plus one
+    NBP x NBP count accumulator per pair of more than one 16-row band;
+    forward-only still holds the band boundaries ((ceil((Lx+1)/16) - 1)
+    (Ly+1)(NBP+1) doubles per pair)."""
     from tkfmi import _abi, mixdom
     L = _abi.lib()
     st = list(mixdom.state_types(8, 4))  # 160 emitting states: NBP = 160
@@ -124,7 +125,7 @@ def test_banded_workspace_holds_lattice_boundaries_and_accumulators():
     p, k = _problem(st, [0] * len(st), xl, yl, G=1)
     full = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), 0)
     fo = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), _abi.FORWARD_ONLY)
-    lat = sum((x + 1) * (y + 1) * 161 * 8 for x, y in zip(xl, yl))
+    lat = sum((((x + 16) // 16) * y + x + 1) * 16 * 161 * 8 for x, y in zip(xl, yl))
     acc = sum(160 * 160 * 8 for x in xl if x + 1 > 16)
     assert abs((full - fo) - lat - acc) < 4096
     bnd = sum(((x + 16) // 16 - 1) * (y + 1) * 161 * 8 for x, y in zip(xl, yl))

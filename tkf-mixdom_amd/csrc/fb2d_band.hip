@@ -6,32 +6,38 @@
 // :1536-1692, counts :1930-1971).
 //
 // Design:
-//  * A pair's lattice rows are cut into bands of R = 16 rows; each band is
-//    one workgroup of 8 waves, so a long pair spreads over many compute units
-//    instead of running on one.  Band g needs the last row of band g - 1
-//    (forward) or the first row of band g + 1 (backward): it receives those
-//    rows through a boundary buffer in global memory, written with
-//    device-scope stores and published every CQ columns with a progress
-//    counter (bounded waits; workgroups take tickets in dependency order, so a
-//    wait is only ever on a workgroup that is already running).
+//  * A pair's lattice rows are cut into bands of R rows (R = 16 for 96..160
+//    padded states, 32 for 64, 64 for 32); each band is one workgroup of 8
+//    waves, so a long pair spreads over many compute units.  Band g needs
+//    the last row of band g - 1 (forward) or the first row of band g + 1
+//    (backward): it receives it through a boundary buffer in global memory,
+//    written with device-scope stores and published every CQ columns with a
+//    progress counter (bounded waits).  Workgroups take their band from one
+//    of eight ticket queues, the one of the XCD they run on; a pair's bands
+//    sit in one queue, in dependency order, so a wait is only ever on a
+//    workgroup that is already running, and a pair's transition block and
+//    tables stay in that XCD's L2.
 //  * Inside a band, step t handles cell (row r, column t - r) of every row:
-//    the 16 cells of a step are independent.  Per step:
+//    the R cells of a step are independent.  Per step:
 //      A (elementwise, one wave per cell, states over the lanes): the cell's
 //        state vector from the values its predecessors pushed to it, times
-//        the emissions, normalised to one binary exponent per cell; written to
-//        the HBM lattice (forward) and to LDS;
-//      B (matrix cores): the transition contraction of all 16 cells as one
-//        (16 x NBP) . (NBP x NBP) product with v_mfma_f64_16x16x4f64, P read
-//        from L2; each output value is pushed straight to the one successor
-//        that uses it (M-type states to (r+1, j+1), I-type to (r, j+1), D-type
-//        to (r+1, j)), into a two-slot LDS ring -- or, for the band's last
-//        row, to the boundary buffer;
+//        the emissions (loaded one step ahead), normalised to one binary
+//        exponent per cell with a single wave reduction; written to the HBM
+//        lattice (forward) and to LDS;
+//      B (matrix cores): the transition contraction of the step's cells as
+//        (R x NBP) . (NBP x NBP) with v_mfma_f64_16x16x4f64.  The transition
+//        block is held in registers for the whole sweep: each wave owns
+//        column tiles over a slice of the contraction (KS slices), and each
+//        output value is pushed straight to the one successor that uses it
+//        (M-type states to (r+1, j+1), I-type to (r, j+1), D-type to
+//        (r+1, j)) as KS partial sums in a two-slot LDS ring, summed in a
+//        fixed order by the successor -- or, for the band's last row, to the
+//        boundary buffer;
 //      backward: B = g P^T the same way, the expected counts sum_c F_c g_c^T
 //        as a second product (K = the step's cells) into 16 x 16 register
 //        tiles held for the whole sweep, then a third elementwise phase
 //        (posteriors, emission counts, q = e . B pushed to the predecessors).
-//    Only the lattice and the boundary rows go to HBM; the per-step buffers
-//    stay in LDS.
+//    Only the lattice and the boundary rows go to HBM.
 //  * Probability space with one binary exponent per cell (as the other
 //    kernels): no exp/log in the inner loops.
 #include <hip/hip_runtime.h>
@@ -55,12 +61,28 @@ constexpr int TAB_SZ = TKFMI_TAB_SZ;
 constexpr int EXP_NONE = -(1 << 29);
 constexpr int kWaves = 8;
 constexpr int kThr = kWaves * 64;
-constexpr int R = 16;                  // rows per band
 constexpr int CQ = 16;                 // boundary columns per published chunk
+constexpr int kYR = 256;               // LDS ring of descendant codes (chunks of 128)
+constexpr int kQueues = 8;             // ticket queues, one per XCD
 constexpr int kSpinMax = 1 << 22;      // bounded waits (each spin sleeps ~0.2 us + a load)
 constexpr double LN2 = 0.69314718055994530942;
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+
+// Geometry per padded width NBP = 16 NT: rows per band R = 16 RT, the
+// transition contraction's K (NBP / 4 chunks of 4) split into KS slices, so
+// that NT x KS blocks (column tile, slice) spread evenly over the 4 SIMDs.
+__host__ __device__ constexpr int band_rt(int NT) { return NT == 2 ? 4 : (NT == 4 ? 2 : 1); }
+__host__ __device__ constexpr int band_ks(int NT) { return NT == 8 ? 1 : 2; }
+template <int NT>
+struct Geo {
+  static constexpr int NBP = 16 * NT, NBPP = NBP + 1, KR = (NBP + 63) / 64;
+  static constexpr int RT = band_rt(NT), R = 16 * RT, CPW = R / kWaves;
+  static constexpr int KS = band_ks(NT), KCH = NBP / 4, KB = KCH / KS;
+  static constexpr int NBLK = NT * KS, BPW = (NBLK + kWaves - 1) / kWaves;
+  static_assert(kWaves % KS == 0, "a wave's blocks share one slice (ks = w mod KS)");
+  static constexpr int NTILE = (NT * NT + kWaves - 1) / kWaves;  // count tiles per wave
+};
 
 struct BDesc {
   int32_t pair, Lx, Ly, i0;    // pair geometry; first row of the band
@@ -68,7 +90,8 @@ struct BDesc {
   int32_t fin, fout;           // progress counters in / out (ints into the flag region; -1: none)
   int32_t ain, aout;           // backward: count-accumulator flags in (from band g+1) / out (to g-1)
   int64_t bin_off, bout_off;   // doubles into ws: boundary rows in / out ((Ly+1) x (NBP+1); -1: none)
-  int64_t lat_off;             // doubles into ws: the pair's lattice [(Lx+1)(Ly+1)][NBP], then exponents
+  int64_t lat_off;             // doubles into ws: the band's lattice [T][R][NBP] by step and row, then
+                               // its exponents [T][R] (a step's cells are contiguous)
   int64_t pm_off, tab_off;     // doubles into ws: transition block, probability-space tables
   int64_t x_off, y_off, post_off;
   int64_t acc_off;             // doubles into ws: the pair's count accumulator [NBP][NBP]
@@ -80,13 +103,14 @@ struct BParams {
   const int32_t* bidx;   // [NBP] state index of emitting slot k (padding: 0)
   const int32_t* btype;  // [NBP] type (0 for padding)
   const int32_t* bgrp;   // [NBP] emission group
+  const int32_t* qoff;   // [kQueues + 1] first descriptor of each ticket queue
   const uint8_t* x_cat;
   const uint8_t* y_cat;
   const double* log_emit;  // dense mode: pair b's [i][j][state] log table at desc.post_off
   const BDesc* desc;
   double* ws;
   int32_t* flagv;   // boundary progress counters
-  int32_t* ticket;  // this launch's ticket counter
+  int32_t* ticket;  // this launch's kQueues ticket counters
   int32_t* abortf;  // raised by a timed-out wait
   double* pinfo;    // [B][2]: mantissa and binary exponent of P(x, y)
   double* out_logP;
@@ -97,25 +121,46 @@ struct BParams {
 
 __host__ __device__ inline int64_t pm_stride(int NBP) { return 2 * (int64_t)NBP * NBP + 2 * NBP + 8; }
 
-__device__ inline double wave_max(double v) {
-  for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-  return v;
-}
 __device__ inline double wave_sum(double v) {
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
   return v;
 }
 __device__ inline int frexp_exp(double v) { return __builtin_amdgcn_frexp_exp(v); }
 
-__device__ inline double emit(const double* tab, int ty, int g, int xi, int yj, int i, int j) {
-  const double* t = tab + (size_t)g * TAB_SZ;
-  if (ty == ST_M) return (i >= 1 && j >= 1) ? t[xi * A_PAD + yj] : 0.0;
-  if (ty == ST_I) return (j >= 1) ? t[A_PAD * A_PAD + yj] : 0.0;
-  if (ty == ST_D) return (i >= 1) ? t[A_PAD * A_PAD + A_PAD + xi] : 0.0;
-  return 0.0;
+// Max over the wave of an int, in DPP steps (row shifts within each row of
+// 16 lanes, then the row broadcasts of lanes 15 and 31), read from lane 63.
+// Lanes whose DPP source is outside the row keep their own value, which max
+// leaves unchanged.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp_max_step(int v) {
+  return max(v, __builtin_amdgcn_update_dpp(v, v, CTRL, ROW_MASK, 0xf, false));
 }
+__device__ __forceinline__ int wave_max_i32(int v) {
+  v = dpp_max_step<0x111, 0xf>(v);  // row_shr:1
+  v = dpp_max_step<0x112, 0xf>(v);  // row_shr:2
+  v = dpp_max_step<0x114, 0xf>(v);  // row_shr:4
+  v = dpp_max_step<0x118, 0xf>(v);  // row_shr:8
+  v = dpp_max_step<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v = dpp_max_step<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  return __builtin_amdgcn_readlane(v, 63);
+}
+// Binary exponent of the largest positive value over the wave (frexp
+// convention), or EXP_NONE if no lane holds one.
+__device__ __forceinline__ int wave_max_exp(double m) {
+  return wave_max_i32(m > 0.0 ? frexp_exp(m) : EXP_NONE);
+}
+
 __device__ inline int emit_entry(int ty, int xi, int yj) {
   return ty == ST_M ? xi * A_PAD + yj : (ty == ST_I ? A_PAD * A_PAD + yj : A_PAD * A_PAD + A_PAD + xi);
+}
+__device__ inline bool emit_ok(int ty, int i, int j) {
+  return ty == ST_M ? (i >= 1 && j >= 1) : (ty == ST_I ? j >= 1 : (ty == ST_D && i >= 1));
+}
+// Emission probability of a state of type ty and group g at cell (i, j):
+// one load per lane, the entry chosen by selects (no branch per type, so
+// that the loads of several cells stay in flight together).
+__device__ inline double emit(const double* tab, int ty, int g, int xi, int yj, int i, int j) {
+  return emit_ok(ty, i, j) ? tab[(size_t)g * TAB_SZ + emit_entry(ty, xi, yj)] : 0.0;
 }
 
 // ---- cross-workgroup boundary (as in the chained 17-state kernels): entries
@@ -132,8 +177,14 @@ __device__ __forceinline__ void publish(int* flag, int count) {
   __hip_atomic_store(flag, count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // Waits (whole wave) until *flag >= need; false if the launch was aborted.
+// The counter is read with relaxed loads: everything it publishes is read
+// with device-scope loads at the coherence point (bnd_load), after this
+// wave has seen the count, and the producer raised the count only once its
+// stores had completed there.  An acquire would invalidate the XCD's L2
+// (the transition blocks, tables and lattice lines of every workgroup on
+// it) at each poll.
 __device__ bool wait_ge(int* flag, int need, int* abortf, int spin_max) {
-  int v = __builtin_amdgcn_readfirstlane(__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+  int v = __builtin_amdgcn_readfirstlane(__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
   int spins = 0;
   while (v < need) {
     if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
@@ -143,20 +194,35 @@ __device__ bool wait_ge(int* flag, int need, int* abortf, int spin_max) {
       return false;
     }
     __builtin_amdgcn_s_sleep(8);
-    v = __builtin_amdgcn_readfirstlane(__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+    v = __builtin_amdgcn_readfirstlane(__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
   }
   return true;
 }
 
-// The launch's next ticket, the same for the whole workgroup; a ticket past
-// the grid (a counter that was not zeroed) raises the abort flag and ends the
-// waves without touching a descriptor.
-__device__ __forceinline__ int take_ticket(int32_t* counter, int32_t* abortf) {
+// The workgroup's band: the next ticket of its XCD's queue (hardware register
+// XCC_ID), or of the next queue that still has bands.  Every queue lists its
+// pairs' bands in dependency order, so the band a workgroup waits on holds an
+// earlier ticket of the same queue and is already running.  The grid has one
+// workgroup per band, so a workgroup always finds one; if none is left (a
+// counter that was not zeroed) the abort flag is raised and the waves end
+// without touching a descriptor.
+__device__ __forceinline__ int take_band(int32_t* counters, const int32_t* qoff, int32_t* abortf) {
   __shared__ int s_ticket;
-  if (threadIdx.x == 0) s_ticket = atomicAdd(counter, 1);
+  if (threadIdx.x == 0) {
+    const int xcc = __builtin_amdgcn_s_getreg((20 << 0) | (0 << 6) | (3 << 11)) & (kQueues - 1);
+    int d = -1;
+    for (int q = 0; q < kQueues && d < 0; ++q) {
+      const int x = (xcc + q) & (kQueues - 1);
+      const int n = qoff[x + 1] - qoff[x];
+      if (n <= 0) continue;
+      const int t = atomicAdd(counters + x, 1);
+      if (t < n) d = qoff[x] + t;
+    }
+    s_ticket = d;
+  }
   __syncthreads();
   const int t = __builtin_amdgcn_readfirstlane(s_ticket);
-  if (t < 0 || t >= (int)gridDim.x) {
+  if (t < 0) {
     if (threadIdx.x == 0) __hip_atomic_store(abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_endpgm();
@@ -164,27 +230,56 @@ __device__ __forceinline__ int take_ticket(int32_t* counter, int32_t* abortf) {
   return t;
 }
 
-// out[16 x NBP tile nt of rows 0..15] = A (LDS, rows stride NBPP) . Bm (global
-// NBP x NBP, row-major) for column tile nt: acc[v] = C[4 v + lane/16][16 nt +
-// lane%16].  B fragments are loaded one batch of 4 k-chunks ahead.
+// All of wave w's blocks (w + kWaves u, u < nb; they share the slice
+// ks = w mod KS, hence the A operands) for row tile rt, interleaved so that
+// the wave has nb independent accumulation chains.
+template <int NT, int NBW>
+__device__ __forceinline__ void wave_gemm(const double* __restrict__ sA, int rt, int ks,
+                                          const double (&fr)[NBW][Geo<NT>::KB], int nb, d4 (&acc)[NBW],
+                                          int lane) {
+  using Gm = Geo<NT>;
+  static_assert(Gm::KB % 4 == 0, "chunks in batches of 4");
+  const double* ap = sA + (16 * rt + (lane & 15)) * Gm::NBPP + 4 * ks * Gm::KB + (lane >> 4);
+#pragma unroll
+  for (int u = 0; u < NBW; ++u) acc[u] = d4{0.0, 0.0, 0.0, 0.0};
+  double a[4], an[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) a[q] = ap[4 * q];
+#pragma unroll
+  for (int cb = 0; cb < Gm::KB; cb += 4) {
+    if (cb + 4 < Gm::KB) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) an[q] = ap[4 * (cb + 4 + q)];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int u = 0; u < NBW; ++u)
+        if (u < nb) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], fr[u][cb + q], acc[u], 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a[q] = an[q];
+  }
+}
+
 template <int NT>
-__device__ __forceinline__ d4 tile_gemm(const double* __restrict__ sA, const double* __restrict__ Bm, int nt,
-                                        int lane) {
-  constexpr int NBP = 16 * NT, NBPP = NBP + 1, KC = NBP / 4;
-  const double* ap = sA + (lane & 15) * NBPP + (lane >> 4);
-  const double* bp = Bm + (size_t)(lane >> 4) * NBP + 16 * nt + (lane & 15);
+__device__ __forceinline__ d4 block_gemm_stream(const double* __restrict__ sA, int rt, int nt, int ks,
+                                                const double* __restrict__ Bm, int lane) {
+  using Gm = Geo<NT>;
+  static_assert(Gm::KB % 4 == 0, "chunks in batches of 4");
+  const double* ap = sA + (16 * rt + (lane & 15)) * Gm::NBPP + 4 * ks * Gm::KB + (lane >> 4);
+  const double* bp = Bm + (size_t)(4 * ks * Gm::KB + (lane >> 4)) * Gm::NBP + 16 * nt + (lane & 15);
   d4 acc = {0.0, 0.0, 0.0, 0.0};
   double bn[4];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) bn[q] = bp[(size_t)4 * q * NBP];
+  for (int q = 0; q < 4; ++q) bn[q] = bp[(size_t)4 * q * Gm::NBP];
 #pragma unroll 1
-  for (int kb = 0; kb < KC; kb += 4) {
+  for (int kb = 0; kb < Gm::KB; kb += 4) {
     double b[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) b[q] = bn[q];
-    if (kb + 4 < KC) {
+    if (kb + 4 < Gm::KB) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) bn[q] = bp[(size_t)4 * (kb + 4 + q) * NBP];
+      for (int q = 0; q < 4; ++q) bn[q] = bp[(size_t)4 * (kb + 4 + q) * Gm::NBP];
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ap[4 * (kb + q)], b[q], acc, 0, 0, 0);
@@ -192,18 +287,70 @@ __device__ __forceinline__ d4 tile_gemm(const double* __restrict__ sA, const dou
   return acc;
 }
 
+template <int NT>
+__device__ __forceinline__ void load_frag(const double* __restrict__ Bm, int b, int lane, double* fr) {
+  using Gm = Geo<NT>;
+  const int nt = b / Gm::KS, ks = b - nt * Gm::KS;
+  const double* bp = Bm + (size_t)(4 * ks * Gm::KB + (lane >> 4)) * Gm::NBP + 16 * nt + (lane & 15);
+#pragma unroll
+  for (int c = 0; c < Gm::KB; ++c) fr[c] = bp[(size_t)4 * c * Gm::NBP];
+}
+
+// Boundary entries [u0, u1) (entry u at column u, or Ly - u if reversed:
+// NBP values and an exponent) into the LDS ring s_bin of CQ + 1 entries
+// (slot u mod (CQ + 1)), which then holds the chunk and the last entry of
+// the previous one.  Whole workgroup, two entries per wave, every load in
+// flight at once.
+template <int NBP>
+__device__ __forceinline__ void fetch_bnd(double (*s_bin)[NBP + 1], const double* src, int u0, int u1,
+                                          bool reversed, int Ly) {
+  static_assert(CQ == 2 * kWaves, "two entries per wave");
+  constexpr int K1 = (NBP + 1 + 63) / 64;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double v[2][K1];
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const int u = u0 + w + kWaves * e;
+    const double* src_e = src + (size_t)(reversed ? Ly - u : u) * (NBP + 1);
+#pragma unroll
+    for (int q = 0; q < K1; ++q) {
+      const int k = lane + 64 * q;
+      v[e][q] = (u < u1 && k <= NBP) ? bnd_load(src_e + k) : 0.0;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const int u = u0 + w + kWaves * e;
+#pragma unroll
+    for (int q = 0; q < K1; ++q) {
+      const int k = lane + 64 * q;
+      if (u < u1 && k <= NBP) s_bin[u % (CQ + 1)][k] = v[e][q];
+    }
+  }
+}
+
+// Descendant codes ys[u] in an LDS ring of kYR: chunk [c0, c0 + 128) is
+// loaded 64 steps before the first step that reads it.
+__device__ __forceinline__ void load_y(uint8_t* s_y, const uint8_t* ys, int c0, int Ly) {
+  const int u = c0 + (int)threadIdx.x;
+  if (threadIdx.x < 128 && u < Ly) s_y[u & (kYR - 1)] = ys[u];
+}
+
 // ====================================================================== forward
 template <int NT, bool DENSE>
 __global__ __launch_bounds__(kThr) void band_fwd_kernel(BParams Pr) {
-  constexpr int NBP = 16 * NT, NBPP = NBP + 1, KR = (NBP + 63) / 64;
-  static_assert(NBP % 16 == 0 && (NBP / 4) % 4 == 0, "k-chunks in batches of 4");
-  __shared__ double s_pre[2][R][NBPP];  // values pushed to the cells of steps t (slot t&1), t+1
-  __shared__ double s_F[R][NBPP];       // this step's normalised forward values (GEMM operand)
-  __shared__ double s_last[NBPP];       // the last row's pushes for the band below, and its exponent
+  using Gm = Geo<NT>;
+  constexpr int NBP = Gm::NBP, NBPP = Gm::NBPP, KR = Gm::KR, R = Gm::R, CPW = Gm::CPW;
+  constexpr int KS = Gm::KS, KB = Gm::KB, NBLK = Gm::NBLK, BPW = Gm::BPW;
+  __shared__ double s_pre[2][KS][R][NBPP];  // partial pushes to the cells of steps t (slot t&1), t+1
+  __shared__ double s_F[R][NBPP];           // this step's normalised forward values (GEMM operand)
+  __shared__ double s_last[KS][NBP];        // the last row's partial pushes for the band below
+  __shared__ double s_bin[CQ + 1][NBP + 1]; // the band above's last-row pushes (row 0's wave)
   __shared__ int s_eM[2][R], s_eI[2][R], s_eD[2][R], s_E[R];
-  __shared__ int s_ok;                  // cleared when a boundary wait gave up
+  __shared__ int s_lastE, s_ok;             // their exponent; cleared when a boundary wait gave up
+  __shared__ uint8_t s_y[DENSE ? 1 : kYR];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const BDesc pd = Pr.desc[take_ticket(Pr.ticket, Pr.abortf)];
+  const BDesc pd = Pr.desc[take_band(Pr.ticket, Pr.qoff, Pr.abortf)];
   const int Lx = pd.Lx, Ly = pd.Ly, W1 = Ly + 1, NB = Pr.NB, S = Pr.S, nr = pd.nr, i0 = pd.i0;
   const double* pm = Pr.ws + pd.pm_off;
   const double* P = pm;
@@ -217,7 +364,7 @@ __global__ __launch_bounds__(kThr) void band_fwd_kernel(BParams Pr) {
   const bool want_lat = !(Pr.flags & TKFMI_FORWARD_ONLY);
   const bool want_counts = want_lat && !(Pr.flags & TKFMI_NO_COUNTS) && Pr.out_n != nullptr;
   double* lat = Pr.ws + pd.lat_off;
-  double* latE = lat + (size_t)(Lx + 1) * W1 * NBP;
+  double* latE = lat + (size_t)pd.T * R * NBP;
   const double* bin = pd.bin_off >= 0 ? Pr.ws + pd.bin_off : nullptr;
   double* bout = pd.bout_off >= 0 ? Pr.ws + pd.bout_off : nullptr;
   double* out_n = want_counts ? Pr.out_n + (size_t)pd.pair * S * S : nullptr;
@@ -229,102 +376,159 @@ __global__ __launch_bounds__(kThr) void band_fwd_kernel(BParams Pr) {
     gr[q] = (k < NB) ? Pr.bgrp[k] : 0;
     sx[q] = (k < NB) ? Pr.bidx[k] : 0;
   }
-  for (int q = threadIdx.x; q < 2 * R * NBPP; q += kThr) (&s_pre[0][0][0])[q] = 0.0;
+  // the wave's transition blocks, held for the whole sweep, and the types of
+  // the output columns it writes
+  double fr[BPW][KB];
+  int tkb[BPW];
+  double psb[BPW];  // the origin's pushes p(S -> k) for the slice-0 blocks
+  const int nbw = (NBLK - w + kWaves - 1) / kWaves;  // the wave's blocks
+#pragma unroll
+  for (int u = 0; u < BPW; ++u) {
+    const int b = w + kWaves * u, k = 16 * (b / KS) + (lane & 15);
+    if (b < NBLK) load_frag<NT>(P, b, lane, fr[u]);
+    tkb[u] = (b < NBLK && k < NB) ? Pr.btype[k] : 0;
+    psb[u] = (b < NBLK && k < NB && b % KS == 0) ? p_start[k] : 0.0;
+  }
+  int xr[CPW];  // ancestor codes of the wave's rows
+#pragma unroll
+  for (int c = 0; c < CPW; ++c) {
+    const int i = i0 + w + kWaves * c;
+    xr[c] = (!DENSE && w + kWaves * c < nr && i >= 1) ? (int)min((int)xs[i - 1], A_PAD - 1) : 0;
+  }
+  for (int q = threadIdx.x; q < 2 * KS * R * NBPP; q += kThr) (&s_pre[0][0][0][0])[q] = 0.0;
   for (int q = threadIdx.x; q < R * NBPP; q += kThr) (&s_F[0][0])[q] = 0.0;
-  for (int q = threadIdx.x; q < NBPP; q += kThr) s_last[q] = 0.0;
-  if (threadIdx.x == 0) s_ok = 1;
+  for (int q = threadIdx.x; q < KS * NBP; q += kThr) (&s_last[0][0])[q] = 0.0;
+  if (threadIdx.x == 0) { s_ok = 1; s_lastE = 0; }
   if (threadIdx.x < 2 * R) {
     (&s_eM[0][0])[threadIdx.x] = EXP_NONE;
     (&s_eI[0][0])[threadIdx.x] = EXP_NONE;
     (&s_eD[0][0])[threadIdx.x] = EXP_NONE;
   }
+  if constexpr (!DENSE) load_y(s_y, ys, 0, Ly);
   __syncthreads();
+  // emissions (dense: log emissions) of the wave's cells at step tt
+  double en[CPW][KR];
+  auto prefetch = [&](int tt) {
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+      const int r = w + kWaves * c, j = tt - r, i = i0 + r;
+      const bool in = r < nr && j >= 0 && j <= Ly;
+      const int yj = (!DENSE && in && j >= 1) ? (int)min((int)s_y[(j - 1) & (kYR - 1)], A_PAD - 1) : 0;
+#pragma unroll
+      for (int q = 0; q < KR; ++q) {
+        if constexpr (DENSE) en[c][q] = (in && ty[q] != 0) ? dem[((size_t)i * W1 + j) * S + sx[q]] : 0.0;
+        else en[c][q] = in ? emit(tab, ty[q], gr[q], xr[c], yj, i, j) : 0.0;
+      }
+    }
+  };
+  prefetch(0);
   bool ok_run = true;
   for (int t = 0; t < pd.T; ++t) {
     const int sl = t & 1, sl1 = sl ^ 1;
     ok_run = ok_run && s_ok;
-    // ---- the last row's pushes of the previous step -> boundary (wave 7);
-    // a band whose input wait gave up stops publishing, so that the bands
-    // below it give up too instead of reading values computed from garbage
-    if (bout && w == kWaves - 1 && ok_run) {
-      const int jp = t - 1 - (nr - 1);
-      if (jp >= 0 && jp <= Ly) {
-        double* e = bout + (size_t)jp * (NBP + 1);
-        for (int k = lane; k <= NBP; k += 64) bnd_store(e + k, s_last[k]);
-        if (((jp + 1) % CQ) == 0 || jp == Ly) publish(Pr.flagv + pd.fout, jp + 1);
-      }
+    if constexpr (!DENSE) {
+      if (((t + 64) & 127) == 0) load_y(s_y, ys, t + 64, Ly);
     }
-    // ---- A: the cells of this step
-#pragma unroll 1
-    for (int r = w; r < R; r += kWaves) {
+    // ---- row 0's next chunk of the band above's pushes: wave 0 waits for
+    // it, then the workgroup copies it to LDS
+    if (bin != nullptr && (t % CQ) == 0 && t <= Ly) {
+      const int u1 = min(t + CQ, Ly + 1);
+      if (w == 0 && ok_run) {
+        ok_run = wait_ge(Pr.flagv + pd.fin, u1, Pr.abortf, Pr.spin_max);
+        if (!ok_run && lane == 0) s_ok = 0;
+      }
+      __syncthreads();
+      ok_run = ok_run && s_ok;
+      if (ok_run) fetch_bnd<NBP>(s_bin, bin, t, u1, false, Ly);
+      __syncthreads();
+    }
+    // ---- the last row's pushes of the previous step -> boundary (wave 7,
+    // published after the matrix phase); a band whose input wait gave up
+    // stops publishing, so that the bands below it give up too instead of
+    // reading values computed from garbage
+    const int jp = t - 1 - (nr - 1);
+    const bool put = bout != nullptr && w == kWaves - 1 && ok_run && jp >= 0 && jp <= Ly;
+    if (put) {
+      double* e = bout + (size_t)jp * (NBP + 1);
+      for (int k = lane; k < NBP; k += 64) {
+        double h = s_last[0][k];
+#pragma unroll
+        for (int ks = 1; ks < KS; ++ks) h += s_last[ks][k];
+        bnd_store(e + k, h);
+      }
+      if (lane == 0) bnd_store(e + NBP, (double)s_lastE);
+    }
+    // ---- A: the wave's cells of this step, in stages across the cells:
+    // F' = v e 2^(e_type - E0) (E0: the largest predecessor exponent), the
+    // cell's exponent from the largest F', then F normalised
+    double Fp[CPW][KR];
+    int E0c[CPW], ex[CPW];
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+      const int r = w + kWaves * c;
+      const int j = t - r, i = i0 + r;
+      const bool in = r < nr && j >= 0 && j <= Ly;
+      const bool top = (r == 0) && bin != nullptr;
+      const double* bM = s_bin[(j + CQ) % (CQ + 1)];  // entry j - 1
+      const double* bD = s_bin[j % (CQ + 1)];
+      const bool okM = in && i >= 1 && j >= 1, okI = in && j >= 1, okD = in && i >= 1;
+      // predecessor exponents: M from (i-1, j-1), I from (i, j-1), D from (i-1, j)
+      int eM = EXP_NONE, eI = EXP_NONE, eD = EXP_NONE;
+      if (okM) eM = top ? (int)bM[NBP] : s_eM[sl][r];
+      if (okI) eI = s_eI[sl][r];
+      if (okD) eD = top ? (int)bD[NBP] : s_eD[sl][r];
+      const int E0 = max(eM, max(eI, eD));
+      E0c[c] = E0;
+      double m = 0.0;
+#pragma unroll
+      for (int q = 0; q < KR; ++q) {
+        // every read unconditional (row r's pushes, the boundary entries),
+        // the lane's value chosen by selects
+        const int k = min(lane + 64 * q, NBP - 1);
+        double pre = s_pre[sl][0][r][k];
+#pragma unroll
+        for (int ks = 1; ks < KS; ++ks) pre += s_pre[sl][ks][r][k];
+        const double bmv = bM[k], bdv = bD[k];
+        const int tq = ty[q];
+        double v = 0.0;
+        if (tq == ST_M) v = okM ? (top ? bmv : pre) : 0.0;
+        else if (tq == ST_I) v = okI ? pre : 0.0;
+        else if (tq == ST_D) v = okD ? (top ? bdv : pre) : 0.0;
+        const int et = tq == ST_M ? eM : (tq == ST_I ? eI : eD);
+        double e;
+        if constexpr (DENSE) e = (v != 0.0) ? exp(en[c][q]) : 0.0;
+        else e = en[c][q];
+        const double f = (v != 0.0) ? ldexp(v * e, et - E0) : 0.0;
+        Fp[c][q] = f;
+        m = fmax(m, f);
+      }
+      ex[c] = wave_max_exp(m);
+    }
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+      const int r = w + kWaves * c;
       const int j = t - r, i = i0 + r;
       if (r >= nr || j < 0 || j > Ly) {
         for (int k = lane; k < NBP; k += 64) s_F[r][k] = 0.0;
         continue;
       }
-      const bool top = (r == 0) && bin != nullptr;
-      if (top && (j % CQ) == 0 && ok_run) {
-        ok_run = wait_ge(Pr.flagv + pd.fin, min(j + CQ, Ly + 1), Pr.abortf, Pr.spin_max);
-        if (!ok_run && lane == 0) s_ok = 0;
-      }
-      const bool okM = i >= 1 && j >= 1, okI = j >= 1, okD = i >= 1;
-      // predecessor exponents: M from (i-1, j-1), I from (i, j-1), D from (i-1, j)
-      int eM = EXP_NONE, eI = EXP_NONE, eD = EXP_NONE;
-      if (okM) eM = top ? (int)bnd_load(bin + (size_t)(j - 1) * (NBP + 1) + NBP) : s_eM[sl][r];
-      if (okI) eI = s_eI[sl][r];
-      if (okD) eD = top ? (int)bnd_load(bin + (size_t)j * (NBP + 1) + NBP) : s_eD[sl][r];
-      const int xi = (!DENSE && i >= 1) ? (int)min((int)xs[i - 1], A_PAD - 1) : 0;
-      const int yj = (!DENSE && j >= 1) ? (int)min((int)ys[j - 1], A_PAD - 1) : 0;
-      double raw[KR];
-#pragma unroll
-      for (int q = 0; q < KR; ++q) {
-        const int k = lane + 64 * q;
-        double v = 0.0;
-        if (ty[q] == ST_M && okM) v = top ? bnd_load(bin + (size_t)(j - 1) * (NBP + 1) + k) : s_pre[sl][r][k];
-        else if (ty[q] == ST_I && okI) v = s_pre[sl][r][k];
-        else if (ty[q] == ST_D && okD) v = top ? bnd_load(bin + (size_t)j * (NBP + 1) + k) : s_pre[sl][r][k];
-        if (v != 0.0) {
-          double e;
-          if constexpr (DENSE) e = exp(dem[((size_t)i * W1 + j) * S + sx[q]]);
-          else e = emit(tab, ty[q], gr[q], xi, yj, i, j);
-          v *= e;
-        }
-        raw[q] = v;
-      }
-      double mM = 0.0, mI = 0.0, mD = 0.0;
-#pragma unroll
-      for (int q = 0; q < KR; ++q) {
-        if (ty[q] == ST_M) mM = fmax(mM, raw[q]);
-        else if (ty[q] == ST_I) mI = fmax(mI, raw[q]);
-        else if (ty[q] == ST_D) mD = fmax(mD, raw[q]);
-      }
-      mM = wave_max(mM);
-      mI = wave_max(mI);
-      mD = wave_max(mD);
-      const int cM = (mM > 0.0) ? eM + frexp_exp(mM) : EXP_NONE;
-      const int cI = (mI > 0.0) ? eI + frexp_exp(mI) : EXP_NONE;
-      const int cD = (mD > 0.0) ? eD + frexp_exp(mD) : EXP_NONE;
-      int E = max(cM, max(cI, cD));
-      if (E == EXP_NONE) E = 0;
       const bool origin = (i == 0 && j == 0);
+      const bool any = ex[c] != EXP_NONE;
+      const int sh = any ? ex[c] : 0;
+      const int E = (any && !origin) ? E0c[c] + sh : 0;
       double F[KR];
 #pragma unroll
       for (int q = 0; q < KR; ++q) {
-        const int sh = (ty[q] == ST_M) ? eM - E : ((ty[q] == ST_I) ? eI - E : eD - E);
-        F[q] = (origin || raw[q] == 0.0) ? 0.0 : ldexp(raw[q], sh);
-      }
-      if (origin) E = 0;
-#pragma unroll
-      for (int q = 0; q < KR; ++q) {
         const int k = lane + 64 * q;
+        F[q] = origin ? 0.0 : ldexp(Fp[c][q], -sh);
         if (k < NBP) {
           s_F[r][k] = F[q];
-          if (want_lat) lat[((size_t)i * W1 + j) * NBP + k] = F[q];
+          if (want_lat) lat[((size_t)t * R + r) * NBP + k] = F[q];
         }
       }
       if (lane == 0) {
         s_E[r] = E;
-        if (want_lat) latE[(size_t)i * W1 + j] = (double)E;
+        if (want_lat) latE[(size_t)t * R + r] = (double)E;
       }
       if (i == Lx && j == Ly) {  // log P and the E column
         double part = 0.0;
@@ -355,23 +559,30 @@ __global__ __launch_bounds__(kThr) void band_fwd_kernel(BParams Pr) {
         }
       }
     }
+    if (t + 1 < pd.T) prefetch(t + 1);
     __syncthreads();
-    // ---- B: h = F P for the step's cells, pushed to the successors
+    // ---- B: h = F P for the step's cells, pushed to the successors as partial sums
     const bool origin_step = (i0 == 0 && t == 0);
-#pragma unroll 1
-    for (int nt = w; nt < NT; nt += kWaves) {
-      const d4 acc = tile_gemm<NT>(&s_F[0][0], P, nt, lane);
-      const int k = 16 * nt + (lane & 15);
-      const int tk = (k < NB) ? Pr.btype[k] : 0;
+    const int ks = w % KS;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = 4 * v + (lane >> 4);
-        const int j = t - r;
-        if (r >= nr || j < 0 || j > Ly || tk == 0) continue;
-        const double h = (origin_step && r == 0) ? p_start[k] : acc[v];
-        if (tk == ST_I) s_pre[sl1][r][k] = h;
-        else if (r + 1 < nr) s_pre[tk == ST_M ? sl : sl1][r + 1][k] = h;
-        else s_last[k] = h;
+    for (int rt = 0; rt < Gm::RT; ++rt) {
+      d4 acc[BPW];
+      wave_gemm<NT, BPW>(&s_F[0][0], rt, ks, fr, nbw, acc, lane);
+#pragma unroll
+      for (int u = 0; u < BPW; ++u) {
+        if (u >= nbw) break;
+        const int k = 16 * ((w + kWaves * u) / KS) + (lane & 15);
+        const int tk = tkb[u];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = 16 * rt + 4 * v + (lane >> 4);
+          const int j = t - r;
+          if (r >= nr || j < 0 || j > Ly || tk == 0) continue;
+          const double h = (origin_step && r == 0) ? psb[u] : acc[u][v];
+          if (tk == ST_I) s_pre[sl1][ks][r][k] = h;
+          else if (r + 1 < nr) s_pre[tk == ST_M ? sl : sl1][ks][r + 1][k] = h;
+          else s_last[ks][k] = h;
+        }
       }
     }
     if (w == 0 && lane < nr) {  // the pushed values' exponents
@@ -383,10 +594,11 @@ __global__ __launch_bounds__(kThr) void band_fwd_kernel(BParams Pr) {
           s_eD[sl1][r + 1] = E;
           s_eM[sl][r + 1] = E;
         } else {
-          s_last[NBP] = (double)E;
+          s_lastE = E;
         }
       }
     }
+    if (put && (((jp + 1) % CQ) == 0 || jp == Ly)) publish(Pr.flagv + pd.fout, jp + 1);
     __syncthreads();
   }
   // ---- the last pushes -> boundary
@@ -395,29 +607,42 @@ __global__ __launch_bounds__(kThr) void band_fwd_kernel(BParams Pr) {
     const int jp = pd.T - 1 - (nr - 1);
     if (jp >= 0 && jp <= Ly) {
       double* e = bout + (size_t)jp * (NBP + 1);
-      for (int k = lane; k <= NBP; k += 64) bnd_store(e + k, s_last[k]);
+      for (int k = lane; k < NBP; k += 64) {
+        double h = s_last[0][k];
+#pragma unroll
+        for (int ks = 1; ks < KS; ++ks) h += s_last[ks][k];
+        bnd_store(e + k, h);
+      }
+      if (lane == 0) bnd_store(e + NBP, (double)s_lastE);
       publish(Pr.flagv + pd.fout, jp + 1);
     }
   }
 }
 
 // ===================================================================== backward
-// Count tiles (NT x NT blocks of 16 x 16) dealt out to the waves: tile
-// w + kWaves q to wave w.
-__host__ __device__ constexpr int count_ntile(int NT) { return (NT * NT + kWaves - 1) / kWaves; }
+// The transition blocks are held in registers where they fit beside the
+// count tiles (up to 96 padded states); above, they are read from L2 per
+// step.
+template <int NT>
+__host__ __device__ constexpr bool bwd_breg() { return NT <= 6; }
 
 template <int NT, bool DENSE>
 __global__ __launch_bounds__(kThr) void band_bwd_kernel(BParams Pr) {
-  constexpr int NBP = 16 * NT, NBPP = NBP + 1, KR = (NBP + 63) / 64, NTILE = count_ntile(NT);
+  using Gm = Geo<NT>;
+  constexpr int NBP = Gm::NBP, NBPP = Gm::NBPP, KR = Gm::KR, R = Gm::R, CPW = Gm::CPW;
+  constexpr int KS = Gm::KS, KB = Gm::KB, NBLK = Gm::NBLK, BPW = Gm::BPW, NTILE = Gm::NTILE;
+  constexpr bool BREG = bwd_breg<NT>();
   __shared__ double s_pre[2][R][NBPP];  // q values pushed to the cells of steps t, t+1
   __shared__ double s_G[R][NBPP];       // g of this step's cells (successor-pulled q)
   __shared__ double s_FS[R][NBPP];      // their scaled forward values
-  __shared__ double s_B[R][NBPP];       // B = g P^T
-  __shared__ double s_last[NBPP];       // the first row's pushes for the band above, and its exponent
+  __shared__ double s_B[KS][R][NBPP];   // partial sums of B = g P^T
+  __shared__ double s_last[NBP];        // the first row's pushes for the band above
+  __shared__ double s_bin[CQ + 1][NBP + 1];  // the band below's first-row pushes (row nr-1's wave)
   __shared__ int s_eM[2][R], s_eI[2][R], s_eD[2][R], s_Eg[R];
-  __shared__ int s_ok;                  // cleared when a boundary wait gave up
+  __shared__ int s_lastE, s_ok;         // their exponent; cleared when a boundary wait gave up
+  __shared__ uint8_t s_y[DENSE ? 1 : kYR];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const BDesc pd = Pr.desc[take_ticket(Pr.ticket, Pr.abortf)];
+  const BDesc pd = Pr.desc[take_band(Pr.ticket, Pr.qoff, Pr.abortf)];
   const int Lx = pd.Lx, Ly = pd.Ly, W1 = Ly + 1, NB = Pr.NB, S = Pr.S, nr = pd.nr, i0 = pd.i0;
   const double* pm = Pr.ws + pd.pm_off;
   const double* P = pm;
@@ -432,7 +657,7 @@ __global__ __launch_bounds__(kThr) void band_bwd_kernel(BParams Pr) {
   const bool want_post = (Pr.flags & TKFMI_WANT_POSTERIORS) && Pr.out_post != nullptr;
   const bool want_ecnt = !DENSE && (Pr.flags & TKFMI_WANT_EMIT_COUNTS) && Pr.out_ecnt != nullptr;
   const double* lat = Pr.ws + pd.lat_off;
-  const double* latE = lat + (size_t)(Lx + 1) * W1 * NBP;
+  const double* latE = lat + (size_t)pd.T * R * NBP;
   const double* bin = pd.bin_off >= 0 ? Pr.ws + pd.bin_off : nullptr;
   double* bout = pd.bout_off >= 0 ? Pr.ws + pd.bout_off : nullptr;
   double* out_n = want_counts ? Pr.out_n + (size_t)pd.pair * S * S : nullptr;
@@ -450,38 +675,106 @@ __global__ __launch_bounds__(kThr) void band_bwd_kernel(BParams Pr) {
     gr[q] = (k < NB) ? Pr.bgrp[k] : 0;
     sx[q] = (k < NB) ? Pr.bidx[k] : 0;
   }
+  double fr[BREG ? BPW : 1][BREG ? KB : 1];
+  if constexpr (BREG) {
+#pragma unroll
+    for (int u = 0; u < BPW; ++u)
+      if (w + kWaves * u < NBLK) load_frag<NT>(PT, w + kWaves * u, lane, fr[u]);
+  }
+  int xr[CPW];  // ancestor codes of the wave's rows (reversed: rr = w + 8 c)
+#pragma unroll
+  for (int c = 0; c < CPW; ++c) {
+    const int rr = w + kWaves * c, i = i0 + nr - 1 - rr;
+    xr[c] = (!DENSE && rr < nr && i >= 1) ? (int)min((int)xs[i - 1], A_PAD - 1) : 0;
+  }
   for (int q = threadIdx.x; q < 2 * R * NBPP; q += kThr) (&s_pre[0][0][0])[q] = 0.0;
   for (int q = threadIdx.x; q < R * NBPP; q += kThr) {
     (&s_G[0][0])[q] = 0.0;
     (&s_FS[0][0])[q] = 0.0;
   }
-  for (int q = threadIdx.x; q < NBPP; q += kThr) s_last[q] = 0.0;
-  if (threadIdx.x == 0) s_ok = 1;
+  for (int q = threadIdx.x; q < NBP; q += kThr) s_last[q] = 0.0;
+  if (threadIdx.x == 0) { s_ok = 1; s_lastE = 0; }
   if (threadIdx.x < 2 * R) {
     (&s_eM[0][0])[threadIdx.x] = EXP_NONE;
     (&s_eI[0][0])[threadIdx.x] = EXP_NONE;
     (&s_eD[0][0])[threadIdx.x] = EXP_NONE;
   }
+  // Reversed coordinates: rr = nr - 1 - r (rr = 0 is the band's last row),
+  // jj = Ly - j; step t handles (rr, jj = t - rr).  Successors of (i, j):
+  // M (i+1, j+1) = (rr-1, jj-1), I (i, j+1) = (rr, jj-1), D (i+1, j) = (rr-1, jj).
+  // The y ring holds ys[Ly - 1 - u] at u: cell (rr, jj) reads u = jj.
+  auto load_yr = [&](int c0) {
+    const int u = c0 + (int)threadIdx.x;
+    if (threadIdx.x < 128 && u < Ly) s_y[u & (kYR - 1)] = ys[Ly - 1 - u];
+  };
+  if constexpr (!DENSE) load_yr(0);
   d4 cacc[NTILE];
 #pragma unroll
   for (int q = 0; q < NTILE; ++q) cacc[q] = d4{0.0, 0.0, 0.0, 0.0};
   __syncthreads();
-  constexpr int CPW = R / kWaves;  // cells per wave and step
+  // forward values and exponent of the wave's cells at step tt (the
+  // forward's step T - 1 - tt, row nr - 1 - rr)
+  double lf[CPW][KR];
+  int lfE[CPW];
+  auto prefetch_lat = [&](int tt) {
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+      const int rr = w + kWaves * c, jj = tt - rr;
+      const bool in = rr < nr && jj >= 0 && jj <= Ly;
+      const size_t cell = (size_t)(pd.T - 1 - tt) * R + (nr - 1 - rr);
+#pragma unroll
+      for (int q = 0; q < KR; ++q) {
+        const int k = lane + 64 * q;
+        lf[c][q] = (in && k < NBP) ? lat[cell * NBP + k] : 0.0;
+      }
+      lfE[c] = in ? (int)latE[cell] : 0;
+    }
+  };
+  // emissions (dense: log emissions) of the wave's cells at step tt
+  double en[CPW][KR];
+  auto prefetch_em = [&](int tt) {
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+      const int rr = w + kWaves * c, jj = tt - rr, i = i0 + nr - 1 - rr, j = Ly - jj;
+      const bool in = rr < nr && jj >= 0 && jj <= Ly;
+      const int yj = (!DENSE && in && j >= 1) ? (int)min((int)s_y[jj & (kYR - 1)], A_PAD - 1) : 0;
+#pragma unroll
+      for (int q = 0; q < KR; ++q) {
+        if constexpr (DENSE) en[c][q] = (in && ty[q] != 0) ? dem[((size_t)i * W1 + j) * S + sx[q]] : 0.0;
+        else en[c][q] = in ? emit(tab, ty[q], gr[q], xr[c], yj, i, j) : 0.0;
+      }
+    }
+  };
+  prefetch_lat(0);
+  prefetch_em(0);
   bool ok_run = true;
-  // Reversed coordinates: rr = nr - 1 - r (rr = 0 is the band's last row),
-  // jj = Ly - j; step t handles (rr, jj = t - rr).  Successors of (i, j):
-  // M (i+1, j+1) = (rr-1, jj-1), I (i, j+1) = (rr, jj-1), D (i+1, j) = (rr-1, jj).
   for (int t = 0; t < pd.T; ++t) {
     const int sl = t & 1, sl1 = sl ^ 1;
     ok_run = ok_run && s_ok;
-    // ---- the first row's pushes of the previous step -> boundary (wave 7)
-    if (bout && w == kWaves - 1 && ok_run) {
-      const int jjp = t - 1 - (nr - 1);
-      if (jjp >= 0 && jjp <= Ly) {
-        double* e = bout + (size_t)(Ly - jjp) * (NBP + 1);
-        for (int k = lane; k <= NBP; k += 64) bnd_store(e + k, s_last[k]);
-        if (((jjp + 1) % CQ) == 0 || jjp == Ly) publish(Pr.flagv + pd.fout, jjp + 1);
+    if constexpr (!DENSE) {
+      if (((t + 64) & 127) == 0) load_yr(t + 64);
+    }
+    // ---- the bottom row's next chunk of the band below's pushes: wave 0
+    // waits for it, then the workgroup copies it to LDS
+    if (bin != nullptr && (t % CQ) == 0 && t <= Ly) {
+      const int u1 = min(t + CQ, Ly + 1);
+      if (w == 0 && ok_run) {
+        ok_run = wait_ge(Pr.flagv + pd.fin, u1, Pr.abortf, Pr.spin_max);
+        if (!ok_run && lane == 0) s_ok = 0;
       }
+      __syncthreads();
+      ok_run = ok_run && s_ok;
+      if (ok_run) fetch_bnd<NBP>(s_bin, bin, t, u1, true, Ly);
+      __syncthreads();
+    }
+    // ---- the first row's pushes of the previous step -> boundary (wave 7,
+    // published after the matrix phase)
+    const int jjp = t - 1 - (nr - 1);
+    const bool put = bout != nullptr && w == kWaves - 1 && ok_run && jjp >= 0 && jjp <= Ly;
+    if (put) {
+      double* e = bout + (size_t)(Ly - jjp) * (NBP + 1);
+      for (int k = lane; k < NBP; k += 64) bnd_store(e + k, s_last[k]);
+      if (lane == 0) bnd_store(e + NBP, (double)s_lastE);
     }
     // ---- A: g and the scaled forward values of this step's cells
 #pragma unroll
@@ -493,34 +786,31 @@ __global__ __launch_bounds__(kThr) void band_bwd_kernel(BParams Pr) {
         continue;
       }
       const bool bot = (rr == 0) && bin != nullptr;
-      if (bot && (jj % CQ) == 0 && ok_run) {
-        ok_run = wait_ge(Pr.flagv + pd.fin, min(jj + CQ, Ly + 1), Pr.abortf, Pr.spin_max);
-        if (!ok_run && lane == 0) s_ok = 0;
-      }
+      const double* bM = s_bin[(jj + CQ) % (CQ + 1)];  // column j + 1 (reversed entry jj - 1)
+      const double* bD = s_bin[jj % (CQ + 1)];
       const bool okM = i < Lx && j < Ly, okI = j < Ly, okD = i < Lx;
       int eM = EXP_NONE, eI = EXP_NONE, eD = EXP_NONE;
-      if (okM) eM = bot ? (int)bnd_load(bin + (size_t)(j + 1) * (NBP + 1) + NBP) : s_eM[sl][rr];
+      if (okM) eM = bot ? (int)bM[NBP] : s_eM[sl][rr];
       if (okI) eI = s_eI[sl][rr];
-      if (okD) eD = bot ? (int)bnd_load(bin + (size_t)j * (NBP + 1) + NBP) : s_eD[sl][rr];
+      if (okD) eD = bot ? (int)bD[NBP] : s_eD[sl][rr];
       int Eg = max(eM, max(eI, eD));
       if (Eg == EXP_NONE) Eg = 0;
-      const size_t cell = (size_t)i * W1 + j;
-      const int EF = (int)latE[cell];
-      const double fsc = ldexp(inv_mP, EF + Eg - EP);
+      const double fsc = ldexp(inv_mP, lfE[c] + Eg - EP);
       double g[KR];
 #pragma unroll
       for (int q = 0; q < KR; ++q) {
         const int k = lane + 64 * q;
+        const int kc = min(k, NBP - 1);
+        const double pre = s_pre[sl][rr][kc], bmv = bM[kc], bdv = bD[kc];
+        const int tq = ty[q];
         double v = 0.0;
-        if (ty[q] == ST_M && okM)
-          v = ldexp(bot ? bnd_load(bin + (size_t)(j + 1) * (NBP + 1) + k) : s_pre[sl][rr][k], eM - Eg);
-        else if (ty[q] == ST_I && okI) v = ldexp(s_pre[sl][rr][k], eI - Eg);
-        else if (ty[q] == ST_D && okD)
-          v = ldexp(bot ? bnd_load(bin + (size_t)j * (NBP + 1) + k) : s_pre[sl][rr][k], eD - Eg);
+        if (tq == ST_M) v = okM ? ldexp(bot ? bmv : pre, eM - Eg) : 0.0;
+        else if (tq == ST_I) v = okI ? ldexp(pre, eI - Eg) : 0.0;
+        else if (tq == ST_D) v = okD ? ldexp(bot ? bdv : pre, eD - Eg) : 0.0;
         g[q] = v;
         if (k < NBP) {
           s_G[rr][k] = v;
-          s_FS[rr][k] = lat[cell * NBP + k] * fsc;
+          s_FS[rr][k] = lf[c][q] * fsc;
         }
       }
       if (lane == 0) s_Eg[rr] = Eg;
@@ -539,71 +829,106 @@ __global__ __launch_bounds__(kThr) void band_bwd_kernel(BParams Pr) {
         if (want_post && lane == 0) post[0] = (i == Lx && j == Ly) ? 1.0 : part * scale;
       }
     }
+    if (t + 1 < pd.T) prefetch_lat(t + 1);
     __syncthreads();
-    // ---- B: B = g P^T, counts += Fs^T g
-#pragma unroll 1
-    for (int nt = w; nt < NT; nt += kWaves) {
-      const d4 bacc = tile_gemm<NT>(&s_G[0][0], PT, nt, lane);
+    // ---- B: B = g P^T (partial sums per slice), counts += Fs^T g
+    if constexpr (BREG) {
+      const int ks = w % KS, nbw = (NBLK - w + kWaves - 1) / kWaves;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) s_B[4 * v + (lane >> 4)][16 * nt + (lane & 15)] = bacc[v];
+      for (int rt = 0; rt < Gm::RT; ++rt) {
+        d4 acc[BPW];
+        wave_gemm<NT, BPW>(&s_G[0][0], rt, ks, fr, nbw, acc, lane);
+#pragma unroll
+        for (int u = 0; u < BPW; ++u) {
+          if (u >= nbw) break;
+          const int nt = (w + kWaves * u) / KS;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) s_B[ks][16 * rt + 4 * v + (lane >> 4)][16 * nt + (lane & 15)] = acc[u][v];
+        }
+      }
+    } else {
+#pragma unroll 1
+      for (int b = w; b < NBLK; b += kWaves) {
+        const int nt = b / KS, ks = b - nt * KS;
+#pragma unroll 1
+        for (int rt = 0; rt < Gm::RT; ++rt) {
+          const d4 acc = block_gemm_stream<NT>(&s_G[0][0], rt, nt, ks, PT, lane);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) s_B[ks][16 * rt + 4 * v + (lane >> 4)][16 * nt + (lane & 15)] = acc[v];
+        }
+      }
     }
     if (want_counts) {
 #pragma unroll
       for (int kc = 0; kc < R / 4; ++kc) {
-        const double* fr = &s_FS[4 * kc + (lane >> 4)][lane & 15];
-        const double* gr_ = &s_G[4 * kc + (lane >> 4)][lane & 15];
+        const double* fa = &s_FS[4 * kc + (lane >> 4)][lane & 15];
+        const double* ga = &s_G[4 * kc + (lane >> 4)][lane & 15];
 #pragma unroll
         for (int q = 0; q < NTILE; ++q) {
           const int tile = w + kWaves * q;
           if (tile < NT * NT)
-            cacc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr[16 * (tile / NT)], gr_[16 * (tile % NT)], cacc[q], 0, 0, 0);
+            cacc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[16 * (tile / NT)], ga[16 * (tile % NT)], cacc[q], 0, 0, 0);
         }
       }
     }
     __syncthreads();
-    // ---- C: normalise B, posteriors, emission counts, q = e . B to the predecessors
+    if (put && (((jjp + 1) % CQ) == 0 || jjp == Ly)) publish(Pr.flagv + pd.fout, jjp + 1);
+    // ---- C: normalise B, posteriors, emission counts, q = e . B to the
+    // predecessors; B's exponents of the wave's cells first, together
+    double Bv[CPW][KR];
+    int ex[CPW];
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+      const int rr = w + kWaves * c;
+      const int jj = t - rr, i = i0 + nr - 1 - rr, j = Ly - jj;
+      const bool in = rr < nr && jj >= 0 && jj <= Ly;
+      const bool corner = (i == Lx && j == Ly);
+      double mx = 0.0;
+#pragma unroll
+      for (int q = 0; q < KR; ++q) {
+        const int k = lane + 64 * q;
+        double bsum = 0.0;
+        if (in && k < NBP) {
+          if (corner) bsum = (k < NB) ? p_end[k] : 0.0;
+          else {
+            bsum = s_B[0][rr][k];
+#pragma unroll
+            for (int ks = 1; ks < KS; ++ks) bsum += s_B[ks][rr][k];
+          }
+        }
+        Bv[c][q] = bsum;
+        mx = fmax(mx, bsum);
+      }
+      ex[c] = wave_max_exp(mx);
+    }
 #pragma unroll
     for (int c = 0; c < CPW; ++c) {
       const int rr = w + kWaves * c;
       const int jj = t - rr, r = nr - 1 - rr, i = i0 + r, j = Ly - jj;
       if (rr >= nr || jj < 0 || jj > Ly) continue;
       const bool corner = (i == Lx && j == Ly);
-      double Bv[KR];
-      double mx = 0.0;
-#pragma unroll
-      for (int q = 0; q < KR; ++q) {
-        const int k = lane + 64 * q;
-        Bv[q] = (k < NBP) ? (corner ? (k < NB ? p_end[k] : 0.0) : s_B[rr][k]) : 0.0;
-        mx = fmax(mx, Bv[q]);
-      }
-      mx = wave_max(mx);
-      const int sh = (mx > 0.0) ? frexp_exp(mx) : 0;
+      const int sh = (ex[c] != EXP_NONE) ? ex[c] : 0;
       const int EB = (corner ? 0 : s_Eg[rr]) + sh;
-      const int xi = (!DENSE && i >= 1) ? (int)min((int)xs[i - 1], A_PAD - 1) : 0;
-      const int yj = (!DENSE && j >= 1) ? (int)min((int)ys[j - 1], A_PAD - 1) : 0;
+      const int yj = (!DENSE && j >= 1) ? (int)min((int)s_y[jj & (kYR - 1)], A_PAD - 1) : 0;
       const size_t cellS = ((size_t)i * W1 + j) * S;
 #pragma unroll
       for (int q = 0; q < KR; ++q) {
         const int k = lane + 64 * q;
         if (k >= NBP) continue;
-        const double b = ldexp(Bv[q], -sh);
+        const double b = ldexp(Bv[c][q], -sh);
         double e = 0.0;
         if (k < NB) {
-          if constexpr (DENSE) {
-            const bool ok = (ty[q] == ST_M) ? (i >= 1 && j >= 1) : (ty[q] == ST_I) ? (j >= 1) : (i >= 1);
-            e = ok ? exp(dem[cellS + sx[q]]) : 0.0;
-          } else {
-            e = emit(tab, ty[q], gr[q], xi, yj, i, j);
-          }
+          if constexpr (DENSE) e = emit_ok(ty[q], i, j) ? exp(en[c][q]) : 0.0;
+          else e = en[c][q];
         }
         const double qv = e * b;
         if (k < NB && (want_post || want_ecnt)) {
           // F B / P(x, y) = Fs (scaled by 2^(EF + Eg - EP) / mP) times B
           // before its normalisation (relative to 2^Eg; the corner: Eg = 0)
-          const double pv = s_FS[rr][k] * Bv[q];
+          const double pv = s_FS[rr][k] * Bv[c][q];
           if (want_post) post[cellS + sx[q]] = pv;
           if (want_ecnt && pv != 0.0 && e != 0.0)
-            atomicAdd(ecnt + (size_t)gr[q] * TAB_SZ + emit_entry(ty[q], xi, yj), pv);
+            atomicAdd(ecnt + (size_t)gr[q] * TAB_SZ + emit_entry(ty[q], xr[c], yj), pv);
         }
         // push q: M-type to (i-1, j-1), I-type to (i, j-1), D-type to (i-1, j)
         if (ty[q] == ST_I) s_pre[sl1][rr][k] = qv;
@@ -618,10 +943,11 @@ __global__ __launch_bounds__(kThr) void band_bwd_kernel(BParams Pr) {
           s_eD[sl1][rr + 1] = EB;
           s_eM[sl][rr + 1] = EB;
         } else {
-          s_last[NBP] = (double)EB;
+          s_lastE = EB;
         }
       }
     }
+    if (t + 1 < pd.T) prefetch_em(t + 1);
     __syncthreads();
   }
   // ---- the last pushes -> boundary
@@ -630,7 +956,8 @@ __global__ __launch_bounds__(kThr) void band_bwd_kernel(BParams Pr) {
     const int jjp = pd.T - 1 - (nr - 1);
     if (jjp >= 0 && jjp <= Ly) {
       double* e = bout + (size_t)(Ly - jjp) * (NBP + 1);
-      for (int k = lane; k <= NBP; k += 64) bnd_store(e + k, s_last[k]);
+      for (int k = lane; k < NBP; k += 64) bnd_store(e + k, s_last[k]);
+      if (lane == 0) bnd_store(e + NBP, (double)s_lastE);
       publish(Pr.flagv + pd.fout, jjp + 1);
     }
   }
@@ -690,18 +1017,38 @@ __global__ void zero_ints_kernel(int32_t* p, int n) {
 }
 
 // ---------------------------------------------------------------- host side
+// Flag region (ints): forward ticket counters [kQueues], backward [kQueues],
+// abort flag, padding to 4 x kQueues, then forward / backward / accumulator
+// counters per band boundary.
+constexpr int kFlagHead = 4 * kQueues;
+
 struct BPlan {
   bool ok = false;
   std::string why;
-  int S = 0, NB = 0, NT = 0, NBP = 0, e_idx = -1, G = 0;
+  int S = 0, NB = 0, NT = 0, NBP = 0, R = 0, e_idx = -1, G = 0;
   std::vector<int32_t> bidx, btype, bgrp;
+  std::vector<int32_t> qoff_f, qoff_b;  // kQueues + 1 each
   std::vector<BDesc> fwd, bwd;
-  int n_flags = 0;  // ints: 2 tickets, abort, pad, then fwd / bwd / accumulator counters per boundary
+  int n_flags = 0;
   size_t idx_b = 0, desc_b = 0, pm_b = 0, tab_b = 0, pinfo_b = 0, flag_b = 0, data_b = 0;
   size_t total() const { return idx_b + desc_b + pm_b + tab_b + pinfo_b + flag_b + data_b; }
 };
 
 size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+// Compute units per ticket queue (per XCD): the current device's count / 8,
+// or 32 (MI355X: 256 / 8) where no device can be queried.
+double cus_per_queue() {
+  static int cus = [] {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+      return n;
+    (void)hipGetLastError();
+    return 256;
+  }();
+  return std::max(1.0, cus / (double)kQueues);
+}
 
 BPlan plan(const tkfmi_problem* p, uint32_t flags) {
   BPlan pl;
@@ -722,6 +1069,8 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
   const int nt = (pl.NB + 15) / 16;
   pl.NT = (nt <= 2) ? 2 : (nt <= 4 ? 4 : (nt <= 6 ? 6 : (nt <= 8 ? 8 : 10)));
   pl.NBP = 16 * pl.NT;
+  pl.R = 16 * band_rt(pl.NT);
+  const int R = pl.R;
   while ((int)pl.bidx.size() < pl.NBP) { pl.bidx.push_back(0); pl.btype.push_back(0); pl.bgrp.push_back(0); }
   pl.G = p->n_groups;
   for (int k = 0; k < pl.NB; ++k)
@@ -732,14 +1081,12 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
   const int B = p->B;
   if (B > 0 && (!p->h_x_len || !p->h_y_len || !p->h_x_off || !p->h_y_off)) { pl.why = "host arrays missing"; return pl; }
   const bool fwd_only = (flags & TKFMI_FORWARD_ONLY) != 0;
+  const bool counts = !fwd_only && !(flags & TKFMI_NO_COUNTS);
   const int NBP = pl.NBP;
-  pl.idx_b = align256(3 * (size_t)NBP * 4);
+  pl.idx_b = align256((3 * (size_t)NBP + 2 * (kQueues + 1)) * 4);
   pl.pm_b = align256((size_t)pm_stride(NBP) * 8 * p->n_trans);
   pl.tab_b = dense ? 0 : align256((size_t)p->n_tab * pl.G * TAB_SZ * 8);
   pl.pinfo_b = align256((size_t)std::max(B, 1) * 16);
-  // bands, boundaries, lattice
-  int64_t off = 0;  // doubles into the data region
-  int nbnd = 0;
   std::vector<int64_t> post_off(B);
   int64_t ap = 0;
   for (int b = 0; b < B; ++b) {
@@ -748,70 +1095,114 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
     post_off[b] = ap;
     ap += (int64_t)(lx + 1) * (ly + 1) * p->S;
   }
-  const bool counts = !fwd_only && !(flags & TKFMI_NO_COUNTS);
-  struct PairBands { int b, G; int64_t lat, bnd0, acc; int flag0; };
+  // per pair: lattice, band boundaries, count accumulator; flags
+  int64_t off = 0;  // doubles into the data region
+  int nbnd = 0;
+  struct PairBands { int G; int64_t lat, bnd0, acc; int flag0; double cost; };
   std::vector<PairBands> pb(B);
   for (int b = 0; b < B; ++b) {
     const int lx = p->h_x_len[b], ly = p->h_y_len[b];
     const int G = (lx + R) / R;  // ceil((Lx + 1) / R)
-    pb[b].b = b;
     pb[b].G = G;
     pb[b].lat = off;
-    if (!fwd_only) off += (int64_t)(lx + 1) * (ly + 1) * (NBP + 1);
+    // band g's lattice: T_g = Ly + nr_g steps of R rows
+    if (!fwd_only) off += ((int64_t)G * ly + lx + 1) * R * (NBP + 1);
     pb[b].bnd0 = off;
     off += (int64_t)(G - 1) * (ly + 1) * (NBP + 1);
     pb[b].acc = off;
     if (counts && G > 1) off += (int64_t)NBP * NBP;
-    pb[b].flag0 = 4 + 3 * nbnd;
+    pb[b].flag0 = kFlagHead + 3 * nbnd;
     nbnd += G - 1;
+    pb[b].cost = (double)G * (ly + R);
   }
-  pl.n_flags = 4 + 3 * nbnd;
+  pl.n_flags = kFlagHead + 3 * nbnd;
   pl.flag_b = align256((size_t)pl.n_flags * 4);
+  // Pairs to queues (one per XCD), longest first to the least loaded queue.
+  // Within a queue, the pairs are taken in groups just large enough to keep
+  // the XCD's compute units busy: a band can start about lag = R + CQ steps
+  // after the band it reads from, so one pair can feed T / lag compute units
+  // (T: steps per band), and a group holds pairs until the sum reaches 1.5
+  // times the XCD's compute units.  A group's bands are listed layer by
+  // layer (band 0 of each pair, then band 1, ...; the backward from the
+  // last band), so that a band's ticket comes about when its input is
+  // ready, and a few pairs at a time share the XCD's L2.
+  std::vector<int> order(B);
+  for (int b = 0; b < B; ++b) order[b] = b;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return pb[a].cost > pb[c].cost; });
+  std::vector<std::vector<int>> queue(kQueues);
+  std::vector<double> load(kQueues, 0.0);
+  for (int b : order) {
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    queue[x].push_back(b);
+    load[x] += pb[b].cost;
+  }
+  const double lag = R + CQ, feed = 1.5 * cus_per_queue();
+  pl.qoff_f.assign(kQueues + 1, 0);
+  pl.qoff_b.assign(kQueues + 1, 0);
+  std::vector<BDesc> fwd_all, bwd_all;  // per pair, bands 0..G-1
+  std::vector<int64_t> first(B, 0);
   for (int b = 0; b < B; ++b) {
-    const int lx = p->h_x_len[b], ly = p->h_y_len[b], G = pb[b].G;
-    for (int g = 0; g < G; ++g) {
-      BDesc d;
-      memset(&d, 0, sizeof(d));
-      d.pair = b; d.Lx = lx; d.Ly = ly; d.i0 = g * R; d.nr = std::min(R, lx + 1 - g * R);
-      d.g = g; d.G = G; d.T = ly + d.nr;
-      d.lat_off = pb[b].lat;
-      d.pm_off = (p->n_trans > 1 ? (int64_t)b : 0) * pm_stride(NBP);
-      d.tab_off = (!dense && p->n_tab > 1 ? (int64_t)b : 0) * pl.G * TAB_SZ;
-      d.x_off = p->h_x_off[b]; d.y_off = p->h_y_off[b];
-      d.post_off = post_off[b];
-      d.acc_off = pb[b].acc;
+    {
+      const int lx = p->h_x_len[b], ly = p->h_y_len[b], G = pb[b].G;
       const int64_t bsz = (int64_t)(ly + 1) * (NBP + 1);
-      // forward: boundary g-1 in, g out; backward: boundary g in, g-1 out
-      BDesc f = d, k = d;
       const int fl = pb[b].flag0;
-      f.bin_off = g > 0 ? pb[b].bnd0 + (int64_t)(g - 1) * bsz : -1;
-      f.fin = g > 0 ? fl + 3 * (g - 1) : -1;
-      f.bout_off = g < G - 1 ? pb[b].bnd0 + (int64_t)g * bsz : -1;
-      f.fout = g < G - 1 ? fl + 3 * g : -1;
-      f.ain = f.aout = -1;
-      k.bin_off = g < G - 1 ? pb[b].bnd0 + (int64_t)g * bsz : -1;
-      k.fin = g < G - 1 ? fl + 3 * g + 1 : -1;
-      k.bout_off = g > 0 ? pb[b].bnd0 + (int64_t)(g - 1) * bsz : -1;
-      k.fout = g > 0 ? fl + 3 * (g - 1) + 1 : -1;
-      k.ain = (counts && g < G - 1) ? fl + 3 * g + 2 : -1;
-      k.aout = (counts && g > 0) ? fl + 3 * (g - 1) + 2 : -1;
-      pl.fwd.push_back(f);
-      pl.bwd.push_back(k);
+      first[b] = (int64_t)fwd_all.size();
+      int64_t lat_band = pb[b].lat;
+      for (int g = 0; g < G; ++g) {
+        BDesc d;
+        memset(&d, 0, sizeof(d));
+        d.pair = b; d.Lx = lx; d.Ly = ly; d.i0 = g * R; d.nr = std::min(R, lx + 1 - g * R);
+        d.g = g; d.G = G; d.T = ly + d.nr;
+        d.lat_off = lat_band;
+        lat_band += (int64_t)d.T * R * (NBP + 1);
+        d.pm_off = (p->n_trans > 1 ? (int64_t)b : 0) * pm_stride(NBP);
+        d.tab_off = (!dense && p->n_tab > 1 ? (int64_t)b : 0) * pl.G * TAB_SZ;
+        d.x_off = p->h_x_off[b]; d.y_off = p->h_y_off[b];
+        d.post_off = post_off[b];
+        d.acc_off = pb[b].acc;
+        // forward: boundary g-1 in, g out
+        d.bin_off = g > 0 ? pb[b].bnd0 + (int64_t)(g - 1) * bsz : -1;
+        d.fin = g > 0 ? fl + 3 * (g - 1) : -1;
+        d.bout_off = g < G - 1 ? pb[b].bnd0 + (int64_t)g * bsz : -1;
+        d.fout = g < G - 1 ? fl + 3 * g : -1;
+        d.ain = d.aout = -1;
+        fwd_all.push_back(d);
+        // backward: boundary g in, g-1 out (the forward's boundary rows are
+        // no longer needed once the forward launch has ended)
+        d.bin_off = g < G - 1 ? pb[b].bnd0 + (int64_t)g * bsz : -1;
+        d.fin = g < G - 1 ? fl + 3 * g + 1 : -1;
+        d.bout_off = g > 0 ? pb[b].bnd0 + (int64_t)(g - 1) * bsz : -1;
+        d.fout = g > 0 ? fl + 3 * (g - 1) + 1 : -1;
+        d.ain = (counts && g < G - 1) ? fl + 3 * g + 2 : -1;
+        d.aout = (counts && g > 0) ? fl + 3 * (g - 1) + 2 : -1;
+        bwd_all.push_back(d);
+      }
     }
   }
-  // Ticket order: by the step at which a band can start working (a band
-  // starts lag steps after the band it reads from), then by the length of
-  // the chain still ahead of it.  Producers get lower tickets than their
-  // consumers, and a band is not resident long before its input arrives.
-  const int64_t lag = R + CQ;
-  std::stable_sort(pl.fwd.begin(), pl.fwd.end(), [&](const BDesc& a, const BDesc& c) {
-    if (a.g != c.g) return a.g < c.g;
-    return (int64_t)(a.G - 1 - a.g) * lag + a.T > (int64_t)(c.G - 1 - c.g) * lag + c.T;
-  });
-  std::stable_sort(pl.bwd.begin(), pl.bwd.end(), [&](const BDesc& a, const BDesc& c) {
-    if (a.G - a.g != c.G - c.g) return a.G - a.g < c.G - c.g;
-    return (int64_t)a.g * lag + a.T > (int64_t)c.g * lag + c.T;
-  });
+  for (int x = 0; x < kQueues; ++x) {
+    const std::vector<int>& qp = queue[x];
+    size_t a = 0;
+    while (a < qp.size()) {
+      size_t e = a;
+      double fed = 0.0;
+      int maxG = 0;
+      while (e < qp.size() && (e == a || fed < feed)) {
+        fed += (p->h_y_len[qp[e]] + R) / lag;
+        maxG = std::max(maxG, pb[qp[e]].G);
+        ++e;
+      }
+      for (int layer = 0; layer < maxG; ++layer)
+        for (size_t q = a; q < e; ++q) {
+          const int b = qp[q], G = pb[b].G;
+          if (layer >= G) continue;
+          pl.fwd.push_back(fwd_all[first[b] + layer]);
+          pl.bwd.push_back(bwd_all[first[b] + G - 1 - layer]);
+        }
+      a = e;
+    }
+    pl.qoff_f[x + 1] = (int32_t)pl.fwd.size();
+    pl.qoff_b[x + 1] = (int32_t)pl.bwd.size();
+  }
   pl.desc_b = align256(sizeof(BDesc) * std::max<size_t>(pl.fwd.size() + pl.bwd.size(), 1));
   pl.data_b = (size_t)off * 8;
   // offsets of the data region, relative to ws (in doubles)
@@ -893,6 +1284,8 @@ int tkfmi_band_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
   memcpy(blob.data(), pl.bidx.data(), pl.NBP * 4);
   memcpy(blob.data() + pl.NBP * 4, pl.btype.data(), pl.NBP * 4);
   memcpy(blob.data() + 2 * pl.NBP * 4, pl.bgrp.data(), pl.NBP * 4);
+  memcpy(blob.data() + 3 * pl.NBP * 4, pl.qoff_f.data(), (kQueues + 1) * 4);
+  memcpy(blob.data() + (3 * pl.NBP + kQueues + 1) * 4, pl.qoff_b.data(), (kQueues + 1) * 4);
   memcpy(blob.data() + pl.idx_b, pl.fwd.data(), sizeof(BDesc) * nf);
   memcpy(blob.data() + pl.idx_b + sizeof(BDesc) * nf, pl.bwd.data(), sizeof(BDesc) * nbw);
   int rc = (flags & TKFMI_REUSE_DESCRIPTORS) ? TKFMI_OK : tkfmi_stage_upload(wsb, blob.data(), blob.size(), stream, err);
@@ -919,7 +1312,7 @@ int tkfmi_band_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
   prm.bidx = d_idx; prm.btype = d_idx + pl.NBP; prm.bgrp = d_idx + 2 * pl.NBP;
   prm.x_cat = p->x_cat; prm.y_cat = p->y_cat; prm.log_emit = p->log_emit;
   prm.ws = wsd;
-  prm.flagv = flagv; prm.abortf = flagv + 2;
+  prm.flagv = flagv; prm.abortf = flagv + 2 * kQueues;
   prm.pinfo = pinfo;
   prm.out_logP = o->log_prob;
   prm.out_n = (flags & TKFMI_NO_COUNTS) ? nullptr : o->n_trans;
@@ -930,7 +1323,8 @@ int tkfmi_band_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
   const bool dense = p->log_emit != nullptr;
   for (int dir = 0; dir < ((flags & TKFMI_FORWARD_ONLY) ? 1 : 2); ++dir) {
     BParams pc = prm;
-    pc.ticket = flagv + dir;
+    pc.ticket = flagv + dir * kQueues;
+    pc.qoff = d_idx + 3 * pl.NBP + dir * (kQueues + 1);
     pc.desc = reinterpret_cast<const BDesc*>(wsb + pl.idx_b) + (dir ? nf : 0);
     const std::string name = std::string(dir ? "band_bwd_kernel<" : "band_fwd_kernel<") + std::to_string(pl.NT) + ">";
     const int slot = tkfmi_ktime_start(name.c_str(), B, cells, stream, err);
