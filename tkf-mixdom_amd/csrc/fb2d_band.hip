@@ -65,6 +65,8 @@ constexpr int CQ = 16;                 // boundary columns per published chunk
 constexpr int kYR = 256;               // LDS ring of descendant codes (chunks of 128)
 constexpr int kQueues = 8;             // ticket queues, one per XCD
 constexpr int kSpinMax = 1 << 22;      // bounded waits (each spin sleeps ~0.2 us + a load)
+constexpr int kEcMaxG = 40;            // emission groups up to which counts are binned in LDS
+constexpr int kEcM = A_PAD * A_PAD;    // M entries (x, y) of a group's table
 constexpr double LN2 = 0.69314718055994530942;
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -72,7 +74,11 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // Geometry per padded width NBP = 16 NT: rows per band R = 16 RT, the
 // transition contraction's K (NBP / 4 chunks of 4) split into KS slices, so
 // that NT x KS blocks (column tile, slice) spread evenly over the 4 SIMDs.
-__host__ __device__ constexpr int band_rt(int NT) { return NT == 2 ? 4 : (NT == 4 ? 2 : 1); }
+__host__ __device__ constexpr int band_rt(int NT) { return NT == 2 ? 2 : 1; }
+// Workgroups per compute unit: two for up to 64 padded states (their
+// per-step chains are latency-bound; a second workgroup fills the gaps),
+// one above (LDS and registers).
+__host__ __device__ constexpr int band_occ(int NT) { return NT <= 4 ? 2 : 1; }
 __host__ __device__ constexpr int band_ks(int NT) { return NT == 8 ? 1 : 2; }
 template <int NT>
 struct Geo {
@@ -95,11 +101,14 @@ struct BDesc {
   int64_t pm_off, tab_off;     // doubles into ws: transition block, probability-space tables
   int64_t x_off, y_off, post_off;
   int64_t acc_off;             // doubles into ws: the pair's count accumulator [NBP][NBP]
+  int64_t mb_off;              // doubles into ws: the band's M-state posterior masses [T][R][nM]
 };
 
 struct BParams {
   int32_t S, NB, NBP, flags;
-  int32_t e_idx, G, spin_max, pad;
+  int32_t e_idx, G, spin_max, nM;
+  int32_t ec_mode, pad1, pad2, pad3;  // emission counts: 0 none, 1 binned (LDS + M masses), 2 atomics
+  const int32_t* mrank;  // [NBP] rank of slot k among the M-type slots (-1: other types)
   const int32_t* bidx;   // [NBP] state index of emitting slot k (padding: 0)
   const int32_t* btype;  // [NBP] type (0 for padding)
   const int32_t* bgrp;   // [NBP] emission group
@@ -338,7 +347,8 @@ __device__ __forceinline__ void load_y(uint8_t* s_y, const uint8_t* ys, int c0, 
 
 // ====================================================================== forward
 template <int NT, bool DENSE>
-__global__ __launch_bounds__(kThr) void band_fwd_kernel(BParams Pr) {
+__global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_occ(NT), 8))) void band_fwd_kernel(
+    BParams Pr) {
   using Gm = Geo<NT>;
   constexpr int NBP = Gm::NBP, NBPP = Gm::NBPP, KR = Gm::KR, R = Gm::R, CPW = Gm::CPW;
   constexpr int KS = Gm::KS, KB = Gm::KB, NBLK = Gm::NBLK, BPW = Gm::BPW;
@@ -627,7 +637,8 @@ template <int NT>
 __host__ __device__ constexpr bool bwd_breg() { return NT <= 6; }
 
 template <int NT, bool DENSE>
-__global__ __launch_bounds__(kThr) void band_bwd_kernel(BParams Pr) {
+__global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_occ(NT), 8))) void band_bwd_kernel(
+    BParams Pr) {
   using Gm = Geo<NT>;
   constexpr int NBP = Gm::NBP, NBPP = Gm::NBPP, KR = Gm::KR, R = Gm::R, CPW = Gm::CPW;
   constexpr int KS = Gm::KS, KB = Gm::KB, NBLK = Gm::NBLK, BPW = Gm::BPW, NTILE = Gm::NTILE;
@@ -655,7 +666,15 @@ __global__ __launch_bounds__(kThr) void band_bwd_kernel(BParams Pr) {
   const uint8_t* ys = DENSE ? nullptr : Pr.y_cat + pd.y_off;
   const bool want_counts = !(Pr.flags & TKFMI_NO_COUNTS) && Pr.out_n != nullptr;
   const bool want_post = (Pr.flags & TKFMI_WANT_POSTERIORS) && Pr.out_post != nullptr;
-  const bool want_ecnt = !DENSE && (Pr.flags & TKFMI_WANT_EMIT_COUNTS) && Pr.out_ecnt != nullptr;
+  const bool want_ecnt = !DENSE && Pr.ec_mode != 0 && Pr.out_ecnt != nullptr;
+  const bool ec_bin = want_ecnt && Pr.ec_mode == 1;
+  // binned emission counts of this band: I-type entries [G][A_PAD] (by the
+  // descendant residue), D-type [R][G] (by the row, whose ancestor residue
+  // is fixed); M-type masses go to the band's buffer [T][R][nM]
+  extern __shared__ double s_ec[];
+  double* s_ecI = s_ec;
+  double* s_ecD = s_ec + (size_t)Pr.G * A_PAD;
+  double* mbuf = ec_bin ? Pr.ws + pd.mb_off : nullptr;
   const double* lat = Pr.ws + pd.lat_off;
   const double* latE = lat + (size_t)pd.T * R * NBP;
   const double* bin = pd.bin_off >= 0 ? Pr.ws + pd.bin_off : nullptr;
@@ -693,6 +712,8 @@ __global__ __launch_bounds__(kThr) void band_bwd_kernel(BParams Pr) {
     (&s_FS[0][0])[q] = 0.0;
   }
   for (int q = threadIdx.x; q < NBP; q += kThr) s_last[q] = 0.0;
+  if (ec_bin)
+    for (int q = threadIdx.x; q < Pr.G * (A_PAD + R); q += kThr) s_ec[q] = 0.0;
   if (threadIdx.x == 0) { s_ok = 1; s_lastE = 0; }
   if (threadIdx.x < 2 * R) {
     (&s_eM[0][0])[threadIdx.x] = EXP_NONE;
@@ -927,8 +948,17 @@ __global__ __launch_bounds__(kThr) void band_bwd_kernel(BParams Pr) {
           // before its normalisation (relative to 2^Eg; the corner: Eg = 0)
           const double pv = s_FS[rr][k] * Bv[c][q];
           if (want_post) post[cellS + sx[q]] = pv;
-          if (want_ecnt && pv != 0.0 && e != 0.0)
+          if (ec_bin) {
+            const double pe = (e != 0.0) ? pv : 0.0;
+            if (ty[q] == ST_M) {
+              if (i >= 1 && j >= 1) mbuf[((size_t)(pd.T - 1 - t) * R + r) * Pr.nM + Pr.mrank[k]] = pe;
+            } else if (pe != 0.0) {
+              if (ty[q] == ST_I) atomicAdd(s_ecI + gr[q] * A_PAD + yj, pe);
+              else atomicAdd(s_ecD + r * Pr.G + gr[q], pe);
+            }
+          } else if (want_ecnt && pv != 0.0 && e != 0.0) {
             atomicAdd(ecnt + (size_t)gr[q] * TAB_SZ + emit_entry(ty[q], xr[c], yj), pv);
+          }
         }
         // push q: M-type to (i-1, j-1), I-type to (i, j-1), D-type to (i-1, j)
         if (ty[q] == ST_I) s_pre[sl1][rr][k] = qv;
@@ -959,6 +989,19 @@ __global__ __launch_bounds__(kThr) void band_bwd_kernel(BParams Pr) {
       for (int k = lane; k < NBP; k += 64) bnd_store(e + k, s_last[k]);
       if (lane == 0) bnd_store(e + NBP, (double)s_lastE);
       publish(Pr.flagv + pd.fout, jjp + 1);
+    }
+  }
+  if (ec_bin) {  // this band's binned I- and D-type emission counts
+    __syncthreads();
+    for (int q = threadIdx.x; q < Pr.G * A_PAD; q += kThr) {
+      const double v = s_ecI[q];
+      if (v != 0.0) atomicAdd(ecnt + (size_t)(q / A_PAD) * TAB_SZ + kEcM + q % A_PAD, v);
+    }
+    for (int q = threadIdx.x; q < nr * Pr.G; q += kThr) {
+      const double v = s_ecD[q];
+      const int r = q / Pr.G, i = i0 + r;
+      if (v != 0.0 && i >= 1)
+        atomicAdd(ecnt + (size_t)(q % Pr.G) * TAB_SZ + kEcM + A_PAD + min((int)xs[i - 1], A_PAD - 1), v);
     }
   }
   if (!want_counts) {
@@ -1006,6 +1049,50 @@ __global__ __launch_bounds__(kThr) void band_bwd_kernel(BParams Pr) {
   }
 }
 
+// M-type emission counts of one pair (one workgroup per pair): the
+// backward's per-cell M masses binned by (group, ancestor residue,
+// descendant residue) in LDS, then written to the pair's tables (their M
+// entries; the I and D entries come from the bands' atomics).
+struct EcPair {
+  int32_t pair, Lx, Ly, G;
+  int64_t x_off, y_off, mb_off;  // mb_off: band 0's masses; band g's follow band g-1's
+};
+
+__global__ __launch_bounds__(512) void band_ecnt_m_kernel(const EcPair* __restrict__ ep, int R, int nM, int G,
+                                                          const int32_t* __restrict__ mgrp,
+                                                          const uint8_t* __restrict__ x_cat,
+                                                          const uint8_t* __restrict__ y_cat,
+                                                          const double* __restrict__ ws, const double* __restrict__ pinfo,
+                                                          double* __restrict__ out_ecnt) {
+  extern __shared__ double s_acc[];  // [G][A_PAD * A_PAD]
+  const EcPair e = ep[blockIdx.x];
+  if (!(pinfo[2 * e.pair] > 0.0)) return;  // impossible pair: its backward wrote no masses
+  for (int q = threadIdx.x; q < G * kEcM; q += blockDim.x) s_acc[q] = 0.0;
+  __syncthreads();
+  const uint8_t* xs = x_cat + e.x_off;
+  const uint8_t* ys = y_cat + e.y_off;
+  const double* mb = ws + e.mb_off;
+  for (int g = 0; g < e.G; ++g) {
+    const int nr = min(R, e.Lx + 1 - g * R), T = e.Ly + nr;
+    const int64_t n = (int64_t)T * R * nM;
+    for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
+      const int m = (int)(q % nM);
+      const int64_t cell = q / nM;
+      const int r = (int)(cell % R), t = (int)(cell / R);
+      const int i = g * R + r, j = t - r;
+      if (r >= nr || j < 1 || j > e.Ly || i < 1) continue;
+      const double v = mb[q];
+      if (v == 0.0) continue;
+      const int x = min((int)xs[i - 1], A_PAD - 1), y = min((int)ys[j - 1], A_PAD - 1);
+      atomicAdd(s_acc + mgrp[m] * kEcM + x * A_PAD + y, v);
+    }
+    mb += n;
+  }
+  __syncthreads();
+  double* out = out_ecnt + (size_t)e.pair * G * TAB_SZ;
+  for (int q = threadIdx.x; q < G * kEcM; q += blockDim.x) out[(size_t)(q / kEcM) * TAB_SZ + q % kEcM] = s_acc[q];
+}
+
 // status[0] = 1 if the call's abort flag was raised.
 __global__ void raise_status_kernel(const int32_t* __restrict__ abortf, int32_t* __restrict__ status) {
   if (threadIdx.x == 0 && __hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
@@ -1027,8 +1114,11 @@ struct BPlan {
   std::string why;
   int S = 0, NB = 0, NT = 0, NBP = 0, R = 0, e_idx = -1, G = 0;
   std::vector<int32_t> bidx, btype, bgrp;
+  std::vector<int32_t> mrank, mgrp;     // [NBP] M rank of slot k; [NBP] group of M rank m
   std::vector<int32_t> qoff_f, qoff_b;  // kQueues + 1 each
   std::vector<BDesc> fwd, bwd;
+  std::vector<EcPair> ec;               // binned emission counts: one entry per pair
+  int nM = 0, ec_mode = 0;
   int n_flags = 0;
   size_t idx_b = 0, desc_b = 0, pm_b = 0, tab_b = 0, pinfo_b = 0, flag_b = 0, data_b = 0;
   size_t total() const { return idx_b + desc_b + pm_b + tab_b + pinfo_b + flag_b + data_b; }
@@ -1083,7 +1173,15 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
   const bool fwd_only = (flags & TKFMI_FORWARD_ONLY) != 0;
   const bool counts = !fwd_only && !(flags & TKFMI_NO_COUNTS);
   const int NBP = pl.NBP;
-  pl.idx_b = align256((3 * (size_t)NBP + 2 * (kQueues + 1)) * 4);
+  // emission counts: binned (I and D types in LDS per band, M masses per
+  // cell binned per pair) up to kEcMaxG groups, else atomics per cell
+  const bool ecnt = (flags & TKFMI_WANT_EMIT_COUNTS) && !fwd_only && !dense;
+  pl.ec_mode = !ecnt ? 0 : (pl.G <= kEcMaxG ? 1 : 2);
+  pl.mrank.assign(NBP, -1);
+  pl.mgrp.assign(NBP, 0);
+  for (int k = 0; k < pl.NB; ++k)
+    if (pl.btype[k] == ST_M) { pl.mgrp[pl.nM] = pl.bgrp[k]; pl.mrank[k] = pl.nM++; }
+  pl.idx_b = align256((5 * (size_t)NBP + 2 * (kQueues + 1)) * 4);
   pl.pm_b = align256((size_t)pm_stride(NBP) * 8 * p->n_trans);
   pl.tab_b = dense ? 0 : align256((size_t)p->n_tab * pl.G * TAB_SZ * 8);
   pl.pinfo_b = align256((size_t)std::max(B, 1) * 16);
@@ -1098,7 +1196,7 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
   // per pair: lattice, band boundaries, count accumulator; flags
   int64_t off = 0;  // doubles into the data region
   int nbnd = 0;
-  struct PairBands { int G; int64_t lat, bnd0, acc; int flag0; double cost; };
+  struct PairBands { int G; int64_t lat, bnd0, acc, mb; int flag0; double cost; };
   std::vector<PairBands> pb(B);
   for (int b = 0; b < B; ++b) {
     const int lx = p->h_x_len[b], ly = p->h_y_len[b];
@@ -1111,6 +1209,8 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
     off += (int64_t)(G - 1) * (ly + 1) * (NBP + 1);
     pb[b].acc = off;
     if (counts && G > 1) off += (int64_t)NBP * NBP;
+    pb[b].mb = off;
+    if (pl.ec_mode == 1) off += ((int64_t)G * ly + lx + 1) * R * pl.nM;
     pb[b].flag0 = kFlagHead + 3 * nbnd;
     nbnd += G - 1;
     pb[b].cost = (double)G * (ly + R);
@@ -1136,7 +1236,7 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
     queue[x].push_back(b);
     load[x] += pb[b].cost;
   }
-  const double lag = R + CQ, feed = 1.5 * cus_per_queue();
+  const double lag = R + CQ, feed = 1.5 * cus_per_queue() * band_occ(pl.NT);
   pl.qoff_f.assign(kQueues + 1, 0);
   pl.qoff_b.assign(kQueues + 1, 0);
   std::vector<BDesc> fwd_all, bwd_all;  // per pair, bands 0..G-1
@@ -1147,7 +1247,14 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
       const int64_t bsz = (int64_t)(ly + 1) * (NBP + 1);
       const int fl = pb[b].flag0;
       first[b] = (int64_t)fwd_all.size();
-      int64_t lat_band = pb[b].lat;
+      int64_t lat_band = pb[b].lat, mb_band = pb[b].mb;
+      if (pl.ec_mode == 1) {
+        EcPair e;
+        memset(&e, 0, sizeof(e));
+        e.pair = b; e.Lx = lx; e.Ly = ly; e.G = G;
+        e.x_off = p->h_x_off[b]; e.y_off = p->h_y_off[b]; e.mb_off = pb[b].mb;
+        pl.ec.push_back(e);
+      }
       for (int g = 0; g < G; ++g) {
         BDesc d;
         memset(&d, 0, sizeof(d));
@@ -1155,6 +1262,8 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
         d.g = g; d.G = G; d.T = ly + d.nr;
         d.lat_off = lat_band;
         lat_band += (int64_t)d.T * R * (NBP + 1);
+        d.mb_off = mb_band;
+        mb_band += (int64_t)d.T * R * pl.nM;
         d.pm_off = (p->n_trans > 1 ? (int64_t)b : 0) * pm_stride(NBP);
         d.tab_off = (!dense && p->n_tab > 1 ? (int64_t)b : 0) * pl.G * TAB_SZ;
         d.x_off = p->h_x_off[b]; d.y_off = p->h_y_off[b];
@@ -1203,7 +1312,8 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
     pl.qoff_f[x + 1] = (int32_t)pl.fwd.size();
     pl.qoff_b[x + 1] = (int32_t)pl.bwd.size();
   }
-  pl.desc_b = align256(sizeof(BDesc) * std::max<size_t>(pl.fwd.size() + pl.bwd.size(), 1));
+  pl.desc_b = align256(sizeof(BDesc) * std::max<size_t>(pl.fwd.size() + pl.bwd.size(), 1) +
+                       sizeof(EcPair) * pl.ec.size());
   pl.data_b = (size_t)off * 8;
   // offsets of the data region, relative to ws (in doubles)
   const int64_t data0 = (int64_t)(pl.idx_b + pl.desc_b + pl.pm_b + pl.tab_b + pl.pinfo_b + pl.flag_b) / 8;
@@ -1215,31 +1325,41 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
       if (d.bin_off >= 0) d.bin_off += data0;
       if (d.bout_off >= 0) d.bout_off += data0;
       d.acc_off += data0;
+      d.mb_off += data0;
       d.pm_off += pm0;
       d.tab_off += tab0;
     }
+  for (EcPair& e : pl.ec) e.mb_off += data0;
   pl.ok = true;
   return pl;
 }
 
 template <int NT, bool DENSE>
-int launch_nt(bool bwd, const BParams& prm, int n, hipStream_t s, std::string& err) {
+int launch_nt(bool bwd, const BParams& prm, int n, size_t dyn, hipStream_t s, std::string& err) {
   if (n == 0) return TKFMI_OK;
-  if (bwd) hipLaunchKernelGGL((band_bwd_kernel<NT, DENSE>), dim3(n), dim3(kThr), 0, s, prm);
-  else hipLaunchKernelGGL((band_fwd_kernel<NT, DENSE>), dim3(n), dim3(kThr), 0, s, prm);
+  if (bwd) {
+    if (dyn > 0) {
+      const hipError_t ea = hipFuncSetAttribute((const void*)band_bwd_kernel<NT, DENSE>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+      if (ea != hipSuccess) { err = std::string("hipFuncSetAttribute: ") + hipGetErrorString(ea); return TKFMI_EHIP; }
+    }
+    hipLaunchKernelGGL((band_bwd_kernel<NT, DENSE>), dim3(n), dim3(kThr), dyn, s, prm);
+  } else {
+    hipLaunchKernelGGL((band_fwd_kernel<NT, DENSE>), dim3(n), dim3(kThr), 0, s, prm);
+  }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) { err = std::string("band launch: ") + hipGetErrorString(e); return TKFMI_EHIP; }
   return TKFMI_OK;
 }
 
 template <bool DENSE>
-int launch_d(int NT, bool bwd, const BParams& prm, int n, hipStream_t s, std::string& err) {
+int launch_d(int NT, bool bwd, const BParams& prm, int n, size_t dyn, hipStream_t s, std::string& err) {
   switch (NT) {
-    case 2: return launch_nt<2, DENSE>(bwd, prm, n, s, err);
-    case 4: return launch_nt<4, DENSE>(bwd, prm, n, s, err);
-    case 6: return launch_nt<6, DENSE>(bwd, prm, n, s, err);
-    case 8: return launch_nt<8, DENSE>(bwd, prm, n, s, err);
-    default: return launch_nt<10, DENSE>(bwd, prm, n, s, err);
+    case 2: return launch_nt<2, DENSE>(bwd, prm, n, dyn, s, err);
+    case 4: return launch_nt<4, DENSE>(bwd, prm, n, dyn, s, err);
+    case 6: return launch_nt<6, DENSE>(bwd, prm, n, dyn, s, err);
+    case 8: return launch_nt<8, DENSE>(bwd, prm, n, dyn, s, err);
+    default: return launch_nt<10, DENSE>(bwd, prm, n, dyn, s, err);
   }
 }
 
@@ -1286,8 +1406,11 @@ int tkfmi_band_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
   memcpy(blob.data() + 2 * pl.NBP * 4, pl.bgrp.data(), pl.NBP * 4);
   memcpy(blob.data() + 3 * pl.NBP * 4, pl.qoff_f.data(), (kQueues + 1) * 4);
   memcpy(blob.data() + (3 * pl.NBP + kQueues + 1) * 4, pl.qoff_b.data(), (kQueues + 1) * 4);
+  memcpy(blob.data() + (3 * pl.NBP + 2 * (kQueues + 1)) * 4, pl.mrank.data(), pl.NBP * 4);
+  memcpy(blob.data() + (4 * pl.NBP + 2 * (kQueues + 1)) * 4, pl.mgrp.data(), pl.NBP * 4);
   memcpy(blob.data() + pl.idx_b, pl.fwd.data(), sizeof(BDesc) * nf);
   memcpy(blob.data() + pl.idx_b + sizeof(BDesc) * nf, pl.bwd.data(), sizeof(BDesc) * nbw);
+  memcpy(blob.data() + pl.idx_b + sizeof(BDesc) * (nf + nbw), pl.ec.data(), sizeof(EcPair) * pl.ec.size());
   int rc = (flags & TKFMI_REUSE_DESCRIPTORS) ? TKFMI_OK : tkfmi_stage_upload(wsb, blob.data(), blob.size(), stream, err);
   if (rc != TKFMI_OK) return rc;
   const int32_t* d_idx = reinterpret_cast<const int32_t*>(wsb);
@@ -1304,7 +1427,8 @@ int tkfmi_band_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
   BParams prm;
   memset(&prm, 0, sizeof(prm));
   prm.S = p->S; prm.NB = pl.NB; prm.NBP = pl.NBP; prm.flags = (int32_t)flags;
-  prm.e_idx = pl.e_idx; prm.G = pl.G;
+  prm.e_idx = pl.e_idx; prm.G = pl.G; prm.nM = pl.nM; prm.ec_mode = pl.ec_mode;
+  prm.mrank = d_idx + 3 * pl.NBP + 2 * (kQueues + 1);
   {
     const char* e = getenv("TKFMI_CHAIN_SPIN_LIMIT");
     prm.spin_max = e ? std::max(0, atoi(e)) : kSpinMax;
@@ -1329,9 +1453,23 @@ int tkfmi_band_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
     const std::string name = std::string(dir ? "band_bwd_kernel<" : "band_fwd_kernel<") + std::to_string(pl.NT) + ">";
     const int slot = tkfmi_ktime_start(name.c_str(), B, cells, stream, err);
     if (slot == -2) return TKFMI_EHIP;
-    rc = dense ? launch_d<true>(pl.NT, dir == 1, pc, (int)(dir ? nbw : nf), stream, err)
-               : launch_d<false>(pl.NT, dir == 1, pc, (int)(dir ? nbw : nf), stream, err);
+    const size_t dyn = (dir == 1 && pl.ec_mode == 1) ? (size_t)pl.G * (A_PAD + pl.R) * 8 : 0;
+    rc = dense ? launch_d<true>(pl.NT, dir == 1, pc, (int)(dir ? nbw : nf), dyn, stream, err)
+               : launch_d<false>(pl.NT, dir == 1, pc, (int)(dir ? nbw : nf), dyn, stream, err);
     if (rc != TKFMI_OK) return rc;
+    if (tkfmi_ktime_stop(slot, stream, err) != TKFMI_OK) return TKFMI_EHIP;
+  }
+  if (pl.ec_mode == 1 && !pl.ec.empty()) {  // M-type emission counts, binned per pair
+    const size_t smem = (size_t)pl.G * kEcM * 8;
+    hipError_t e = hipFuncSetAttribute((const void*)band_ecnt_m_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)smem);
+    if (e != hipSuccess) { err = std::string("hipFuncSetAttribute: ") + hipGetErrorString(e); return TKFMI_EHIP; }
+    const int slot = tkfmi_ktime_start("band_ecnt_m_kernel", B, cells, stream, err);
+    if (slot == -2) return TKFMI_EHIP;
+    hipLaunchKernelGGL(band_ecnt_m_kernel, dim3((unsigned)pl.ec.size()), dim3(512), smem, stream,
+                       reinterpret_cast<const EcPair*>(wsb + pl.idx_b + sizeof(BDesc) * (nf + nbw)), pl.R, pl.nM,
+                       pl.G, d_idx + 4 * pl.NBP + 2 * (kQueues + 1), p->x_cat, p->y_cat, wsd, pinfo, o->emit_counts);
+    if (hipGetLastError() != hipSuccess) { err = "emission-count launch failed"; return TKFMI_EHIP; }
     if (tkfmi_ktime_stop(slot, stream, err) != TKFMI_OK) return TKFMI_EHIP;
   }
   if (o->status) {
