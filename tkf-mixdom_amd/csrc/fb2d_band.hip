@@ -315,7 +315,7 @@ __device__ __forceinline__ void fetch_bnd(double (*s_bin)[NBP + 1], const double
                                           bool reversed, int Ly) {
   static_assert(CQ == 2 * kWaves, "two entries per wave");
   constexpr int K1 = (NBP + 1 + 63) / 64;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // w: wave-uniform (SGPR)
   double v[2][K1];
 #pragma unroll
   for (int e = 0; e < 2; ++e) {
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
   __shared__ int s_eM[2][R], s_eI[2][R], s_eD[2][R], s_E[R];
   __shared__ int s_lastE, s_ok;             // their exponent; cleared when a boundary wait gave up
   __shared__ uint8_t s_y[DENSE ? 1 : kYR];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // w: wave-uniform (SGPR)
   const BDesc pd = Pr.desc[take_band(Pr.ticket, Pr.qoff, Pr.abortf)];
   const int Lx = pd.Lx, Ly = pd.Ly, W1 = Ly + 1, NB = Pr.NB, S = Pr.S, nr = pd.nr, i0 = pd.i0;
   const double* pm = Pr.ws + pd.pm_off;
@@ -500,10 +500,10 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
         for (int ks = 1; ks < KS; ++ks) pre += s_pre[sl][ks][r][k];
         const double bmv = bM[k], bdv = bD[k];
         const int tq = ty[q];
-        double v = 0.0;
-        if (tq == ST_M) v = okM ? (top ? bmv : pre) : 0.0;
-        else if (tq == ST_I) v = okI ? pre : 0.0;
-        else if (tq == ST_D) v = okD ? (top ? bdv : pre) : 0.0;
+        double v = 0.0;  // selects, no branch per type
+        v = (tq == ST_I && okI) ? pre : v;
+        v = (tq == ST_M && okM) ? (top ? bmv : pre) : v;
+        v = (tq == ST_D && okD) ? (top ? bdv : pre) : v;
         const int et = tq == ST_M ? eM : (tq == ST_I ? eI : eD);
         double e;
         if constexpr (DENSE) e = (v != 0.0) ? exp(en[c][q]) : 0.0;
@@ -589,9 +589,13 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
           const int j = t - r;
           if (r >= nr || j < 0 || j > Ly || tk == 0) continue;
           const double h = (origin_step && r == 0) ? psb[u] : acc[u][v];
-          if (tk == ST_I) s_pre[sl1][ks][r][k] = h;
-          else if (r + 1 < nr) s_pre[tk == ST_M ? sl : sl1][ks][r + 1][k] = h;
-          else s_last[ks][k] = h;
+          // one store, its address chosen by selects: I-type to (r, j+1)
+          // (slot t+1), M-type to (r+1, j+1) (slot t+2), D-type to (r+1, j)
+          // (slot t+1); the last row's M and D pushes to the boundary row
+          const bool last = tk != ST_I && r + 1 >= nr;
+          double* dst = last ? &s_last[ks][k]
+                             : &s_pre[tk == ST_M ? sl : sl1][ks][tk == ST_I ? r : r + 1][k];
+          *dst = h;
         }
       }
     }
@@ -652,7 +656,7 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
   __shared__ int s_eM[2][R], s_eI[2][R], s_eD[2][R], s_Eg[R];
   __shared__ int s_lastE, s_ok;         // their exponent; cleared when a boundary wait gave up
   __shared__ uint8_t s_y[DENSE ? 1 : kYR];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // w: wave-uniform (SGPR)
   const BDesc pd = Pr.desc[take_band(Pr.ticket, Pr.qoff, Pr.abortf)];
   const int Lx = pd.Lx, Ly = pd.Ly, W1 = Ly + 1, NB = Pr.NB, S = Pr.S, nr = pd.nr, i0 = pd.i0;
   const double* pm = Pr.ws + pd.pm_off;
@@ -824,10 +828,12 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
         const int kc = min(k, NBP - 1);
         const double pre = s_pre[sl][rr][kc], bmv = bM[kc], bdv = bD[kc];
         const int tq = ty[q];
-        double v = 0.0;
-        if (tq == ST_M) v = okM ? ldexp(bot ? bmv : pre, eM - Eg) : 0.0;
-        else if (tq == ST_I) v = okI ? ldexp(pre, eI - Eg) : 0.0;
-        else if (tq == ST_D) v = okD ? ldexp(bot ? bdv : pre, eD - Eg) : 0.0;
+        double v = 0.0;  // selects, no branch per type
+        v = (tq == ST_I && okI) ? pre : v;
+        v = (tq == ST_M && okM) ? (bot ? bmv : pre) : v;
+        v = (tq == ST_D && okD) ? (bot ? bdv : pre) : v;
+        const int et = tq == ST_M ? eM : (tq == ST_I ? eI : eD);
+        v = ldexp(v, et - Eg);
         g[q] = v;
         if (k < NBP) {
           s_G[rr][k] = v;
@@ -961,11 +967,11 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
           }
         }
         // push q: M-type to (i-1, j-1), I-type to (i, j-1), D-type to (i-1, j)
-        if (ty[q] == ST_I) s_pre[sl1][rr][k] = qv;
-        else if (ty[q] == ST_M || ty[q] == ST_D) {
-          if (rr + 1 < nr) s_pre[ty[q] == ST_M ? sl : sl1][rr + 1][k] = qv;
-          else s_last[k] = qv;
-        }
+        // (one store, its address chosen by selects)
+        const int tq = ty[q];
+        const bool last = tq != ST_I && rr + 1 >= nr;
+        double* dst = last ? &s_last[k] : &s_pre[tq == ST_M ? sl : sl1][tq == ST_I ? rr : rr + 1][k];
+        if (tq != 0) *dst = qv;
       }
       if (lane == 0) {
         s_eI[sl1][rr] = EB;
