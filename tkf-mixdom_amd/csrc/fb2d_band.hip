@@ -84,6 +84,10 @@ template <int NT>
 struct Geo {
   static constexpr int NBP = 16 * NT, NBPP = NBP + 1, KR = (NBP + 63) / 64;
   static constexpr int RT = band_rt(NT), R = 16 * RT, CPW = R / kWaves;
+  // elementwise phases: LPC lanes per cell, CPI cells per wave instruction
+  // (32 padded states: two cells per instruction, lanes 0-31 and 32-63),
+  // NG groups of CPI cells per wave and step
+  static constexpr int LPC = NBP <= 32 ? 32 : 64, CPI = 64 / LPC, NG = CPW / CPI;
   static constexpr int KS = band_ks(NT), KCH = NBP / 4, KB = KCH / KS;
   static constexpr int NBLK = NT * KS, BPW = (NBLK + kWaves - 1) / kWaves;
   static_assert(kWaves % KS == 0, "a wave's blocks share one slice (ks = w mod KS)");
@@ -130,33 +134,40 @@ struct BParams {
 
 __host__ __device__ inline int64_t pm_stride(int NBP) { return 2 * (int64_t)NBP * NBP + 2 * NBP + 8; }
 
-__device__ inline double wave_sum(double v) {
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
 __device__ inline int frexp_exp(double v) { return __builtin_amdgcn_frexp_exp(v); }
 
-// Max over the wave of an int, in DPP steps (row shifts within each row of
-// 16 lanes, then the row broadcasts of lanes 15 and 31), read from lane 63.
-// Lanes whose DPP source is outside the row keep their own value, which max
-// leaves unchanged.
+// One step of a max over lanes with DPP (row shifts within each row of 16
+// lanes, row broadcasts of lanes 15 and 31); lanes whose DPP source is
+// outside the row keep their own value, which max leaves unchanged.
 template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ int dpp_max_step(int v) {
   return max(v, __builtin_amdgcn_update_dpp(v, v, CTRL, ROW_MASK, 0xf, false));
 }
-__device__ __forceinline__ int wave_max_i32(int v) {
+
+// Binary exponent of the largest positive value over the lanes of a cell
+// (the wave, or its half when CPI = 2; frexp convention), or EXP_NONE if no
+// lane holds one.
+template <int CPI>
+__device__ __forceinline__ int cell_max_exp(double m) {
+  int v = m > 0.0 ? frexp_exp(m) : EXP_NONE;
   v = dpp_max_step<0x111, 0xf>(v);  // row_shr:1
   v = dpp_max_step<0x112, 0xf>(v);  // row_shr:2
   v = dpp_max_step<0x114, 0xf>(v);  // row_shr:4
   v = dpp_max_step<0x118, 0xf>(v);  // row_shr:8
   v = dpp_max_step<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
-  v = dpp_max_step<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
-  return __builtin_amdgcn_readlane(v, 63);
+  if constexpr (CPI == 1) {
+    v = dpp_max_step<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return __builtin_amdgcn_readlane(v, 63);
+  } else {
+    const int a = __builtin_amdgcn_readlane(v, 31), b = __builtin_amdgcn_readlane(v, 63);
+    return (threadIdx.x & 32) ? b : a;
+  }
 }
-// Binary exponent of the largest positive value over the wave (frexp
-// convention), or EXP_NONE if no lane holds one.
-__device__ __forceinline__ int wave_max_exp(double m) {
-  return wave_max_i32(m > 0.0 ? frexp_exp(m) : EXP_NONE);
+// Sum over the lanes of a cell (the wave, or its half when CPI = 2).
+template <int CPI>
+__device__ __forceinline__ double cell_sum(double v) {
+  for (int o = 32 / CPI; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  return v;
 }
 
 __device__ inline int emit_entry(int ty, int xi, int yj) {
@@ -378,10 +389,13 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
   const double* bin = pd.bin_off >= 0 ? Pr.ws + pd.bin_off : nullptr;
   double* bout = pd.bout_off >= 0 ? Pr.ws + pd.bout_off : nullptr;
   double* out_n = want_counts ? Pr.out_n + (size_t)pd.pair * S * S : nullptr;
+  constexpr int LPC = Gm::LPC, CPI = Gm::CPI, NG = Gm::NG;
+  const int hc = (CPI == 2) ? (lane >> 5) : 0;  // the lane's cell within a group
+  const int kl = lane & (LPC - 1);              // the lane's state slot (q = 0)
   int ty[KR], gr[KR], sx[KR];
 #pragma unroll
   for (int q = 0; q < KR; ++q) {
-    const int k = lane + 64 * q;
+    const int k = kl + 64 * q;
     ty[q] = (k < NB) ? Pr.btype[k] : 0;
     gr[q] = (k < NB) ? Pr.bgrp[k] : 0;
     sx[q] = (k < NB) ? Pr.bidx[k] : 0;
@@ -399,11 +413,12 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
     tkb[u] = (b < NBLK && k < NB) ? Pr.btype[k] : 0;
     psb[u] = (b < NBLK && k < NB && b % KS == 0) ? p_start[k] : 0.0;
   }
-  int xr[CPW];  // ancestor codes of the wave's rows
+  // the lane's row in group cg: r = w + kWaves (CPI cg + hc)
+  int xr[NG];  // ancestor codes of the lane's rows
 #pragma unroll
-  for (int c = 0; c < CPW; ++c) {
-    const int i = i0 + w + kWaves * c;
-    xr[c] = (!DENSE && w + kWaves * c < nr && i >= 1) ? (int)min((int)xs[i - 1], A_PAD - 1) : 0;
+  for (int cg = 0; cg < NG; ++cg) {
+    const int r = w + kWaves * (CPI * cg + hc), i = i0 + r;
+    xr[cg] = (!DENSE && r < nr && i >= 1) ? (int)min((int)xs[i - 1], A_PAD - 1) : 0;
   }
   for (int q = threadIdx.x; q < 2 * KS * R * NBPP; q += kThr) (&s_pre[0][0][0][0])[q] = 0.0;
   for (int q = threadIdx.x; q < R * NBPP; q += kThr) (&s_F[0][0])[q] = 0.0;
@@ -416,18 +431,18 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
   }
   if constexpr (!DENSE) load_y(s_y, ys, 0, Ly);
   __syncthreads();
-  // emissions (dense: log emissions) of the wave's cells at step tt
-  double en[CPW][KR];
+  // emissions (dense: log emissions) of the lane's cells at step tt
+  double en[NG][KR];
   auto prefetch = [&](int tt) {
 #pragma unroll
-    for (int c = 0; c < CPW; ++c) {
-      const int r = w + kWaves * c, j = tt - r, i = i0 + r;
+    for (int cg = 0; cg < NG; ++cg) {
+      const int r = w + kWaves * (CPI * cg + hc), j = tt - r, i = i0 + r;
       const bool in = r < nr && j >= 0 && j <= Ly;
       const int yj = (!DENSE && in && j >= 1) ? (int)min((int)s_y[(j - 1) & (kYR - 1)], A_PAD - 1) : 0;
 #pragma unroll
       for (int q = 0; q < KR; ++q) {
-        if constexpr (DENSE) en[c][q] = (in && ty[q] != 0) ? dem[((size_t)i * W1 + j) * S + sx[q]] : 0.0;
-        else en[c][q] = in ? emit(tab, ty[q], gr[q], xr[c], yj, i, j) : 0.0;
+        if constexpr (DENSE) en[cg][q] = (in && ty[q] != 0) ? dem[((size_t)i * W1 + j) * S + sx[q]] : 0.0;
+        else en[cg][q] = in ? emit(tab, ty[q], gr[q], xr[cg], yj, i, j) : 0.0;
       }
     }
   };
@@ -468,36 +483,37 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
       }
       if (lane == 0) bnd_store(e + NBP, (double)s_lastE);
     }
-    // ---- A: the wave's cells of this step, in stages across the cells:
-    // F' = v e 2^(e_type - E0) (E0: the largest predecessor exponent), the
-    // cell's exponent from the largest F', then F normalised
-    double Fp[CPW][KR];
-    int E0c[CPW], ex[CPW];
+    // ---- A: the wave's cells of this step (CPI per instruction), in stages
+    // across the groups: F' = v e 2^(e_type - E0) (E0: the largest
+    // predecessor exponent), the cell's exponent from the largest F', then
+    // F normalised
+    double Fp[NG][KR];
+    int E0c[NG], ex[NG];
 #pragma unroll
-    for (int c = 0; c < CPW; ++c) {
-      const int r = w + kWaves * c;
+    for (int cg = 0; cg < NG; ++cg) {
+      const int r = w + kWaves * (CPI * cg + hc);
       const int j = t - r, i = i0 + r;
       const bool in = r < nr && j >= 0 && j <= Ly;
       const bool top = (r == 0) && bin != nullptr;
       const double* bM = s_bin[(j + CQ) % (CQ + 1)];  // entry j - 1
       const double* bD = s_bin[j % (CQ + 1)];
+      const int rc = min(r, R - 1);
       const bool okM = in && i >= 1 && j >= 1, okI = in && j >= 1, okD = in && i >= 1;
       // predecessor exponents: M from (i-1, j-1), I from (i, j-1), D from (i-1, j)
-      int eM = EXP_NONE, eI = EXP_NONE, eD = EXP_NONE;
-      if (okM) eM = top ? (int)bM[NBP] : s_eM[sl][r];
-      if (okI) eI = s_eI[sl][r];
-      if (okD) eD = top ? (int)bD[NBP] : s_eD[sl][r];
+      const int eM = okM ? (top ? (int)bM[NBP] : s_eM[sl][rc]) : EXP_NONE;
+      const int eI = okI ? s_eI[sl][rc] : EXP_NONE;
+      const int eD = okD ? (top ? (int)bD[NBP] : s_eD[sl][rc]) : EXP_NONE;
       const int E0 = max(eM, max(eI, eD));
-      E0c[c] = E0;
+      E0c[cg] = E0;
       double m = 0.0;
 #pragma unroll
       for (int q = 0; q < KR; ++q) {
         // every read unconditional (row r's pushes, the boundary entries),
         // the lane's value chosen by selects
-        const int k = min(lane + 64 * q, NBP - 1);
-        double pre = s_pre[sl][0][r][k];
+        const int k = min(kl + 64 * q, NBP - 1);
+        double pre = s_pre[sl][0][rc][k];
 #pragma unroll
-        for (int ks = 1; ks < KS; ++ks) pre += s_pre[sl][ks][r][k];
+        for (int ks = 1; ks < KS; ++ks) pre += s_pre[sl][ks][rc][k];
         const double bmv = bM[k], bdv = bD[k];
         const int tq = ty[q];
         double v = 0.0;  // selects, no branch per type
@@ -506,64 +522,65 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
         v = (tq == ST_D && okD) ? (top ? bdv : pre) : v;
         const int et = tq == ST_M ? eM : (tq == ST_I ? eI : eD);
         double e;
-        if constexpr (DENSE) e = (v != 0.0) ? exp(en[c][q]) : 0.0;
-        else e = en[c][q];
+        if constexpr (DENSE) e = (v != 0.0) ? exp(en[cg][q]) : 0.0;
+        else e = en[cg][q];
         const double f = (v != 0.0) ? ldexp(v * e, et - E0) : 0.0;
-        Fp[c][q] = f;
+        Fp[cg][q] = f;
         m = fmax(m, f);
       }
-      ex[c] = wave_max_exp(m);
+      ex[cg] = cell_max_exp<CPI>(m);
     }
 #pragma unroll
-    for (int c = 0; c < CPW; ++c) {
-      const int r = w + kWaves * c;
+    for (int cg = 0; cg < NG; ++cg) {
+      const int r = w + kWaves * (CPI * cg + hc);
       const int j = t - r, i = i0 + r;
-      if (r >= nr || j < 0 || j > Ly) {
-        for (int k = lane; k < NBP; k += 64) s_F[r][k] = 0.0;
-        continue;
-      }
+      const bool in = r < nr && j >= 0 && j <= Ly;
+      const int rc = min(r, R - 1);
       const bool origin = (i == 0 && j == 0);
-      const bool any = ex[c] != EXP_NONE;
-      const int sh = any ? ex[c] : 0;
-      const int E = (any && !origin) ? E0c[c] + sh : 0;
+      const bool any = ex[cg] != EXP_NONE;
+      const int sh = any ? ex[cg] : 0;
+      const int E = (any && !origin) ? E0c[cg] + sh : 0;
       double F[KR];
 #pragma unroll
       for (int q = 0; q < KR; ++q) {
-        const int k = lane + 64 * q;
-        F[q] = origin ? 0.0 : ldexp(Fp[c][q], -sh);
-        if (k < NBP) {
-          s_F[r][k] = F[q];
-          if (want_lat) lat[((size_t)t * R + r) * NBP + k] = F[q];
+        const int k = kl + 64 * q;
+        F[q] = (origin || !in) ? 0.0 : ldexp(Fp[cg][q], -sh);
+        if (k < NBP && r < R) {
+          s_F[rc][k] = F[q];
+          if (want_lat && in) lat[((size_t)t * R + r) * NBP + k] = F[q];
         }
       }
-      if (lane == 0) {
+      if (kl == 0 && in) {
         s_E[r] = E;
         if (want_lat) latE[(size_t)t * R + r] = (double)E;
       }
-      if (i == Lx && j == Ly) {  // log P and the E column
+      const bool corner = in && i == Lx && j == Ly;
+      if (__builtin_amdgcn_ballot_w64(corner)) {  // log P and the E column (the corner's cell)
         double part = 0.0;
 #pragma unroll
         for (int q = 0; q < KR; ++q) {
-          const int k = lane + 64 * q;
+          const int k = kl + 64 * q;
           if (k < NB) part = fma(F[q], p_end[k], part);
         }
-        double tot = wave_sum(part);
-        double lp;
-        if (origin) { tot = p_se; lp = log(p_se); }
-        else lp = log(tot) + (double)E * LN2;
-        if (!ok_run) { tot = 0.0; lp = __builtin_nan(""); }
-        if (lane == 0) {
-          Pr.out_logP[pd.pair] = (tot > 0.0 || !ok_run) ? lp : -1e30;
-          Pr.pinfo[2 * pd.pair] = (tot > 0.0) ? ldexp(tot, -frexp_exp(tot)) : 0.0;
-          Pr.pinfo[2 * pd.pair + 1] = (tot > 0.0) ? (double)(E + frexp_exp(tot)) : 0.0;
-        }
-        if (want_counts && tot > 0.0) {
-          if (origin) { if (lane == 0) out_n[Pr.e_idx] = 1.0; }
-          else {
+        double tot = cell_sum<CPI>(part);
+        if (corner) {
+          double lp;
+          if (origin) { tot = p_se; lp = log(p_se); }
+          else lp = log(tot) + (double)E * LN2;
+          if (!ok_run) { tot = 0.0; lp = __builtin_nan(""); }
+          if (kl == 0) {
+            Pr.out_logP[pd.pair] = (tot > 0.0 || !ok_run) ? lp : -1e30;
+            Pr.pinfo[2 * pd.pair] = (tot > 0.0) ? ldexp(tot, -frexp_exp(tot)) : 0.0;
+            Pr.pinfo[2 * pd.pair + 1] = (tot > 0.0) ? (double)(E + frexp_exp(tot)) : 0.0;
+          }
+          if (want_counts && tot > 0.0) {
+            if (origin) { if (kl == 0) out_n[Pr.e_idx] = 1.0; }
+            else {
 #pragma unroll
-            for (int q = 0; q < KR; ++q) {
-              const int k = lane + 64 * q;
-              if (k < NB) out_n[Pr.bidx[k] * S + Pr.e_idx] = F[q] * p_end[k] / tot;
+              for (int q = 0; q < KR; ++q) {
+                const int k = kl + 64 * q;
+                if (k < NB) out_n[Pr.bidx[k] * S + Pr.e_idx] = F[q] * p_end[k] / tot;
+              }
             }
           }
         }
@@ -690,10 +707,13 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
   const int EP = (int)Pr.pinfo[2 * pd.pair + 1];
   if (!(mP > 0.0)) return;  // impossible or aborted pair: every band of it returns here
   const double inv_mP = 1.0 / mP;
+  constexpr int LPC = Gm::LPC, CPI = Gm::CPI, NG = Gm::NG;
+  const int hc = (CPI == 2) ? (lane >> 5) : 0;  // the lane's cell within a group
+  const int kl = lane & (LPC - 1);              // the lane's state slot (q = 0)
   int ty[KR], gr[KR], sx[KR];
 #pragma unroll
   for (int q = 0; q < KR; ++q) {
-    const int k = lane + 64 * q;
+    const int k = kl + 64 * q;
     ty[q] = (k < NB) ? Pr.btype[k] : 0;
     gr[q] = (k < NB) ? Pr.bgrp[k] : 0;
     sx[q] = (k < NB) ? Pr.bidx[k] : 0;
@@ -704,11 +724,12 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
     for (int u = 0; u < BPW; ++u)
       if (w + kWaves * u < NBLK) load_frag<NT>(PT, w + kWaves * u, lane, fr[u]);
   }
-  int xr[CPW];  // ancestor codes of the wave's rows (reversed: rr = w + 8 c)
+  // the lane's reversed row in group cg: rr = w + kWaves (CPI cg + hc)
+  int xr[NG];  // ancestor codes of the lane's rows
 #pragma unroll
-  for (int c = 0; c < CPW; ++c) {
-    const int rr = w + kWaves * c, i = i0 + nr - 1 - rr;
-    xr[c] = (!DENSE && rr < nr && i >= 1) ? (int)min((int)xs[i - 1], A_PAD - 1) : 0;
+  for (int cg = 0; cg < NG; ++cg) {
+    const int rr = w + kWaves * (CPI * cg + hc), i = i0 + nr - 1 - rr;
+    xr[cg] = (!DENSE && rr < nr && i >= 1) ? (int)min((int)xs[i - 1], A_PAD - 1) : 0;
   }
   for (int q = threadIdx.x; q < 2 * R * NBPP; q += kThr) (&s_pre[0][0][0])[q] = 0.0;
   for (int q = threadIdx.x; q < R * NBPP; q += kThr) {
@@ -739,34 +760,34 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
   __syncthreads();
   // forward values and exponent of the wave's cells at step tt (the
   // forward's step T - 1 - tt, row nr - 1 - rr)
-  double lf[CPW][KR];
-  int lfE[CPW];
+  double lf[NG][KR];
+  int lfE[NG];
   auto prefetch_lat = [&](int tt) {
 #pragma unroll
-    for (int c = 0; c < CPW; ++c) {
-      const int rr = w + kWaves * c, jj = tt - rr;
+    for (int cg = 0; cg < NG; ++cg) {
+      const int rr = w + kWaves * (CPI * cg + hc), jj = tt - rr;
       const bool in = rr < nr && jj >= 0 && jj <= Ly;
       const size_t cell = (size_t)(pd.T - 1 - tt) * R + (nr - 1 - rr);
 #pragma unroll
       for (int q = 0; q < KR; ++q) {
-        const int k = lane + 64 * q;
-        lf[c][q] = (in && k < NBP) ? lat[cell * NBP + k] : 0.0;
+        const int k = kl + 64 * q;
+        lf[cg][q] = (in && k < NBP) ? lat[cell * NBP + k] : 0.0;
       }
-      lfE[c] = in ? (int)latE[cell] : 0;
+      lfE[cg] = in ? (int)latE[cell] : 0;
     }
   };
-  // emissions (dense: log emissions) of the wave's cells at step tt
-  double en[CPW][KR];
+  // emissions (dense: log emissions) of the lane's cells at step tt
+  double en[NG][KR];
   auto prefetch_em = [&](int tt) {
 #pragma unroll
-    for (int c = 0; c < CPW; ++c) {
-      const int rr = w + kWaves * c, jj = tt - rr, i = i0 + nr - 1 - rr, j = Ly - jj;
+    for (int cg = 0; cg < NG; ++cg) {
+      const int rr = w + kWaves * (CPI * cg + hc), jj = tt - rr, i = i0 + nr - 1 - rr, j = Ly - jj;
       const bool in = rr < nr && jj >= 0 && jj <= Ly;
       const int yj = (!DENSE && in && j >= 1) ? (int)min((int)s_y[jj & (kYR - 1)], A_PAD - 1) : 0;
 #pragma unroll
       for (int q = 0; q < KR; ++q) {
-        if constexpr (DENSE) en[c][q] = (in && ty[q] != 0) ? dem[((size_t)i * W1 + j) * S + sx[q]] : 0.0;
-        else en[c][q] = in ? emit(tab, ty[q], gr[q], xr[c], yj, i, j) : 0.0;
+        if constexpr (DENSE) en[cg][q] = (in && ty[q] != 0) ? dem[((size_t)i * W1 + j) * S + sx[q]] : 0.0;
+        else en[cg][q] = in ? emit(tab, ty[q], gr[q], xr[cg], yj, i, j) : 0.0;
       }
     }
   };
@@ -803,28 +824,24 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
     }
     // ---- A: g and the scaled forward values of this step's cells
 #pragma unroll
-    for (int c = 0; c < CPW; ++c) {
-      const int rr = w + kWaves * c;
+    for (int cg = 0; cg < NG; ++cg) {
+      const int rr = w + kWaves * (CPI * cg + hc);
       const int jj = t - rr, r = nr - 1 - rr, i = i0 + r, j = Ly - jj;
-      if (rr >= nr || jj < 0 || jj > Ly) {
-        for (int k = lane; k < NBP; k += 64) { s_G[rr][k] = 0.0; s_FS[rr][k] = 0.0; }
-        continue;
-      }
+      const bool in = rr < nr && jj >= 0 && jj <= Ly;
       const bool bot = (rr == 0) && bin != nullptr;
       const double* bM = s_bin[(jj + CQ) % (CQ + 1)];  // column j + 1 (reversed entry jj - 1)
       const double* bD = s_bin[jj % (CQ + 1)];
-      const bool okM = i < Lx && j < Ly, okI = j < Ly, okD = i < Lx;
-      int eM = EXP_NONE, eI = EXP_NONE, eD = EXP_NONE;
-      if (okM) eM = bot ? (int)bM[NBP] : s_eM[sl][rr];
-      if (okI) eI = s_eI[sl][rr];
-      if (okD) eD = bot ? (int)bD[NBP] : s_eD[sl][rr];
+      const bool okM = in && i < Lx && j < Ly, okI = in && j < Ly, okD = in && i < Lx;
+      const int eM = okM ? (bot ? (int)bM[NBP] : s_eM[sl][rr]) : EXP_NONE;
+      const int eI = okI ? s_eI[sl][rr] : EXP_NONE;
+      const int eD = okD ? (bot ? (int)bD[NBP] : s_eD[sl][rr]) : EXP_NONE;
       int Eg = max(eM, max(eI, eD));
       if (Eg == EXP_NONE) Eg = 0;
-      const double fsc = ldexp(inv_mP, lfE[c] + Eg - EP);
+      const double fsc = ldexp(inv_mP, lfE[cg] + Eg - EP);
       double g[KR];
 #pragma unroll
       for (int q = 0; q < KR; ++q) {
-        const int k = lane + 64 * q;
+        const int k = kl + 64 * q;
         const int kc = min(k, NBP - 1);
         const double pre = s_pre[sl][rr][kc], bmv = bM[kc], bdv = bD[kc];
         const int tq = ty[q];
@@ -837,23 +854,24 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
         g[q] = v;
         if (k < NBP) {
           s_G[rr][k] = v;
-          s_FS[rr][k] = lf[c][q] * fsc;
+          s_FS[rr][k] = in ? lf[cg][q] * fsc : 0.0;
         }
       }
-      if (lane == 0) s_Eg[rr] = Eg;
-      if (i == 0 && j == 0) {  // start state: S -> body counts, posterior of S at (0, 0)
+      if (kl == 0) s_Eg[rr] = Eg;
+      const bool start = in && i == 0 && j == 0;
+      if (__builtin_amdgcn_ballot_w64(start)) {  // start state: S -> body counts, posterior of S at (0, 0)
         const double scale = ldexp(inv_mP, Eg - EP);
         double part = 0.0;
 #pragma unroll
         for (int q = 0; q < KR; ++q) {
-          const int k = lane + 64 * q;
+          const int k = kl + 64 * q;
           if (k < NB) {
             part = fma(p_start[k], g[q], part);
-            if (want_counts) out_n[Pr.bidx[k]] = p_start[k] * g[q] * scale;
+            if (want_counts && start) out_n[Pr.bidx[k]] = p_start[k] * g[q] * scale;
           }
         }
-        part = wave_sum(part);
-        if (want_post && lane == 0) post[0] = (i == Lx && j == Ly) ? 1.0 : part * scale;
+        part = cell_sum<CPI>(part);
+        if (want_post && start && kl == 0) post[0] = (i == Lx && j == Ly) ? 1.0 : part * scale;
       }
     }
     if (t + 1 < pd.T) prefetch_lat(t + 1);
@@ -901,58 +919,55 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
     __syncthreads();
     if (put && (((jjp + 1) % CQ) == 0 || jjp == Ly)) publish(Pr.flagv + pd.fout, jjp + 1);
     // ---- C: normalise B, posteriors, emission counts, q = e . B to the
-    // predecessors; B's exponents of the wave's cells first, together
-    double Bv[CPW][KR];
-    int ex[CPW];
+    // predecessors; B's exponents of the lane's cells first, together
+    double Bv[NG][KR];
+    int ex[NG];
 #pragma unroll
-    for (int c = 0; c < CPW; ++c) {
-      const int rr = w + kWaves * c;
+    for (int cg = 0; cg < NG; ++cg) {
+      const int rr = w + kWaves * (CPI * cg + hc);
       const int jj = t - rr, i = i0 + nr - 1 - rr, j = Ly - jj;
       const bool in = rr < nr && jj >= 0 && jj <= Ly;
       const bool corner = (i == Lx && j == Ly);
       double mx = 0.0;
 #pragma unroll
       for (int q = 0; q < KR; ++q) {
-        const int k = lane + 64 * q;
-        double bsum = 0.0;
-        if (in && k < NBP) {
-          if (corner) bsum = (k < NB) ? p_end[k] : 0.0;
-          else {
-            bsum = s_B[0][rr][k];
+        const int k = kl + 64 * q;
+        const int kc = min(k, NBP - 1);
+        double bsum = s_B[0][rr][kc];
 #pragma unroll
-            for (int ks = 1; ks < KS; ++ks) bsum += s_B[ks][rr][k];
-          }
-        }
-        Bv[c][q] = bsum;
+        for (int ks = 1; ks < KS; ++ks) bsum += s_B[ks][rr][kc];
+        if (corner) bsum = (k < NB) ? p_end[kc] : 0.0;
+        bsum = (in && k < NBP) ? bsum : 0.0;
+        Bv[cg][q] = bsum;
         mx = fmax(mx, bsum);
       }
-      ex[c] = wave_max_exp(mx);
+      ex[cg] = cell_max_exp<CPI>(mx);
     }
 #pragma unroll
-    for (int c = 0; c < CPW; ++c) {
-      const int rr = w + kWaves * c;
+    for (int cg = 0; cg < NG; ++cg) {
+      const int rr = w + kWaves * (CPI * cg + hc);
       const int jj = t - rr, r = nr - 1 - rr, i = i0 + r, j = Ly - jj;
-      if (rr >= nr || jj < 0 || jj > Ly) continue;
+      const bool in = rr < nr && jj >= 0 && jj <= Ly;
       const bool corner = (i == Lx && j == Ly);
-      const int sh = (ex[c] != EXP_NONE) ? ex[c] : 0;
+      const int sh = (ex[cg] != EXP_NONE) ? ex[cg] : 0;
       const int EB = (corner ? 0 : s_Eg[rr]) + sh;
-      const int yj = (!DENSE && j >= 1) ? (int)min((int)s_y[jj & (kYR - 1)], A_PAD - 1) : 0;
+      const int yj = (!DENSE && in && j >= 1) ? (int)min((int)s_y[jj & (kYR - 1)], A_PAD - 1) : 0;
       const size_t cellS = ((size_t)i * W1 + j) * S;
 #pragma unroll
       for (int q = 0; q < KR; ++q) {
-        const int k = lane + 64 * q;
-        if (k >= NBP) continue;
-        const double b = ldexp(Bv[c][q], -sh);
+        const int k = kl + 64 * q;
+        if (k >= NBP || !in) continue;
+        const double b = ldexp(Bv[cg][q], -sh);
         double e = 0.0;
         if (k < NB) {
-          if constexpr (DENSE) e = emit_ok(ty[q], i, j) ? exp(en[c][q]) : 0.0;
-          else e = en[c][q];
+          if constexpr (DENSE) e = emit_ok(ty[q], i, j) ? exp(en[cg][q]) : 0.0;
+          else e = en[cg][q];
         }
         const double qv = e * b;
         if (k < NB && (want_post || want_ecnt)) {
           // F B / P(x, y) = Fs (scaled by 2^(EF + Eg - EP) / mP) times B
           // before its normalisation (relative to 2^Eg; the corner: Eg = 0)
-          const double pv = s_FS[rr][k] * Bv[c][q];
+          const double pv = s_FS[rr][k] * Bv[cg][q];
           if (want_post) post[cellS + sx[q]] = pv;
           if (ec_bin) {
             const double pe = (e != 0.0) ? pv : 0.0;
@@ -963,7 +978,7 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
               else atomicAdd(s_ecD + r * Pr.G + gr[q], pe);
             }
           } else if (want_ecnt && pv != 0.0 && e != 0.0) {
-            atomicAdd(ecnt + (size_t)gr[q] * TAB_SZ + emit_entry(ty[q], xr[c], yj), pv);
+            atomicAdd(ecnt + (size_t)gr[q] * TAB_SZ + emit_entry(ty[q], xr[cg], yj), pv);
           }
         }
         // push q: M-type to (i-1, j-1), I-type to (i, j-1), D-type to (i-1, j)
@@ -973,7 +988,7 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
         double* dst = last ? &s_last[k] : &s_pre[tq == ST_M ? sl : sl1][tq == ST_I ? rr : rr + 1][k];
         if (tq != 0) *dst = qv;
       }
-      if (lane == 0) {
+      if (kl == 0 && in) {
         s_eI[sl1][rr] = EB;
         if (rr + 1 < nr) {
           s_eD[sl1][rr + 1] = EB;
@@ -1078,21 +1093,27 @@ __global__ __launch_bounds__(512) void band_ecnt_m_kernel(const EcPair* __restri
   const uint8_t* xs = x_cat + e.x_off;
   const uint8_t* ys = y_cat + e.y_off;
   const double* mb = ws + e.mb_off;
+  // thread -> (cell slot, M rank): nMp = nM rounded up to a power of two,
+  // 512 / nMp cells per pass; R is a power of two (16, 32)
+  int nMp = 1;
+  while (nMp < nM) nMp <<= 1;
+  const int m = threadIdx.x & (nMp - 1), cs = threadIdx.x / nMp, cpp = blockDim.x / nMp;
+  const int lr = __builtin_ctz(R);
+  const int gm = (m < nM) ? mgrp[m] : 0;
   for (int g = 0; g < e.G; ++g) {
-    const int nr = min(R, e.Lx + 1 - g * R), T = e.Ly + nr;
-    const int64_t n = (int64_t)T * R * nM;
-    for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
-      const int m = (int)(q % nM);
-      const int64_t cell = q / nM;
-      const int r = (int)(cell % R), t = (int)(cell / R);
-      const int i = g * R + r, j = t - r;
-      if (r >= nr || j < 1 || j > e.Ly || i < 1) continue;
-      const double v = mb[q];
-      if (v == 0.0) continue;
-      const int x = min((int)xs[i - 1], A_PAD - 1), y = min((int)ys[j - 1], A_PAD - 1);
-      atomicAdd(s_acc + mgrp[m] * kEcM + x * A_PAD + y, v);
+    const int nr = min(R, e.Lx + 1 - g * R), T = e.Ly + nr, ncell = T * R;
+    if (m < nM) {
+      for (int cell = cs; cell < ncell; cell += cpp) {
+        const int r = cell & (R - 1), t = cell >> lr;
+        const int i = g * R + r, j = t - r;
+        if (r >= nr || j < 1 || j > e.Ly || i < 1) continue;
+        const double v = mb[cell * nM + m];
+        if (v == 0.0) continue;
+        const int x = min((int)xs[i - 1], A_PAD - 1), y = min((int)ys[j - 1], A_PAD - 1);
+        atomicAdd(s_acc + gm * kEcM + x * A_PAD + y, v);
+      }
     }
-    mb += n;
+    mb += (size_t)ncell * nM;
   }
   __syncthreads();
   double* out = out_ecnt + (size_t)e.pair * G * TAB_SZ;
