@@ -79,8 +79,10 @@ __host__ __device__ constexpr int band_rt(int NT) { return NT == 2 ? 2 : 1; }
 // per-step chains are latency-bound; a second workgroup fills the gaps),
 // one above (LDS and registers).
 __host__ __device__ constexpr int band_occ(int NT) { return NT <= 4 ? 2 : 1; }
-__host__ __device__ constexpr int band_ks(int NT) { return NT == 8 ? 1 : 2; }
-template <int NT>
+// K slices of the forward's and the backward's products (the backward keeps
+// one for 32 padded states: its LDS holds the emission-count tables too)
+__host__ __device__ constexpr int band_ks(int NT, bool bwd) { return (NT == 8 || (bwd && NT == 2)) ? 1 : 2; }
+template <int NT, bool BWD = false>
 struct Geo {
   static constexpr int NBP = 16 * NT, NBPP = NBP + 1, KR = (NBP + 63) / 64;
   static constexpr int RT = band_rt(NT), R = 16 * RT, CPW = R / kWaves;
@@ -88,9 +90,17 @@ struct Geo {
   // (32 padded states: two cells per instruction, lanes 0-31 and 32-63),
   // NG groups of CPI cells per wave and step
   static constexpr int LPC = NBP <= 32 ? 32 : 64, CPI = 64 / LPC, NG = CPW / CPI;
-  static constexpr int KS = band_ks(NT), KCH = NBP / 4, KB = KCH / KS;
+  static constexpr int KS = band_ks(NT, BWD), KCH = NBP / 4, KB = KCH / KS;
   static constexpr int NBLK = NT * KS, BPW = (NBLK + kWaves - 1) / kWaves;
   static_assert(kWaves % KS == 0, "a wave's blocks share one slice (ks = w mod KS)");
+  // fewer blocks than waves (32 padded states): wave w takes block w mod
+  // NBLK for the row tiles rt = w / NBLK (mod RPW), so every wave has work
+  static constexpr bool SPLIT = NBLK < kWaves;
+  static constexpr int RPW = SPLIT ? kWaves / NBLK : 1;
+  static_assert(!SPLIT || kWaves % NBLK == 0, "blocks divide the waves");
+  __host__ __device__ static constexpr int block(int w, int u) { return SPLIT ? w % NBLK : w + kWaves * u; }
+  __host__ __device__ static constexpr int nblocks(int w) { return SPLIT ? 1 : (NBLK - w + kWaves - 1) / kWaves; }
+  __host__ __device__ static constexpr int rt0(int w) { return SPLIT ? w / NBLK : 0; }
   static constexpr int NTILE = (NT * NT + kWaves - 1) / kWaves;  // count tiles per wave
 };
 
@@ -111,7 +121,8 @@ struct BDesc {
 struct BParams {
   int32_t S, NB, NBP, flags;
   int32_t e_idx, G, spin_max, nM;
-  int32_t ec_mode, pad1, pad2, pad3;  // emission counts: 0 none, 1 binned (LDS + M masses), 2 atomics
+  int32_t ec_mode, pad1, pad2, pad3;  // emission counts: 0 none, 1 binned (I, D in LDS; M masses
+                                      // to a buffer), 2 atomics, 3 binned (I, D, M in LDS)
   const int32_t* mrank;  // [NBP] rank of slot k among the M-type slots (-1: other types)
   const int32_t* bidx;   // [NBP] state index of emitting slot k (padding: 0)
   const int32_t* btype;  // [NBP] type (0 for padding)
@@ -253,11 +264,10 @@ __device__ __forceinline__ int take_band(int32_t* counters, const int32_t* qoff,
 // All of wave w's blocks (w + kWaves u, u < nb; they share the slice
 // ks = w mod KS, hence the A operands) for row tile rt, interleaved so that
 // the wave has nb independent accumulation chains.
-template <int NT, int NBW>
+template <class Gm, int NBW>
 __device__ __forceinline__ void wave_gemm(const double* __restrict__ sA, int rt, int ks,
-                                          const double (&fr)[NBW][Geo<NT>::KB], int nb, d4 (&acc)[NBW],
+                                          const double (&fr)[NBW][Gm::KB], int nb, d4 (&acc)[NBW],
                                           int lane) {
-  using Gm = Geo<NT>;
   static_assert(Gm::KB % 4 == 0, "chunks in batches of 4");
   const double* ap = sA + (16 * rt + (lane & 15)) * Gm::NBPP + 4 * ks * Gm::KB + (lane >> 4);
 #pragma unroll
@@ -281,10 +291,9 @@ __device__ __forceinline__ void wave_gemm(const double* __restrict__ sA, int rt,
   }
 }
 
-template <int NT>
+template <class Gm>
 __device__ __forceinline__ d4 block_gemm_stream(const double* __restrict__ sA, int rt, int nt, int ks,
                                                 const double* __restrict__ Bm, int lane) {
-  using Gm = Geo<NT>;
   static_assert(Gm::KB % 4 == 0, "chunks in batches of 4");
   const double* ap = sA + (16 * rt + (lane & 15)) * Gm::NBPP + 4 * ks * Gm::KB + (lane >> 4);
   const double* bp = Bm + (size_t)(4 * ks * Gm::KB + (lane >> 4)) * Gm::NBP + 16 * nt + (lane & 15);
@@ -307,9 +316,8 @@ __device__ __forceinline__ d4 block_gemm_stream(const double* __restrict__ sA, i
   return acc;
 }
 
-template <int NT>
+template <class Gm>
 __device__ __forceinline__ void load_frag(const double* __restrict__ Bm, int b, int lane, double* fr) {
-  using Gm = Geo<NT>;
   const int nt = b / Gm::KS, ks = b - nt * Gm::KS;
   const double* bp = Bm + (size_t)(4 * ks * Gm::KB + (lane >> 4)) * Gm::NBP + 16 * nt + (lane & 15);
 #pragma unroll
@@ -405,11 +413,11 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
   double fr[BPW][KB];
   int tkb[BPW];
   double psb[BPW];  // the origin's pushes p(S -> k) for the slice-0 blocks
-  const int nbw = (NBLK - w + kWaves - 1) / kWaves;  // the wave's blocks
+  const int nbw = Gm::nblocks(w);  // the wave's blocks
 #pragma unroll
   for (int u = 0; u < BPW; ++u) {
-    const int b = w + kWaves * u, k = 16 * (b / KS) + (lane & 15);
-    if (b < NBLK) load_frag<NT>(P, b, lane, fr[u]);
+    const int b = Gm::block(w, u), k = 16 * (b / KS) + (lane & 15);
+    if (b < NBLK) load_frag<Gm>(P, b, lane, fr[u]);
     tkb[u] = (b < NBLK && k < NB) ? Pr.btype[k] : 0;
     psb[u] = (b < NBLK && k < NB && b % KS == 0) ? p_start[k] : 0.0;
   }
@@ -592,13 +600,13 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
     const bool origin_step = (i0 == 0 && t == 0);
     const int ks = w % KS;
 #pragma unroll
-    for (int rt = 0; rt < Gm::RT; ++rt) {
+    for (int rt = Gm::rt0(w); rt < Gm::RT; rt += Gm::RPW) {
       d4 acc[BPW];
-      wave_gemm<NT, BPW>(&s_F[0][0], rt, ks, fr, nbw, acc, lane);
+      wave_gemm<Gm, BPW>(&s_F[0][0], rt, ks, fr, nbw, acc, lane);
 #pragma unroll
       for (int u = 0; u < BPW; ++u) {
         if (u >= nbw) break;
-        const int k = 16 * ((w + kWaves * u) / KS) + (lane & 15);
+        const int k = 16 * (Gm::block(w, u) / KS) + (lane & 15);
         const int tk = tkb[u];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -660,7 +668,7 @@ __host__ __device__ constexpr bool bwd_breg() { return NT <= 6; }
 template <int NT, bool DENSE>
 __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_occ(NT), 8))) void band_bwd_kernel(
     BParams Pr) {
-  using Gm = Geo<NT>;
+  using Gm = Geo<NT, true>;
   constexpr int NBP = Gm::NBP, NBPP = Gm::NBPP, KR = Gm::KR, R = Gm::R, CPW = Gm::CPW;
   constexpr int KS = Gm::KS, KB = Gm::KB, NBLK = Gm::NBLK, BPW = Gm::BPW, NTILE = Gm::NTILE;
   constexpr bool BREG = bwd_breg<NT>();
@@ -688,14 +696,16 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
   const bool want_counts = !(Pr.flags & TKFMI_NO_COUNTS) && Pr.out_n != nullptr;
   const bool want_post = (Pr.flags & TKFMI_WANT_POSTERIORS) && Pr.out_post != nullptr;
   const bool want_ecnt = !DENSE && Pr.ec_mode != 0 && Pr.out_ecnt != nullptr;
-  const bool ec_bin = want_ecnt && Pr.ec_mode == 1;
+  const bool ec_bin = want_ecnt && (Pr.ec_mode == 1 || Pr.ec_mode == 3);
+  const bool ec_mlds = want_ecnt && Pr.ec_mode == 3;
   // binned emission counts of this band: I-type entries [G][A_PAD] (by the
   // descendant residue), D-type [R][G] (by the row, whose ancestor residue
   // is fixed); M-type masses go to the band's buffer [T][R][nM]
   extern __shared__ double s_ec[];
   double* s_ecI = s_ec;
   double* s_ecD = s_ec + (size_t)Pr.G * A_PAD;
-  double* mbuf = ec_bin ? Pr.ws + pd.mb_off : nullptr;
+  double* s_ecM = s_ec + (size_t)Pr.G * (A_PAD + R);  // [G][A_PAD * A_PAD] (mode 3)
+  double* mbuf = (ec_bin && !ec_mlds) ? Pr.ws + pd.mb_off : nullptr;
   const double* lat = Pr.ws + pd.lat_off;
   const double* latE = lat + (size_t)pd.T * R * NBP;
   const double* bin = pd.bin_off >= 0 ? Pr.ws + pd.bin_off : nullptr;
@@ -722,7 +732,7 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
   if constexpr (BREG) {
 #pragma unroll
     for (int u = 0; u < BPW; ++u)
-      if (w + kWaves * u < NBLK) load_frag<NT>(PT, w + kWaves * u, lane, fr[u]);
+      if (Gm::block(w, u) < NBLK) load_frag<Gm>(PT, Gm::block(w, u), lane, fr[u]);
   }
   // the lane's reversed row in group cg: rr = w + kWaves (CPI cg + hc)
   int xr[NG];  // ancestor codes of the lane's rows
@@ -738,7 +748,7 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
   }
   for (int q = threadIdx.x; q < NBP; q += kThr) s_last[q] = 0.0;
   if (ec_bin)
-    for (int q = threadIdx.x; q < Pr.G * (A_PAD + R); q += kThr) s_ec[q] = 0.0;
+    for (int q = threadIdx.x; q < Pr.G * (A_PAD + R + (ec_mlds ? kEcM : 0)); q += kThr) s_ec[q] = 0.0;
   if (threadIdx.x == 0) { s_ok = 1; s_lastE = 0; }
   if (threadIdx.x < 2 * R) {
     (&s_eM[0][0])[threadIdx.x] = EXP_NONE;
@@ -878,15 +888,15 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
     __syncthreads();
     // ---- B: B = g P^T (partial sums per slice), counts += Fs^T g
     if constexpr (BREG) {
-      const int ks = w % KS, nbw = (NBLK - w + kWaves - 1) / kWaves;
+      const int ks = w % KS, nbw = Gm::nblocks(w);
 #pragma unroll
-      for (int rt = 0; rt < Gm::RT; ++rt) {
+      for (int rt = Gm::rt0(w); rt < Gm::RT; rt += Gm::RPW) {
         d4 acc[BPW];
-        wave_gemm<NT, BPW>(&s_G[0][0], rt, ks, fr, nbw, acc, lane);
+        wave_gemm<Gm, BPW>(&s_G[0][0], rt, ks, fr, nbw, acc, lane);
 #pragma unroll
         for (int u = 0; u < BPW; ++u) {
           if (u >= nbw) break;
-          const int nt = (w + kWaves * u) / KS;
+          const int nt = Gm::block(w, u) / KS;
 #pragma unroll
           for (int v = 0; v < 4; ++v) s_B[ks][16 * rt + 4 * v + (lane >> 4)][16 * nt + (lane & 15)] = acc[u][v];
         }
@@ -897,7 +907,7 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
         const int nt = b / KS, ks = b - nt * KS;
 #pragma unroll 1
         for (int rt = 0; rt < Gm::RT; ++rt) {
-          const d4 acc = block_gemm_stream<NT>(&s_G[0][0], rt, nt, ks, PT, lane);
+          const d4 acc = block_gemm_stream<Gm>(&s_G[0][0], rt, nt, ks, PT, lane);
 #pragma unroll
           for (int v = 0; v < 4; ++v) s_B[ks][16 * rt + 4 * v + (lane >> 4)][16 * nt + (lane & 15)] = acc[v];
         }
@@ -972,7 +982,11 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
           if (ec_bin) {
             const double pe = (e != 0.0) ? pv : 0.0;
             if (ty[q] == ST_M) {
-              if (i >= 1 && j >= 1) mbuf[((size_t)(pd.T - 1 - t) * R + r) * Pr.nM + Pr.mrank[k]] = pe;
+              if (ec_mlds) {
+                if (pe != 0.0) atomicAdd(s_ecM + gr[q] * kEcM + xr[cg] * A_PAD + yj, pe);
+              } else if (i >= 1 && j >= 1) {
+                mbuf[((size_t)(pd.T - 1 - t) * R + r) * Pr.nM + Pr.mrank[k]] = pe;
+              }
             } else if (pe != 0.0) {
               if (ty[q] == ST_I) atomicAdd(s_ecI + gr[q] * A_PAD + yj, pe);
               else atomicAdd(s_ecD + r * Pr.G + gr[q], pe);
@@ -1012,8 +1026,13 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
       publish(Pr.flagv + pd.fout, jjp + 1);
     }
   }
-  if (ec_bin) {  // this band's binned I- and D-type emission counts
+  if (ec_bin) {  // this band's binned I- and D-type (and in mode 3 M-type) emission counts
     __syncthreads();
+    if (ec_mlds)
+      for (int q = threadIdx.x; q < Pr.G * kEcM; q += kThr) {
+        const double v = s_ecM[q];
+        if (v != 0.0) atomicAdd(ecnt + (size_t)(q / kEcM) * TAB_SZ + q % kEcM, v);
+      }
     for (int q = threadIdx.x; q < Pr.G * A_PAD; q += kThr) {
       const double v = s_ecI[q];
       if (v != 0.0) atomicAdd(ecnt + (size_t)(q / A_PAD) * TAB_SZ + kEcM + q % A_PAD, v);
@@ -1103,14 +1122,22 @@ __global__ __launch_bounds__(512) void band_ecnt_m_kernel(const EcPair* __restri
   for (int g = 0; g < e.G; ++g) {
     const int nr = min(R, e.Lx + 1 - g * R), T = e.Ly + nr, ncell = T * R;
     if (m < nM) {
-      for (int cell = cs; cell < ncell; cell += cpp) {
-        const int r = cell & (R - 1), t = cell >> lr;
-        const int i = g * R + r, j = t - r;
-        if (r >= nr || j < 1 || j > e.Ly || i < 1) continue;
-        const double v = mb[cell * nM + m];
-        if (v == 0.0) continue;
-        const int x = min((int)xs[i - 1], A_PAD - 1), y = min((int)ys[j - 1], A_PAD - 1);
-        atomicAdd(s_acc + gm * kEcM + x * A_PAD + y, v);
+      constexpr int U = 4;  // cells per thread and pass, their loads in flight together
+      for (int base = cs; base < ncell; base += U * cpp) {
+        double v[U];
+        int xy[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int cell = base + u * cpp;
+          const int r = cell & (R - 1), t = cell >> lr;
+          const int i = g * R + r, j = t - r;
+          const bool ok = cell < ncell && r < nr && j >= 1 && j <= e.Ly && i >= 1;
+          v[u] = ok ? mb[cell * nM + m] : 0.0;
+          xy[u] = ok ? min((int)xs[i - 1], A_PAD - 1) * A_PAD + min((int)ys[j - 1], A_PAD - 1) : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (v[u] != 0.0) atomicAdd(s_acc + gm * kEcM + xy[u], v[u]);
       }
     }
     mb += (size_t)ncell * nM;
@@ -1135,6 +1162,13 @@ __global__ void zero_ints_kernel(int32_t* p, int n) {
 // abort flag, padding to 4 x kQueues, then forward / backward / accumulator
 // counters per band boundary.
 constexpr int kFlagHead = 4 * kQueues;
+
+// LDS of band_bwd_kernel<NT> before its dynamic emission-count tables (the
+// sizes of its __shared__ arrays, rounded up).
+size_t bwd_static_lds(int NT) {
+  const size_t NBP = 16 * NT, NBPP = NBP + 1, R = 16 * band_rt(NT), KS = band_ks(NT, true);
+  return (2 * R * NBPP + 2 * R * NBPP + KS * R * NBPP + NBP + (CQ + 1) * (NBP + 1)) * 8 + (7 * R + 4) * 4 + kYR + 64;
+}
 
 struct BPlan {
   bool ok = false;
@@ -1203,7 +1237,12 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
   // emission counts: binned (I and D types in LDS per band, M masses per
   // cell binned per pair) up to kEcMaxG groups, else atomics per cell
   const bool ecnt = (flags & TKFMI_WANT_EMIT_COUNTS) && !fwd_only && !dense;
-  pl.ec_mode = !ecnt ? 0 : (pl.G <= kEcMaxG ? 1 : 2);
+  // mode 3 (every type binned in the backward's LDS) where the tables fit
+  // beside the kernel's own LDS at its workgroups per CU; else mode 1 up to
+  // kEcMaxG groups (the M reduction kernel's LDS), else atomics per cell
+  const size_t lds_m3 = bwd_static_lds(pl.NT) + (size_t)pl.G * (A_PAD + pl.R + kEcM) * 8;
+  const bool m3 = lds_m3 + 1024 <= (size_t)(160 * 1024) / band_occ(pl.NT);
+  pl.ec_mode = !ecnt ? 0 : (m3 ? 3 : (pl.G <= kEcMaxG ? 1 : 2));
   pl.mrank.assign(NBP, -1);
   pl.mgrp.assign(NBP, 0);
   for (int k = 0; k < pl.NB; ++k)
@@ -1480,7 +1519,9 @@ int tkfmi_band_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
     const std::string name = std::string(dir ? "band_bwd_kernel<" : "band_fwd_kernel<") + std::to_string(pl.NT) + ">";
     const int slot = tkfmi_ktime_start(name.c_str(), B, cells, stream, err);
     if (slot == -2) return TKFMI_EHIP;
-    const size_t dyn = (dir == 1 && pl.ec_mode == 1) ? (size_t)pl.G * (A_PAD + pl.R) * 8 : 0;
+    const size_t dyn = (dir == 1 && pl.ec_mode == 1)   ? (size_t)pl.G * (A_PAD + pl.R) * 8
+                       : (dir == 1 && pl.ec_mode == 3) ? (size_t)pl.G * (A_PAD + pl.R + kEcM) * 8
+                                                       : 0;
     rc = dense ? launch_d<true>(pl.NT, dir == 1, pc, (int)(dir ? nbw : nf), dyn, stream, err)
                : launch_d<false>(pl.NT, dir == 1, pc, (int)(dir ? nbw : nf), dyn, stream, err);
     if (rc != TKFMI_OK) return rc;
