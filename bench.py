@@ -39,6 +39,7 @@ for _p in (ROOT, os.path.join(ROOT, "tkf-mixdom_amd")):
 METRIC = "fwd+bwd+counts DP cell-states/s, MixDom2 K=3 batched vs plain; p99 latency"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_STATE = 16.0
+CHECKPOINT_BUDGET = 256 << 20  # the library's default per-pair lattice budget
 # HBM bytes per launch, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
 # workload (KiB per dispatch, scaled by the copy-kernel calibration: fetch x2.00,
 # write x1.00), keyed by (launch, pairs in it): profiles/fbc_kernels_c1_mixdom2_k3.md
@@ -240,9 +241,13 @@ def c0_bench(steps):
             "pairs": w.B, "cell_states": cs}
 
 
-def c2_bench(pairs):
-    """c2: MixDom2 K=8 F=4 C=4 (162 states) fwd+bwd+counts on the wide
-    kernel, lengths U{100..1000}; one timed call."""
+def c2_bench(pairs, steps=2, budget=0):
+    """c2: MixDom2 K=8 F=4 C=4 (162 states) forward + backward + transition
+    and emission counts, lengths U{100..1000}, on the banded kernel: with
+    full lattices (checkpoint budget 0: 121 GiB of workspace for the 256
+    pairs), or with the pairs whose lattice exceeds ``budget`` bytes
+    checkpointed (each backward band recomputes its lattice).  One warm-up
+    call, then the mean of ``steps`` timed calls."""
     import numpy as np
     import torch
     from tkfmi import _abi, synth
@@ -250,26 +255,32 @@ def c2_bench(pairs):
     w = synth.make_mixdom2(seed=2, B=pairs, K=8, F=4, C=4, lengths=rng.integers(100, 1001, pairs),
                            name="c2")
     dev = torch.device("cuda", 0)
-    r = _abi.BatchRunner(w.log_trans.to(dev), w.state_types, w.state_group, w.log_tab.to(dev),
-                         _abi.PackedPairs(w.xs, w.ys, dev), time_kernels=True)
-    ms = _time_calls(r.run, 1)
-    kern = {k["name"]: round(k["ms"], 2) for k in _abi.last_kernel_times()}
+    _abi.set_checkpoint_budget(budget)
+    try:
+        r = _abi.BatchRunner(w.log_trans.to(dev), w.state_types, w.state_group, w.log_tab.to(dev),
+                             _abi.PackedPairs(w.xs, w.ys, dev), want_emit_counts=True)
+        ms = _time_calls(r.run, steps)
+        r.flags |= _abi.TIME_KERNELS
+        kern = _kernel_ms(r.run)
+        ws = r.ws_bytes
+        del r
+    finally:
+        _abi.set_checkpoint_budget(_abi.CHECKPOINT_BUDGET_DEFAULT)
+    torch.cuda.empty_cache()
     cs = w.cell_states()
     # 6 S flops per cell-state (forward, backward, count products)
     tf = 6.0 * w.S * cs / (ms / 1e3) / 1e12
-    out = {"value": cs / (ms / 1e3), "unit": "cell-states/s", "ms": ms, "pairs": w.B,
-           "cell_states": cs, "fp64_tflops": tf, "frac_of_fp64_mfma_peak": tf / 78.6,
-           "kernels_ms": kern, "workspace_GiB": r.ws_bytes / 2**30}
-    del r
-    torch.cuda.empty_cache()
-    return out
+    return {"value": cs / (ms / 1e3), "unit": "cell-states/s", "ms": ms, "pairs": w.B,
+            "cell_states": cs, "fp64_tflops": tf, "frac_of_fp64_mfma_peak": tf / 78.6,
+            "emission_counts": True, "checkpoint_budget": budget, "kernels_ms": kern,
+            "workspace_GiB": ws / 2**30}
 
 
 def mid_bench(steps, K=3, F=2):
     """The c1 workload's shape (1024 pairs, LogNormal(ln200,0.5) lengths in
-    [30,800]) on a mid-size MixDom2 layout, K=3 F=2 (32 states, the size of
-    the reference's trained d3f2 models): fwd+bwd+transition and emission
-    counts in one call."""
+    [30,800]) on a mid-size MixDom2 layout: K=3 F=2 (32 states), K=5 F=2 (52)
+    or K=8 F=2 (82), the sizes of the reference's trained models:
+    fwd+bwd+transition and emission counts in one call (banded kernel)."""
     import torch
     from tkfmi import _abi, synth
     w = synth.make_mixdom2(seed=1, B=1024, K=K, F=F, C=1, name=f"c1_k{K}f{F}")
@@ -514,13 +525,19 @@ def main():
             cfg["c3"] = c3_bench(max(1, args.steps // 2))
             _log(f"c3: {cfg['c3']['ms']:.1f} ms per gradient")
         if "mid" in sel:
-            _log("c1-shaped batch at K=3 F=2 (32 states)")
-            cfg["mid_k3f2"] = mid_bench(max(1, args.steps // 2))
-            _log(f"mid K=3 F=2: {cfg['mid_k3f2']['ms']:.1f} ms, {cfg['mid_k3f2']['value']:.3e} cell-states/s")
+            for K in (3, 5, 8):
+                _log(f"c1-shaped batch at K={K} F=2 ({2 + 10 * K} states)")
+                key = f"mid_k{K}f2"
+                cfg[key] = mid_bench(max(1, args.steps // 2), K=K, F=2)
+                _log(f"mid K={K} F=2: {cfg[key]['ms']:.1f} ms, {cfg[key]['value']:.3e} cell-states/s")
         if "c2" in sel and args.c2_pairs > 0:
             _log(f"c2 162-state wide kernel, {args.c2_pairs} pairs")
             cfg["c2"] = c2_bench(args.c2_pairs)
             _log(f"c2: {cfg['c2']['ms']:.0f} ms, {cfg['c2']['fp64_tflops']:.2f} TF/s")
+            _log("c2 with the default per-pair checkpoint budget (banded recomputation)")
+            cfg["c2_checkpointed"] = c2_bench(args.c2_pairs, budget=CHECKPOINT_BUDGET)
+            _log(f"c2 checkpointed: {cfg['c2_checkpointed']['ms']:.0f} ms in "
+                 f"{cfg['c2_checkpointed']['workspace_GiB']:.1f} GiB of workspace")
         out["configs"] = cfg
     loads = [float(v) for v in args.serve_loads.split(",") if v.strip()]
     if loads:

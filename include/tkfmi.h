@@ -53,18 +53,27 @@ extern "C" {
                                     workspace: skip their host-to-device
                                     upload, so the call is only device
                                     work (e.g. captured into a HIP graph)  */
-#define TKFMI_CHECKPOINT 256u    /* antidiagonal checkpointing for every
-                                    pair (the reference's gradient path
-                                    recomputes the lattice under
-                                    jax.checkpoint,
+#define TKFMI_CHECKPOINT 256u    /* checkpointing for every pair (the
+                                    reference's gradient path recomputes
+                                    the lattice under jax.checkpoint,
                                     python/tkfmixdom/jax/train/vjp.py
-                                    remat): the forward keeps the restart
+                                    remat).  Banded kernel, when the batch
+                                    has enough bands: the forward keeps
+                                    only its boundary rows, and each
+                                    backward band recomputes its own
+                                    lattice from the row above into one of
+                                    as many slots as workgroups fit the
+                                    device (one extra forward sweep,
+                                    lattice memory one band per resident
+                                    workgroup).  Otherwise the antidiagonal
+                                    kernel: the forward keeps the restart
                                     state of every K-th antidiagonal, K ~
                                     sqrt(2 (Lx+Ly+1)), and the backward
                                     recomputes one K-antidiagonal segment
-                                    at a time; identical results, one extra
-                                    forward sweep, lattice memory about
-                                    3 K / (Ly+1) of the full one.  Without
+                                    at a time (lattice memory about
+                                    3 K / (Ly+1) of the full one).  Results
+                                    are identical to the full-lattice
+                                    run's.  Without
                                     the flag, pairs whose lattice exceeds
                                     the budget (tkfmi_checkpoint_budget,
                                     256 MiB by default) are checkpointed.
@@ -180,10 +189,11 @@ size_t tkfmi_fb2d_workspace_bytes(const tkfmi_problem* prob, uint32_t flags);
  *     TKFMI_GENERIC_KERNEL: the generic kernel;
  *   - 16..160 emitting states: MFMA transition and count contractions, by
  *     default as chains of workgroups per pair (a pair's 16-row bands on
- *     separate compute units, ticketed in dependency order; full lattice),
- *     or one workgroup per pair sweeping antidiagonals, which is what runs
- *     with TKFMI_CLASS_KERNEL and whenever a pair is checkpointed
- *     (TKFMI_CHECKPOINT, or a lattice above tkfmi_checkpoint_budget).
+ *     separate compute units, ticketed in dependency order; checkpointed
+ *     bands recompute their lattice in the backward), or one workgroup per
+ *     pair sweeping antidiagonals, which is what runs with
+ *     TKFMI_CLASS_KERNEL and when checkpointing a batch of few bands
+ *     (recomputation slots there would save less than half the lattice).
  * If a chained launch ever times out on an internal wait, the affected
  * pairs' log_prob is NaN and outputs.status (if given) is set.
  * Thread safety: calls from several host threads are safe as long as each

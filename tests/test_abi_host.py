@@ -134,6 +134,32 @@ def test_banded_workspace_holds_lattice_boundaries_and_accumulators():
     assert L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), _abi.CLASS_KERNEL) != full
 
 
+def test_banded_checkpoint_workspace_holds_slots_not_lattices():
+    """TKFMI_CHECKPOINT on the banded kernel: no lattice is kept; the
+    workspace holds the forward's boundary rows, a second set for the
+    backward's pushes, the count accumulators and one lattice slot per
+    workgroup the device holds at once (one band: (Ly + 16) steps of 16
+    rows of NBP + 1 doubles; 256 slots when no device is visible, the
+    MI355X's CU count).  A batch whose slots would save less than half of
+    its lattices (one pair) goes to the antidiagonal kernel instead."""
+    from tkfmi import _abi, mixdom
+    L = _abi.lib()
+    st = list(mixdom.state_types(8, 4))  # NBP = 160: one workgroup per CU
+    n, x, y = 300, 200, 150
+    p, k = _problem(st, [0] * len(st), [x] * n, [y] * n, G=1)
+    ck = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), _abi.CHECKPOINT)
+    fo = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), _abi.FORWARD_ONLY)
+    full = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), 0)
+    bnd = n * ((x + 16) // 16 - 1) * (y + 1) * 161 * 8
+    slots = 256 * (y + 16) * 16 * 161 * 8
+    acc = n * 160 * 160 * 8
+    assert abs((ck - fo) - bnd - slots - acc) < 4096
+    assert ck < full / 3
+    p1, k1 = _problem(st, [0] * len(st), [x], [y], G=1)
+    assert (L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p1), _abi.CHECKPOINT)
+            == L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p1), _abi.CHECKPOINT | _abi.CLASS_KERNEL))
+
+
 def test_checkpoint_budget_applies_per_pair():
     """Without TKFMI_CHECKPOINT, a 162-state pair whose full lattice
     ((Lx+1)(Ly+1)(NBP+1) x 8 B) exceeds the budget is checkpointed and a

@@ -185,3 +185,43 @@ def test_band_wait_timeout_aborts_cleanly(monkeypatch):
     good = _run(w)
     assert int(good["status"].item()) == 0
     assert np.array_equal(good["log_p"].cpu().numpy(), lr)
+
+
+@pytest.mark.parametrize("K,F,C,n,lo,hi", [pytest.param(3, 2, 1, 240, 150, 400, id="S32"),
+                                           pytest.param(8, 4, 4, 300, 120, 260, id="S162")])
+def test_band_checkpoint_recomputation_is_bit_identical(K, F, C, n, lo, hi):
+    """TKFMI_CHECKPOINT on the banded kernel: the forward keeps no lattice,
+    each backward band recomputes its own from the forward's boundary row
+    into a slot (batches with many more bands than resident workgroups, so
+    that the slots save most of the lattice memory); log P, transition
+    counts and posteriors equal the full-lattice run bit for bit, emission
+    counts to the atomics' summation order."""
+    from tkfmi import _abi, synth
+    rng = np.random.default_rng(70 + K)
+    w = synth.make_mixdom2(seed=80 + K, B=n, K=K, F=F, C=C, lengths=rng.integers(lo, hi, n))
+    dev = torch.device("cuda", 0)
+    pairs = _abi.PackedPairs(w.xs, w.ys, dev)
+    out = {}
+    try:
+        _abi.set_checkpoint_budget(0)
+        for ck in (False, True):
+            r = _abi.BatchRunner(w.log_trans.to(dev), w.state_types, w.state_group, w.log_tab.to(dev), pairs,
+                                 want_posteriors=True, want_emit_counts=True, checkpoint=ck, time_kernels=True)
+            r.run()
+            names = [k["name"] for k in _abi.last_kernel_times()]
+            print(f"K{K} F{F} checkpoint={ck}: workspace {r.ws_bytes / 2**30:.2f} GiB, kernels {names}", flush=True)
+            assert any(nm.startswith("band_bwd_kernel") for nm in names), names
+            res = r.results()
+            _abi.check_status(res)
+            out[ck] = ({k: (v.cpu().numpy() if torch.is_tensor(v) else v) for k, v in res.items()}, r.ws_bytes)
+            del r
+    finally:
+        _abi.set_checkpoint_budget(_abi.CHECKPOINT_BUDGET_DEFAULT)
+    full, ws_full = out[False]
+    ck, ws_ck = out[True]
+    assert ws_ck < ws_full / 2, (ws_ck, ws_full)
+    assert np.array_equal(full["log_p"], ck["log_p"])
+    assert np.array_equal(full["n"], ck["n"])
+    assert np.array_equal(full["posteriors"], ck["posteriors"])
+    ok, worst = _close(ck["emit_counts"], full["emit_counts"], rtol=1e-12)
+    assert ok, worst

@@ -116,12 +116,14 @@ struct BDesc {
   int64_t x_off, y_off, post_off;
   int64_t acc_off;             // doubles into ws: the pair's count accumulator [NBP][NBP]
   int64_t mb_off;              // doubles into ws: the band's M-state posterior masses [T][R][nM]
+  int64_t fbin_off;            // backward of a checkpointed band (lat_off < 0): the forward's
+                               // boundary row g-1, from which the band's lattice is recomputed
 };
 
 struct BParams {
   int32_t S, NB, NBP, flags;
   int32_t e_idx, G, spin_max, nM;
-  int32_t ec_mode, pad1, pad2, pad3;  // emission counts: 0 none, 1 binned (I, D in LDS; M masses
+  int32_t ec_mode, nslots, pad2, pad3;  // emission counts: 0 none, 1 binned (I, D in LDS; M masses
                                       // to a buffer), 2 atomics, 3 binned (I, D, M in LDS)
   const int32_t* mrank;  // [NBP] rank of slot k among the M-type slots (-1: other types)
   const int32_t* bidx;   // [NBP] state index of emitting slot k (padding: 0)
@@ -137,6 +139,9 @@ struct BParams {
   int32_t* ticket;  // this launch's kQueues ticket counters
   int32_t* abortf;  // raised by a timed-out wait
   double* pinfo;    // [B][2]: mantissa and binary exponent of P(x, y)
+  double* slots;    // checkpointed bands' recomputed lattices: nslots slots of slot_stride doubles
+  int64_t slot_stride;
+  int32_t* slotf;   // [nslots] slot in use
   double* out_logP;
   double* out_n;
   double* out_post;
@@ -365,21 +370,42 @@ __device__ __forceinline__ void load_y(uint8_t* s_y, const uint8_t* ys, int c0, 
 }
 
 // ====================================================================== forward
-template <int NT, bool DENSE>
-__global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_occ(NT), 8))) void band_fwd_kernel(
-    BParams Pr) {
+// LDS of the forward sweep.
+template <int NT>
+struct FwdLds {
+  using Gm = Geo<NT>;
+  double s_pre[2][Gm::KS][Gm::R][Gm::NBPP];  // partial pushes to the cells of steps t (slot t&1), t+1
+  double s_F[Gm::R][Gm::NBPP];               // this step's normalised forward values (GEMM operand)
+  double s_last[Gm::KS][Gm::NBP];            // the last row's partial pushes for the band below
+  double s_bin[CQ + 1][Gm::NBP + 1];         // the band above's last-row pushes (row 0's wave)
+  int s_eM[2][Gm::R], s_eI[2][Gm::R], s_eD[2][Gm::R], s_E[Gm::R];
+  int s_lastE, s_ok;                         // their exponent; cleared when a boundary wait gave up
+  uint8_t s_y[kYR];
+};
+
+// One band's forward sweep.  RECOMP: the backward's recomputation of a
+// checkpointed band -- its lattice goes to the workgroup's slot `lat`, the
+// band above's pushes are read from the forward's boundary row `bin` (the
+// forward launch has completed: no waits), and nothing is published.
+template <int NT, bool DENSE, bool RECOMP>
+__device__ __forceinline__ void fwd_sweep(const BParams& Pr, const BDesc& pd, unsigned char* smem, double* lat,
+                                          const double* bin) {
   using Gm = Geo<NT>;
   constexpr int NBP = Gm::NBP, NBPP = Gm::NBPP, KR = Gm::KR, R = Gm::R, CPW = Gm::CPW;
   constexpr int KS = Gm::KS, KB = Gm::KB, NBLK = Gm::NBLK, BPW = Gm::BPW;
-  __shared__ double s_pre[2][KS][R][NBPP];  // partial pushes to the cells of steps t (slot t&1), t+1
-  __shared__ double s_F[R][NBPP];           // this step's normalised forward values (GEMM operand)
-  __shared__ double s_last[KS][NBP];        // the last row's partial pushes for the band below
-  __shared__ double s_bin[CQ + 1][NBP + 1]; // the band above's last-row pushes (row 0's wave)
-  __shared__ int s_eM[2][R], s_eI[2][R], s_eD[2][R], s_E[R];
-  __shared__ int s_lastE, s_ok;             // their exponent; cleared when a boundary wait gave up
-  __shared__ uint8_t s_y[DENSE ? 1 : kYR];
+  FwdLds<NT>& L = *reinterpret_cast<FwdLds<NT>*>(smem);
+  auto& s_pre = L.s_pre;
+  auto& s_F = L.s_F;
+  auto& s_last = L.s_last;
+  auto& s_bin = L.s_bin;
+  auto& s_eM = L.s_eM;
+  auto& s_eI = L.s_eI;
+  auto& s_eD = L.s_eD;
+  auto& s_E = L.s_E;
+  auto& s_lastE = L.s_lastE;
+  auto& s_ok = L.s_ok;
+  uint8_t* s_y = L.s_y;
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // w: wave-uniform (SGPR)
-  const BDesc pd = Pr.desc[take_band(Pr.ticket, Pr.qoff, Pr.abortf)];
   const int Lx = pd.Lx, Ly = pd.Ly, W1 = Ly + 1, NB = Pr.NB, S = Pr.S, nr = pd.nr, i0 = pd.i0;
   const double* pm = Pr.ws + pd.pm_off;
   const double* P = pm;
@@ -390,12 +416,11 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
   const double* dem = DENSE ? Pr.log_emit + pd.post_off : nullptr;
   const uint8_t* xs = DENSE ? nullptr : Pr.x_cat + pd.x_off;
   const uint8_t* ys = DENSE ? nullptr : Pr.y_cat + pd.y_off;
-  const bool want_lat = !(Pr.flags & TKFMI_FORWARD_ONLY);
-  const bool want_counts = want_lat && !(Pr.flags & TKFMI_NO_COUNTS) && Pr.out_n != nullptr;
-  double* lat = Pr.ws + pd.lat_off;
-  double* latE = lat + (size_t)pd.T * R * NBP;
-  const double* bin = pd.bin_off >= 0 ? Pr.ws + pd.bin_off : nullptr;
-  double* bout = pd.bout_off >= 0 ? Pr.ws + pd.bout_off : nullptr;
+  const bool want_lat = lat != nullptr;
+  // the end transitions' counts come from the corner cell, also of a checkpointed pair (no lattice)
+  const bool want_counts = !RECOMP && !(Pr.flags & (TKFMI_NO_COUNTS | TKFMI_FORWARD_ONLY)) && Pr.out_n != nullptr;
+  double* latE = want_lat ? lat + (size_t)pd.T * R * NBP : nullptr;
+  double* bout = (!RECOMP && pd.bout_off >= 0) ? Pr.ws + pd.bout_off : nullptr;
   double* out_n = want_counts ? Pr.out_n + (size_t)pd.pair * S * S : nullptr;
   constexpr int LPC = Gm::LPC, CPI = Gm::CPI, NG = Gm::NG;
   const int hc = (CPI == 2) ? (lane >> 5) : 0;  // the lane's cell within a group
@@ -466,7 +491,7 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
     // it, then the workgroup copies it to LDS
     if (bin != nullptr && (t % CQ) == 0 && t <= Ly) {
       const int u1 = min(t + CQ, Ly + 1);
-      if (w == 0 && ok_run) {
+      if (!RECOMP && w == 0 && ok_run) {
         ok_run = wait_ge(Pr.flagv + pd.fin, u1, Pr.abortf, Pr.spin_max);
         if (!ok_run && lane == 0) s_ok = 0;
       }
@@ -562,7 +587,7 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
         s_E[r] = E;
         if (want_lat) latE[(size_t)t * R + r] = (double)E;
       }
-      const bool corner = in && i == Lx && j == Ly;
+      const bool corner = !RECOMP && in && i == Lx && j == Ly;
       if (__builtin_amdgcn_ballot_w64(corner)) {  // log P and the E column (the corner's cell)
         double part = 0.0;
 #pragma unroll
@@ -658,6 +683,16 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
   }
 }
 
+template <int NT, bool DENSE>
+__global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_occ(NT), 8))) void band_fwd_kernel(
+    BParams Pr) {
+  __shared__ __align__(16) unsigned char smem[sizeof(FwdLds<NT>)];
+  const BDesc pd = Pr.desc[take_band(Pr.ticket, Pr.qoff, Pr.abortf)];
+  // checkpointed pairs (lat_off < 0) keep no lattice: the backward recomputes it
+  double* lat = (!(Pr.flags & TKFMI_FORWARD_ONLY) && pd.lat_off >= 0) ? Pr.ws + pd.lat_off : nullptr;
+  fwd_sweep<NT, DENSE, false>(Pr, pd, smem, lat, pd.bin_off >= 0 ? Pr.ws + pd.bin_off : nullptr);
+}
+
 // ===================================================================== backward
 // The transition blocks are held in registers where they fit beside the
 // count tiles (up to 96 padded states); above, they are read from L2 per
@@ -665,22 +700,68 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
 template <int NT>
 __host__ __device__ constexpr bool bwd_breg() { return NT <= 6; }
 
-template <int NT, bool DENSE>
+// LDS of the backward sweep.
+template <int NT>
+struct BwdLds {
+  using Gm = Geo<NT, true>;
+  // (largest first: this order keeps the backward's address registers, and
+  // with them its spills, lowest at 160 padded states)
+  double s_B[Gm::KS][Gm::R][Gm::NBPP];  // partial sums of B = g P^T
+  double s_pre[2][Gm::R][Gm::NBPP];   // q values pushed to the cells of steps t, t+1
+  double s_bin[CQ + 1][Gm::NBP + 1];  // the band below's first-row pushes (row nr-1's wave)
+  double s_G[Gm::R][Gm::NBPP];        // g of this step's cells (successor-pulled q)
+  double s_FS[Gm::R][Gm::NBPP];       // their scaled forward values
+  double s_last[Gm::NBP];             // the first row's pushes for the band above
+  uint8_t s_y[kYR];
+  int s_eM[2][Gm::R], s_eI[2][Gm::R], s_eD[2][Gm::R], s_Eg[Gm::R];
+  int s_lastE, s_ok;                  // their exponent; cleared when a boundary wait gave up
+};
+// The backward kernel's static LDS: its own, or the forward sweep's for a
+// checkpointed band's recomputation (the two are used one after the other).
+template <int NT>
+__host__ __device__ constexpr size_t bwd_smem() {
+  return sizeof(FwdLds<NT>) > sizeof(BwdLds<NT>) ? sizeof(FwdLds<NT>) : sizeof(BwdLds<NT>);
+}
+
+// A free slot for a checkpointed band's lattice (the launch never has more
+// such bands resident than slots), or -1 if the wait gave up.  The slot's
+// previous holder may have run on another XCD, whose L2 is not coherent with
+// this one's: it releases the slot with an agent-scope release (its dirty L2
+// lines written back) and the taker acquires it (this XCD's stale copies
+// dropped), so the lattice writes and reads below never mix two holders'
+// lines.
+__device__ int take_slot(int32_t* slotf, int nslots, int32_t* abortf, int spin_max) {
+  __shared__ int s_slot;
+  if (threadIdx.x == 0) {
+    int got = -1;
+    for (int spins = 0; got < 0 && spins <= spin_max; ++spins) {
+      for (int q = 0; q < nslots && got < 0; ++q) {
+        const int sl = (int)((blockIdx.x + q) % (unsigned)nslots);
+        int expect = 0;
+        if (__hip_atomic_load(slotf + sl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
+            __hip_atomic_compare_exchange_strong(slotf + sl, &expect, 1, __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+          got = sl;
+      }
+      if (got < 0) __builtin_amdgcn_s_sleep(8);
+    }
+    if (got < 0) __hip_atomic_store(abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_slot = got;
+  }
+  __syncthreads();
+  return __builtin_amdgcn_readfirstlane(s_slot);
+}
+
+// CK: the batch has checkpointed bands (a separate instantiation: the
+// recomputation's registers would otherwise crowd the backward sweep's).
+template <int NT, bool DENSE, bool CK>
 __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_occ(NT), 8))) void band_bwd_kernel(
     BParams Pr) {
   using Gm = Geo<NT, true>;
   constexpr int NBP = Gm::NBP, NBPP = Gm::NBPP, KR = Gm::KR, R = Gm::R, CPW = Gm::CPW;
   constexpr int KS = Gm::KS, KB = Gm::KB, NBLK = Gm::NBLK, BPW = Gm::BPW, NTILE = Gm::NTILE;
   constexpr bool BREG = bwd_breg<NT>();
-  __shared__ double s_pre[2][R][NBPP];  // q values pushed to the cells of steps t, t+1
-  __shared__ double s_G[R][NBPP];       // g of this step's cells (successor-pulled q)
-  __shared__ double s_FS[R][NBPP];      // their scaled forward values
-  __shared__ double s_B[KS][R][NBPP];   // partial sums of B = g P^T
-  __shared__ double s_last[NBP];        // the first row's pushes for the band above
-  __shared__ double s_bin[CQ + 1][NBP + 1];  // the band below's first-row pushes (row nr-1's wave)
-  __shared__ int s_eM[2][R], s_eI[2][R], s_eD[2][R], s_Eg[R];
-  __shared__ int s_lastE, s_ok;         // their exponent; cleared when a boundary wait gave up
-  __shared__ uint8_t s_y[DENSE ? 1 : kYR];
+  __shared__ __align__(16) unsigned char smem[bwd_smem<NT>()];
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // w: wave-uniform (SGPR)
   const BDesc pd = Pr.desc[take_band(Pr.ticket, Pr.qoff, Pr.abortf)];
   const int Lx = pd.Lx, Ly = pd.Ly, W1 = Ly + 1, NB = Pr.NB, S = Pr.S, nr = pd.nr, i0 = pd.i0;
@@ -706,8 +787,6 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
   double* s_ecD = s_ec + (size_t)Pr.G * A_PAD;
   double* s_ecM = s_ec + (size_t)Pr.G * (A_PAD + R);  // [G][A_PAD * A_PAD] (mode 3)
   double* mbuf = (ec_bin && !ec_mlds) ? Pr.ws + pd.mb_off : nullptr;
-  const double* lat = Pr.ws + pd.lat_off;
-  const double* latE = lat + (size_t)pd.T * R * NBP;
   const double* bin = pd.bin_off >= 0 ? Pr.ws + pd.bin_off : nullptr;
   double* bout = pd.bout_off >= 0 ? Pr.ws + pd.bout_off : nullptr;
   double* out_n = want_counts ? Pr.out_n + (size_t)pd.pair * S * S : nullptr;
@@ -717,6 +796,44 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
   const int EP = (int)Pr.pinfo[2 * pd.pair + 1];
   if (!(mP > 0.0)) return;  // impossible or aborted pair: every band of it returns here
   const double inv_mP = 1.0 / mP;
+  // the band's forward lattice: stored by the forward, or (checkpointed
+  // band) recomputed here into a slot from the forward's boundary row
+  const double* lat;
+  int slot = -1;
+  if (!CK || pd.lat_off >= 0) {
+    lat = Pr.ws + pd.lat_off;
+  } else {
+    slot = take_slot(Pr.slotf, Pr.nslots, Pr.abortf, Pr.spin_max);
+    if (slot < 0) {  // no slot (cannot happen with the planned count): give up on the pair
+      if (threadIdx.x == 0) Pr.out_logP[pd.pair] = __builtin_nan("");
+      return;
+    }
+    double* sl = Pr.slots + (size_t)slot * Pr.slot_stride;
+    fwd_sweep<NT, DENSE, true>(Pr, pd, smem, sl, pd.fbin_off >= 0 ? Pr.ws + pd.fbin_off : nullptr);
+    __syncthreads();
+    lat = sl;
+  }
+  const double* latE = lat + (size_t)pd.T * R * NBP;
+  auto release = [&] {  // the slot back to the pool, once the band has read it
+    if (slot >= 0) {
+      __syncthreads();
+      if (threadIdx.x == 0) __hip_atomic_store(Pr.slotf + slot, 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  };
+  BwdLds<NT>& L = *reinterpret_cast<BwdLds<NT>*>(smem);
+  auto& s_pre = L.s_pre;
+  auto& s_G = L.s_G;
+  auto& s_FS = L.s_FS;
+  auto& s_B = L.s_B;
+  auto& s_last = L.s_last;
+  auto& s_bin = L.s_bin;
+  auto& s_eM = L.s_eM;
+  auto& s_eI = L.s_eI;
+  auto& s_eD = L.s_eD;
+  auto& s_Eg = L.s_Eg;
+  auto& s_lastE = L.s_lastE;
+  auto& s_ok = L.s_ok;
+  uint8_t* s_y = L.s_y;
   constexpr int LPC = Gm::LPC, CPI = Gm::CPI, NG = Gm::NG;
   const int hc = (CPI == 2) ? (lane >> 5) : 0;  // the lane's cell within a group
   const int kl = lane & (LPC - 1);              // the lane's state slot (q = 0)
@@ -1015,6 +1132,7 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
     if (t + 1 < pd.T) prefetch_em(t + 1);
     __syncthreads();
   }
+  release();  // the lattice is read no more
   // ---- the last pushes -> boundary
   ok_run = ok_run && s_ok;
   if (bout && w == kWaves - 1 && ok_run) {
@@ -1163,11 +1281,16 @@ __global__ void zero_ints_kernel(int32_t* p, int n) {
 // counters per band boundary.
 constexpr int kFlagHead = 4 * kQueues;
 
-// LDS of band_bwd_kernel<NT> before its dynamic emission-count tables (the
-// sizes of its __shared__ arrays, rounded up).
+// LDS of band_bwd_kernel<NT> before its dynamic emission-count tables.
 size_t bwd_static_lds(int NT) {
-  const size_t NBP = 16 * NT, NBPP = NBP + 1, R = 16 * band_rt(NT), KS = band_ks(NT, true);
-  return (2 * R * NBPP + 2 * R * NBPP + KS * R * NBPP + NBP + (CQ + 1) * (NBP + 1)) * 8 + (7 * R + 4) * 4 + kYR + 64;
+  const size_t extra = 64;  // the ticket and slot words
+  switch (NT) {
+    case 2: return bwd_smem<2>() + extra;
+    case 4: return bwd_smem<4>() + extra;
+    case 6: return bwd_smem<6>() + extra;
+    case 8: return bwd_smem<8>() + extra;
+    default: return bwd_smem<10>() + extra;
+  }
 }
 
 struct BPlan {
@@ -1180,6 +1303,9 @@ struct BPlan {
   std::vector<BDesc> fwd, bwd;
   std::vector<EcPair> ec;               // binned emission counts: one entry per pair
   int nM = 0, ec_mode = 0;
+  int nslots = 0;                       // checkpointed bands' lattice slots
+  int64_t slot_stride = 0, slot0 = 0;   // doubles per slot; first slot (data region)
+  int64_t ck_lat = 0, ck_saving = 0;    // checkpointed pairs' lattices; what the slots save of them
   int n_flags = 0;
   size_t idx_b = 0, desc_b = 0, pm_b = 0, tab_b = 0, pinfo_b = 0, flag_b = 0, data_b = 0;
   size_t total() const { return idx_b + desc_b + pm_b + tab_b + pinfo_b + flag_b + data_b; }
@@ -1262,17 +1388,36 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
   // per pair: lattice, band boundaries, count accumulator; flags
   int64_t off = 0;  // doubles into the data region
   int nbnd = 0;
-  struct PairBands { int G; int64_t lat, bnd0, acc, mb; int flag0; double cost; };
+  // A pair whose lattice exceeds the checkpoint budget (or every pair with
+  // TKFMI_CHECKPOINT) keeps none: its backward bands recompute their own
+  // lattices into slots (one per workgroup that can be resident), from the
+  // forward's boundary rows, which a second set of boundary rows then keeps
+  // intact for the backward's own pushes.
+  const size_t budget = tkfmi_checkpoint_budget_bytes();
+  struct PairBands { int G; bool ck; int64_t lat, bnd0, bbnd0, acc, mb; int flag0; double cost; };
   std::vector<PairBands> pb(B);
+  int64_t slot_stride = 0, ck_bands = 0, ck_lat = 0;
   for (int b = 0; b < B; ++b) {
     const int lx = p->h_x_len[b], ly = p->h_y_len[b];
     const int G = (lx + R) / R;  // ceil((Lx + 1) / R)
     pb[b].G = G;
-    pb[b].lat = off;
     // band g's lattice: T_g = Ly + nr_g steps of R rows
-    if (!fwd_only) off += ((int64_t)G * ly + lx + 1) * R * (NBP + 1);
+    const int64_t lat_d = ((int64_t)G * ly + lx + 1) * R * (NBP + 1);
+    pb[b].ck = !fwd_only && ((flags & TKFMI_CHECKPOINT) || (budget > 0 && (size_t)lat_d * 8 > budget));
+    pb[b].lat = (fwd_only || pb[b].ck) ? -1 : off;
+    if (pb[b].lat >= 0) off += lat_d;
+    if (pb[b].ck) {
+      slot_stride = std::max<int64_t>(slot_stride, ((int64_t)ly + std::min(R, lx + 1)) * R * (NBP + 1));
+      ck_bands += G;
+      ck_lat += lat_d;
+    }
     pb[b].bnd0 = off;
     off += (int64_t)(G - 1) * (ly + 1) * (NBP + 1);
+    pb[b].bbnd0 = pb[b].bnd0;
+    if (pb[b].ck) {
+      pb[b].bbnd0 = off;
+      off += (int64_t)(G - 1) * (ly + 1) * (NBP + 1);
+    }
     pb[b].acc = off;
     if (counts && G > 1) off += (int64_t)NBP * NBP;
     pb[b].mb = off;
@@ -1281,7 +1426,15 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
     nbnd += G - 1;
     pb[b].cost = (double)G * (ly + R);
   }
-  pl.n_flags = kFlagHead + 3 * nbnd;
+  // slots: no more checkpointed bands are resident at once than there are
+  // such bands, or workgroups the device holds (band_occ(NT) per CU)
+  pl.nslots = (int)std::min<int64_t>(ck_bands, (int64_t)(cus_per_queue() * kQueues) * band_occ(pl.NT));
+  pl.ck_saving = ck_lat - (int64_t)pl.nslots * slot_stride;
+  pl.ck_lat = ck_lat;
+  pl.slot_stride = slot_stride;
+  pl.slot0 = off;
+  off += (int64_t)pl.nslots * slot_stride;
+  pl.n_flags = kFlagHead + 3 * nbnd + pl.nslots;
   pl.flag_b = align256((size_t)pl.n_flags * 4);
   // Pairs to queues (one per XCD), longest first to the least loaded queue.
   // Within a queue, the pairs are taken in groups just large enough to keep
@@ -1327,7 +1480,7 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
         d.pair = b; d.Lx = lx; d.Ly = ly; d.i0 = g * R; d.nr = std::min(R, lx + 1 - g * R);
         d.g = g; d.G = G; d.T = ly + d.nr;
         d.lat_off = lat_band;
-        lat_band += (int64_t)d.T * R * (NBP + 1);
+        if (lat_band >= 0) lat_band += (int64_t)d.T * R * (NBP + 1);
         d.mb_off = mb_band;
         mb_band += (int64_t)d.T * R * pl.nM;
         d.pm_off = (p->n_trans > 1 ? (int64_t)b : 0) * pm_stride(NBP);
@@ -1341,12 +1494,15 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
         d.bout_off = g < G - 1 ? pb[b].bnd0 + (int64_t)g * bsz : -1;
         d.fout = g < G - 1 ? fl + 3 * g : -1;
         d.ain = d.aout = -1;
+        d.fbin_off = -1;
         fwd_all.push_back(d);
-        // backward: boundary g in, g-1 out (the forward's boundary rows are
-        // no longer needed once the forward launch has ended)
-        d.bin_off = g < G - 1 ? pb[b].bnd0 + (int64_t)g * bsz : -1;
+        // backward: boundary g in, g-1 out (over the forward's boundary rows
+        // once the forward launch has ended, unless they are kept for the
+        // recomputation of a checkpointed pair)
+        d.fbin_off = (pb[b].ck && g > 0) ? pb[b].bnd0 + (int64_t)(g - 1) * bsz : -1;
+        d.bin_off = g < G - 1 ? pb[b].bbnd0 + (int64_t)g * bsz : -1;
         d.fin = g < G - 1 ? fl + 3 * g + 1 : -1;
-        d.bout_off = g > 0 ? pb[b].bnd0 + (int64_t)(g - 1) * bsz : -1;
+        d.bout_off = g > 0 ? pb[b].bbnd0 + (int64_t)(g - 1) * bsz : -1;
         d.fout = g > 0 ? fl + 3 * (g - 1) + 1 : -1;
         d.ain = (counts && g < G - 1) ? fl + 3 * g + 2 : -1;
         d.aout = (counts && g > 0) ? fl + 3 * (g - 1) + 2 : -1;
@@ -1387,7 +1543,8 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
   const int64_t tab0 = (int64_t)(pl.idx_b + pl.desc_b + pl.pm_b) / 8;
   for (std::vector<BDesc>* v : {&pl.fwd, &pl.bwd})
     for (BDesc& d : *v) {
-      d.lat_off += data0;
+      if (d.lat_off >= 0) d.lat_off += data0;
+      if (d.fbin_off >= 0) d.fbin_off += data0;
       if (d.bin_off >= 0) d.bin_off += data0;
       if (d.bout_off >= 0) d.bout_off += data0;
       d.acc_off += data0;
@@ -1396,20 +1553,29 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
       d.tab_off += tab0;
     }
   for (EcPair& e : pl.ec) e.mb_off += data0;
+  pl.slot0 += data0;
   pl.ok = true;
   return pl;
+}
+
+template <int NT, bool DENSE, bool CK>
+int launch_bwd(const BParams& prm, int n, size_t dyn, hipStream_t s, std::string& err) {
+  if (dyn > 0) {
+    const hipError_t ea = hipFuncSetAttribute((const void*)band_bwd_kernel<NT, DENSE, CK>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+    if (ea != hipSuccess) { err = std::string("hipFuncSetAttribute: ") + hipGetErrorString(ea); return TKFMI_EHIP; }
+  }
+  hipLaunchKernelGGL((band_bwd_kernel<NT, DENSE, CK>), dim3(n), dim3(kThr), dyn, s, prm);
+  return TKFMI_OK;
 }
 
 template <int NT, bool DENSE>
 int launch_nt(bool bwd, const BParams& prm, int n, size_t dyn, hipStream_t s, std::string& err) {
   if (n == 0) return TKFMI_OK;
   if (bwd) {
-    if (dyn > 0) {
-      const hipError_t ea = hipFuncSetAttribute((const void*)band_bwd_kernel<NT, DENSE>,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
-      if (ea != hipSuccess) { err = std::string("hipFuncSetAttribute: ") + hipGetErrorString(ea); return TKFMI_EHIP; }
-    }
-    hipLaunchKernelGGL((band_bwd_kernel<NT, DENSE>), dim3(n), dim3(kThr), dyn, s, prm);
+    const int rc = prm.nslots > 0 ? launch_bwd<NT, DENSE, true>(prm, n, dyn, s, err)
+                                  : launch_bwd<NT, DENSE, false>(prm, n, dyn, s, err);
+    if (rc != TKFMI_OK) return rc;
   } else {
     hipLaunchKernelGGL((band_fwd_kernel<NT, DENSE>), dim3(n), dim3(kThr), 0, s, prm);
   }
@@ -1429,18 +1595,6 @@ int launch_d(int NT, bool bwd, const BParams& prm, int n, size_t dyn, hipStream_
   }
 }
 
-bool needs_checkpoint(const tkfmi_problem* p, uint32_t flags, int NBP) {
-  if (flags & TKFMI_FORWARD_ONLY) return false;
-  if (flags & TKFMI_CHECKPOINT) return true;
-  const size_t budget = tkfmi_checkpoint_budget_bytes();
-  if (budget == 0) return false;
-  for (int b = 0; b < p->B; ++b) {
-    const size_t cells = (size_t)(p->h_x_len[b] + 1) * (p->h_y_len[b] + 1);
-    if (cells * (NBP + 1) * 8 > budget) return true;
-  }
-  return false;
-}
-
 }  // namespace tkfmi_band
 
 using namespace tkfmi_band;
@@ -1448,8 +1602,10 @@ using namespace tkfmi_band;
 bool tkfmi_band_applies(const tkfmi_problem* p, uint32_t flags) {
   if (flags & TKFMI_CLASS_KERNEL) return false;  // the one-workgroup-per-pair kernel
   const BPlan pl = plan(p, flags);
-  if (!pl.ok) return false;
-  return !needs_checkpoint(p, flags, pl.NBP);
+  // checkpointed pairs: the slots (a band lattice per resident workgroup)
+  // must at least halve their lattices; otherwise (a few very long pairs)
+  // the one-workgroup-per-pair kernel's K-antidiagonal checkpoints are used
+  return pl.ok && (pl.ck_lat == 0 || 2 * pl.ck_saving >= pl.ck_lat);
 }
 
 size_t tkfmi_band_workspace(const tkfmi_problem* p, uint32_t flags) {
@@ -1503,6 +1659,10 @@ int tkfmi_band_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
   prm.x_cat = p->x_cat; prm.y_cat = p->y_cat; prm.log_emit = p->log_emit;
   prm.ws = wsd;
   prm.flagv = flagv; prm.abortf = flagv + 2 * kQueues;
+  prm.nslots = pl.nslots;
+  prm.slots = wsd + pl.slot0;
+  prm.slot_stride = pl.slot_stride;
+  prm.slotf = flagv + (pl.n_flags - pl.nslots);
   prm.pinfo = pinfo;
   prm.out_logP = o->log_prob;
   prm.out_n = (flags & TKFMI_NO_COUNTS) ? nullptr : o->n_trans;
@@ -1516,7 +1676,8 @@ int tkfmi_band_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
     pc.ticket = flagv + dir * kQueues;
     pc.qoff = d_idx + 3 * pl.NBP + dir * (kQueues + 1);
     pc.desc = reinterpret_cast<const BDesc*>(wsb + pl.idx_b) + (dir ? nf : 0);
-    const std::string name = std::string(dir ? "band_bwd_kernel<" : "band_fwd_kernel<") + std::to_string(pl.NT) + ">";
+    const std::string name = std::string(dir ? "band_bwd_kernel<" : "band_fwd_kernel<") + std::to_string(pl.NT) +
+                             (dir && pl.nslots > 0 ? ",recompute>" : ">");
     const int slot = tkfmi_ktime_start(name.c_str(), B, cells, stream, err);
     if (slot == -2) return TKFMI_EHIP;
     const size_t dyn = (dir == 1 && pl.ec_mode == 1)   ? (size_t)pl.G * (A_PAD + pl.R) * 8

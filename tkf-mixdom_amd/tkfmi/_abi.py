@@ -351,9 +351,10 @@ def fb2d_batched(log_trans, state_types, state_group, log_tab, pairs, *,
     pairs: PackedPairs.
     Returns a dict: 'log_p' (B,), and unless forward_only 'n' (B,S,S)
     (want_counts), 'posteriors' (flat, want_posteriors), 'emit_counts'
-    (B,G,TAB_SZ, want_emit_counts).  checkpoint: antidiagonal checkpointing
+    (B,G,TAB_SZ, want_emit_counts).  checkpoint: lattice checkpointing
     (TKFMI_CHECKPOINT: less lattice memory, one extra forward sweep; used by
-    the 17..162-state kernel).  log_emit: flat dense log emissions in the
+    the 17..162-state kernels: banded recomputation per band, or
+    antidiagonal segments in the one-workgroup-per-pair kernel).  log_emit: flat dense log emissions in the
     posterior layout instead of the tables (see BatchRunner).
     """
     r = BatchRunner(log_trans, state_types, state_group, log_tab, pairs,

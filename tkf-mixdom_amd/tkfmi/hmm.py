@@ -98,9 +98,10 @@ def fb2d_batch(log_trans, state_types, xs, ys, log_tab, state_group=None, *,
     ``generic=True`` (or kernel="generic") forces the any-layout kernel; the
     TKF and MixDom K*F<=3 layouts otherwise run on the cell-per-lane path,
     chained workgroups per pair by default or one workgroup per pair with
-    kernel="class".  ``checkpoint=True`` keeps only every K-th antidiagonal's
-    forward state and recomputes the rest in the backward (the 17..162-state
-    kernel; same results, less lattice memory).
+    kernel="class".  ``checkpoint=True`` keeps no full lattice and recomputes
+    it in the backward (the 17..162-state kernels: per band from the forward's
+    boundary rows, or per K-antidiagonal segment with kernel="class" and for
+    batches of few bands; same results, less lattice memory).
     ``log_emit``: a list of dense (Lx+1, Ly+1, S) log-emission tables, one
     per pair (the reference's log_emit_table), used instead of ``log_tab``
     (then None); xs / ys give only the lengths.  Runs on the generic kernel

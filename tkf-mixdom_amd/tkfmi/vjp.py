@@ -71,7 +71,7 @@ def log_prob_batch(log_trans, state_types, xs, ys, log_tab, state_group=None, ge
                    kernel=None, checkpoint=False):
     """Differentiable (B,) log-likelihoods of a ragged batch (one call).
     ``generic`` / ``kernel`` select the kernel and ``checkpoint`` turns on
-    antidiagonal checkpointing (the reference's rematerialisation in its
+    lattice checkpointing (the reference's rematerialisation in its
     gradient path) as in tkfmi.hmm.fb2d_batch."""
     dev = log_trans.device if isinstance(log_trans, torch.Tensor) else hmm.device()
     spec = _Spec(state_types, state_group, xs, ys, dev, generic, kernel, checkpoint)
