@@ -141,7 +141,9 @@ typedef struct tkfmi_outputs {
 /* Lattice bytes of one pair above which the 16..160-state kernel
  * checkpoints that pair without TKFMI_CHECKPOINT (default 256 MiB, the
  * budget of the reference's gradient configuration; 0 = never without the
- * flag).  Process-wide; query workspace sizes after changing it.  No
+ * flag).  Exception: when the banded kernel's recomputation slots would not
+ * halve the selected pairs' lattices (a few long pairs) and those lattices
+ * total at most 16 budgets, they are kept whole (the faster choice).  Process-wide; query workspace sizes after changing it.  No
  * counterpart in the reference (its remat is always on in the VJP). */
 int tkfmi_checkpoint_budget(size_t bytes);
 

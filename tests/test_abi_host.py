@@ -161,24 +161,36 @@ def test_banded_checkpoint_workspace_holds_slots_not_lattices():
 
 
 def test_checkpoint_budget_applies_per_pair():
-    """Without TKFMI_CHECKPOINT, a 162-state pair whose full lattice
-    ((Lx+1)(Ly+1)(NBP+1) x 8 B) exceeds the budget is checkpointed and a
-    smaller one is not: with the default 256 MiB budget the 800 x 700 pair
-    (0.9 GB) is, the 200 x 150 pair (39 MB) is not; budget 0 turns it off."""
+    """Without TKFMI_CHECKPOINT, 162-state pairs whose lattice exceeds the
+    budget (default 256 MiB) are checkpointed, smaller ones are not, and
+    budget 0 turns it off.  Many such pairs: the banded kernel's
+    recomputation slots, as with the flag.  One 3000 x 2500 pair (9.7 GB,
+    which slots cannot halve): the antidiagonal kernel's checkpoints, as
+    with the flag.  One 800 x 700 pair (0.9 GB, at most 16 budgets): its
+    lattice is kept whole, as with budget 0.  A 200 x 150 pair (39 MB) is
+    never checkpointed."""
     from tkfmi import _abi, mixdom
     L = _abi.lib()
     st = list(mixdom.state_types(8, 4))
-    ws = lambda xl, yl, f=0: L.tkfmi_fb2d_workspace_bytes(ctypes.byref(_problem(st, [0] * len(st), xl, yl)[0]), f)
+
+    def ws(xl, yl, f=0):
+        p, keep = _problem(st, [0] * len(st), xl, yl)  # keep: the arrays p points into
+        return L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), f)
+
+    many = ([600] * 300, [600] * 300)
     try:
         _abi.set_checkpoint_budget(_abi.CHECKPOINT_BUDGET_DEFAULT)
-        auto_big, auto_small = ws([800], [700]), ws([200], [150])
-        assert auto_big == ws([800], [700], _abi.CHECKPOINT)
-        assert auto_small < ws([200], [150], _abi.CHECKPOINT) * 10 and auto_small > ws([200], [150], _abi.CHECKPOINT)
-        both = ws([800, 200], [700, 150])
+        auto = {"many": ws(*many), "huge": ws([3000], [2500]), "mid": ws([800], [700]),
+                "small": ws([200], [150])}
+        assert auto["many"] == ws(*many, _abi.CHECKPOINT)
+        assert auto["huge"] == ws([3000], [2500], _abi.CHECKPOINT)
+        assert auto["huge"] == ws([3000], [2500], _abi.CHECKPOINT | _abi.CLASS_KERNEL)
+        assert auto["mid"] > ws([800], [700], _abi.CHECKPOINT)
         _abi.set_checkpoint_budget(0)
-        assert ws([800], [700]) > 4 * auto_big
-        assert ws([200], [150]) == auto_small
-        assert ws([800, 200], [700, 150]) > both
+        assert ws(*many) > 5 * auto["many"]
+        assert ws([3000], [2500]) > 4 * auto["huge"]
+        assert ws([800], [700]) == auto["mid"]
+        assert ws([200], [150]) == auto["small"]
     finally:
         _abi.set_checkpoint_budget(_abi.CHECKPOINT_BUDGET_DEFAULT)
 

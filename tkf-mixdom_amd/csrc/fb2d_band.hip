@@ -1393,24 +1393,40 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
   // lattices into slots (one per workgroup that can be resident), from the
   // forward's boundary rows, which a second set of boundary rows then keeps
   // intact for the backward's own pushes.
+  // Slots: no more checkpointed bands are resident at once than there are
+  // such bands, or workgroups the device holds (band_occ(NT) per CU).  When
+  // the budget (not TKFMI_CHECKPOINT) selects pairs whose lattices the slots
+  // would not halve (a few long pairs among short ones) and those lattices
+  // total at most 16 budgets, they are kept whole.
   const size_t budget = tkfmi_checkpoint_budget_bytes();
   struct PairBands { int G; bool ck; int64_t lat, bnd0, bbnd0, acc, mb; int flag0; double cost; };
   std::vector<PairBands> pb(B);
   int64_t slot_stride = 0, ck_bands = 0, ck_lat = 0;
   for (int b = 0; b < B; ++b) {
     const int lx = p->h_x_len[b], ly = p->h_y_len[b];
-    const int G = (lx + R) / R;  // ceil((Lx + 1) / R)
-    pb[b].G = G;
+    pb[b].G = (lx + R) / R;  // ceil((Lx + 1) / R)
     // band g's lattice: T_g = Ly + nr_g steps of R rows
-    const int64_t lat_d = ((int64_t)G * ly + lx + 1) * R * (NBP + 1);
+    const int64_t lat_d = ((int64_t)pb[b].G * ly + lx + 1) * R * (NBP + 1);
     pb[b].ck = !fwd_only && ((flags & TKFMI_CHECKPOINT) || (budget > 0 && (size_t)lat_d * 8 > budget));
-    pb[b].lat = (fwd_only || pb[b].ck) ? -1 : off;
-    if (pb[b].lat >= 0) off += lat_d;
     if (pb[b].ck) {
       slot_stride = std::max<int64_t>(slot_stride, ((int64_t)ly + std::min(R, lx + 1)) * R * (NBP + 1));
-      ck_bands += G;
+      ck_bands += pb[b].G;
       ck_lat += lat_d;
     }
+  }
+  int nslots = (int)std::min<int64_t>(ck_bands, (int64_t)(cus_per_queue() * kQueues) * band_occ(pl.NT));
+  if (!(flags & TKFMI_CHECKPOINT) && ck_lat > 0 && 2 * (ck_lat - (int64_t)nslots * slot_stride) < ck_lat &&
+      (size_t)ck_lat * 8 <= 16 * budget) {
+    for (int b = 0; b < B; ++b) pb[b].ck = false;
+    nslots = 0;
+    slot_stride = ck_lat = 0;
+  }
+  for (int b = 0; b < B; ++b) {
+    const int lx = p->h_x_len[b], ly = p->h_y_len[b];
+    const int G = pb[b].G;
+    const int64_t lat_d = ((int64_t)G * ly + lx + 1) * R * (NBP + 1);
+    pb[b].lat = (fwd_only || pb[b].ck) ? -1 : off;
+    if (pb[b].lat >= 0) off += lat_d;
     pb[b].bnd0 = off;
     off += (int64_t)(G - 1) * (ly + 1) * (NBP + 1);
     pb[b].bbnd0 = pb[b].bnd0;
@@ -1426,9 +1442,7 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
     nbnd += G - 1;
     pb[b].cost = (double)G * (ly + R);
   }
-  // slots: no more checkpointed bands are resident at once than there are
-  // such bands, or workgroups the device holds (band_occ(NT) per CU)
-  pl.nslots = (int)std::min<int64_t>(ck_bands, (int64_t)(cus_per_queue() * kQueues) * band_occ(pl.NT));
+  pl.nslots = nslots;
   pl.ck_saving = ck_lat - (int64_t)pl.nslots * slot_stride;
   pl.ck_lat = ck_lat;
   pl.slot_stride = slot_stride;
