@@ -143,8 +143,9 @@ typedef struct tkfmi_outputs {
  * budget of the reference's gradient configuration; 0 = never without the
  * flag).  Exception: when the banded kernel's recomputation slots would not
  * halve the selected pairs' lattices (a few long pairs) and those lattices
- * total at most 16 budgets, they are kept whole (the faster choice).  Process-wide; query workspace sizes after changing it.  No
- * counterpart in the reference (its remat is always on in the VJP). */
+ * total at most 16 budgets, they are kept whole (the faster choice).
+ * Process-wide; query workspace sizes after changing it.  No counterpart in
+ * the reference (its remat is always on in the VJP). */
 int tkfmi_checkpoint_budget(size_t bytes);
 
 /* Version of this header's ABI (TKFMI_ABI_VERSION). */
