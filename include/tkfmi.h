@@ -515,6 +515,28 @@ int tkfmi_last_kernel_times(tkfmi_kernel_time* out, int32_t max);
  */
 int tkfmi_chain_config(int32_t long_w, int32_t main_w, double long_frac);
 
+/*
+ * Schedule of the chained 2D kernels; no counterpart in the reference (it
+ * runs a pair's forward, then its backward: dp/hmm.py:1834-1973).  In the
+ * one-launch schedule the forward and the backward of every pair run at the
+ * same time, from opposite corners, and meet at antidiagonal
+ * C = round(cut (Lx + Ly)): the backward takes the expected counts of the
+ * cells i + j <= C, the forward those of the cells past C, both scaled by
+ * the P(x, y) of the transitions crossing the cut.  A pair then takes about
+ * max(C, Lx + Ly - C) steps of each sweep instead of Lx + Ly of both in
+ * turn, but all its workgroups stay resident together, so the schedule is
+ * used only when the call's workgroups (two per band group of each pair)
+ * number at most max_workgroups (-1: twice the device's compute units, the
+ * number it holds at once); larger calls, calls asking for posteriors or
+ * for log P only run the forward launch, then the backward launch.  cut = 0
+ * never uses it.  Process-wide, applies to later calls (query the workspace
+ * size again: the one-launch schedule adds 256 (Lx + 1) bytes per pair and
+ * a second set of boundary rows).  Defaults 0.45 and -1, or TKFMI_CHAIN_CUT
+ * and TKFMI_CHAIN_BIDI_MAX_WG.  Returns TKFMI_OK or TKFMI_EINVAL (cut
+ * outside [0, 1]).
+ */
+int tkfmi_chain_schedule(double cut, int32_t max_workgroups);
+
 /* Result of tkfmi_autotune: the candidates timed, in order (the first is
  * the configuration in force before the call), their ms per call (-1: not
  * run because its workspace exceeded the one given) and the chosen one. */

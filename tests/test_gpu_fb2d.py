@@ -320,12 +320,18 @@ def test_chain_wait_timeout_aborts_cleanly(monkeypatch):
     """A wait that times out (forced from the host with a spin limit of 1)
     must end the launch: pairs computed without a wait keep their values,
     pairs whose chain waited either completed or report NaN, never a wrong
-    number; the next call with the normal limit is exact."""
-    from tkfmi import synth
+    number; the next call with the normal limit is exact.  Two-launch
+    schedule (forward, then backward), where one-band pairs never wait; the
+    one-launch schedule's version is in test_gpu_bidi.py."""
+    from tkfmi import autotune, synth
     w = synth.make_mixdom2(seed=31, B=8, lengths=[40, 700, 20, 650, 60, 800, 30, 500])
     ref = _run(w, want_post=False, kernel="generic")
     monkeypatch.setenv("TKFMI_CHAIN_SPIN_LIMIT", "1")
-    bad = _run(w, want_post=False, kernel="chain")
+    autotune.set_chain_schedule(0.0)
+    try:
+        bad = _run(w, want_post=False, kernel="chain")
+    finally:
+        autotune.set_chain_schedule(*autotune.DEFAULT_SCHEDULE)
     monkeypatch.delenv("TKFMI_CHAIN_SPIN_LIMIT")
     lr, lb = ref["log_p"].cpu().numpy(), bad["log_p"].cpu().numpy()
     for b in range(w.B):

@@ -34,6 +34,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -44,179 +45,16 @@
 #include <vector>
 
 #include "fb2d_cell.h"
+#include "fb2d_chain.h"
 #include "fb2d_internal.h"
 #include "tkfmi.h"
 
 namespace tkfmi_v2 {
 
-constexpr int CQ = 16;                 // boundary entries per published / fetched chunk
-// Backward count staging per recursion wave: its 64 cells' F-scaled and g
-// vectors, 16 doubles each (16 KB).
-constexpr int kStageDoubles = 2048;
-constexpr int kSpinMax = 1 << 22;      // bounded waits (each spin sleeps ~0.2 us + a load)
-// TKFMI_CHAIN_SPIN_LIMIT overrides kSpinMax; tests set it tiny so that
-// waits time out and the abort path runs with every kernel still ending.
-
-struct CDesc {
-  int32_t pair, Lx, Ly, band0;  // pair geometry; first band of this workgroup
-  int32_t nw, g, G, T;          // waves holding a band (<= W), chain index, chain length, steps
-  int64_t x_off, y_off;
-  int64_t lat_off, post_off, pm_off;
-  int64_t xin_off, xout_off;    // doubles into the boundary region (-1: none)
-  int32_t fin, fout;            // progress counters (ints into the flag region; -1: none)
-};
-
-struct CParams {
-  int32_t S, flags, n_tab, G_tab;
-  int32_t gmap[16];  // layout group -> caller table group
-  const double* log_tab;
-  const uint8_t* x_cat;
-  const uint8_t* y_cat;
-  double* lattice;
-  double* xring;      // boundary rows between the workgroups of a chain
-  int32_t* flagv;     // progress counters
-  int32_t* ticket;    // this launch's ticket counter
-  int32_t* abortf;    // raised by a timed-out wait
-  int32_t spin_max;   // spins before a wait times out
-  double* pinfo;      // [B][2]: mantissa and binary exponent of P(x, y)
-  double* out_logP;
-  double* out_n;
-  double* out_post;
-  double* out_ecnt;
-};
-
-__device__ __forceinline__ int ld_acquire(int* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Waits (whole wave, uniform) until *flag >= need.  Returns false when the
-// launch was aborted, by this wait timing out or by another one.
-__device__ bool wait_ge(int* flag, int need, int* abortf, int spin_max) {
-  int v = __builtin_amdgcn_readfirstlane(ld_acquire(flag));
-  int spins = 0;
-  while (v < need) {
-    if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
-      return false;
-    if (++spins > spin_max) {
-      __hip_atomic_store(abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return false;
-    }
-    __builtin_amdgcn_s_sleep(8);
-    v = __builtin_amdgcn_readfirstlane(ld_acquire(flag));
-  }
-  return true;
-}
-
-// Boundary entries between the workgroups of a chain are written and read
-// as device-scope relaxed atomics, which gfx950 performs at the device's
-// coherence point (past the per-XCD L2s).  Publishing then only has to wait
-// until this lane's entry stores have completed before it raises the counter:
-// an agent-scope release fence would also write back every dirty line of the
-// XCD's L2, and there the forward's lattice stores keep megabytes dirty, so
-// each publish would wait for a flush of the whole cache.
-__device__ __forceinline__ void ring_store(double* p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Publishes `count` boundary entries (this lane's ring_store calls before it)
-// to the next workgroup of the chain.
-__device__ __forceinline__ void publish(int* flag, int count) {
-  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the entry stores have completed
-  __hip_atomic_store(flag, count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Copies boundary entries [e, e_end) into the LDS chunk buffer (whole wave),
-// after wait_ge has acquired their progress counter.
-__device__ __forceinline__ void fetch_chunk(double* s_xc, const double* src, int n) {
-  for (int q = threadIdx.x & 63; q < n; q += 64)
-    s_xc[q] = __hip_atomic_load(src + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  wave_lds_sync();
-}
-
-// Writes one boundary entry (the M and D parts of v, then the exponent) to
-// the LDS ring of the next wave or, for the next workgroup, to global memory.
-template <class L, int NX>
-__device__ __forceinline__ void put_entry(double* e, bool global, const double* v, int E) {
-  constexpr int NB = L::NB, nM = count_type<L, ST_M>();
-#pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    int o = -1;
-    if (L::type(b) == ST_M) o = rank_in_type<L, ST_M>(b);
-    if (L::type(b) == ST_D) o = nM + rank_in_type<L, ST_D>(b);
-    if (o >= 0) {
-      if (global) ring_store(e + o, v[b]);
-      else e[o] = v[b];
-    }
-  }
-  if (global) ring_store(e + NX - 1, (double)E);
-  else e[NX - 1] = (double)E;
-}
-
 // status[0] = 1 if the call's abort flag was raised (after every class joined).
 __global__ void raise_status_kernel(const int32_t* __restrict__ abortf, int32_t* __restrict__ status) {
   if (threadIdx.x == 0 && __hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
     status[0] = 1;
-}
-
-// The launch's next ticket, the same for the whole workgroup.  A ticket past
-// the grid means the counter was not zeroed before the launch (a caller
-// error, e.g. a workspace shared by overlapping calls): the workgroup raises
-// the abort flag and its waves end here (every wave sees the same ticket),
-// without touching a descriptor.
-__device__ __forceinline__ int take_ticket(int32_t* counter, int32_t* abortf) {
-  __shared__ int s_ticket;
-  if (threadIdx.x == 0) s_ticket = atomicAdd(counter, 1);
-  __syncthreads();
-  const int t = __builtin_amdgcn_readfirstlane(s_ticket);
-  if (t < 0 || t >= (int)gridDim.x) {
-    if (threadIdx.x == 0) __hip_atomic_store(abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_endpgm();
-  }
-  return t;
-}
-
-// LDS layout (doubles): tables | start / end transitions | emission counts
-// (backward) | short rings | chunk buffer | count staging (backward, 64 cells
-// x 32 doubles per recursion wave) | bytes: x rows of the workgroup, y.
-template <class L, int W>
-struct CLds {
-  static constexpr int NX = count_type<L, ST_M>() + count_type<L, ST_D>() + 1;
-  static constexpr int tab = 0;
-  static constexpr int misc = L::NG * TAB_SZ;  // p_start[NB], p_end[NB], P[S->E]
-  static constexpr int ecnt = misc + 32;
-  static constexpr int ring(bool bwd) { return ecnt + (bwd ? L::NG * TAB_SZ : 0); }
-  static constexpr int xc(bool bwd) { return ring(bwd) + (W - 1) * 2 * NX; }
-  static constexpr int stage(bool bwd) { return ((xc(bwd) + CQ * NX) + 1) & ~1; }
-  static constexpr int seq_bytes(bool bwd) { return 8 * (stage(bwd) + (bwd ? W * kStageDoubles : 0)); }
-  static constexpr int xbytes = 64 * W + 8;
-  static size_t total(bool bwd, int max_ly) { return (size_t)seq_bytes(bwd) + xbytes + ((max_ly + 8) & ~7) + 16; }
-};
-
-// Stages the exponentiated emission tables and the workgroup's x rows / y.
-// The start / end transitions go to LDS too: a global load inside the sweep
-// would make the compiler wait for every outstanding lattice store (stores
-// and loads share one counter) at the next use of its register.
-template <class L, int W, int NTHR = W * 64>
-__device__ __forceinline__ void stage_inputs(const CParams& Pr, const CDesc& pd, const double* pm,
-                                             double* s_tab, uint8_t* s_x, uint8_t* s_y) {
-  constexpr int NG = L::NG, NB = L::NB;
-  static_assert(2 * NB + 1 <= 32, "start / end staging");
-  double* s_misc = s_tab + NG * TAB_SZ;
-  for (int q = threadIdx.x; q < 2 * NB + 1; q += NTHR) s_misc[q] = pm[2 * NB * NB + q];
-  const double* ltab = Pr.log_tab + (Pr.n_tab > 1 ? (size_t)pd.pair * Pr.G_tab * TAB_SZ : 0);
-  for (int q = threadIdx.x; q < NG * TAB_SZ; q += NTHR) {
-    const int g = q / TAB_SZ, o = q - g * TAB_SZ;
-    s_tab[q] = exp(ltab[Pr.gmap[g] * TAB_SZ + o]);
-  }
-  // s_x[q] = x[64 band0 + q - 1] (the character of row 64 band0 + q)
-  const int xb = 64 * pd.band0 - 1;
-  for (int q = threadIdx.x; q < 64 * W + 1; q += NTHR) {
-    const int xi = xb + q;
-    s_x[q] = (xi >= 0 && xi < pd.Lx) ? (uint8_t)min((int)Pr.x_cat[pd.x_off + xi], A_PAD - 1) : 0;
-  }
-  for (int q = threadIdx.x; q < pd.Ly; q += NTHR) s_y[q] = min((int)Pr.y_cat[pd.y_off + q], A_PAD - 1);
-  if (threadIdx.x == 0) s_y[pd.Ly] = 0;
 }
 
 // ============================================================== forward
@@ -416,55 +254,6 @@ __global__ __launch_bounds__(W * 64) void fbc_fwd_kernel(CParams Pr, const doubl
   if (!ok_run) {
     __hip_atomic_store(Pr.abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (threadIdx.x == 0) Pr.out_logP[pd.pair] = __builtin_nan("");
-  }
-}
-
-// Count wave, one step: the expected-count MFMAs of the cells the recursion
-// waves staged at the previous step.  On gfx950 an FP64 MFMA occupies its
-// SIMD's FP64 pipe for 64 cycles, with no overlap with VALU FP64 work, so a
-// separate wave (on its own SIMD) keeps them off the recursion's dependent
-// chain.  Each stage (64 cells) is read in two halves of 32 cells into
-// registers; the reads of the next half are issued before the 8 MFMAs of the
-// current one (4 cells each, two accumulator chains: acc (+acc2) += F^T g),
-// so LDS latency hides behind the matrix pipe.  When both halves of stage v
-// are in registers the wave releases it (s_cflag[v] = t): recursion wave v
-// may then overwrite it with its cells of step t.  Stages of waves that were
-// not active at step t - 1 are read but not counted.
-template <int W>
-__device__ __forceinline__ void count_step(const double* s_stage, int NW, int Ly, int t, int lane, int* s_cflag,
-                                           d4& acc, d4& acc2) {
-  int ln = lane;
-  asm volatile("" : "+v"(ln));  // recompute the addresses here (hoisted, they would take 48 VGPRs)
-  auto load = [&](int i, double (&a)[8], double (&b)[8]) {
-    const double* sf = s_stage + (size_t)(i >> 1) * kStageDoubles;
-    const double* sg = sf + kStageDoubles / 2;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int c = 4 * (8 * (i & 1) + k) + (ln >> 4);
-      const int o = c * 16 + ((ln & 15) ^ (c & 15));
-      a[k] = sf[o];
-      b[k] = sg[o];
-    }
-  };
-  double ca[8], cb[8], na[8], nb[8];
-  load(0, ca, cb);
-#pragma unroll
-  for (int i = 0; i < 2 * W; ++i) {
-    const int v = i >> 1;
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // half i is in registers
-    if ((i & 1) && lane == 0) __hip_atomic_store(s_cflag + v, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (i + 1 < 2 * W) load(i + 1, na, nb);
-    __builtin_amdgcn_sched_barrier(0);
-    if (v < NW && t >= 64 * v + 1 && t <= 64 * v + 64 + Ly) {  // wave v staged at step t - 1
-#pragma unroll
-      for (int k = 0; k < 8; k += 2) {
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[k], cb[k], acc, 0, 0, 0);
-        acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[k + 1], cb[k + 1], acc2, 0, 0, 0);
-      }
-    }
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) { ca[k] = na[k]; cb[k] = nb[k]; }
   }
 }
 
@@ -830,6 +619,34 @@ ChainConfig g_cfg = [] {
   if (const char* e = getenv("TKFMI_CHAIN_LONG")) c.long_frac = atof(e);
   return c;
 }();
+// The one-launch schedule (fb2d_bidi.hip): a pair's forward and backward
+// meet at C = cut (Lx + Ly); the forward counts the cells past C, the
+// backward those up to it.  It halves a pair's critical path but keeps all of
+// a pair's workgroups resident at once, and the workgroups of a chain's
+// later links then wait for their producers in slots that other pairs could
+// use: on a GPU it saturates it is slower than two launches (the 1024-pair
+// K=3 batch: 12.1 against 8.9 ms), on one it leaves room it is faster (the
+// batch's longest pair alone: 4.0 against 6.8 ms).  So it runs when every
+// workgroup of the call fits the GPU at once: at most max_wg workgroups (2
+// per band group of each pair; -1: 2 per compute unit).  cut = 0: never.
+// tkfmi_chain_schedule, TKFMI_CHAIN_CUT, TKFMI_CHAIN_BIDI_MAX_WG.
+std::atomic<double> g_cut{[] {
+  const char* e = getenv("TKFMI_CHAIN_CUT");
+  return e ? std::max(0.0, std::min(1.0, atof(e))) : 0.45;
+}()};
+std::atomic<int> g_bidi_max_wg{[] {
+  const char* e = getenv("TKFMI_CHAIN_BIDI_MAX_WG");
+  return e ? atoi(e) : -1;
+}()};
+// Workgroups of the one-launch kernel that the current device holds at once
+// (two per compute unit: VGPR- and LDS-bound).
+int bidi_resident_wg() {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    return 0;
+  return 2 * cus;
+}
 // Configurations chosen by tkfmi_autotune, per (layout, length bin of the
 // batch's longest sequence); a batch with no entry uses g_cfg.
 constexpr int kLenBins = 6;  // longest sequence < 128, 256, 512, 1024, 2048, more
@@ -873,14 +690,30 @@ struct ChainPlan {
   bool ok = false;
   std::string why;
   std::vector<CDesc> fwd[NCLS], bwd[NCLS];
+  std::vector<CDesc> both[NCLS];  // one-launch schedule: each pair's forward and backward groups together
+  bool bidi = false;
+  int fail_base = 0;  // one-launch schedule: flag index of pair 0's failure flag
   int64_t cls_cells[NCLS] = {0, 0, 0, 0};
   int cls_pairs[NCLS] = {0, 0, 0, 0};
   int max_ly[NCLS] = {0, 0, 0, 0};
-  size_t desc_b = 0, pm_b = 0, lat_b = 0, ring_b = 0, flag_b = 0, pinfo_b = 0;
-  int n_flags = 0;  // ints: NCLS classes x 2 tickets, abort, pad, then 2 per boundary
+  size_t desc_b = 0, pm_b = 0, lat_b = 0, ring_b = 0, flag_b = 0, pinfo_b = 0, cut_b = 0, pslot_b = 0;
+  // ints: NCLS classes x 2 tickets, abort, pad, then 2 per boundary (and,
+  // one-launch, per pair 1 partial counter + 1 cut flag per band group)
+  int n_flags = 0;
   int W[NCLS] = {3, 3, 2, 1};  // waves per workgroup of each class
-  size_t total() const { return desc_b + pm_b + lat_b + ring_b + flag_b + pinfo_b; }
+  size_t total() const { return desc_b + pm_b + lat_b + ring_b + flag_b + pinfo_b + cut_b + pslot_b; }
 };
+
+// The one-launch schedule applies when the backward has counts to take, no
+// posterior lattice is asked for and the call's n_wg workgroups fit the
+// limit (see g_cut).
+bool bidi_wanted(uint32_t flags, int64_t n_wg) {
+  if (flags & (TKFMI_FORWARD_ONLY | TKFMI_WANT_POSTERIORS)) return false;
+  if ((flags & TKFMI_NO_COUNTS) && !(flags & TKFMI_WANT_EMIT_COUNTS)) return false;
+  if (!(g_cut.load() > 0.0)) return false;
+  const int lim = g_bidi_max_wg.load();
+  return n_wg <= (int64_t)(lim < 0 ? bidi_resident_wg() : lim);
+}
 
 // cfg: the class configuration to plan with (null: the tuned or default one
 // for the batch's layout and length bin).
@@ -908,11 +741,22 @@ ChainPlan plan_chain(const tkfmi_problem* p, const Plan& pl, uint32_t flags, con
     post_off[b] = ap;
     ap += (int64_t)(p->h_x_len[b] + 1) * (p->h_y_len[b] + 1) * p->S;
   }
-  int64_t aring = 0;
+  int64_t aring = 0, acut = 0, aslot = 0;
   int nfl = 2 * NCLS + 8;
+  const double cut = g_cut.load();
+  std::vector<std::pair<int64_t, std::vector<CDesc>>> pair_groups[NCLS];  // one-launch: (critical path, groups)
   const int64_t cells_max = B > 0 ? (int64_t)(p->h_x_len[order[0]] + 1) * (p->h_y_len[order[0]] + 1) : 0;
   const ChainConfig cf = cfg ? *cfg : chain_config((int)pl.lay, length_bin(max_seq_len(p)));
   for (int c = 0; c < NCLS; ++c) cp.W[c] = cf.w[c];
+  {
+    int64_t n_wg = 0;  // workgroups of a one-launch call: 2 per band group
+    for (int b = 0; b < B; ++b) {
+      const int nb = (p->h_x_len[b] + 64) / 64;
+      const int c = ccls_of(cf, nb, (int64_t)(p->h_x_len[b] + 1) * (p->h_y_len[b] + 1), cells_max);
+      n_wg += 2 * ((nb + cf.w[c] - 1) / cf.w[c]);
+    }
+    cp.bidi = bidi_wanted(flags, n_wg);
+  }
   for (int q = 0; q < B; ++q) {
     const int b = order[q];
     const int Lx = p->h_x_len[b], Ly = p->h_y_len[b];
@@ -925,7 +769,10 @@ ChainPlan plan_chain(const tkfmi_problem* p, const Plan& pl, uint32_t flags, con
     cp.max_ly[c] = std::max(cp.max_ly[c], Ly);
     const int64_t ring0 = aring;
     const int flag0 = nfl;
-    aring += (int64_t)(G - 1) * (Ly + 1) * NX;
+    // the backward's boundary rows: the same region as the forward's when
+    // the two run in turn, a region of their own when they run together
+    const int64_t bring0 = cp.bidi ? ring0 + (int64_t)(G - 1) * (Ly + 1) * NX : ring0;
+    aring += (int64_t)(cp.bidi ? 2 : 1) * (G - 1) * (Ly + 1) * NX;
     nfl += 2 * (G - 1);
     std::vector<CDesc> ch(G);
     for (int g = 0; g < G; ++g) {
@@ -951,11 +798,57 @@ ChainPlan plan_chain(const tkfmi_problem* p, const Plan& pl, uint32_t flags, con
     for (int g = G - 1; g >= 0; --g) {  // backward: boundary g in, g-1 out
       CDesc d = ch[g];
       d.T = Ly + 64 * (d.nw - 1) + 64;
-      d.xin_off = g < G - 1 ? ring0 + (int64_t)g * (Ly + 1) * NX : -1;
+      d.xin_off = g < G - 1 ? bring0 + (int64_t)g * (Ly + 1) * NX : -1;
       d.fin = g < G - 1 ? flag0 + 2 * g + 1 : -1;
-      d.xout_off = g > 0 ? ring0 + (int64_t)(g - 1) * (Ly + 1) * NX : -1;
+      d.xout_off = g > 0 ? bring0 + (int64_t)(g - 1) * (Ly + 1) * NX : -1;
       d.fout = g > 0 ? flag0 + 2 * (g - 1) + 1 : -1;
       cp.bwd[c].push_back(d);
+    }
+    if (cp.bidi) {
+      // the meeting antidiagonal, the pair's partial counter, one cut flag
+      // per band group, partial slots (16 doubles apart: one cache line
+      // each) and the cut buffer ([2][16][Lx + 1] doubles)
+      const int D = Lx + Ly;
+      const int Cm = std::max(0, std::min(D, (int)llround(cut * D)));
+      const int cnt = nfl++;
+      const int bf0 = nfl;
+      nfl += G;
+      const int ps0 = (int)aslot;
+      aslot += 16 * G;
+      const int64_t cut0 = acut;
+      acut += 32 * (int64_t)(Lx + 1);
+      std::vector<CDesc>& fw = cp.fwd[c];
+      std::vector<CDesc>& bw = cp.bwd[c];
+      std::vector<CDesc> grp;
+      for (int k = 0; k < G; ++k) {  // chain heads first: forward 0, backward G-1, forward 1, ...
+        for (int dir = 0; dir < 2; ++dir) {
+          CDesc d = dir ? bw[bw.size() - G + k] : fw[fw.size() - G + k];
+          d.dir = dir;
+          d.C = Cm;
+          d.cnt = cnt;
+          d.bflag = bf0 + d.g;
+          d.pslot = ps0;
+          d.gpair = G;
+          d.cut_off = cut0;
+          grp.push_back(d);
+        }
+      }
+      const int lag = 64 * (cp.W[c] - 1) + 64 + CQ;
+      const CDesc& f0 = fw[fw.size() - G];
+      pair_groups[c].push_back({(int64_t)(G - 1) * lag + f0.T, grp});
+      fw.resize(fw.size() - G);
+      bw.resize(bw.size() - G);
+    }
+  }
+  if (cp.bidi) {  // per pair, a flag raised by its timed-out waits
+    cp.fail_base = nfl;
+    nfl += B;
+  }
+  if (cp.bidi) {  // pairs by critical path, each pair's groups consecutive
+    for (int c = 0; c < NCLS; ++c) {
+      std::stable_sort(pair_groups[c].begin(), pair_groups[c].end(),
+                       [](const auto& a, const auto& b) { return a.first > b.first; });
+      for (auto& pg : pair_groups[c]) cp.both[c].insert(cp.both[c].end(), pg.second.begin(), pg.second.end());
     }
   }
   // Critical-path ticket order.  A workgroup's critical path is the number
@@ -980,7 +873,7 @@ ChainPlan plan_chain(const tkfmi_problem* p, const Plan& pl, uint32_t flags, con
     }
   }
   size_t nd = 0;
-  for (int c = 0; c < NCLS; ++c) nd += cp.fwd[c].size() + cp.bwd[c].size();
+  for (int c = 0; c < NCLS; ++c) nd += cp.fwd[c].size() + cp.bwd[c].size() + cp.both[c].size();
   cp.desc_b = align256(sizeof(CDesc) * std::max<size_t>(nd, 1));
   cp.pm_b = align256(stride_for(pl.NB) * 8 * (size_t)p->n_trans);
   cp.lat_b = align256((size_t)al * 8);
@@ -988,6 +881,8 @@ ChainPlan plan_chain(const tkfmi_problem* p, const Plan& pl, uint32_t flags, con
   cp.n_flags = nfl;
   cp.flag_b = align256((size_t)nfl * 4);
   cp.pinfo_b = align256((size_t)std::max(B, 1) * 16);
+  cp.cut_b = cp.bidi ? align256((size_t)std::max<int64_t>(acut, 1) * 8) : 0;
+  cp.pslot_b = cp.bidi ? align256((size_t)std::max<int64_t>(aslot, 1) * 8) : 0;
   for (int c = 0; c < NCLS; ++c)
     if (cp.cls_pairs[c] && CLds<LMix<3, 1>, 4>::total(true, cp.max_ly[c]) > 160 * 1024 - 512) {
       cp.why = "LDS budget exceeded (sequences too long)";
@@ -1105,12 +1000,16 @@ int chain_run_impl(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
   double* xring = reinterpret_cast<double*>(wsb + cp.desc_b + cp.pm_b + cp.lat_b);
   int32_t* flagv = reinterpret_cast<int32_t*>(wsb + cp.desc_b + cp.pm_b + cp.lat_b + cp.ring_b);
   double* pinfo = reinterpret_cast<double*>(wsb + cp.desc_b + cp.pm_b + cp.lat_b + cp.ring_b + cp.flag_b);
-  // descriptors: [class][fwd | bwd]
+  double* cutbuf = reinterpret_cast<double*>(wsb + cp.desc_b + cp.pm_b + cp.lat_b + cp.ring_b + cp.flag_b + cp.pinfo_b);
+  double* pslot = reinterpret_cast<double*>(wsb + cp.desc_b + cp.pm_b + cp.lat_b + cp.ring_b + cp.flag_b +
+                                            cp.pinfo_b + cp.cut_b);
+  // descriptors: [class][fwd | bwd] (two launches) or [class][both] (one launch)
   std::vector<CDesc> all;
   size_t base[NCLS][2];
   for (int c = 0; c < NCLS; ++c) {
     base[c][0] = all.size();
     all.insert(all.end(), cp.fwd[c].begin(), cp.fwd[c].end());
+    all.insert(all.end(), cp.both[c].begin(), cp.both[c].end());
     base[c][1] = all.size();
     all.insert(all.end(), cp.bwd[c].begin(), cp.bwd[c].end());
   }
@@ -1139,6 +1038,9 @@ int chain_run_impl(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
   prm.log_tab = p->log_tab; prm.x_cat = p->x_cat; prm.y_cat = p->y_cat;
   prm.lattice = lattice; prm.xring = xring; prm.flagv = flagv; prm.abortf = flagv + 2 * NCLS;
   prm.pinfo = pinfo;
+  prm.cutbuf = cutbuf;
+  prm.pslot = pslot;
+  prm.fail_base = cp.fail_base;
   {
     const char* e = getenv("TKFMI_CHAIN_SPIN_LIMIT");
     prm.spin_max = e ? std::max(0, atoi(e)) : kSpinMax;
@@ -1154,7 +1056,19 @@ int chain_run_impl(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
     if (!cp.cls_pairs[c]) continue;
     hipStream_t s = one_stream ? stream : sts->s[c];
     if (!one_stream && hipStreamWaitEvent(s, sts->start, 0) != hipSuccess) { err = "hipStreamWaitEvent failed"; return TKFMI_EHIP; }
-    for (int dir = 0; dir < (fwd_only ? 1 : 2); ++dir) {
+    if (cp.bidi) {  // forward and backward of every pair in one launch
+      CParams pc = prm;
+      pc.ticket = flagv + 2 * c;
+      const std::string kname = std::string("fbc_bidi_kernel<") + layout_name(pl) + "," + std::to_string(cp.W[c]) +
+                                ">" + (c == 0 ? " long" : "");
+      const int slot = tkfmi_ktime_start(kname.c_str(), cp.cls_pairs[c], cp.cls_cells[c], s, err);
+      if (slot == -2) return TKFMI_EHIP;
+      rc = launch_bidi(pl, cp.W[c], pc, pmat, ddesc + base[c][0], (int)cp.both[c].size(), cp.max_ly[c],
+                       wpc[c][1], s, err);
+      if (rc != TKFMI_OK) return rc;
+      if (tkfmi_ktime_stop(slot, s, err) != TKFMI_OK) return TKFMI_EHIP;
+    }
+    for (int dir = 0; dir < (cp.bidi ? 0 : (fwd_only ? 1 : 2)); ++dir) {
       CParams pc = prm;
       pc.ticket = flagv + 2 * c + dir;
       const int n = (int)(dir ? cp.bwd[c].size() : cp.fwd[c].size());
@@ -1169,6 +1083,10 @@ int chain_run_impl(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
     if (one_stream) continue;
     if (hipEventRecord(sts->done[c], s) != hipSuccess) { err = "hipEventRecord failed"; return TKFMI_EHIP; }
     if (hipStreamWaitEvent(stream, sts->done[c], 0) != hipSuccess) { err = "hipStreamWaitEvent failed"; return TKFMI_EHIP; }
+  }
+  if (cp.bidi) {
+    rc = launch_mark_failed(flagv + cp.fail_base, p->B, o->log_prob, stream, err);
+    if (rc != TKFMI_OK) return rc;
   }
   if (o->status) {
     hipLaunchKernelGGL(raise_status_kernel, dim3(1), dim3(64), 0, stream, prm.abortf, o->status);
@@ -1187,6 +1105,14 @@ extern "C" int tkfmi_chain_config(int32_t long_w, int32_t main_w, double long_fr
   g_cfg.w[1] = main_w;
   g_cfg.long_frac = long_frac;
   g_tuned.clear();
+  return TKFMI_OK;
+}
+
+extern "C" int tkfmi_chain_schedule(double cut, int32_t max_workgroups) {
+  if (!(cut >= 0.0 && cut <= 1.0))
+    return tkfmi_fail(TKFMI_EINVAL, "tkfmi_chain_schedule: cut must be in [0, 1]");
+  g_cut.store(cut);
+  g_bidi_max_wg.store(max_workgroups < 0 ? -1 : max_workgroups);
   return TKFMI_OK;
 }
 

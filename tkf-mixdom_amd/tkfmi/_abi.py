@@ -177,7 +177,7 @@ _lock = threading.Lock()
 EXPORTED = ("tkfmi_abi_version", "tkfmi_last_error",
             "tkfmi_fb2d_workspace_bytes", "tkfmi_fb2d_batched",
             "tkfmi_fb1d_workspace_bytes", "tkfmi_fb1d_batched",
-            "tkfmi_last_kernel_times", "tkfmi_chain_config", "tkfmi_suffstats_size",
+            "tkfmi_last_kernel_times", "tkfmi_chain_config", "tkfmi_chain_schedule", "tkfmi_suffstats_size",
             "tkfmi_restore_workspace_bytes", "tkfmi_restore_counts_batched",
             "tkfmi_bdi_stats_batched", "tkfmi_tkf92_estep_stats", "tkfmi_hr_stats_batched",
             "tkfmi_build_workspace_bytes", "tkfmi_build_params_batched",
@@ -231,6 +231,8 @@ def lib():
                                              _c_p, ctypes.c_size_t, _c_p, ctypes.c_uint32]
             L.tkfmi_chain_config.restype = ctypes.c_int
             L.tkfmi_chain_config.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_double]
+            L.tkfmi_chain_schedule.restype = ctypes.c_int
+            L.tkfmi_chain_schedule.argtypes = [ctypes.c_double, ctypes.c_int32]
             L.tkfmi_last_kernel_times.restype = ctypes.c_int
             L.tkfmi_last_kernel_times.argtypes = [ctypes.POINTER(KernelTime), ctypes.c_int32]
             L.tkfmi_suffstats_size.restype = ctypes.c_int32

@@ -93,3 +93,19 @@ def tune_chain(make_runner, candidates=CANDIDATES, steps=3, margin=0.03):
         best = fastest
     set_chain_config(*best)
     return {"best": best, "ms": timings}
+
+
+# The chained kernels' one-launch schedule: where a pair's forward and
+# backward meet (fraction of Lx + Ly; 0: never one launch) and the largest
+# call (in workgroups) that uses it (-1: what the GPU holds at once).
+DEFAULT_SCHEDULE = (0.45, -1)  # the library's own defaults
+
+
+def set_chain_schedule(cut, max_workgroups=-1):
+    """Set the chained kernels' schedule (process-wide): calls of at most
+    ``max_workgroups`` workgroups (-1: twice the device's compute units) run
+    each pair's forward and backward in one launch, meeting at antidiagonal
+    round(cut (Lx + Ly)); larger calls, and every call with cut = 0, run the
+    forward launch, then the backward launch."""
+    L = _abi.lib()
+    _abi.check(L.tkfmi_chain_schedule(float(cut), int(max_workgroups)))
