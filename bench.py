@@ -299,9 +299,11 @@ def mid_bench(steps, K=3, F=2):
 
 
 def critical_path(w, emit):
-    """The longest pair of the batch alone, forward + backward launch times
-    (HIP events): a call cannot end before its longest pair's chain of
-    antidiagonal steps does, whatever the batch around it."""
+    """The longest pair of the batch alone, its launch times (HIP events): a
+    call cannot end before its longest pair's chain of antidiagonal steps
+    does, whatever the batch around it.  A lone pair fits the GPU, so it runs
+    on the chained kernels' one-launch schedule (forward and backward
+    together, meeting at an antidiagonal)."""
     import numpy as np
     import torch
     from tkfmi import _abi
@@ -315,11 +317,12 @@ def critical_path(w, emit):
     ks = []
     for _ in range(5):
         r.run()
-        ks.append(sum(k["ms"] for k in _abi.last_kernel_times()))
+        kt = _abi.last_kernel_times()
+        ks.append(sum(k["ms"] for k in kt))
     torch.cuda.synchronize()
     Lx, Ly = len(w.xs[b]), len(w.ys[b])
     return {"pair": [Lx, Ly], "ms": float(np.median(ks)),
-            "antidiagonal_steps": Lx + Ly + 1}
+            "antidiagonal_steps": Lx + Ly + 1, "kernels": sorted({k["name"] for k in kt})}
 
 
 def c3_bench(steps):
@@ -503,7 +506,7 @@ def main():
         out["roofline"]["critical_path"] = {
             "longest_pair": crit["pair"], "antidiagonal_steps": crit["antidiagonal_steps"],
             "ms": crit["ms"], "us_per_step": 1e3 * crit["ms"] / crit["antidiagonal_steps"],
-            "frac_of_step": crit["ms"] / step_ms}
+            "frac_of_step": crit["ms"] / step_ms, "kernels": crit["kernels"]}
         _log(f"critical path: longest pair {crit['pair']} alone {crit['ms']:.3f} ms "
              f"({crit['ms'] / step_ms:.2f} of the step)")
     if not args.no_plain:
