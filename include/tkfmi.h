@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define TKFMI_ABI_VERSION 4
+#define TKFMI_ABI_VERSION 5
 
 /* error codes */
 #define TKFMI_OK 0
@@ -509,11 +509,15 @@ int tkfmi_last_kernel_times(tkfmi_kernel_time* out, int32_t max);
  * long_frac x the batch's largest pair runs as a chain of long_w-wave
  * workgroups on its own stream; the other pairs with at least main_w bands
  * as chains of main_w-wave workgroups; shorter pairs with 2- and 1-wave
- * workgroups.  long_w in 1..4, main_w in 2..4, long_frac >= 0 (0: no long
- * class).  Defaults 3, 3, 0.  tkfmi.autotune chooses them by timing the
- * caller's batch.  Returns TKFMI_OK or TKFMI_EINVAL.
+ * workgroups.  long_one_launch != 0 runs that long class on the one-launch
+ * schedule (see tkfmi_chain_schedule) whatever the call's size, the other
+ * classes as that schedule decides.  long_w in 1..4, main_w in 2..4,
+ * long_frac >= 0 (0: no long class).  Defaults 3, 3, 0.7, 1 (the few
+ * longest pairs of a batch on the one-launch schedule).  tkfmi.autotune
+ * chooses them by timing the caller's batch.  Returns TKFMI_OK or
+ * TKFMI_EINVAL.
  */
-int tkfmi_chain_config(int32_t long_w, int32_t main_w, double long_frac);
+int tkfmi_chain_config(int32_t long_w, int32_t main_w, double long_frac, int32_t long_one_launch);
 
 /*
  * Schedule of the chained 2D kernels; no counterpart in the reference (it
@@ -549,6 +553,7 @@ typedef struct tkfmi_tune_result {
   int32_t main_w[16];
   double long_frac[16];
   double ms[16];
+  int32_t long_one[16]; /* the long class on the one-launch schedule         */
 } tkfmi_tune_result;
 
 /* Workspace tkfmi_autotune needs for `prob` (the largest over candidates). */

@@ -318,8 +318,11 @@ def test_pad_to_bin_matches_reference_bins(monkeypatch):
 def test_chain_config_validates_arguments():
     from tkfmi import _abi, autotune
     L = _abi.lib()
-    assert L.tkfmi_chain_config(5, 3, 0.3) < 0
-    assert L.tkfmi_chain_config(4, 1, 0.3) < 0
-    assert L.tkfmi_chain_config(4, 3, -1.0) < 0
+    assert L.tkfmi_chain_config(5, 3, 0.3, 0) < 0
+    assert L.tkfmi_chain_config(4, 1, 0.3, 0) < 0
+    assert L.tkfmi_chain_config(4, 3, -1.0, 1) < 0
     assert b"tkfmi_chain_config" in L.tkfmi_last_error()
+    assert L.tkfmi_chain_schedule(1.5, -1) < 0
+    assert L.tkfmi_chain_schedule(-0.1, -1) < 0
+    assert b"tkfmi_chain_schedule" in L.tkfmi_last_error()
     autotune.set_chain_config(*autotune.DEFAULT)

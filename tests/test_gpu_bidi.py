@@ -118,7 +118,8 @@ def test_mixdom_layouts(K, F, C):
 
 @pytest.mark.parametrize("cfg", [(4, 3, 0.3), (2, 2, 0.0), (3, 2, 0.3)])
 def test_workgroup_widths(cfg):
-    """Other workgroup widths per length class (1- to 4-wave chains)."""
+    """Other workgroup widths per length class (1- to 4-wave chains), every
+    class on the one-launch schedule."""
     from tkfmi import autotune
     w, emit, lt_of = _mixdom(3, 1, 1, [5, 64, 65, 130, 260, 390, 520, 700], seed=23)
     try:
@@ -192,8 +193,9 @@ def test_wait_timeout_marks_pairs_nan(monkeypatch):
 
 def test_schedule_follows_call_size():
     """By default a call that the GPU holds at once (here 6 pairs) runs in
-    one launch and a call that does not (the 1024-pair benchmark batch) in
-    two; both give the two-launch results."""
+    one launch; in a call that does not (the 1024-pair benchmark batch) only
+    the long class does (its few longest pairs), the others in two launches;
+    both give the two-launch results."""
     from tkfmi import _abi, autotune, hmm, synth
     autotune.set_chain_schedule(*autotune.DEFAULT_SCHEDULE)
     w = synth.make_mixdom2(seed=5, B=6, lengths=[30, 200, 700, 64, 65, 400])
@@ -215,9 +217,27 @@ def test_schedule_follows_call_size():
                          time_kernels=True)
     r.run()
     kt = _abi.last_kernel_times()
-    assert not any("fbc_bidi_kernel" in k["name"] for k in kt), kt
+    assert any(k["name"].startswith("fbc_fwd_kernel") for k in kt), kt
+    assert all(k["name"].endswith(" long") for k in kt if "fbc_bidi_kernel" in k["name"]), kt
 
 
 def torch_device():
     import torch
     return torch.device("cuda:0")
+
+
+@pytest.mark.parametrize("cfg", [(3, 3, 0.5, 1), (4, 3, 0.3, 1)])
+def test_long_class_one_launch(cfg):
+    """The long class on the one-launch schedule and the other classes in two
+    launches, in one call (default schedule limits): the oracle's results."""
+    from tkfmi import autotune, hmm
+    w, emit, lt_of = _mixdom(3, 1, 1, [5, 64, 65, 130, 260, 390, 520, 700, 800], seed=27,
+                             ys={7: 760, 8: 640})
+    autotune.set_chain_schedule(*autotune.DEFAULT_SCHEDULE)
+    try:
+        autotune.set_chain_config(*cfg)
+        res = hmm.fb2d_batch(w.log_trans, w.state_types, w.xs, w.ys, w.log_tab, w.state_group,
+                             want_emit_counts=True, kernel="chain")
+    finally:
+        autotune.set_chain_config(*autotune.DEFAULT)
+    _check_vs_oracle(w, res, emit, lt_of)

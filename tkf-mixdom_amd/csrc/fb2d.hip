@@ -631,16 +631,21 @@ size_t plan_ws_bytes(const Plan& pl) {
 
 // Pinned staging buffers for host -> device uploads (per-pair descriptors,
 // statistic programs).  A buffer is reused only once the copy out of it has
-// completed (its event), so an upload never waits for earlier work on any
-// stream; the pool grows while every buffer is still in flight.
+// completed (its event).  The pool grows while every buffer is still in
+// flight, up to kMaxStages per host thread; beyond that the call waits for
+// the earliest upload.  Growing costs a hipHostMalloc (milliseconds), so a
+// small pool, filled by a caller's first calls, keeps a steady loop of calls
+// from stalling on allocations while the host runs ahead of the GPU.
 struct Stage {
   void* buf = nullptr;
   size_t cap = 0;
   hipEvent_t ev = nullptr;
   bool pending = false;
+  uint64_t seq = 0;  // submission order of its last upload
 };
-constexpr int kMaxStages = 64;
+constexpr int kMaxStages = 8;
 thread_local std::vector<Stage> g_stages;
+thread_local uint64_t g_stage_seq = 0;
 
 template <int GL, int NB, int W, bool DENSE>
 int launch(const Plan& pl, DevParams dp, int B, size_t smem, hipStream_t stream) {
@@ -688,8 +693,10 @@ int tkfmi_stage_upload(void* dst, const void* src, size_t bytes, hipStream_t str
     if ((int)g_stages.size() < kMaxStages) {
       g_stages.emplace_back();
       st = &g_stages.back();
-    } else {  // every buffer in flight: wait for the oldest
+    } else {  // every buffer in flight: wait for the earliest upload
       st = &g_stages.front();
+      for (Stage& c : g_stages)
+        if (c.seq < st->seq) st = &c;
       hipError_t e = hipEventSynchronize(st->ev);
       if (e != hipSuccess) { err = std::string("hipEventSynchronize: ") + hipGetErrorString(e); return TKFMI_EHIP; }
       st->pending = false;
@@ -713,6 +720,7 @@ int tkfmi_stage_upload(void* dst, const void* src, size_t bytes, hipStream_t str
   e = hipEventRecord(st->ev, stream);
   if (e != hipSuccess) { err = std::string("hipEventRecord: ") + hipGetErrorString(e); return TKFMI_EHIP; }
   st->pending = true;
+  st->seq = ++g_stage_seq;
   return TKFMI_OK;
 }
 

@@ -608,15 +608,20 @@ __global__ void zero_ints_kernel(int32_t* p, int n) {
 constexpr int NCLS = 4;
 struct ChainConfig {
   int w[NCLS] = {3, 3, 2, 1};
-  // cells >= long_frac x the batch's largest pair; 0 (default) = no long
-  // class: on a saturated GPU it does not shorten the call (measured on the
-  // 1024-pair K=3 batch), the autotuner can still switch it on
-  double long_frac = 0.0;
+  // cells >= long_frac x the batch's largest pair; 0 = no long class
+  double long_frac = 0.7;
+  // 1 (default): the long class runs on the one-launch schedule
+  // (fb2d_bidi.hip) while the other classes keep two launches, so the
+  // longest pairs stop setting the call's length (the 1024-pair K=3 batch:
+  // 8.1-8.3 ms with its 6-15 longest pairs so, against 8.8 ms with no long
+  // class; a long class in two launches does not shorten the call)
+  int long_one = 1;
 };
 std::mutex g_cfg_mu;
 ChainConfig g_cfg = [] {
   ChainConfig c;
   if (const char* e = getenv("TKFMI_CHAIN_LONG")) c.long_frac = atof(e);
+  if (const char* e = getenv("TKFMI_CHAIN_LONG_ONE")) c.long_one = atoi(e) != 0;
   return c;
 }();
 // The one-launch schedule (fb2d_bidi.hip): a pair's forward and backward
@@ -691,7 +696,8 @@ struct ChainPlan {
   std::string why;
   std::vector<CDesc> fwd[NCLS], bwd[NCLS];
   std::vector<CDesc> both[NCLS];  // one-launch schedule: each pair's forward and backward groups together
-  bool bidi = false;
+  bool bidi = false;              // some class runs on the one-launch schedule
+  bool bidi_cls[NCLS] = {false, false, false, false};
   int fail_base = 0;  // one-launch schedule: flag index of pair 0's failure flag
   int64_t cls_cells[NCLS] = {0, 0, 0, 0};
   int cls_pairs[NCLS] = {0, 0, 0, 0};
@@ -746,7 +752,7 @@ ChainPlan plan_chain(const tkfmi_problem* p, const Plan& pl, uint32_t flags, con
   const double cut = g_cut.load();
   std::vector<std::pair<int64_t, std::vector<CDesc>>> pair_groups[NCLS];  // one-launch: (critical path, groups)
   const int64_t cells_max = B > 0 ? (int64_t)(p->h_x_len[order[0]] + 1) * (p->h_y_len[order[0]] + 1) : 0;
-  const ChainConfig cf = cfg ? *cfg : chain_config((int)pl.lay, length_bin(max_seq_len(p)));
+  ChainConfig cf = cfg ? *cfg : chain_config((int)pl.lay, length_bin(max_seq_len(p)));
   for (int c = 0; c < NCLS; ++c) cp.W[c] = cf.w[c];
   {
     int64_t n_wg = 0;  // workgroups of a one-launch call: 2 per band group
@@ -755,7 +761,18 @@ ChainPlan plan_chain(const tkfmi_problem* p, const Plan& pl, uint32_t flags, con
       const int c = ccls_of(cf, nb, (int64_t)(p->h_x_len[b] + 1) * (p->h_y_len[b] + 1), cells_max);
       n_wg += 2 * ((nb + cf.w[c] - 1) / cf.w[c]);
     }
-    cp.bidi = bidi_wanted(flags, n_wg);
+    const bool all = bidi_wanted(flags, n_wg);
+    // A long class on one launch exists only in calls of many more
+    // workgroups than the GPU holds, where the longest pairs' chains set the
+    // call's length; elsewhere its pairs stay in the main class (measured on
+    // the serving loop's calls: a class of its own, on a stream of its own,
+    // made latency worse).
+    const int res = bidi_resident_wg();
+    const bool big = res > 0 && n_wg > 4 * (int64_t)res;
+    const bool long_one = cf.long_one && cf.long_frac > 0.0 && big && !all && bidi_wanted(flags, 0);
+    if (cf.long_one && !long_one) cf.long_frac = 0.0;
+    for (int c = 0; c < NCLS; ++c) cp.bidi_cls[c] = all || (c == 0 && long_one);
+    cp.bidi = cp.bidi_cls[0] || cp.bidi_cls[1] || cp.bidi_cls[2] || cp.bidi_cls[3];
   }
   for (int q = 0; q < B; ++q) {
     const int b = order[q];
@@ -771,8 +788,9 @@ ChainPlan plan_chain(const tkfmi_problem* p, const Plan& pl, uint32_t flags, con
     const int flag0 = nfl;
     // the backward's boundary rows: the same region as the forward's when
     // the two run in turn, a region of their own when they run together
-    const int64_t bring0 = cp.bidi ? ring0 + (int64_t)(G - 1) * (Ly + 1) * NX : ring0;
-    aring += (int64_t)(cp.bidi ? 2 : 1) * (G - 1) * (Ly + 1) * NX;
+    const bool bidi = cp.bidi_cls[c];
+    const int64_t bring0 = bidi ? ring0 + (int64_t)(G - 1) * (Ly + 1) * NX : ring0;
+    aring += (int64_t)(bidi ? 2 : 1) * (G - 1) * (Ly + 1) * NX;
     nfl += 2 * (G - 1);
     std::vector<CDesc> ch(G);
     for (int g = 0; g < G; ++g) {
@@ -804,7 +822,7 @@ ChainPlan plan_chain(const tkfmi_problem* p, const Plan& pl, uint32_t flags, con
       d.fout = g > 0 ? flag0 + 2 * (g - 1) + 1 : -1;
       cp.bwd[c].push_back(d);
     }
-    if (cp.bidi) {
+    if (bidi) {
       // the meeting antidiagonal, the pair's partial counter, one cut flag
       // per band group, partial slots (16 doubles apart: one cache line
       // each) and the cut buffer ([2][16][Lx + 1] doubles)
@@ -1054,9 +1072,13 @@ int chain_run_impl(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
   const bool fwd_only = (flags & TKFMI_FORWARD_ONLY) != 0;
   for (int c = 0; c < NCLS; ++c) {
     if (!cp.cls_pairs[c]) continue;
-    hipStream_t s = one_stream ? stream : sts->s[c];
+    // Classes 2 and 3 (2- and 1-band pairs, short) share a library stream:
+    // with the caller's stream that keeps a call within the GPU's 4 hardware
+    // queues, so the long class's launch never shares one with the main
+    // class's (two streams on one queue run their kernels in turn).
+    hipStream_t s = one_stream ? stream : sts->s[c == 3 ? 2 : c];
     if (!one_stream && hipStreamWaitEvent(s, sts->start, 0) != hipSuccess) { err = "hipStreamWaitEvent failed"; return TKFMI_EHIP; }
-    if (cp.bidi) {  // forward and backward of every pair in one launch
+    if (cp.bidi_cls[c]) {  // forward and backward of every pair in one launch
       CParams pc = prm;
       pc.ticket = flagv + 2 * c;
       const std::string kname = std::string("fbc_bidi_kernel<") + layout_name(pl) + "," + std::to_string(cp.W[c]) +
@@ -1068,7 +1090,7 @@ int chain_run_impl(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
       if (rc != TKFMI_OK) return rc;
       if (tkfmi_ktime_stop(slot, s, err) != TKFMI_OK) return TKFMI_EHIP;
     }
-    for (int dir = 0; dir < (cp.bidi ? 0 : (fwd_only ? 1 : 2)); ++dir) {
+    for (int dir = 0; dir < (cp.bidi_cls[c] ? 0 : (fwd_only ? 1 : 2)); ++dir) {
       CParams pc = prm;
       pc.ticket = flagv + 2 * c + dir;
       const int n = (int)(dir ? cp.bwd[c].size() : cp.fwd[c].size());
@@ -1097,9 +1119,10 @@ int chain_run_impl(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
 
 }  // namespace
 
-extern "C" int tkfmi_chain_config(int32_t long_w, int32_t main_w, double long_frac) {
+extern "C" int tkfmi_chain_config(int32_t long_w, int32_t main_w, double long_frac, int32_t long_one_launch) {
   if (long_w < 1 || long_w > 4 || main_w < 2 || main_w > 4 || !(long_frac >= 0.0))
     return tkfmi_fail(TKFMI_EINVAL, "tkfmi_chain_config: waves must be 1..4 (main 2..4), long_frac >= 0");
+  g_cfg.long_one = long_one_launch != 0;
   std::lock_guard<std::mutex> lk(g_cfg_mu);
   g_cfg.w[0] = long_w;
   g_cfg.w[1] = main_w;
@@ -1117,15 +1140,16 @@ extern "C" int tkfmi_chain_schedule(double cut, int32_t max_workgroups) {
 }
 
 namespace {
-// Candidate (long_w, main_w, long_frac) settings timed by tkfmi_autotune; the
+// Candidate (long_w, main_w, long_frac, long_one) settings timed by tkfmi_autotune; the
 // current default for the batch's key is always timed first.
 const ChainConfig kCandidates[] = {
-    {{3, 3, 2, 1}, 0.0}, {{3, 3, 2, 1}, 0.15}, {{3, 3, 2, 1}, 0.3}, {{4, 3, 2, 1}, 0.3},
-    {{4, 4, 2, 1}, 0.3}, {{4, 4, 2, 1}, 0.0},  {{3, 2, 2, 1}, 0.0}, {{2, 2, 2, 1}, 0.0}};
+    {{3, 3, 2, 1}, 0.7, 1},  {{3, 3, 2, 1}, 0.5, 1},  {{4, 3, 2, 1}, 0.6, 1},  {{3, 3, 2, 1}, 0.0, 0},
+    {{3, 3, 2, 1}, 0.15, 0}, {{3, 3, 2, 1}, 0.3, 0},  {{4, 3, 2, 1}, 0.3, 0},  {{4, 4, 2, 1}, 0.3, 0},
+    {{4, 4, 2, 1}, 0.0, 0},  {{3, 2, 2, 1}, 0.0, 0},  {{2, 2, 2, 1}, 0.0, 0}};
 constexpr int kNCand = (int)(sizeof(kCandidates) / sizeof(kCandidates[0]));
 
 bool same_cfg(const ChainConfig& a, const ChainConfig& b) {
-  return a.w[0] == b.w[0] && a.w[1] == b.w[1] && a.long_frac == b.long_frac;
+  return a.w[0] == b.w[0] && a.w[1] == b.w[1] && a.long_frac == b.long_frac && a.long_one == b.long_one;
 }
 }  // namespace
 
@@ -1175,6 +1199,7 @@ extern "C" int tkfmi_autotune(const tkfmi_problem* p, const tkfmi_outputs* o, vo
     res->long_w[c] = cand[c].w[0];
     res->main_w[c] = cand[c].w[1];
     res->long_frac[c] = cand[c].long_frac;
+    res->long_one[c] = cand[c].long_one;
     res->ms[c] = -1.0;  // -1: not run (its workspace exceeds ws_bytes)
   }
   res->n = nc;

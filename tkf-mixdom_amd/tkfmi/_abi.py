@@ -14,7 +14,7 @@ import torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libtkfmi.so")
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 FORWARD_ONLY = 1
 WANT_POSTERIORS = 2
 NO_COUNTS = 4
@@ -157,6 +157,7 @@ class TuneResult(ctypes.Structure):
         ("main_w", ctypes.c_int32 * 16),
         ("long_frac", ctypes.c_double * 16),
         ("ms", ctypes.c_double * 16),
+        ("long_one", ctypes.c_int32 * 16),
     ]
 
 
@@ -230,7 +231,7 @@ def lib():
             L.tkfmi_fb1d_batched.argtypes = [ctypes.POINTER(Problem1D), ctypes.POINTER(Outputs),
                                              _c_p, ctypes.c_size_t, _c_p, ctypes.c_uint32]
             L.tkfmi_chain_config.restype = ctypes.c_int
-            L.tkfmi_chain_config.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_double]
+            L.tkfmi_chain_config.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_int32]
             L.tkfmi_chain_schedule.restype = ctypes.c_int
             L.tkfmi_chain_schedule.argtypes = [ctypes.c_double, ctypes.c_int32]
             L.tkfmi_last_kernel_times.restype = ctypes.c_int

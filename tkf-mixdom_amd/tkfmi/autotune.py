@@ -18,12 +18,14 @@ import torch
 
 from . import _abi
 
-# (long_w, main_w, long_frac): waves per workgroup of the longest pairs'
-# class, of the other pairs' class, and the cell fraction of the batch's
-# largest pair from which a pair counts as long (0: no long class).
-DEFAULT = (3, 3, 0.0)  # the library's own defaults
-CANDIDATES = ((3, 3, 0.0), (3, 3, 0.3), (4, 3, 0.3), (4, 4, 0.3), (3, 3, 0.15), (3, 3, 0.5), (4, 4, 0.0),
-              (3, 2, 0.0), (2, 2, 0.0))
+# (long_w, main_w, long_frac, long_one): waves per workgroup of the longest
+# pairs' class, of the other pairs' class, the cell fraction of the batch's
+# largest pair from which a pair counts as long (0: no long class), and 1 to
+# run that long class on the one-launch schedule (forward and backward of a
+# pair together, see set_chain_schedule).
+DEFAULT = (3, 3, 0.7, 1)  # the library's own defaults
+CANDIDATES = ((3, 3, 0.7, 1), (3, 3, 0.5, 1), (4, 3, 0.6, 1), (3, 3, 0.0, 0), (3, 3, 0.3, 0), (4, 3, 0.3, 0),
+              (4, 4, 0.3, 0), (3, 3, 0.15, 0), (4, 4, 0.0, 0), (3, 2, 0.0, 0), (2, 2, 0.0, 0))
 
 
 def tune(runner, reps=3):
@@ -32,7 +34,7 @@ def tune(runner, reps=3):
     and keeps the fastest for the batch's key (state layout, length bin of
     its longest sequence), which later calls with that key then use.
     Resizes the runner's workspace for the chosen configuration.  Returns
-    {"key": (layout, length_bin), "best": (long_w, main_w, long_frac),
+    {"key": (layout, length_bin), "best": (long_w, main_w, long_frac, long_one),
     "ms": {config: ms per call}} (empty "ms" when the layout has a single
     configuration)."""
     L = _abi.lib()
@@ -48,16 +50,16 @@ def tune(runner, reps=3):
                                 _abi.ctypes.c_void_p(stream.cuda_stream), flags, int(reps),
                                 _abi.ctypes.byref(res)))
     runner.resize_workspace()
-    cfgs = [(res.long_w[c], res.main_w[c], res.long_frac[c]) for c in range(res.n)]
+    cfgs = [(res.long_w[c], res.main_w[c], res.long_frac[c], res.long_one[c]) for c in range(res.n)]
     return {"key": (res.layout, res.length_bin),
             "best": cfgs[res.best] if res.n else None,
             "ms": {c: res.ms[i] for i, c in enumerate(cfgs)}}
 
 
-def set_chain_config(long_w, main_w, long_frac):
+def set_chain_config(long_w, main_w, long_frac, long_one=0):
     """Set the library's chained-kernel configuration (process-wide)."""
     L = _abi.lib()
-    _abi.check(L.tkfmi_chain_config(int(long_w), int(main_w), float(long_frac)))
+    _abi.check(L.tkfmi_chain_config(int(long_w), int(main_w), float(long_frac), int(long_one)))
 
 
 def _time_runner(runner, steps):
@@ -78,7 +80,7 @@ def tune_chain(make_runner, candidates=CANDIDATES, steps=3, margin=0.03):
     force when it is built).  The first candidate (the library default) is
     kept unless another is faster by more than ``margin`` (a fraction), so
     that timing noise does not flip the choice.  Returns {"best": (long_w,
-    main_w, long_frac), "ms": {config: ms per call}}; the library is left set
+    main_w, long_frac, long_one), "ms": {config: ms per call}}; the library is left set
     to the best.
     """
     timings = {}
