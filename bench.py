@@ -354,7 +354,7 @@ def c3_bench(steps):
                                         w.xs, w.ys)
         lp.sum().backward()
 
-    ms = _time_calls(step, steps)
+    ms = _time_calls(step, steps, warmup=5)
     cs = w.cell_states()
     return {"value": cs / (ms / 1e3), "unit": "cell-states/s (value + full gradient)", "ms": ms,
             "pairs": w.B, "cell_states": cs}

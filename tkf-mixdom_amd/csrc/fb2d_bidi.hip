@@ -68,7 +68,7 @@ template <int NTHR>
 __device__ void publish_partial(double cv, int ce, double* s_red, double* slot, int32_t* cnt) {
   s_red[2 * threadIdx.x] = cv;
   s_red[2 * threadIdx.x + 1] = (double)ce;
-  __syncthreads();
+  wg_release_agent();  // the group's lattice rows go with its partial
   if (threadIdx.x < 64) {
     double v = 0.0;
     int e = 0;
@@ -593,7 +593,7 @@ __device__ __forceinline__ void bidi_bwd(const CParams& Pr, const double* __rest
       publish(Pr.flagv + pd.fout, put_cnt);
       pub_cnt = put_cnt;
     }
-    __syncthreads();
+    wg_release_agent();
     if (threadIdx.x == 0) __hip_atomic_store(Pr.flagv + pd.bflag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     if (ok_run) ok_run = wait_P(Pr, pd, mP, EP);
     inv_mP = (mP > 0.0) ? 1.0 / mP : 0.0;
@@ -863,7 +863,7 @@ __device__ __forceinline__ void bidi_bwd(const CParams& Pr, const double* __rest
   }
   for (; t < T; ++t) step(std::integral_constant<int, 2>(), t);
   if (tC >= T) {  // the cut lies before this group's cells: only its cut flag
-    __syncthreads();
+    wg_release_agent();
     if (threadIdx.x == 0) __hip_atomic_store(Pr.flagv + pd.bflag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
   if (counter && want_counts) count_step<W>(s_stage, NW, Ly, T, lane, s_cflag, cacc, cacc2, tC);

@@ -61,14 +61,14 @@ struct CParams {
   int32_t fail_base;  // one-launch schedule: flagv[fail_base + pair] = 1 when a wait of the pair timed out
 };
 
-__device__ __forceinline__ int ld_acquire(int* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Waits (whole wave, uniform) until *flag >= need.  Returns false when the
-// launch was aborted, by this wait timing out or by another one.
+// Waits (whole wave, uniform) until *flag >= need, then acquires (agent
+// scope) what was released with the flag.  Returns false when the launch was
+// aborted, by this wait timing out or by another one.  The flag is polled
+// with relaxed loads and the acquire is one fence after the last: an
+// agent-scope acquire invalidates the XCD's L2, and a poll that acquired
+// would invalidate it for every workgroup on the XCD once per spin.
 __device__ inline bool wait_ge(int* flag, int need, int* abortf, int spin_max) {
-  int v = __builtin_amdgcn_readfirstlane(ld_acquire(flag));
+  int v = __builtin_amdgcn_readfirstlane(__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
   int spins = 0;
   while (v < need) {
     if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
@@ -78,9 +78,19 @@ __device__ inline bool wait_ge(int* flag, int need, int* abortf, int spin_max) {
       return false;
     }
     __builtin_amdgcn_s_sleep(8);
-    v = __builtin_amdgcn_readfirstlane(ld_acquire(flag));
+    v = __builtin_amdgcn_readfirstlane(__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
   }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return true;
+}
+
+// Before one thread releases, at agent scope, data that every wave of the
+// workgroup wrote: each wave's own release fence (its stores complete, the
+// XCD's L2 written back), then the barrier.  A barrier alone does not wait
+// for the other waves' stores.
+__device__ __forceinline__ void wg_release_agent() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
 }
 
 // Boundary entries between the workgroups of a chain are written and read

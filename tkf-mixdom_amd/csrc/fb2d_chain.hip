@@ -762,14 +762,14 @@ ChainPlan plan_chain(const tkfmi_problem* p, const Plan& pl, uint32_t flags, con
       n_wg += 2 * ((nb + cf.w[c] - 1) / cf.w[c]);
     }
     const bool all = bidi_wanted(flags, n_wg);
-    // A long class on one launch exists only in calls of many more
-    // workgroups than the GPU holds, where the longest pairs' chains set the
-    // call's length; elsewhere its pairs stay in the main class (measured on
-    // the serving loop's calls: a class of its own, on a stream of its own,
-    // made latency worse).
-    const int res = bidi_resident_wg();
-    const bool big = res > 0 && n_wg > 4 * (int64_t)res;
-    const bool long_one = cf.long_one && cf.long_frac > 0.0 && big && !all && bidi_wanted(flags, 0);
+    // A long class on one launch exists only in calls too large for the
+    // whole call to run on it: there the longest pairs' chains set the
+    // call's length (c1-shaped batches of 200-1024 pairs, 630-3300
+    // workgroups: 4.1-8.2 ms against 5.1-8.4 with every pair on two
+    // launches and 4.6-12.2 with every pair on one); in calls that fit, its
+    // pairs stay in the main class (a class of its own, on a stream of its
+    // own, made the serving loop's latency worse).
+    const bool long_one = cf.long_one && cf.long_frac > 0.0 && !all && bidi_wanted(flags, 0);
     if (cf.long_one && !long_one) cf.long_frac = 0.0;
     for (int c = 0; c < NCLS; ++c) cp.bidi_cls[c] = all || (c == 0 && long_one);
     cp.bidi = cp.bidi_cls[0] || cp.bidi_cls[1] || cp.bidi_cls[2] || cp.bidi_cls[3];
