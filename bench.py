@@ -332,7 +332,9 @@ def critical_path(w, emit):
 def c3_bench(steps):
     """c3: gradient of the summed MixDom2 K=3 F=1 C=2 log-likelihood of 200
     pairs (LogNormal median 200) w.r.t. every model parameter and t:
-    torch builders + the forward/backward kernel + autograd composition."""
+    torch builders + the forward/backward kernel + autograd composition.
+    Timed with the builders replayed from HIP graphs (graph=True, the
+    fixed-minibatch training step) and, for reference, launched eagerly."""
     import numpy as np
     import torch
     from tkfmi import synth, vjp
@@ -346,17 +348,19 @@ def c3_bench(steps):
           for k, v in p.items()}
     tt = torch.tensor(w.ts, dtype=torch.float64, device=dev, requires_grad=True)
 
-    def step():
+    def step(graph):
         for v in list(tp.values()) + [tt]:
             v.grad = None
         lp = vjp.mixdom2_log_prob_batch(tp["lam"], tp["mu"], tt, tp["dlam"], tp["dmu"], tp["dw"],
                                         tp["frag_w"], tp["ext"], tp["pis"], tp["S"], tp["cd"],
-                                        w.xs, w.ys)
+                                        w.xs, w.ys, graph=graph)
         lp.sum().backward()
 
-    ms = _time_calls(step, steps, warmup=5)
+    ms_eager = _time_calls(lambda: step(False), steps, warmup=5)
+    ms = _time_calls(lambda: step(True), steps, warmup=5)
     cs = w.cell_states()
     return {"value": cs / (ms / 1e3), "unit": "cell-states/s (value + full gradient)", "ms": ms,
+            "ms_eager_builders": ms_eager, "builders": "HIP graphs (graph=True)",
             "pairs": w.B, "cell_states": cs}
 
 
@@ -541,7 +545,8 @@ def main():
         if "c3" in sel:
             _log("c3 gradients, 200 pairs")
             cfg["c3"] = c3_bench(max(1, args.steps // 2))
-            _log(f"c3: {cfg['c3']['ms']:.1f} ms per gradient")
+            _log(f"c3: {cfg['c3']['ms']:.1f} ms per gradient "
+                 f"(builders launched eagerly: {cfg['c3']['ms_eager_builders']:.1f} ms)")
         if "mid" in sel:
             for K in (3, 5, 8):
                 _log(f"c1-shaped batch at K={K} F=2 ({2 + 10 * K} states)")

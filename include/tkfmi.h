@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define TKFMI_ABI_VERSION 5
+#define TKFMI_ABI_VERSION 6
 
 /* error codes */
 #define TKFMI_OK 0
@@ -424,6 +424,72 @@ size_t tkfmi_build_workspace_bytes(const tkfmi_builder* b, int32_t B);
 int tkfmi_build_params_batched(const tkfmi_builder* b, int32_t B, const double* t,
                                double* log_trans, double* log_tab, void* workspace,
                                size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Builders of the gradient path with their adjoints (tkfmi/vjp.py's
+ * differentiable log-likelihoods, the reference's train/vjp.py surfaces).
+ * The reference differentiates its JAX builders by autodiff; these give the
+ * same derivatives from a few kernels per step.  All arrays are device
+ * arrays, row-major, FP64; calls are asynchronous on `stream`.
+ * ---------------------------------------------------------------------- */
+
+/*
+ * For q < n: T[q] (5 x 5, states S M I D E) = TKF91 (mode 0), the
+ * ancestor-length-conditioned TKF91 (mode 1) or TKF92 (mode 2) at
+ * (lam[q], mu[q], t[q], ext[q]) (core/params.py:29-108 with core/bdi.py:40-97:
+ * the same guard constants and the |1 - lambda/mu| < 1e-4 limits), and
+ * J[q] (5 x 5 x 4) = dT / d(lam, mu, t, ext).  ext may be NULL unless
+ * mode = 2.  Replaces autodiff through the builders of train/vjp.py:110-329.
+ */
+int tkfmi_tkf_trans_jac(const double* lam, const double* mu, const double* t, const double* ext,
+                        int64_t n, int32_t mode, double* T, double* J, void* stream);
+
+/* Bytes of device workspace tkfmi_chi_forward / tkfmi_chi_backward need. */
+size_t tkfmi_chi_workspace_bytes(int64_t B, int32_t nk, int32_t nz);
+
+/*
+ * Collapsed MixDom chi of B pairs (models/mixdom.py:252-423, as the reduced
+ * matrix of the exploded machine, models/exact_suffstats_batch.py:61-282):
+ * the exploded matrix T (N0 x N0) has entry flat[e] = r N0 + c equal to
+ * atoms[b, a1[e]] * atoms[b, a2[e]] for e < n_e (no flat index twice) and
+ * 0 elsewhere; kmap[s] / zmap[s] give state s's index among the nk kept /
+ * nz eliminated states (-1 otherwise).  chi [B,nk,nk] = T_KK + T_KZ W with
+ * W [B,nz,nk] = (I - T_ZZ)^-1 T_ZK (kept for the backward).  I - T_ZZ must
+ * be a nonsingular M-matrix (it is for every MixDom machine: the null states
+ * reach End), which lets the solve run without pivoting.
+ */
+int tkfmi_chi_forward(const double* atoms, int64_t B, int32_t n_atoms, const int32_t* a1,
+                      const int32_t* a2, const int32_t* flat, int32_t n_e, const int32_t* kmap,
+                      const int32_t* zmap, int32_t N0, int32_t nk, int32_t nz, double* chi,
+                      double* W, void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * Adjoint of tkfmi_chi_forward: gatoms [B,n_atoms] = d loss / d atoms from
+ * G [B,nk,nk] = d loss / d chi and the forward's W.  nz + n_atoms <= 8192.
+ */
+int tkfmi_chi_backward(const double* atoms, int64_t B, int32_t n_atoms, const int32_t* a1,
+                       const int32_t* a2, const int32_t* flat, int32_t n_e, const int32_t* kmap,
+                       const int32_t* zmap, int32_t N0, int32_t nk, int32_t nz, const double* G,
+                       const double* W, double* gatoms, void* workspace, size_t workspace_bytes,
+                       void* stream);
+
+#define TKFMI_EXPM_MAX_N 40
+
+/*
+ * out[q] = exp(A[q]) for count n x n matrices, n <= TKFMI_EXPM_MAX_N, by the
+ * rule of jax.scipy.linalg.expm (the reference's transition_matrix,
+ * core/ctmc.py:31-60; see tkfmi_build_params_batched).  One workgroup per
+ * matrix.
+ */
+int tkfmi_expm_batched(const double* A, int64_t count, int32_t n, double* out, void* stream);
+
+/*
+ * Adjoint of exp: out[q] = d loss / d A[q] for G[q] = d loss / d exp(A[q]),
+ * i.e. the Frechet derivative of exp at A[q]^T in the direction G[q] (the
+ * upper-right block of exp([[A^T, G], [0, A^T]])), 2 n <= TKFMI_EXPM_MAX_N.
+ */
+int tkfmi_expm_vjp_batched(const double* A, const double* G, int64_t count, int32_t n,
+                           double* out, void* stream);
 
 /* ------------------------------------------------------------------------
  * Precompiled X/A/Y pairs decoded on the device.

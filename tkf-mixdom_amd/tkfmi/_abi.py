@@ -14,7 +14,7 @@ import torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libtkfmi.so")
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 FORWARD_ONLY = 1
 WANT_POSTERIORS = 2
 NO_COUNTS = 4
@@ -184,7 +184,9 @@ EXPORTED = ("tkfmi_abi_version", "tkfmi_last_error",
             "tkfmi_build_workspace_bytes", "tkfmi_build_params_batched",
             "tkfmi_fb2d_batched_noalloc", "tkfmi_pairs_decode_workspace_bytes",
             "tkfmi_pairs_decode", "tkfmi_autotune_workspace_bytes", "tkfmi_autotune",
-            "tkfmi_checkpoint_budget")
+            "tkfmi_checkpoint_budget", "tkfmi_tkf_trans_jac", "tkfmi_chi_workspace_bytes",
+            "tkfmi_chi_forward", "tkfmi_chi_backward", "tkfmi_expm_batched", "tkfmi_expm_vjp_batched")
+EXPM_MAX_N = 40  # TKFMI_EXPM_MAX_N
 NOALLOC_SCRATCH_BYTES = 576 << 20  # TKFMI_NOALLOC_SCRATCH_BYTES
 
 
@@ -258,6 +260,21 @@ def lib():
             L.tkfmi_build_params_batched.restype = ctypes.c_int
             L.tkfmi_build_params_batched.argtypes = [ctypes.POINTER(Builder), ctypes.c_int32, _c_p, _c_p,
                                                      _c_p, _c_p, ctypes.c_size_t, _c_p]
+            L.tkfmi_tkf_trans_jac.restype = ctypes.c_int
+            L.tkfmi_tkf_trans_jac.argtypes = [_c_p, _c_p, _c_p, _c_p, ctypes.c_int64, ctypes.c_int32,
+                                              _c_p, _c_p, _c_p]
+            L.tkfmi_chi_workspace_bytes.restype = ctypes.c_size_t
+            L.tkfmi_chi_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32]
+            chi_args = [_c_p, ctypes.c_int64, ctypes.c_int32, _c_p, _c_p, _c_p, ctypes.c_int32, _c_p, _c_p,
+                        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+            L.tkfmi_chi_forward.restype = ctypes.c_int
+            L.tkfmi_chi_forward.argtypes = chi_args + [_c_p, _c_p, _c_p, ctypes.c_size_t, _c_p]
+            L.tkfmi_chi_backward.restype = ctypes.c_int
+            L.tkfmi_chi_backward.argtypes = chi_args + [_c_p, _c_p, _c_p, _c_p, ctypes.c_size_t, _c_p]
+            L.tkfmi_expm_batched.restype = ctypes.c_int
+            L.tkfmi_expm_batched.argtypes = [_c_p, ctypes.c_int64, ctypes.c_int32, _c_p, _c_p]
+            L.tkfmi_expm_vjp_batched.restype = ctypes.c_int
+            L.tkfmi_expm_vjp_batched.argtypes = [_c_p, _c_p, ctypes.c_int64, ctypes.c_int32, _c_p, _c_p]
             v = L.tkfmi_abi_version()
             if v != ABI_VERSION:
                 raise TkfmiError(f"libtkfmi ABI {v} != expected {ABI_VERSION}")

@@ -54,6 +54,7 @@ class _Plan:
 
     def __init__(self, K, F):
         self.K, self.F = K, F
+        self._on_dev = {}
         N0 = 8 + 8 * K + 15 * K * F
         self.N0 = N0
         # atom offsets: tau_main (25) | tau_dom (K*25) | dom_w (K) | kappa (K) |
@@ -129,6 +130,25 @@ class _Plan:
         mask = np.ones(N0, bool)
         mask[self.keep] = False
         self.elim = np.nonzero(mask)[0]
+        # the native elimination's view (libtkfmi tkfmi_chi_forward): int32
+        # transition lists and each state's index among the kept / eliminated
+        self.kmap = np.full(N0, -1, np.int32)
+        self.kmap[self.keep] = np.arange(len(self.keep), dtype=np.int32)
+        self.zmap = np.full(N0, -1, np.int32)
+        self.zmap[self.elim] = np.arange(len(self.elim), dtype=np.int32)
+        self.flat32, self.a1_32, self.a2_32 = (x.astype(np.int32) for x in (flat, self.a1, self.a2))
+
+
+    def on(self, name, dev):
+        """Index array ``name`` as a tensor on ``dev``, uploaded once: a
+        call then issues no host-to-device copy (and can be captured in a
+        graph)."""
+        key = (name, str(dev))
+        v = self._on_dev.get(key)
+        if v is None:
+            v = torch.as_tensor(getattr(self, name), device=dev)
+            self._on_dev[key] = v
+        return v
 
 
 @functools.lru_cache(maxsize=None)
@@ -141,9 +161,8 @@ def _f64(x, dev):
         else torch.as_tensor(x, dtype=torch.float64, device=dev)
 
 
-def exploded_trans(main_lam, main_mu, t, dom_lam, dom_mu, dom_w, frag_w, ext_rates):
-    """The exploded transition matrix (..., N0, N0) at t (scalar or (B,));
-    ext_rates (K,F) is put on the diagonal of (K,F,F) (models/mixdom.py:284-287)."""
+def _atoms(main_lam, main_mu, t, dom_lam, dom_mu, dom_w, frag_w, ext_rates):
+    """The exploded machine's atoms (..., n_atoms) at t and its plan."""
     if not isinstance(t, torch.Tensor):
         t = torch.as_tensor(t, dtype=torch.float64)
     t = t.to(torch.float64)
@@ -163,25 +182,38 @@ def exploded_trans(main_lam, main_mu, t, dom_lam, dom_mu, dom_w, frag_w, ext_rat
                        notext.reshape(-1), torch.ones(1, dtype=torch.float64, device=dev)])
     atoms = torch.cat([tau0.reshape(bt + (25,)), taud.reshape(bt + (25 * K,)),
                        fixed.expand(bt + fixed.shape)], dim=-1)
-    a1 = torch.as_tensor(pl.a1, device=dev)
-    a2 = torch.as_tensor(pl.a2, device=dev)
-    vals = atoms[..., a1] * atoms[..., a2]
+    return atoms, pl
+
+
+def _scatter(atoms, pl):
+    dev = atoms.device
+    bt = atoms.shape[:-1]
+    vals = atoms[..., pl.on("a1", dev)] * atoms[..., pl.on("a2", dev)]
     T = torch.zeros(bt + (pl.N0 * pl.N0,), dtype=torch.float64, device=dev)
-    T = T.scatter(-1, torch.as_tensor(pl.flat, device=dev).expand(bt + (len(pl.flat),)), vals)
+    T = T.scatter(-1, pl.on("flat", dev).expand(bt + (len(pl.flat),)), vals)
     return T.reshape(bt + (pl.N0, pl.N0))
+
+
+def exploded_trans(main_lam, main_mu, t, dom_lam, dom_mu, dom_w, frag_w, ext_rates):
+    """The exploded transition matrix (..., N0, N0) at t (scalar or (B,));
+    ext_rates (K,F) is put on the diagonal of (K,F,F) (models/mixdom.py:284-287)."""
+    return _scatter(*_atoms(main_lam, main_mu, t, dom_lam, dom_mu, dom_w, frag_w, ext_rates))
 
 
 def build_nested_trans(main_lam, main_mu, t, dom_lam, dom_mu, dom_w, frag_w, ext_rates):
     """Collapsed MixDom chi (..., N, N), N = 2 + 5KF, the reference's
     build_nested_trans (models/mixdom.py:252-423) obtained by eliminating the
     exploded machine's null states.  ext_rates (K,F) is made diagonal
-    (MixDom1 format, models/mixdom.py:284-287) or given as (K,F,F)."""
-    T = exploded_trans(main_lam, main_mu, t, dom_lam, dom_mu, dom_w, frag_w, ext_rates)
-    K, F = torch.as_tensor(frag_w).shape
-    pl = _plan(K, F)
+    (MixDom1 format, models/mixdom.py:284-287) or given as (K,F,F).  On the
+    GPU the elimination and its adjoint are libtkfmi's (tkfmi.gradops)."""
+    atoms, pl = _atoms(main_lam, main_mu, t, dom_lam, dom_mu, dom_w, frag_w, ext_rates)
+    if atoms.is_cuda:
+        from . import gradops
+        return gradops.chi(atoms, pl)
+    T = _scatter(atoms, pl)
     dev = T.device
-    kp = torch.as_tensor(pl.keep, device=dev)
-    ez = torch.as_tensor(pl.elim, device=dev)
+    kp = pl.on("keep", dev)
+    ez = pl.on("elim", dev)
     T_kk = T[..., kp[:, None], kp[None, :]]
     T_kz = T[..., kp[:, None], ez[None, :]]
     T_zk = T[..., ez[:, None], kp[None, :]]

@@ -78,8 +78,17 @@ def _assemble91(a, b, g, k, cond=False):
     return torch.stack([row_b, row_b, row_b, row_d, row_e], dim=-2)
 
 
+def _native(*xs):
+    """On the GPU the closed forms and their derivatives are libtkfmi's
+    (tkfmi.gradops): one kernel instead of a few dozen elementwise ones."""
+    return xs[0].is_cuda
+
+
 def tkf91_trans(lam, mu, t):
     lam, mu, t = _common(lam, mu, t)
+    if _native(lam, mu, t):
+        from . import gradops
+        return gradops.tkf_trans(lam, mu, t, kind="tkf91")
     return _assemble91(tkf_alpha(mu, t), tkf_beta(lam, mu, t), tkf_gamma(lam, mu, t),
                        tkf_kappa(lam, mu))
 
@@ -87,6 +96,9 @@ def tkf91_trans(lam, mu, t):
 def tkf91_trans_cond(lam, mu, t):
     """core/params.py:57-89 (kappa and 1-kappa factors removed)."""
     lam, mu, t = _common(lam, mu, t)
+    if _native(lam, mu, t):
+        from . import gradops
+        return gradops.tkf_trans(lam, mu, t, kind="tkf91_cond")
     return _assemble91(tkf_alpha(mu, t), tkf_beta(lam, mu, t), tkf_gamma(lam, mu, t),
                        tkf_kappa(lam, mu), cond=True)
 
@@ -94,6 +106,10 @@ def tkf91_trans_cond(lam, mu, t):
 def tkf92_trans(lam, mu, t, ext):
     """core/params.py:92-108: S row unchanged; M/I/D rows scaled by (1-ext)
     with ext added on the diagonal."""
+    lam, mu, t, ext = _common(lam, mu, t, ext)
+    if _native(lam, mu, t, ext):
+        from . import gradops
+        return gradops.tkf_trans(lam, mu, t, ext, kind="tkf92")
     t91 = tkf91_trans(lam, mu, t)
     (ext,) = _common(ext) if isinstance(ext, torch.Tensor) else (torch.as_tensor(ext, dtype=torch.float64, device=t91.device),)
     ext = ext.to(t91.device)

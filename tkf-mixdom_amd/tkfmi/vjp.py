@@ -91,12 +91,18 @@ _MAX_SQUARINGS = 16  # 1-norms up to theta_13 * 2**16 (~3.5e5)
 
 
 def expm(A):
-    """exp(A) for a batch of square FP64 matrices: degree-13 Pade with
-    scaling and squaring.  The scaling exponent is chosen per matrix on the
+    """exp(A) for a batch of square FP64 matrices.  On the GPU (matrices up
+    to 20 x 20) libtkfmi's batched kernel, by the rule of
+    jax.scipy.linalg.expm, with the adjoint of exp's Frechet derivative as
+    its backward (tkfmi.gradops).  Otherwise degree-13 Pade with scaling and
+    squaring in torch: the scaling exponent is chosen per matrix on the
     device and the squarings are masked, so the call never waits for the GPU
     (torch.linalg.matrix_exp copies norms to the host to pick its degree);
     autograd differentiates through it.  Matrices with 1-norm above
-    theta_13 * 2**16 get too few squarings."""
+    theta_13 * 2**16 get too few squarings there, NaN on the GPU route."""
+    from . import gradops
+    if gradops.expm_ok(A):
+        return gradops.expm(A)
     b = _PADE13
     eye = torch.eye(A.shape[-1], dtype=A.dtype, device=A.device)
     norm = A.abs().sum(-2).amax(-1)
@@ -137,22 +143,79 @@ def _dev():
     return hmm.device()
 
 
+# --------------------------------------------------------- graphed builders
+class _GraphedBuilder:
+    """A builder's forward and backward captured as two HIP graphs for one
+    set of argument shapes (torch.cuda.make_graphed_callables).  The graphs
+    replay into fixed buffers, so a call's outputs and saved values stay
+    valid only until the next call: a call made while the previous call's
+    backward has not run yet is computed eagerly instead."""
+
+    def __init__(self, fn, args):
+        sample = tuple(a.detach().clone().requires_grad_(a.requires_grad) for a in args)
+        self.g = torch.cuda.make_graphed_callables(fn, sample)
+        self.fn = fn
+        self.pending = False
+
+    def _done(self, grad):
+        self.pending = False
+        return grad
+
+    def __call__(self, args):
+        if self.pending:
+            return self.fn(*args)
+        out = self.g(*args)
+        self.pending = True
+        out[0].register_hook(self._done)
+        return out
+
+
+_GRAPHED = {}
+
+
+def _built(fn, args, graph):
+    """fn(*args) -> (log transitions, emission tables), or, with graph=True
+    and a gradient to take, the same through a _GraphedBuilder for these
+    argument shapes: a training step over a fixed minibatch shape then
+    replays two graphs instead of launching the builders' ~2,600 small
+    kernels one by one."""
+    if not (graph and torch.is_grad_enabled() and any(a.requires_grad for a in args)):
+        return fn(*args)
+    key = (fn, tuple((tuple(a.shape), a.requires_grad, str(a.device)) for a in args))
+    gb = _GRAPHED.get(key)
+    if gb is None:
+        gb = _GRAPHED[key] = _GraphedBuilder(fn, args)
+    return gb(args)
+
+
 def _as(x, dev):
     return x.to(device=dev, dtype=torch.float64) if isinstance(x, torch.Tensor) \
         else torch.as_tensor(x, dtype=torch.float64, device=dev)
 
 
 # ----------------------------------------------------------------- TKF91/92
-def tkf91_log_prob_batch(ins_rate, del_rate, ts, Q, pi, xs, ys, *, cond=False, generic=False, kernel=None):
+def _shared_tab(Q, pi, ts):
+    sub = transition_matrix(Q, ts)                                 # (B,A,A)
+    return tables.shared_tables(sub, pi.expand(sub.shape[:-1]))    # (B,1,TAB)
+
+
+def _tkf91_build(ins_rate, del_rate, ts, Q, pi):
+    return hmm.safe_log(params.tkf91_trans(ins_rate, del_rate, ts)), _shared_tab(Q, pi, ts)
+
+
+def _tkf91c_build(ins_rate, del_rate, ts, Q, pi):
+    return hmm.safe_log(params.tkf91_trans_cond(ins_rate, del_rate, ts)), _shared_tab(Q, pi, ts)
+
+
+def tkf91_log_prob_batch(ins_rate, del_rate, ts, Q, pi, xs, ys, *, cond=False, generic=False, kernel=None,
+                         graph=False):
     """(B,) log P(x_b, y_b | lambda, mu, t_b, Q, pi) under TKF91 with the
     shared emissions of pair_hmm_emissions (train/vjp.py:110-194); ``cond``
-    uses the ancestor-length-conditioned matrix (train/vjp.py:198-249)."""
+    uses the ancestor-length-conditioned matrix (train/vjp.py:198-249).
+    ``graph`` replays the parameter builders from HIP graphs (see _built)."""
     dev = _dev()
-    ts = _as(ts, dev)
-    build = params.tkf91_trans_cond if cond else params.tkf91_trans
-    log_trans = hmm.safe_log(build(_as(ins_rate, dev), _as(del_rate, dev), ts))
-    sub = transition_matrix(_as(Q, dev), ts)                       # (B,A,A)
-    tab = tables.shared_tables(sub, _as(pi, dev).expand(sub.shape[:-1]))  # (B,1,TAB)
+    args = tuple(_as(x, dev) for x in (ins_rate, del_rate, ts, Q, pi))
+    log_trans, tab = _built(_tkf91c_build if cond else _tkf91_build, args, graph)
     return log_prob_batch(log_trans, params.TKF_STATE_TYPES, xs, ys, tab, generic=generic, kernel=kernel)
 
 
@@ -168,14 +231,16 @@ def tkf91_log_prob_cond(ins_rate, del_rate, t, Q, pi, x_seq, y_seq):
                                 [x_seq], [y_seq], cond=True)[0]
 
 
-def tkf92_log_prob_batch(ins_rate, del_rate, ts, ext, Q, pi, xs, ys, *, generic=False, kernel=None):
+def _tkf92_build(ins_rate, del_rate, ts, ext, Q, pi):
+    return hmm.safe_log(params.tkf92_trans(ins_rate, del_rate, ts, ext)), _shared_tab(Q, pi, ts)
+
+
+def tkf92_log_prob_batch(ins_rate, del_rate, ts, ext, Q, pi, xs, ys, *, generic=False, kernel=None,
+                         graph=False):
     """(B,) TKF92 log-likelihoods (train/vjp.py:253-329)."""
     dev = _dev()
-    ts = _as(ts, dev)
-    log_trans = hmm.safe_log(params.tkf92_trans(_as(ins_rate, dev), _as(del_rate, dev), ts,
-                                                _as(ext, dev)))
-    sub = transition_matrix(_as(Q, dev), ts)
-    tab = tables.shared_tables(sub, _as(pi, dev).expand(sub.shape[:-1]))
+    args = tuple(_as(x, dev) for x in (ins_rate, del_rate, ts, ext, Q, pi))
+    log_trans, tab = _built(_tkf92_build, args, graph)
     return log_prob_batch(log_trans, params.TKF_STATE_TYPES, xs, ys, tab, generic=generic, kernel=kernel)
 
 
@@ -187,20 +252,23 @@ def tkf92_log_prob(ins_rate, del_rate, t, ext, Q, pi, x_seq, y_seq):
 
 
 # ------------------------------------------------------------------ MixDom
+def _mixdom_build(main_ins_rate, main_del_rate, ts, dom_ins_rates, dom_del_rates, dom_weights,
+                  frag_weights, ext_rates, Q, pi):
+    chi = mixdom.build_nested_trans(main_ins_rate, main_del_rate, ts, dom_ins_rates, dom_del_rates,
+                                    dom_weights, frag_weights, ext_rates)
+    return hmm.safe_log(chi), _shared_tab(Q, pi, ts)
+
+
 def mixdom_log_prob_batch(main_ins_rate, main_del_rate, ts, dom_ins_rates, dom_del_rates,
-                          dom_weights, frag_weights, ext_rates, Q, pi, xs, ys, *, generic=False, kernel=None):
+                          dom_weights, frag_weights, ext_rates, Q, pi, xs, ys, *, generic=False, kernel=None,
+                          graph=False):
     """(B,) MixDom log-likelihoods with shared emissions (train/vjp.py:351-458)."""
     dev = _dev()
-    ts = _as(ts, dev)
-    chi = mixdom.build_nested_trans(_as(main_ins_rate, dev), _as(main_del_rate, dev), ts,
-                                    _as(dom_ins_rates, dev), _as(dom_del_rates, dev),
-                                    _as(dom_weights, dev), _as(frag_weights, dev),
-                                    _as(ext_rates, dev))
-    K, F = _as(frag_weights, dev).shape
-    st = mixdom.state_types(K, F)
-    sub = transition_matrix(_as(Q, dev), ts)
-    tab = tables.shared_tables(sub, _as(pi, dev).expand(sub.shape[:-1]))
-    return log_prob_batch(hmm.safe_log(chi), st, xs, ys, tab, generic=generic, kernel=kernel)
+    args = tuple(_as(x, dev) for x in (main_ins_rate, main_del_rate, ts, dom_ins_rates, dom_del_rates,
+                                       dom_weights, frag_weights, ext_rates, Q, pi))
+    K, F = args[6].shape
+    log_trans, tab = _built(_mixdom_build, args, graph)
+    return log_prob_batch(log_trans, mixdom.state_types(K, F), xs, ys, tab, generic=generic, kernel=kernel)
 
 
 def mixdom_log_prob(main_ins_rate, main_del_rate, t, dom_ins_rates, dom_del_rates, dom_weights,
@@ -219,23 +287,31 @@ def build_class_subs(class_S_exch, class_pis, ts):
     return transition_matrix(Qc[None], ts[:, None])               # (B,C,A,A)
 
 
+def _mixdom2_build(main_ins_rate, main_del_rate, ts, dom_ins_rates, dom_del_rates, dom_weights,
+                   frag_weights, ext_rates, class_pis, class_S_exch, class_dist):
+    chi = mixdom.build_nested_trans(main_ins_rate, main_del_rate, ts, dom_ins_rates, dom_del_rates,
+                                    dom_weights, frag_weights, ext_rates)
+    subs = build_class_subs(class_S_exch, class_pis, ts)
+    tab = tables.per_class_tables(subs, class_pis, class_dist)     # (B,KF,TAB)
+    return hmm.safe_log(chi), tab
+
+
 def mixdom2_log_prob_batch(main_ins_rate, main_del_rate, ts, dom_ins_rates, dom_del_rates,
                            dom_weights, frag_weights, ext_rates, class_pis, class_S_exch,
-                           class_dist, xs, ys, *, generic=False, kernel=None, checkpoint=False):
+                           class_dist, xs, ys, *, generic=False, kernel=None, checkpoint=False,
+                           graph=False):
     """(B,) MixDom2 log-likelihoods: per-(domain, fragment) class-mixture
-    emissions (train/vjp.py:628-820; dp/hmm.py:673-783)."""
+    emissions (train/vjp.py:628-820; dp/hmm.py:673-783).  ``graph`` replays
+    the parameter builders from HIP graphs (see _built)."""
     dev = _dev()
-    ts = _as(ts, dev)
-    chi = mixdom.build_nested_trans(_as(main_ins_rate, dev), _as(main_del_rate, dev), ts,
-                                    _as(dom_ins_rates, dev), _as(dom_del_rates, dev),
-                                    _as(dom_weights, dev), _as(frag_weights, dev),
-                                    _as(ext_rates, dev))
-    K, F = _as(frag_weights, dev).shape
+    args = tuple(_as(x, dev) for x in (main_ins_rate, main_del_rate, ts, dom_ins_rates, dom_del_rates,
+                                       dom_weights, frag_weights, ext_rates, class_pis, class_S_exch,
+                                       class_dist))
+    K, F = args[6].shape
     st = mixdom.state_types(K, F)
-    subs = build_class_subs(_as(class_S_exch, dev), _as(class_pis, dev), ts)
-    tab = tables.per_class_tables(subs, _as(class_pis, dev), _as(class_dist, dev))  # (B,KF,TAB)
+    log_trans, tab = _built(_mixdom2_build, args, graph)
     groups = tables.state_groups_per_class(len(st), F)
-    return log_prob_batch(hmm.safe_log(chi), st, xs, ys, tab, groups, generic=generic, kernel=kernel,
+    return log_prob_batch(log_trans, st, xs, ys, tab, groups, generic=generic, kernel=kernel,
                           checkpoint=checkpoint)
 
 
