@@ -486,7 +486,9 @@ int tkfmi_expm_batched(const double* A, int64_t count, int32_t n, double* out, v
 /*
  * Adjoint of exp: out[q] = d loss / d A[q] for G[q] = d loss / d exp(A[q]),
  * i.e. the Frechet derivative of exp at A[q]^T in the direction G[q] (the
- * upper-right block of exp([[A^T, G], [0, A^T]])), 2 n <= TKFMI_EXPM_MAX_N.
+ * upper-right block of exp([[A^T, G], [0, A^T]])), computed with the
+ * degree-13 Pade approximant and the squarings exp(A[q]) takes;
+ * 2 n <= TKFMI_EXPM_MAX_N.
  */
 int tkfmi_expm_vjp_batched(const double* A, const double* G, int64_t count, int32_t n,
                            double* out, void* stream);

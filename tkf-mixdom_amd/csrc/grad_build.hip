@@ -16,11 +16,15 @@
 //    I - T_ZZ is a nonsingular M-matrix (the null states reach End), so the
 //    Gauss-Jordan solves need no pivoting.  One workgroup per pair.
 //  * exp(A) of a batch of small matrices (the rule of jax.scipy.linalg.expm,
-//    core/ctmc.py:31-60) and its adjoint, the Frechet derivative of exp at
-//    A^T in the direction G: the upper-right block of
-//    exp([[A^T, G], [0, A^T]]) (Higham, "Functions of Matrices", SIAM 2008,
-//    thm. 3.6), with G scaled by a power of two to the norm of A and the
-//    block scaled back.  One workgroup per matrix, everything in LDS.
+//    core/ctmc.py:31-60) and its adjoint, the Frechet derivative L(A^T, G)
+//    of exp at A^T in the direction G.  L is computed alongside the
+//    degree-13 Pade approximant and its squarings, each product X Y paired
+//    with its derivative X dY + dX Y (Al-Mohy and Higham, "Computing the
+//    Frechet derivative of the matrix exponential", SIAM J. Matrix Anal.
+//    Appl. 30 (2009) 1639-1657, alg. 6.4, with degree 13 throughout):
+//    n x n products only, where exp of the doubled matrix
+//    [[A^T, G], [0, A^T]] would take eight times the work.  One workgroup per
+//    matrix, everything in LDS.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -172,16 +176,23 @@ __device__ void gauss_jordan(double* M, int nz, int nk, double* fac) {
   }
 }
 
+// The solves' nz x (nz + nk) matrix lives in LDS when it fits
+// (kChiLdsDoubles), else in the pair's workspace slice.
+constexpr int kChiLdsDoubles = 14 * 1024;
+
 // chi [B, nk, nk] and W = (I - T_ZZ)^-1 T_ZK [B, nz, nk].
-__global__ __launch_bounds__(256) void chi_fwd_kernel(ChiArgs a, double* __restrict__ chi, double* __restrict__ Wout) {
-  extern __shared__ double s_fac[];
+__global__ __launch_bounds__(256) void chi_fwd_kernel(ChiArgs a, double* __restrict__ chi, double* __restrict__ Wout,
+                                                      int lds_m) {
+  extern __shared__ double s_dyn[];
+  double* s_fac = s_dyn;
   const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
   const int nk = a.nk, nz = a.nz, ld = nz + nk;
   double* Tkk = a.ws + (size_t)b * chi_slice(a);
   double* Tkz = Tkk + (size_t)nk * nk;
-  double* Mz = Tkz + (size_t)nk * nz;  // [I - T_ZZ | T_ZK]
+  double* Mz = lds_m ? s_dyn + nz : Tkz + (size_t)nk * nz;  // [I - T_ZZ | T_ZK]
   const double* at = a.atoms + (size_t)b * a.n_atoms;
-  for (size_t q = tid; q < chi_slice(a); q += nt) Tkk[q] = 0.0;
+  for (size_t q = tid; q < (size_t)nk * nk + (size_t)nk * nz; q += nt) Tkk[q] = 0.0;
+  for (int q = tid; q < nz * ld; q += nt) Mz[q] = 0.0;
   __syncthreads();
   for (int z = tid; z < nz; z += nt) Mz[(size_t)z * ld + z] = 1.0;
   __syncthreads();
@@ -214,18 +225,20 @@ __global__ __launch_bounds__(256) void chi_fwd_kernel(ChiArgs a, double* __restr
 // d loss / d atoms [B, n_atoms] from G = d loss / d chi [B, nk, nk] and the
 // forward's W.
 __global__ __launch_bounds__(256) void chi_bwd_kernel(ChiArgs a, const double* __restrict__ G,
-                                                      const double* __restrict__ Wall, double* __restrict__ gatoms) {
+                                                      const double* __restrict__ Wall, double* __restrict__ gatoms,
+                                                      int lds_m) {
   extern __shared__ double s_dyn[];
   const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
   const int nk = a.nk, nz = a.nz, ld = nz + nk;
   double* s_fac = s_dyn;             // nz
   double* s_ga = s_dyn + nz;         // n_atoms
   double* Tkz = a.ws + (size_t)b * chi_slice(a);
-  double* Mt = Tkz + (size_t)nk * nz;  // [(I - T_ZZ)^T | T_KZ^T G]
+  double* Mt = lds_m ? s_ga + a.n_atoms : Tkz + (size_t)nk * nz;  // [(I - T_ZZ)^T | T_KZ^T G]
   const double* at = a.atoms + (size_t)b * a.n_atoms;
   const double* Gb = G + (size_t)b * nk * nk;
   const double* W = Wall + (size_t)b * nz * nk;
-  for (size_t q = tid; q < (size_t)nk * nz + (size_t)nz * ld; q += nt) Tkz[q] = 0.0;
+  for (size_t q = tid; q < (size_t)nk * nz; q += nt) Tkz[q] = 0.0;
+  for (int q = tid; q < nz * ld; q += nt) Mt[q] = 0.0;
   for (int q = tid; q < a.n_atoms; q += nt) s_ga[q] = 0.0;
   __syncthreads();
   for (int z = tid; z < nz; z += nt) Mt[(size_t)z * ld + z] = 1.0;
@@ -295,59 +308,196 @@ __global__ __launch_bounds__(kExpmThreads) void expm_fwd_kernel(const double* __
   for (int q = threadIdx.x; q < m2; q += blockDim.x) ob[q] = s_out[q];
 }
 
-// out[b] = L(A[b]^T, G[b]), the adjoint of exp's Frechet derivative at A[b].
+// C (+)= X Y (n x n, row-major, LDS); caller syncs.
+__device__ inline void mm_to(const double* X, const double* Y, double* C, int n, bool add) {
+  for (int q = threadIdx.x; q < n * n; q += blockDim.x) {
+    const int r = q / n, c = q % n;
+    double acc = add ? C[q] : 0.0;
+    for (int k = 0; k < n; ++k) acc = fma(X[r * n + k], Y[k * n + c], acc);
+    C[q] = acc;
+  }
+}
+
+constexpr int kFrechetBufs = 18;  // n x n LDS matrices used by expm_vjp_kernel
+size_t frechet_lds_bytes(int n) { return (size_t)kFrechetBufs * n * n * 8 + (size_t)n * 8 + 64; }
+
+// out[b] = L(A[b]^T, G[b]), the adjoint of exp's Frechet derivative at A[b]:
+// with X = A^T / 2^s and E = G / 2^s (s squarings as exp(A) takes them),
+// the degree-13 Pade approximant R = (V - U)^-1 (V + U) and its derivative
+// L = (V - U)^-1 ((LU + LV) + (LU - LV) R), then s times L <- R L + L R,
+// R <- R R.
 __global__ __launch_bounds__(kExpmThreads) void expm_vjp_kernel(const double* __restrict__ A,
                                                                  const double* __restrict__ G, int n,
                                                                  double* __restrict__ out) {
   extern __shared__ double s_dyn[];
-  const int m = 2 * n, mm = m * m, n2 = n * n;
-  double* s_in = s_dyn;
-  double* s_out = s_in + mm;
-  double* s_w = s_out + mm;
-  int* s_sh = reinterpret_cast<int*>(s_w + kExpmBufs * mm);
-  __shared__ double s_norm[2];
-  __shared__ int s_scale;
+  const int n2 = n * n, tid = threadIdx.x, nt = blockDim.x;
+  double* X = s_dyn;
+  double* E = X + n2;
+  double* X2 = E + n2;
+  double* L2 = X2 + n2;
+  double* X4 = L2 + n2;
+  double* L4 = X4 + n2;
+  double* X6 = L4 + n2;
+  double* L6 = X6 + n2;
+  double* T = L6 + n2;
+  double* LT = T + n2;
+  double* W = LT + n2;
+  double* LW = W + n2;
+  double* U = LW + n2;
+  double* LU = U + n2;
+  double* M = LU + n2;  // n x 4n: [V - U | V + U | LU + LV | LU - LV]
+  double* fac = M + 4 * n2;
+  __shared__ int s_ns, s_piv;
   const double* Ab = A + (size_t)blockIdx.x * n2;
   const double* Gb = G + (size_t)blockIdx.x * n2;
-  if (threadIdx.x == 0) {  // 1-norms of A^T and G: the scale 2^k of G that matches them
-    double na = 0.0, ng = 0.0;
-    for (int c = 0; c < n; ++c) {
-      double ca = 0.0, cg = 0.0;
-      for (int r = 0; r < n; ++r) {
-        ca += fabs(Ab[c * n + r]);  // column c of A^T
-        cg += fabs(Gb[r * n + c]);
-      }
-      na = fmax(na, ca);
-      ng = fmax(ng, cg);
-    }
-    s_norm[0] = na;
-    s_norm[1] = ng;
-    int k = 0;
-    if (ng > 0.0 && na > 0.0) k = ilogb(na) - ilogb(ng);
-    else if (ng > 0.0) k = -ilogb(ng);
-    s_scale = k;
+  double* ob = out + (size_t)blockIdx.x * n2;
+  for (int q = tid; q < n2; q += nt) {
+    const int r = q / n, c = q % n;
+    X[q] = Ab[c * n + r];
+    E[q] = Gb[q];
   }
   __syncthreads();
-  double* ob = out + (size_t)blockIdx.x * n2;
-  if (!(s_norm[1] > 0.0)) {  // G = 0 (or NaN: propagate)
-    for (int q = threadIdx.x; q < n2; q += blockDim.x) ob[q] = (s_norm[1] == 0.0) ? 0.0 : __builtin_nan("");
+  if (tid == 0) {  // the squarings exp(A) takes (expm_lds: from the 1-norm of A)
+    double nrm = 0.0;
+    for (int c = 0; c < n; ++c) {
+      double cs = 0.0;
+      for (int r = 0; r < n; ++r) cs += fabs(Ab[r * n + c]);
+      if (!(cs <= nrm)) nrm = cs;
+    }
+    double sq = floor(log2(nrm / kTheta13));
+    sq = (sq > 0.0) ? sq : 0.0;
+    s_ns = (nrm != nrm || sq > (double)kExpmMaxSquarings) ? -1 : (int)sq;
+  }
+  __syncthreads();
+  const int ns = s_ns;
+  if (ns < 0) {  // exp(A) is NaN there (expm_lds): so is its gradient
+    for (int q = tid; q < n2; q += nt) ob[q] = __builtin_nan("");
     return;
   }
-  const int k = s_scale;
-  for (int q = threadIdx.x; q < mm; q += blockDim.x) {
-    const int r = q / m, c = q % m;
-    double v = 0.0;
-    if (r < n && c < n) v = Ab[c * n + r];                      // A^T
-    else if (r >= n && c >= n) v = Ab[(c - n) * n + (r - n)];   // A^T
-    else if (r < n) v = ldexp(Gb[r * n + (c - n)], k);          // 2^k G
-    s_in[q] = v;
+  const double sc = ldexp(1.0, -ns);
+  for (int q = tid; q < n2; q += nt) { X[q] *= sc; E[q] *= sc; }
+  __syncthreads();
+  const double* b = kPadeB[4];
+  mm_to(X, X, X2, n, false);
+  mm_to(X, E, L2, n, false);
+  __syncthreads();
+  mm_to(E, X, L2, n, true);
+  mm_to(X2, X2, X4, n, false);
+  __syncthreads();
+  mm_to(X2, L2, L4, n, false);
+  mm_to(X4, X2, X6, n, false);
+  __syncthreads();
+  mm_to(L2, X2, L4, n, true);
+  mm_to(X4, L2, L6, n, false);
+  __syncthreads();
+  mm_to(L4, X2, L6, n, true);
+  __syncthreads();
+  // odd part: W = X6 (b13 X6 + b11 X4 + b9 X2) + b7 X6 + b5 X4 + b3 X2 + b1 I, U = X W
+  for (int q = tid; q < n2; q += nt) {
+    T[q] = b[13] * X6[q] + b[11] * X4[q] + b[9] * X2[q];
+    LT[q] = b[13] * L6[q] + b[11] * L4[q] + b[9] * L2[q];
   }
   __syncthreads();
-  expm_lds(s_in, 1.0, m, s_out, s_w, s_sh);
-  for (int q = threadIdx.x; q < n2; q += blockDim.x) {
-    const int r = q / n, c = q % n;
-    ob[q] = ldexp(s_out[r * m + n + c], -k);
+  mm_to(X6, T, W, n, false);
+  mm_to(L6, T, LW, n, false);
+  __syncthreads();
+  mm_to(X6, LT, LW, n, true);
+  for (int q = tid; q < n2; q += nt) {
+    const double id = (q / n == q % n) ? 1.0 : 0.0;
+    W[q] += b[7] * X6[q] + b[5] * X4[q] + b[3] * X2[q] + b[1] * id;
   }
+  __syncthreads();
+  for (int q = tid; q < n2; q += nt) LW[q] += b[7] * L6[q] + b[5] * L4[q] + b[3] * L2[q];
+  mm_to(X, W, U, n, false);
+  mm_to(E, W, LU, n, false);
+  __syncthreads();
+  mm_to(X, LW, LU, n, true);
+  // even part: V = X6 (b12 X6 + b10 X4 + b8 X2) + b6 X6 + b4 X4 + b2 X2 + b0 I (into W, LW)
+  for (int q = tid; q < n2; q += nt) {
+    T[q] = b[12] * X6[q] + b[10] * X4[q] + b[8] * X2[q];
+    LT[q] = b[12] * L6[q] + b[10] * L4[q] + b[8] * L2[q];
+  }
+  __syncthreads();
+  mm_to(X6, T, W, n, false);
+  mm_to(L6, T, LW, n, false);
+  __syncthreads();
+  mm_to(X6, LT, LW, n, true);
+  for (int q = tid; q < n2; q += nt) {
+    const double id = (q / n == q % n) ? 1.0 : 0.0;
+    W[q] += b[6] * X6[q] + b[4] * X4[q] + b[2] * X2[q] + b[0] * id;
+  }
+  __syncthreads();
+  const int ld = 4 * n;
+  for (int q = tid; q < n2; q += nt) {
+    LW[q] += b[6] * L6[q] + b[4] * L4[q] + b[2] * L2[q];
+  }
+  __syncthreads();
+  for (int q = tid; q < n2; q += nt) {
+    const int r = q / n, c = q % n;
+    M[r * ld + c] = W[q] - U[q];
+    M[r * ld + n + c] = W[q] + U[q];
+    M[r * ld + 2 * n + c] = LU[q] + LW[q];
+    M[r * ld + 3 * n + c] = LU[q] - LW[q];
+  }
+  __syncthreads();
+  // Gauss-Jordan with partial pivoting on M (the rule expm_lds solves by)
+  for (int k = 0; k < n; ++k) {
+    if (tid == 0) {
+      int p = k;
+      double best = fabs(M[k * ld + k]);
+      for (int r = k + 1; r < n; ++r) {
+        const double v = fabs(M[r * ld + k]);
+        if (v > best) { best = v; p = r; }
+      }
+      s_piv = p;
+    }
+    __syncthreads();
+    const int p = s_piv;
+    if (p != k) {
+      for (int c = tid; c < ld; c += nt) {
+        const double tmp = M[k * ld + c];
+        M[k * ld + c] = M[p * ld + c];
+        M[p * ld + c] = tmp;
+      }
+      __syncthreads();
+    }
+    for (int r = tid; r < n; r += nt) fac[r] = (r == k) ? 0.0 : M[r * ld + k] / M[k * ld + k];
+    __syncthreads();
+    const int w = ld - k - 1;
+    for (int q = tid; q < n * w; q += nt) {
+      const int r = q / w, c = k + 1 + q % w;
+      if (r != k) M[r * ld + c] = fma(-fac[r], M[k * ld + c], M[r * ld + c]);
+    }
+    __syncthreads();
+  }
+  double* R = X2;   // (V - U)^-1 (V + U)
+  double* P = X4;   // (V - U)^-1 (LU + LV)
+  double* Q = X6;   // (V - U)^-1 (LU - LV)
+  for (int q = tid; q < n2; q += nt) {
+    const int r = q / n, c = q % n;
+    const double d = M[r * ld + r];
+    R[q] = M[r * ld + n + c] / d;
+    P[q] = M[r * ld + 2 * n + c] / d;
+    Q[q] = M[r * ld + 3 * n + c] / d;
+  }
+  __syncthreads();
+  double* L = L2;
+  for (int q = tid; q < n2; q += nt) L[q] = P[q];
+  __syncthreads();
+  mm_to(Q, R, L, n, true);  // L = P + Q R
+  __syncthreads();
+  double* R2 = T;
+  double* Ln = LT;
+  for (int i = 0; i < ns; ++i) {  // L <- R L + L R, R <- R R
+    mm_to(R, L, Ln, n, false);
+    mm_to(R, R, R2, n, false);
+    __syncthreads();
+    mm_to(L, R, Ln, n, true);
+    __syncthreads();
+    double* t1 = R; R = R2; R2 = t1;
+    double* t2 = L; L = Ln; Ln = t2;
+  }
+  for (int q = tid; q < n2; q += nt) ob[q] = L[q];
 }
 
 }  // namespace tkfmi_gb
@@ -403,8 +553,12 @@ int tkfmi_chi_forward(const double* atoms, int64_t B, int32_t n_atoms, const int
   if (rc != TKFMI_OK) return rc;
   if (!chi || !W) return tkfmi_fail(TKFMI_EINVAL, "tkfmi_chi_forward: output pointer missing");
   if (B == 0) return TKFMI_OK;
-  hipLaunchKernelGGL(chi_fwd_kernel, dim3((unsigned)B), dim3(256), (size_t)std::max(nz, 1) * 8,
-                     reinterpret_cast<hipStream_t>(stream_v), a, chi, W);
+  const int lds_m = (size_t)nz * (nz + nk) <= (size_t)kChiLdsDoubles;
+  const size_t lds = (size_t)(std::max(nz, 1) + (lds_m ? nz * (nz + nk) : 0)) * 8;
+  hipError_t e = hipFuncSetAttribute((const void*)chi_fwd_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return tkfmi_fail(TKFMI_EHIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+  hipLaunchKernelGGL(chi_fwd_kernel, dim3((unsigned)B), dim3(256), lds, reinterpret_cast<hipStream_t>(stream_v), a,
+                     chi, W, lds_m);
   return launch_check("chi_fwd_kernel");
 }
 
@@ -419,8 +573,12 @@ int tkfmi_chi_backward(const double* atoms, int64_t B, int32_t n_atoms, const in
   if (!G || !W || !gatoms) return tkfmi_fail(TKFMI_EINVAL, "tkfmi_chi_backward: pointer missing");
   if (nz + n_atoms > 8192) return tkfmi_fail(TKFMI_EINVAL, "tkfmi_chi_backward: nz + n_atoms > 8192");
   if (B == 0) return TKFMI_OK;
-  hipLaunchKernelGGL(chi_bwd_kernel, dim3((unsigned)B), dim3(256), (size_t)(nz + n_atoms) * 8,
-                     reinterpret_cast<hipStream_t>(stream_v), a, G, W, gatoms);
+  const int lds_m = (size_t)nz * (nz + nk) + nz + n_atoms <= (size_t)kChiLdsDoubles;
+  const size_t lds = (size_t)(nz + n_atoms + (lds_m ? nz * (nz + nk) : 0)) * 8;
+  hipError_t e = hipFuncSetAttribute((const void*)chi_bwd_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return tkfmi_fail(TKFMI_EHIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+  hipLaunchKernelGGL(chi_bwd_kernel, dim3((unsigned)B), dim3(256), lds, reinterpret_cast<hipStream_t>(stream_v), a, G,
+                     W, gatoms, lds_m);
   return launch_check("chi_bwd_kernel");
 }
 
@@ -443,7 +601,7 @@ int tkfmi_expm_vjp_batched(const double* A, const double* G, int64_t count, int3
     return tkfmi_fail(TKFMI_EINVAL, "tkfmi_expm_vjp_batched: n must be 1..20");
   if (count == 0) return TKFMI_OK;
   if (!A || !G || !out) return tkfmi_fail(TKFMI_EINVAL, "tkfmi_expm_vjp_batched: pointer missing");
-  const size_t lds = expm_lds_bytes(2 * n);
+  const size_t lds = frechet_lds_bytes(n);
   hipError_t e = hipFuncSetAttribute((const void*)expm_vjp_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return tkfmi_fail(TKFMI_EHIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
   hipLaunchKernelGGL(expm_vjp_kernel, dim3((unsigned)count), dim3(kExpmThreads), lds,

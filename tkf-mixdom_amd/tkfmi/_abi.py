@@ -322,8 +322,18 @@ class PackedPairs:
         # cost ~10 us each)
         xcat = normalise_codes(np.concatenate(xs + [np.zeros(1, np.int64)]))
         ycat = normalise_codes(np.concatenate(ys + [np.zeros(1, np.int64)]))
-        self.x_cat = torch.from_numpy(xcat).to(device)
-        self.y_cat = torch.from_numpy(ycat).to(device)
+        if torch.device(device).type == "cuda":
+            # one stream-ordered copy from pinned memory: a copy from pageable
+            # memory would wait for all the GPU work queued before it
+            host = torch.empty(len(xcat) + len(ycat), dtype=torch.uint8, pin_memory=True)
+            hn = host.numpy()
+            hn[:len(xcat)] = xcat
+            hn[len(xcat):] = ycat
+            both = host.to(device, non_blocking=True)
+            self.x_cat, self.y_cat = both[:len(xcat)], both[len(xcat):]
+        else:
+            self.x_cat = torch.from_numpy(xcat)
+            self.y_cat = torch.from_numpy(ycat)
         self.cells = (self.x_len.astype(np.int64) + 1) * (self.y_len.astype(np.int64) + 1)
 
 
