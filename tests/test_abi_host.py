@@ -326,3 +326,41 @@ def test_chain_config_validates_arguments():
     assert L.tkfmi_chain_schedule(-0.1, -1) < 0
     assert b"tkfmi_chain_schedule" in L.tkfmi_last_error()
     autotune.set_chain_config(*autotune.DEFAULT)
+
+
+def test_gradient_builders_validate_arguments_on_the_host():
+    """The gradient path's builder entry points reject bad sizes and missing
+    pointers before any GPU call, with a message; the elimination's
+    workspace size is nk^2 + nk nz + nz (nz + nk) doubles per pair."""
+    from tkfmi import _abi, mixdom
+    L = _abi.lib()
+    P = ctypes.c_void_p
+    fake = P(16)  # never dereferenced: validation fails first
+    assert L.tkfmi_tkf_trans_jac(fake, fake, fake, None, 4, 3, fake, fake, None) == -1
+    assert b"mode" in L.tkfmi_last_error()
+    assert L.tkfmi_tkf_trans_jac(fake, fake, fake, None, 4, 2, fake, fake, None) == -1  # TKF92 needs ext
+    assert b"pointer" in L.tkfmi_last_error()
+    assert L.tkfmi_tkf_trans_jac(None, None, None, None, 0, 0, None, None, None) == 0  # n = 0: nothing to do
+    assert L.tkfmi_expm_batched(fake, 3, 41, fake, None) == -1
+    assert b"1..40" in L.tkfmi_last_error()
+    assert L.tkfmi_expm_vjp_batched(fake, fake, 3, 21, fake, None) == -1
+    assert b"1..20" in L.tkfmi_last_error()
+    assert L.tkfmi_expm_batched(None, 0, 20, None, None) == 0
+    pl = mixdom._plan(3, 1)
+    nk, nz = len(pl.keep), len(pl.elim)
+    assert (nk, nz, pl.N0) == (17, 60, 77)
+    assert L.tkfmi_chi_workspace_bytes(200, nk, nz) == 200 * (nk * nk + nk * nz + nz * (nz + nk)) * 8
+    idx = [P(16)] * 3
+    args = [fake, 5, pl.n_atoms, idx[0], idx[1], idx[2], len(pl.flat), fake, fake, pl.N0, nk, nz]
+    small = L.tkfmi_chi_workspace_bytes(5, nk, nz) - 8
+    assert L.tkfmi_chi_forward(*args, fake, fake, fake, small, None) == -2
+    assert b"workspace too small" in L.tkfmi_last_error()
+    bad = list(args)
+    bad[10], bad[11] = 70, 60  # nk + nz > N0
+    assert L.tkfmi_chi_forward(*bad, fake, fake, fake, 1 << 30, None) == -1
+    assert L.tkfmi_chi_backward(*args, None, fake, fake, fake, 1 << 30, None) == -1
+    assert b"pointer" in L.tkfmi_last_error()
+    # the plan's native view: every state kept or eliminated, maps inverse to the lists
+    assert np.all((pl.kmap >= 0) != (pl.zmap >= 0))
+    assert np.array_equal(pl.kmap[pl.keep], np.arange(nk)) and np.array_equal(pl.zmap[pl.elim], np.arange(nz))
+    assert len(np.unique(pl.flat32)) == len(pl.flat32) and pl.flat32.max() < pl.N0 * pl.N0
