@@ -260,9 +260,7 @@ __device__ __forceinline__ void bidi_fwd(const CParams& Pr, const double* __rest
       double gl[SLOTS];
       if constexpr (PH == 2) {
         const double* a = lat + (valid ? (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + lane : 0);
-#pragma unroll
-        for (int b = 0; b < NB; ++b) gl[b] = a[b * 64];
-        gl[SLOTS - 1] = a[(SLOTS - 1) * 64];
+        load_slot<NB>(a, gl);
       } else if constexpr (PH == 1) {
         const bool okc = valid && (cut2 || (i < Lx && j < Ly));
         const double* a = cutb + (cut2 ? 16 * CS : 0) + (okc ? i : 0);
@@ -324,10 +322,7 @@ __device__ __forceinline__ void bidi_fwd(const CParams& Pr, const double* __rest
       const bool origin = valid && i == 0 && j == 0;  // F = 0, E = 0 there already
       // ---- store F on d <= C, where the backward counts
       if (PH < 2 && valid) {
-        double* dst = lat + (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + lane;
-#pragma unroll
-        for (int b = 0; b < NB; ++b) dst[b * 64] = F[b];
-        dst[(SLOTS - 1) * 64] = (double)E;
+        store_slot<NB>(lat + (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + lane, F, E);
       }
       // ---- push: h[k] = sum_s F[s] P[s][k]
       double h[NB];
@@ -660,9 +655,7 @@ __device__ __forceinline__ void bidi_bwd(const CParams& Pr, const double* __rest
       // after the meeting, which makes them visible).
       if constexpr (PH == 2) {
         const double* a = f_addr(p, i);
-#pragma unroll
-        for (int b = 0; b < NB; ++b) fcur[b] = a[b * 64];
-        fcur[SLOTS - 1] = a[(SLOTS - 1) * 64];
+        load_slot<NB>(a, fcur);
       }
       // ---- receive the successor row's q (lane+1, or a ring for lane 63)
 #pragma unroll
@@ -700,10 +693,7 @@ __device__ __forceinline__ void bidi_bwd(const CParams& Pr, const double* __rest
       }
       if constexpr (PH == 0) {  // past the cut: g for the forward's counts
         if (valid) {
-          double* dst = lat + (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + lane;
-#pragma unroll
-          for (int b = 0; b < NB; ++b) dst[b * 64] = g[b];
-          dst[(SLOTS - 1) * 64] = (double)Eg;
+          store_slot<NB>(lat + (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + lane, g, Eg);
         }
       }
       if constexpr (PH == 1) {  // the cut: g of this cell, the M part of the next cell's
@@ -725,9 +715,7 @@ __device__ __forceinline__ void bidi_bwd(const CParams& Pr, const double* __rest
       meet();
       if (in_range) {
         const double* a = f_addr(p, i);
-#pragma unroll
-        for (int b = 0; b < NB; ++b) fcur[b] = a[b * 64];
-        fcur[SLOTS - 1] = a[(SLOTS - 1) * 64];
+        load_slot<NB>(a, fcur);
       }
     }
     if (in_range) {
