@@ -580,12 +580,18 @@ __device__ __forceinline__ void fwd_sweep(const BParams& Pr, const BDesc& pd, un
         F[q] = (origin || !in) ? 0.0 : ldexp(Fp[cg][q], -sh);
         if (k < NBP && r < R) {
           s_F[rc][k] = F[q];
-          if (want_lat && in) lat[((size_t)t * R + r) * NBP + k] = F[q];
+          if (want_lat && in) {  // written once, read once by the backward: non-temporal
+            if constexpr (RECOMP) lat[((size_t)t * R + r) * NBP + k] = F[q];
+            else __builtin_nontemporal_store(F[q], lat + ((size_t)t * R + r) * NBP + k);
+          }
         }
       }
       if (kl == 0 && in) {
         s_E[r] = E;
-        if (want_lat) latE[(size_t)t * R + r] = (double)E;
+        if (want_lat) {
+          if constexpr (RECOMP) latE[(size_t)t * R + r] = (double)E;
+          else __builtin_nontemporal_store((double)E, latE + (size_t)t * R + r);
+        }
       }
       const bool corner = !RECOMP && in && i == Lx && j == Ly;
       if (__builtin_amdgcn_ballot_w64(corner)) {  // log P and the E column (the corner's cell)
@@ -898,9 +904,9 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
 #pragma unroll
       for (int q = 0; q < KR; ++q) {
         const int k = kl + 64 * q;
-        lf[cg][q] = (in && k < NBP) ? lat[cell * NBP + k] : 0.0;
+        lf[cg][q] = (in && k < NBP) ? __builtin_nontemporal_load(lat + cell * NBP + k) : 0.0;
       }
-      lfE[cg] = in ? (int)latE[cell] : 0;
+      lfE[cg] = in ? (int)__builtin_nontemporal_load(latE + cell) : 0;
     }
   };
   // emissions (dense: log emissions) of the lane's cells at step tt
