@@ -47,7 +47,7 @@ CHECKPOINT_BUDGET = 256 << 20  # the library's default per-pair lattice budget
 # configuration: the 6 longest pairs on the one-launch schedule),
 # profiles/fbc_kernels_c1_mixdom2_k3.md (no long class) and
 # profiles/fbv2_kernel_c1_mixdom2_k3.md (one-workgroup-per-pair kernel).
-TRAFFIC = {("fbc_bwd_kernel<LMix<3,1>,3>", 817): 8.23e9, ("fbc_fwd_kernel<LMix<3,1>,3>", 817): 7.72e9,
+TRAFFIC = {("fbc_bwd_kernel<LMix<3,1>,3>", 817): 8.50e9, ("fbc_fwd_kernel<LMix<3,1>,3>", 817): 7.72e9,
            ("fbc_bidi_kernel<LMix<3,1>,3> long", 6): 0.90e9,
            ("fbc_bwd_kernel<LMix<3,1>,3>", 823): 8.69e9, ("fbc_fwd_kernel<LMix<3,1>,3>", 823): 8.17e9,
            ("fbc_bwd_kernel<LMix<3,1>,2>", 189): 3.25e8, ("fbc_fwd_kernel<LMix<3,1>,2>", 189): 2.74e8,

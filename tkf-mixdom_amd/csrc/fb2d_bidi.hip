@@ -260,7 +260,7 @@ __device__ __forceinline__ void bidi_fwd(const CParams& Pr, const double* __rest
       double gl[SLOTS];
       if constexpr (PH == 2) {
         const double* a = lat + (valid ? (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + lane : 0);
-        load_slot<NB>(a, gl);
+        load_slot<NB, false>(a, gl);
       } else if constexpr (PH == 1) {
         const bool okc = valid && (cut2 || (i < Lx && j < Ly));
         const double* a = cutb + (cut2 ? 16 * CS : 0) + (okc ? i : 0);
@@ -322,7 +322,7 @@ __device__ __forceinline__ void bidi_fwd(const CParams& Pr, const double* __rest
       const bool origin = valid && i == 0 && j == 0;  // F = 0, E = 0 there already
       // ---- store F on d <= C, where the backward counts
       if (PH < 2 && valid) {
-        store_slot<NB>(lat + (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + lane, F, E);
+        store_slot<NB, false>(lat + (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + lane, F, E);
       }
       // ---- push: h[k] = sum_s F[s] P[s][k]
       double h[NB];
@@ -655,7 +655,7 @@ __device__ __forceinline__ void bidi_bwd(const CParams& Pr, const double* __rest
       // after the meeting, which makes them visible).
       if constexpr (PH == 2) {
         const double* a = f_addr(p, i);
-        load_slot<NB>(a, fcur);
+        load_slot<NB, false>(a, fcur);
       }
       // ---- receive the successor row's q (lane+1, or a ring for lane 63)
 #pragma unroll
@@ -693,7 +693,7 @@ __device__ __forceinline__ void bidi_bwd(const CParams& Pr, const double* __rest
       }
       if constexpr (PH == 0) {  // past the cut: g for the forward's counts
         if (valid) {
-          store_slot<NB>(lat + (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + lane, g, Eg);
+          store_slot<NB, false>(lat + (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + lane, g, Eg);
         }
       }
       if constexpr (PH == 1) {  // the cut: g of this cell, the M part of the next cell's
@@ -715,7 +715,7 @@ __device__ __forceinline__ void bidi_bwd(const CParams& Pr, const double* __rest
       meet();
       if (in_range) {
         const double* a = f_addr(p, i);
-        load_slot<NB>(a, fcur);
+        load_slot<NB, false>(a, fcur);
       }
     }
     if (in_range) {

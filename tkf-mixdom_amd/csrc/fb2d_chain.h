@@ -95,26 +95,31 @@ __device__ __forceinline__ void wg_release_agent() {
 
 // Writes one cell's lattice slot: NB values, then the binary exponent in
 // slot SLOTS - 1 (the 64 lanes' slots interleaved, so each store is 512
-// contiguous bytes).  Non-temporal: the lattice is written once and read
-// once, by the backward launch or another workgroup, so holding it in the
-// XCD's L2 only evicts what is reused there (the emission tables, boundary
-// rows and the other launches' working sets).  On the 1024-pair K=3 batch
-// this took the main forward launch from 3.36 to 2.99 ms and the call from
-// 8.1 to 7.7 ms.
-template <int NB>
+// contiguous bytes).  NT: non-temporal, for the two-launch schedule, where
+// the lattice is written once and read once by the next launch, so holding
+// it in the XCD's L2 only evicts what is reused there (the boundary rows and
+// the other launches' working sets): on the 1024-pair K=3 batch the main
+// forward launch went from 3.36 to 2.99 ms and the call from 8.1 to 7.7 ms.
+// The one-launch schedule reads what it wrote within microseconds, from L2:
+// ordinary stores there (non-temporal ones made c1's longest pair alone
+// 4.0 -> 4.3 ms).
+template <int NB, bool NT>
 __device__ __forceinline__ void store_slot(double* dst, const double* v, int E) {
 #pragma unroll
-  for (int b = 0; b < NB; ++b) __builtin_nontemporal_store(v[b], dst + b * 64);
-  __builtin_nontemporal_store((double)E, dst + (SLOTS - 1) * 64);
+  for (int b = 0; b < NB; ++b) {
+    if constexpr (NT) __builtin_nontemporal_store(v[b], dst + b * 64);
+    else dst[b * 64] = v[b];
+  }
+  if constexpr (NT) __builtin_nontemporal_store((double)E, dst + (SLOTS - 1) * 64);
+  else dst[(SLOTS - 1) * 64] = (double)E;
 }
 
-// Reads one cell's lattice slot (store_slot's layout), non-temporal: each
-// slot is read once.
-template <int NB>
+// Reads one cell's lattice slot (store_slot's layout); NT as there.
+template <int NB, bool NT>
 __device__ __forceinline__ void load_slot(const double* a, double* v) {
 #pragma unroll
-  for (int b = 0; b < NB; ++b) v[b] = __builtin_nontemporal_load(a + b * 64);
-  v[SLOTS - 1] = __builtin_nontemporal_load(a + (SLOTS - 1) * 64);
+  for (int b = 0; b < NB; ++b) v[b] = NT ? __builtin_nontemporal_load(a + b * 64) : a[b * 64];
+  v[SLOTS - 1] = NT ? __builtin_nontemporal_load(a + (SLOTS - 1) * 64) : a[(SLOTS - 1) * 64];
 }
 
 // Boundary entries between the workgroups of a chain are written and read

@@ -194,7 +194,7 @@ __global__ __launch_bounds__(W * 64) void fbc_fwd_kernel(CParams Pr, const doubl
       const bool origin = valid && i == 0 && j == 0;  // F = 0, E = 0 there already
       // ---- store the forward lattice (band-local step u = j + lane)
       if (want_lat && valid) {
-        store_slot<NB>(lat + (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + lane, F, E);
+        store_slot<NB, true>(lat + (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + lane, F, E);
       }
       // ---- push: h[k] = sum_s F[s] P[s][k]
       double h[NB];
@@ -378,7 +378,7 @@ __global__ __launch_bounds__((W + 1) * 64, 2) void fbc_bwd_kernel(CParams Pr, co
       double fcur[SLOTS];
       {
         const double* a = f_addr(p, i);
-        load_slot<NB>(a, fcur);
+        load_slot<NB, true>(a, fcur);
       }
       // ---- receive the successor row's q (lane+1, or a ring for lane 63)
       double rD[nD > 0 ? nD : 1], rM[nM > 0 ? nM : 1];
