@@ -1228,13 +1228,17 @@ __global__ __launch_bounds__(512) void band_ecnt_m_kernel(const EcPair* __restri
                                                           const uint8_t* __restrict__ y_cat,
                                                           const double* __restrict__ ws, const double* __restrict__ pinfo,
                                                           double* __restrict__ out_ecnt) {
-  extern __shared__ double s_acc[];  // [G][A_PAD * A_PAD]
+  extern __shared__ double s_acc[];  // [G][A_PAD * A_PAD], then the pair's clamped codes x, y
   const EcPair e = ep[blockIdx.x];
   if (!(pinfo[2 * e.pair] > 0.0)) return;  // impossible pair: its backward wrote no masses
   for (int q = threadIdx.x; q < G * kEcM; q += blockDim.x) s_acc[q] = 0.0;
+  // the codes in LDS: each mass needs its cell's two residues, and global
+  // byte loads beside every mass load held this kernel to a third of HBM speed
+  uint8_t* xs = reinterpret_cast<uint8_t*>(s_acc + G * kEcM);
+  uint8_t* ys = xs + e.Lx;
+  for (int q = threadIdx.x; q < e.Lx; q += blockDim.x) xs[q] = (uint8_t)min((int)x_cat[e.x_off + q], A_PAD - 1);
+  for (int q = threadIdx.x; q < e.Ly; q += blockDim.x) ys[q] = (uint8_t)min((int)y_cat[e.y_off + q], A_PAD - 1);
   __syncthreads();
-  const uint8_t* xs = x_cat + e.x_off;
-  const uint8_t* ys = y_cat + e.y_off;
   const double* mb = ws + e.mb_off;
   // thread -> (cell slot, M rank): nMp = nM rounded up to a power of two,
   // 512 / nMp cells per pass; R is a power of two (16, 32)
@@ -1257,7 +1261,7 @@ __global__ __launch_bounds__(512) void band_ecnt_m_kernel(const EcPair* __restri
           const int i = g * R + r, j = t - r;
           const bool ok = cell < ncell && r < nr && j >= 1 && j <= e.Ly && i >= 1;
           v[u] = ok ? mb[cell * nM + m] : 0.0;
-          xy[u] = ok ? min((int)xs[i - 1], A_PAD - 1) * A_PAD + min((int)ys[j - 1], A_PAD - 1) : 0;
+          xy[u] = ok ? (int)xs[i - 1] * A_PAD + (int)ys[j - 1] : 0;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -1374,7 +1378,11 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
   // kEcMaxG groups (the M reduction kernel's LDS), else atomics per cell
   const size_t lds_m3 = bwd_static_lds(pl.NT) + (size_t)pl.G * (A_PAD + pl.R + kEcM) * 8;
   const bool m3 = lds_m3 + 1024 <= (size_t)(160 * 1024) / band_occ(pl.NT);
-  pl.ec_mode = !ecnt ? 0 : (m3 ? 3 : (pl.G <= kEcMaxG ? 1 : 2));
+  // mode 1's reduction kernel holds the tables and the longest pair's codes
+  int max_xy = 0;
+  for (int b = 0; b < B; ++b) max_xy = std::max(max_xy, p->h_x_len[b] + p->h_y_len[b]);
+  const bool m1 = pl.G <= kEcMaxG && (size_t)pl.G * kEcM * 8 + (size_t)max_xy + 16 <= (size_t)(160 * 1024);
+  pl.ec_mode = !ecnt ? 0 : (m3 ? 3 : (m1 ? 1 : 2));
   pl.mrank.assign(NBP, -1);
   pl.mgrp.assign(NBP, 0);
   for (int k = 0; k < pl.NB; ++k)
@@ -1709,7 +1717,9 @@ int tkfmi_band_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
     if (tkfmi_ktime_stop(slot, stream, err) != TKFMI_OK) return TKFMI_EHIP;
   }
   if (pl.ec_mode == 1 && !pl.ec.empty()) {  // M-type emission counts, binned per pair
-    const size_t smem = (size_t)pl.G * kEcM * 8;
+    int max_xy = 0;
+    for (const EcPair& q : pl.ec) max_xy = std::max(max_xy, q.Lx + q.Ly);
+    const size_t smem = (size_t)pl.G * kEcM * 8 + (((size_t)max_xy + 15) & ~(size_t)15);
     hipError_t e = hipFuncSetAttribute((const void*)band_ecnt_m_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)smem);
     if (e != hipSuccess) { err = std::string("hipFuncSetAttribute: ") + hipGetErrorString(e); return TKFMI_EHIP; }
