@@ -1219,8 +1219,12 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
 // entries; the I and D entries come from the bands' atomics).
 struct EcPair {
   int32_t pair, Lx, Ly, G;
-  int64_t x_off, y_off, mb_off;  // mb_off: band 0's masses; band g's follow band g-1's
+  int32_t g0, g1, pad0, pad1;    // this workgroup's bands [g0, g1) of the pair
+  int64_t x_off, y_off, mb_off;  // mb_off: band g0's masses; band g's follow band g-1's
 };
+// Bands per reduction workgroup: about kEcChunk mass values each, so that a
+// long pair's masses are read by several compute units at once.
+constexpr int64_t kEcChunk = 2 << 20;
 
 __global__ __launch_bounds__(512) void band_ecnt_m_kernel(const EcPair* __restrict__ ep, int R, int nM, int G,
                                                           const int32_t* __restrict__ mgrp,
@@ -1247,7 +1251,7 @@ __global__ __launch_bounds__(512) void band_ecnt_m_kernel(const EcPair* __restri
   const int m = threadIdx.x & (nMp - 1), cs = threadIdx.x / nMp, cpp = blockDim.x / nMp;
   const int lr = __builtin_ctz(R);
   const int gm = (m < nM) ? mgrp[m] : 0;
-  for (int g = 0; g < e.G; ++g) {
+  for (int g = e.g0; g < e.g1; ++g) {
     const int nr = min(R, e.Lx + 1 - g * R), T = e.Ly + nr, ncell = T * R;
     if (m < nM) {
       constexpr int U = 4;  // cells per thread and pass, their loads in flight together
@@ -1271,8 +1275,11 @@ __global__ __launch_bounds__(512) void band_ecnt_m_kernel(const EcPair* __restri
     mb += (size_t)ncell * nM;
   }
   __syncthreads();
-  double* out = out_ecnt + (size_t)e.pair * G * TAB_SZ;
-  for (int q = threadIdx.x; q < G * kEcM; q += blockDim.x) out[(size_t)(q / kEcM) * TAB_SZ + q % kEcM] = s_acc[q];
+  double* out = out_ecnt + (size_t)e.pair * G * TAB_SZ;  // zero-filled by the caller; several workgroups per pair
+  for (int q = threadIdx.x; q < G * kEcM; q += blockDim.x) {
+    const double v = s_acc[q];
+    if (v != 0.0) unsafeAtomicAdd(out + (size_t)(q / kEcM) * TAB_SZ + q % kEcM, v);
+  }
 }
 
 // status[0] = 1 if the call's abort flag was raised.
@@ -1495,12 +1502,26 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
       const int fl = pb[b].flag0;
       first[b] = (int64_t)fwd_all.size();
       int64_t lat_band = pb[b].lat, mb_band = pb[b].mb;
-      if (pl.ec_mode == 1) {
+      if (pl.ec_mode == 1) {  // the pair's bands in chunks of about kEcChunk masses
         EcPair e;
         memset(&e, 0, sizeof(e));
         e.pair = b; e.Lx = lx; e.Ly = ly; e.G = G;
-        e.x_off = p->h_x_off[b]; e.y_off = p->h_y_off[b]; e.mb_off = pb[b].mb;
-        pl.ec.push_back(e);
+        e.x_off = p->h_x_off[b]; e.y_off = p->h_y_off[b];
+        int64_t mo = pb[b].mb, vol = 0;
+        e.g0 = 0;
+        e.mb_off = mo;
+        for (int g = 0; g < G; ++g) {
+          const int64_t band_vol = (int64_t)(ly + std::min(R, lx + 1 - g * R)) * R * pl.nM;
+          vol += band_vol;
+          mo += band_vol;
+          if (vol >= kEcChunk || g == G - 1) {
+            e.g1 = g + 1;
+            pl.ec.push_back(e);
+            e.g0 = g + 1;
+            e.mb_off = mo;
+            vol = 0;
+          }
+        }
       }
       for (int g = 0; g < G; ++g) {
         BDesc d;
