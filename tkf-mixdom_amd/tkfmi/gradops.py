@@ -13,6 +13,7 @@ forward-mode dual arithmetic, the elimination by its linear-solve adjoint,
 exp by the adjoint of its Frechet derivative.
 """
 import torch
+from torch.autograd.function import once_differentiable
 
 from . import _abi
 
@@ -48,6 +49,7 @@ class _TkfTrans(torch.autograd.Function):
         return T
 
     @staticmethod
+    @once_differentiable
     def backward(ctx, G):
         (J,) = ctx.saved_tensors
         g = (G.to(torch.float64)[..., None] * J).sum(dim=(-3, -2))  # (..., 4)
@@ -84,6 +86,7 @@ class _Chi(torch.autograd.Function):
         return chi
 
     @staticmethod
+    @once_differentiable
     def backward(ctx, G):
         a, W = ctx.saved_tensors
         pl = ctx.pl
@@ -123,6 +126,7 @@ class _Expm(torch.autograd.Function):
         return out
 
     @staticmethod
+    @once_differentiable
     def backward(ctx, G):
         (Ac,) = ctx.saved_tensors
         n = Ac.shape[-1]
@@ -134,12 +138,12 @@ class _Expm(torch.autograd.Function):
 
 
 def expm(A):
-    """exp of each n x n matrix of A (..., n, n) on the GPU, 2 n <= 40."""
+    """exp of each n x n matrix of A (..., n, n) on the GPU, n <= 20."""
     return _Expm.apply(A)
 
 
 def expm_ok(A):
-    """Whether expm() takes A: a CUDA tensor of matrices small enough for
-    the adjoint's doubled matrix."""
+    """Whether expm() takes A: a CUDA tensor of matrices of at most 20 x 20
+    (tkfmi_expm_vjp_batched's limit, 2 n <= TKFMI_EXPM_MAX_N)."""
     return isinstance(A, torch.Tensor) and A.is_cuda and A.dim() >= 2 and A.shape[-1] == A.shape[-2] \
         and 2 * A.shape[-1] <= _abi.EXPM_MAX_N and A.numel() > 0

@@ -22,6 +22,7 @@ and has a ``*_batch`` form over a ragged batch with per-pair ``ts`` that runs
 as one kernel call.
 """
 import torch
+from torch.autograd.function import once_differentiable
 
 from . import _abi, hmm, mixdom, params, tables
 
@@ -56,6 +57,7 @@ class FB2DLogProb(torch.autograd.Function):
         return r.log_p
 
     @staticmethod
+    @once_differentiable
     def backward(ctx, g):
         n, ec = ctx.saved_tensors
         g = g.to(torch.float64)[:, None, None]
