@@ -259,7 +259,7 @@ __device__ __forceinline__ void bidi_fwd(const CParams& Pr, const double* __rest
       // from the cut buffer on d = C - 1 (M part) and d = C
       double gl[SLOTS];
       if constexpr (PH == 2) {
-        const double* a = lat + (valid ? (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + lane : 0);
+        const double* a = lat + (valid ? (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + 2 * lane : 0);
         load_slot<NB, false>(a, gl);
       } else if constexpr (PH == 1) {
         const bool okc = valid && (cut2 || (i < Lx && j < Ly));
@@ -322,7 +322,7 @@ __device__ __forceinline__ void bidi_fwd(const CParams& Pr, const double* __rest
       const bool origin = valid && i == 0 && j == 0;  // F = 0, E = 0 there already
       // ---- store F on d <= C, where the backward counts
       if (PH < 2 && valid) {
-        store_slot<NB, false>(lat + (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + lane, F, E);
+        store_slot<NB, false>(lat + (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + 2 * lane, F, E);
       }
       // ---- push: h[k] = sum_s F[s] P[s][k]
       double h[NB];
@@ -616,7 +616,7 @@ __device__ __forceinline__ void bidi_bwd(const CParams& Pr, const double* __rest
   // its count operands are then scaled by 0.
   auto f_addr = [&](int p, int i) -> const double* {
     const bool v = has_band && p >= 0 && p <= Ly && i <= Lx;
-    return lat + (v ? (((size_t)band * BSTRIDE + (Ly - p + lane)) * SLOTS) * 64 + lane : 0);
+    return lat + (v ? (((size_t)band * BSTRIDE + (Ly - p + lane)) * SLOTS) * 64 + 2 * lane : 0);
   };
   const int t_on = 64 * w, t_off = 64 * w + 63 + Ly;
   const bool corner_wave = has_band && band == (Lx + 64) / 64 - 1;
@@ -693,7 +693,7 @@ __device__ __forceinline__ void bidi_bwd(const CParams& Pr, const double* __rest
       }
       if constexpr (PH == 0) {  // past the cut: g for the forward's counts
         if (valid) {
-          store_slot<NB, false>(lat + (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + lane, g, Eg);
+          store_slot<NB, false>(lat + (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + 2 * lane, g, Eg);
         }
       }
       if constexpr (PH == 1) {  // the cut: g of this cell, the M part of the next cell's

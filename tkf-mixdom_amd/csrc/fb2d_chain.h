@@ -93,33 +93,55 @@ __device__ __forceinline__ void wg_release_agent() {
   __syncthreads();
 }
 
-// Writes one cell's lattice slot: NB values, then the binary exponent in
-// slot SLOTS - 1 (the 64 lanes' slots interleaved, so each store is 512
-// contiguous bytes).  NT: non-temporal, for the two-launch schedule, where
-// the lattice is written once and read once by the next launch, so holding
-// it in the XCD's L2 only evicts what is reused there (the boundary rows and
-// the other launches' working sets): on the 1024-pair K=3 batch the main
+// A cell-step's lattice slot: its NB state values and binary exponent as
+// NB + 1 doubles in pairs, in the cell-step's region of SLOTS x 64 doubles:
+// the 64 lanes' pair q (elements 2q, 2q + 1) are 64 contiguous 16-byte
+// entries, so each pair is one 1 KB store or load of the wave (8-byte
+// elements took twice the memory instructions: on the 1024-pair K=3 batch
+// the one-launch longest pairs went 5.27 -> 5.10 ms and the main backward
+// launch 4.61 -> 4.49 ms).  `p` points at the lane's pair 0
+// (the region + 2 lane doubles); element b is at p + 128 (b / 2) + b % 2,
+// the exponent at element NB.
+// NT: non-temporal, for the two-launch schedule, where the lattice is
+// written once by the forward launch and read once by the backward launch:
+// held in the XCD's L2 it only evicted what is reused there (the boundary
+// rows and the other launches' working sets); streamed, that batch's main
 // forward launch went from 3.36 to 2.99 ms and the call from 8.1 to 7.7 ms.
-// The one-launch schedule reads what it wrote within microseconds, from L2:
-// ordinary stores there (non-temporal ones made c1's longest pair alone
-// 4.0 -> 4.3 ms).
+// The one-launch schedule reads back what it wrote within microseconds,
+// from L2: ordinary stores there (streamed, c1's longest pair alone took
+// 4.3 instead of 4.0 ms).
+typedef double slot_d2 __attribute__((ext_vector_type(2)));
+
 template <int NB, bool NT>
-__device__ __forceinline__ void store_slot(double* dst, const double* v, int E) {
+__device__ __forceinline__ void store_slot(double* p, const double* v, int E) {
+  constexpr int NP = (NB + 2) / 2;  // pairs of the NB + 1 elements
 #pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    if constexpr (NT) __builtin_nontemporal_store(v[b], dst + b * 64);
-    else dst[b * 64] = v[b];
+  for (int q = 0; q < NP; ++q) {
+    const int b0 = 2 * q, b1 = 2 * q + 1;
+    slot_d2 x;
+    x.x = b0 == NB ? (double)E : v[b0 < NB ? b0 : 0];
+    x.y = b1 == NB ? (double)E : (b1 < NB ? v[b1] : 0.0);
+    slot_d2* d = reinterpret_cast<slot_d2*>(p + 128 * q);
+    if constexpr (NT) __builtin_nontemporal_store(x, d);
+    else *d = x;
   }
-  if constexpr (NT) __builtin_nontemporal_store((double)E, dst + (SLOTS - 1) * 64);
-  else dst[(SLOTS - 1) * 64] = (double)E;
 }
 
-// Reads one cell's lattice slot (store_slot's layout); NT as there.
+// Reads one cell's lattice slot (store_slot's layout) into v[0..NB) and
+// the exponent into v[SLOTS - 1].
 template <int NB, bool NT>
-__device__ __forceinline__ void load_slot(const double* a, double* v) {
+__device__ __forceinline__ void load_slot(const double* p, double* v) {
+  constexpr int NP = (NB + 2) / 2;
 #pragma unroll
-  for (int b = 0; b < NB; ++b) v[b] = NT ? __builtin_nontemporal_load(a + b * 64) : a[b * 64];
-  v[SLOTS - 1] = NT ? __builtin_nontemporal_load(a + (SLOTS - 1) * 64) : a[(SLOTS - 1) * 64];
+  for (int q = 0; q < NP; ++q) {
+    const slot_d2* s = reinterpret_cast<const slot_d2*>(p + 128 * q);
+    slot_d2 x;
+    if constexpr (NT) x = __builtin_nontemporal_load(s);
+    else x = *s;
+    const int b0 = 2 * q, b1 = 2 * q + 1;
+    if (b0 < NB) v[b0] = x.x; else v[SLOTS - 1] = x.x;
+    if (b1 < NB) v[b1] = x.y; else if (b1 == NB) v[SLOTS - 1] = x.y;
+  }
 }
 
 // Boundary entries between the workgroups of a chain are written and read

@@ -194,7 +194,7 @@ __global__ __launch_bounds__(W * 64) void fbc_fwd_kernel(CParams Pr, const doubl
       const bool origin = valid && i == 0 && j == 0;  // F = 0, E = 0 there already
       // ---- store the forward lattice (band-local step u = j + lane)
       if (want_lat && valid) {
-        store_slot<NB, true>(lat + (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + lane, F, E);
+        store_slot<NB, true>(lat + (((size_t)band * BSTRIDE + (j + lane)) * SLOTS) * 64 + 2 * lane, F, E);
       }
       // ---- push: h[k] = sum_s F[s] P[s][k]
       double h[NB];
@@ -346,7 +346,7 @@ __global__ __launch_bounds__((W + 1) * 64, 2) void fbc_bwd_kernel(CParams Pr, co
   // its count operands are then scaled by 0.
   auto f_addr = [&](int p, int i) -> const double* {
     const bool v = has_band && p >= 0 && p <= Ly && i <= Lx;
-    return lat + (v ? (((size_t)band * BSTRIDE + (Ly - p + lane)) * SLOTS) * 64 + lane : 0);
+    return lat + (v ? (((size_t)band * BSTRIDE + (Ly - p + lane)) * SLOTS) * 64 + 2 * lane : 0);
   };
   const int T = pd.T;
   // A wave's band is in range for steps 64 w .. 64 w + 63 + Ly; outside
