@@ -98,6 +98,7 @@ struct Geo {
   static constexpr bool SPLIT = NBLK < kWaves;
   static constexpr int RPW = SPLIT ? kWaves / NBLK : 1;
   static_assert(!SPLIT || kWaves % NBLK == 0, "blocks divide the waves");
+  static_assert(NG * CPI * kWaves == R, "rows w + kWaves (CPI cg + hc) are exactly 0..R-1");
   __host__ __device__ static constexpr int block(int w, int u) { return SPLIT ? w % NBLK : w + kWaves * u; }
   __host__ __device__ static constexpr int nblocks(int w) { return SPLIT ? 1 : (NBLK - w + kWaves - 1) / kWaves; }
   __host__ __device__ static constexpr int rt0(int w) { return SPLIT ? w / NBLK : 0; }
@@ -381,6 +382,7 @@ struct FwdLds {
   int s_eM[2][Gm::R], s_eI[2][Gm::R], s_eD[2][Gm::R], s_E[Gm::R];
   int s_lastE, s_ok;                         // their exponent; cleared when a boundary wait gave up
   uint8_t s_y[kYR];
+  double s_sink[64];                         // per-lane target of the matrix phase's discarded pushes
 };
 
 // One band's forward sweep.  RECOMP: the backward's recomputation of a
@@ -391,7 +393,7 @@ template <int NT, bool DENSE, bool RECOMP>
 __device__ __forceinline__ void fwd_sweep(const BParams& Pr, const BDesc& pd, unsigned char* smem, double* lat,
                                           const double* bin) {
   using Gm = Geo<NT>;
-  constexpr int NBP = Gm::NBP, NBPP = Gm::NBPP, KR = Gm::KR, R = Gm::R, CPW = Gm::CPW;
+  constexpr int NBP = Gm::NBP, NBPP = Gm::NBPP, KR = Gm::KR, R = Gm::R;
   constexpr int KS = Gm::KS, KB = Gm::KB, NBLK = Gm::NBLK, BPW = Gm::BPW;
   FwdLds<NT>& L = *reinterpret_cast<FwdLds<NT>*>(smem);
   auto& s_pre = L.s_pre;
@@ -520,6 +522,7 @@ __device__ __forceinline__ void fwd_sweep(const BParams& Pr, const BDesc& pd, un
     // across the groups: F' = v e 2^(e_type - E0) (E0: the largest
     // predecessor exponent), the cell's exponent from the largest F', then
     // F normalised
+    const int tb0 = t % (CQ + 1), tb1 = tb0 == 0 ? CQ : tb0 - 1;  // ring entries t, t - 1
     double Fp[NG][KR];
     int E0c[NG], ex[NG];
 #pragma unroll
@@ -527,32 +530,50 @@ __device__ __forceinline__ void fwd_sweep(const BParams& Pr, const BDesc& pd, un
       const int r = w + kWaves * (CPI * cg + hc);
       const int j = t - r, i = i0 + r;
       const bool in = r < nr && j >= 0 && j <= Ly;
-      const bool top = (r == 0) && bin != nullptr;
-      const double* bM = s_bin[(j + CQ) % (CQ + 1)];  // entry j - 1
-      const double* bD = s_bin[j % (CQ + 1)];
-      const int rc = min(r, R - 1);
+      // row 0 (the band above's entries) lies in wave 0's first group only,
+      // where j = t: the entries j - 1 and j of the ring, a scalar index
+      const bool topg = cg == 0 && w == 0 && bin != nullptr;  // wave-uniform
+      const bool top = topg && r == 0;
+      const int rc = r;  // < R: rows w + kWaves (CPI cg + hc) cover 0..R-1
       const bool okM = in && i >= 1 && j >= 1, okI = in && j >= 1, okD = in && i >= 1;
-      // predecessor exponents: M from (i-1, j-1), I from (i, j-1), D from (i-1, j)
-      const int eM = okM ? (top ? (int)bM[NBP] : s_eM[sl][rc]) : EXP_NONE;
-      const int eI = okI ? s_eI[sl][rc] : EXP_NONE;
-      const int eD = okD ? (top ? (int)bD[NBP] : s_eD[sl][rc]) : EXP_NONE;
+      // predecessor exponents: M from (i-1, j-1), I from (i, j-1), D from (i-1, j);
+      // every read unconditional, the lane's value chosen by selects
+      double bmv[KR], bdv[KR];
+      int eMb = 0, eDb = 0;
+      if (topg) {
+        const double* bM = s_bin[tb1];  // entry j - 1
+        const double* bD = s_bin[tb0];
+        eMb = (int)bM[NBP];
+        eDb = (int)bD[NBP];
+#pragma unroll
+        for (int q = 0; q < KR; ++q) {
+          const int k = min(kl + 64 * q, NBP - 1);
+          bmv[q] = bM[k];
+          bdv[q] = bD[k];
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < KR; ++q) bmv[q] = bdv[q] = 0.0;
+      }
+      const int eMl = s_eM[sl][rc], eIl = s_eI[sl][rc], eDl = s_eD[sl][rc];
+      const int eM = okM ? (top ? eMb : eMl) : EXP_NONE;
+      const int eI = okI ? eIl : EXP_NONE;
+      const int eD = okD ? (top ? eDb : eDl) : EXP_NONE;
       const int E0 = max(eM, max(eI, eD));
       E0c[cg] = E0;
       double m = 0.0;
 #pragma unroll
       for (int q = 0; q < KR; ++q) {
-        // every read unconditional (row r's pushes, the boundary entries),
-        // the lane's value chosen by selects
         const int k = min(kl + 64 * q, NBP - 1);
         double pre = s_pre[sl][0][rc][k];
 #pragma unroll
         for (int ks = 1; ks < KS; ++ks) pre += s_pre[sl][ks][rc][k];
-        const double bmv = bM[k], bdv = bD[k];
+        const double bmv_ = bmv[q], bdv_ = bdv[q];
         const int tq = ty[q];
         double v = 0.0;  // selects, no branch per type
         v = (tq == ST_I && okI) ? pre : v;
-        v = (tq == ST_M && okM) ? (top ? bmv : pre) : v;
-        v = (tq == ST_D && okD) ? (top ? bdv : pre) : v;
+        v = (tq == ST_M && okM) ? (top ? bmv_ : pre) : v;
+        v = (tq == ST_D && okD) ? (top ? bdv_ : pre) : v;
         const int et = tq == ST_M ? eM : (tq == ST_I ? eI : eD);
         double e;
         if constexpr (DENSE) e = (v != 0.0) ? exp(en[cg][q]) : 0.0;
@@ -568,7 +589,7 @@ __device__ __forceinline__ void fwd_sweep(const BParams& Pr, const BDesc& pd, un
       const int r = w + kWaves * (CPI * cg + hc);
       const int j = t - r, i = i0 + r;
       const bool in = r < nr && j >= 0 && j <= Ly;
-      const int rc = min(r, R - 1);
+      const int rc = r;
       const bool origin = (i == 0 && j == 0);
       const bool any = ex[cg] != EXP_NONE;
       const int sh = any ? ex[cg] : 0;
@@ -578,7 +599,7 @@ __device__ __forceinline__ void fwd_sweep(const BParams& Pr, const BDesc& pd, un
       for (int q = 0; q < KR; ++q) {
         const int k = kl + 64 * q;
         F[q] = (origin || !in) ? 0.0 : ldexp(Fp[cg][q], -sh);
-        if (k < NBP && r < R) {
+        if (k < NBP) {
           s_F[rc][k] = F[q];
           if (want_lat && in) {  // written once, read once by the backward: non-temporal
             if constexpr (RECOMP) lat[((size_t)t * R + r) * NBP + k] = F[q];
@@ -643,14 +664,17 @@ __device__ __forceinline__ void fwd_sweep(const BParams& Pr, const BDesc& pd, un
         for (int v = 0; v < 4; ++v) {
           const int r = 16 * rt + 4 * v + (lane >> 4);
           const int j = t - r;
-          if (r >= nr || j < 0 || j > Ly || tk == 0) continue;
+          const bool skip = r >= nr || j < 0 || j > Ly || tk == 0;
           const double h = (origin_step && r == 0) ? psb[u] : acc[u][v];
-          // one store, its address chosen by selects: I-type to (r, j+1)
-          // (slot t+1), M-type to (r+1, j+1) (slot t+2), D-type to (r+1, j)
-          // (slot t+1); the last row's M and D pushes to the boundary row
+          // one store, its address chosen by selects (no branch): I-type to
+          // (r, j+1) (slot t+1), M-type to (r+1, j+1) (slot t+2), D-type to
+          // (r+1, j) (slot t+1); the last row's M and D pushes to the
+          // boundary row; a cell outside the band's rows or the pair's
+          // columns, or a padding column, to the lane's sink
           const bool last = tk != ST_I && r + 1 >= nr;
-          double* dst = last ? &s_last[ks][k]
-                             : &s_pre[tk == ST_M ? sl : sl1][ks][tk == ST_I ? r : r + 1][k];
+          double* dst = skip ? &L.s_sink[lane]
+                             : last ? &s_last[ks][k]
+                                    : &s_pre[tk == ST_M ? sl : sl1][ks][tk == ST_I ? r : r + 1][k];
           *dst = h;
         }
       }
@@ -764,7 +788,7 @@ template <int NT, bool DENSE, bool CK>
 __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_occ(NT), 8))) void band_bwd_kernel(
     BParams Pr) {
   using Gm = Geo<NT, true>;
-  constexpr int NBP = Gm::NBP, NBPP = Gm::NBPP, KR = Gm::KR, R = Gm::R, CPW = Gm::CPW;
+  constexpr int NBP = Gm::NBP, NBPP = Gm::NBPP, KR = Gm::KR, R = Gm::R;
   constexpr int KS = Gm::KS, KB = Gm::KB, NBLK = Gm::NBLK, BPW = Gm::BPW, NTILE = Gm::NTILE;
   constexpr bool BREG = bwd_breg<NT>();
   __shared__ __align__(16) unsigned char smem[bwd_smem<NT>()];
@@ -956,18 +980,39 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
       if (lane == 0) bnd_store(e + NBP, (double)s_lastE);
     }
     // ---- A: g and the scaled forward values of this step's cells
+    const int tb0 = t % (CQ + 1), tb1 = tb0 == 0 ? CQ : tb0 - 1;  // ring entries t, t - 1
 #pragma unroll
     for (int cg = 0; cg < NG; ++cg) {
       const int rr = w + kWaves * (CPI * cg + hc);
       const int jj = t - rr, r = nr - 1 - rr, i = i0 + r, j = Ly - jj;
       const bool in = rr < nr && jj >= 0 && jj <= Ly;
-      const bool bot = (rr == 0) && bin != nullptr;
-      const double* bM = s_bin[(jj + CQ) % (CQ + 1)];  // column j + 1 (reversed entry jj - 1)
-      const double* bD = s_bin[jj % (CQ + 1)];
+      // the bottom row (the band below's entries) lies in wave 0's first
+      // group only, where jj = t: the ring's entries jj - 1 and jj, a
+      // scalar index; every read unconditional, values chosen by selects
+      const bool botg = cg == 0 && w == 0 && bin != nullptr;  // wave-uniform
+      const bool bot = botg && rr == 0;
+      double bmv[KR], bdv[KR];
+      int eMb = 0, eDb = 0;
+      if (botg) {
+        const double* bM = s_bin[tb1];  // column j + 1 (reversed entry jj - 1)
+        const double* bD = s_bin[tb0];
+        eMb = (int)bM[NBP];
+        eDb = (int)bD[NBP];
+#pragma unroll
+        for (int q = 0; q < KR; ++q) {
+          const int kc = min(kl + 64 * q, NBP - 1);
+          bmv[q] = bM[kc];
+          bdv[q] = bD[kc];
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < KR; ++q) bmv[q] = bdv[q] = 0.0;
+      }
       const bool okM = in && i < Lx && j < Ly, okI = in && j < Ly, okD = in && i < Lx;
-      const int eM = okM ? (bot ? (int)bM[NBP] : s_eM[sl][rr]) : EXP_NONE;
-      const int eI = okI ? s_eI[sl][rr] : EXP_NONE;
-      const int eD = okD ? (bot ? (int)bD[NBP] : s_eD[sl][rr]) : EXP_NONE;
+      const int eMl = s_eM[sl][rr], eIl = s_eI[sl][rr], eDl = s_eD[sl][rr];
+      const int eM = okM ? (bot ? eMb : eMl) : EXP_NONE;
+      const int eI = okI ? eIl : EXP_NONE;
+      const int eD = okD ? (bot ? eDb : eDl) : EXP_NONE;
       int Eg = max(eM, max(eI, eD));
       if (Eg == EXP_NONE) Eg = 0;
       const double fsc = ldexp(inv_mP, lfE[cg] + Eg - EP);
@@ -976,12 +1021,12 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
       for (int q = 0; q < KR; ++q) {
         const int k = kl + 64 * q;
         const int kc = min(k, NBP - 1);
-        const double pre = s_pre[sl][rr][kc], bmv = bM[kc], bdv = bD[kc];
+        const double pre = s_pre[sl][rr][kc], bmv_ = bmv[q], bdv_ = bdv[q];
         const int tq = ty[q];
         double v = 0.0;  // selects, no branch per type
         v = (tq == ST_I && okI) ? pre : v;
-        v = (tq == ST_M && okM) ? (bot ? bmv : pre) : v;
-        v = (tq == ST_D && okD) ? (bot ? bdv : pre) : v;
+        v = (tq == ST_M && okM) ? (bot ? bmv_ : pre) : v;
+        v = (tq == ST_D && okD) ? (bot ? bdv_ : pre) : v;
         const int et = tq == ST_M ? eM : (tq == ST_I ? eI : eD);
         v = ldexp(v, et - Eg);
         g[q] = v;
