@@ -745,6 +745,7 @@ struct BwdLds {
   uint8_t s_y[kYR];
   int s_eM[2][Gm::R], s_eI[2][Gm::R], s_eD[2][Gm::R], s_Eg[Gm::R];
   int s_lastE, s_ok;                  // their exponent; cleared when a boundary wait gave up
+  double s_sink[64];                  // per-lane target of the padding slots' pushes
 };
 // The backward kernel's static LDS: its own, or the forward sweep's for a
 // checkpointed band's recomputation (the two are used one after the other).
@@ -1149,12 +1150,15 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
           if (want_post) post[cellS + sx[q]] = pv;
           if (ec_bin) {
             const double pe = (e != 0.0) ? pv : 0.0;
-            if (ty[q] == ST_M) {
-              if (ec_mlds) {
-                if (pe != 0.0) atomicAdd(s_ecM + gr[q] * kEcM + xr[cg] * A_PAD + yj, pe);
-              } else if (i >= 1 && j >= 1) {
-                mbuf[((size_t)(pd.T - 1 - t) * R + r) * Pr.nM + Pr.mrank[k]] = pe;
-              }
+            if (ec_mlds) {
+              // every type binned in LDS: one atomic, its bin chosen by
+              // selects (no branch per type)
+              double* bin_ = ty[q] == ST_M ? s_ecM + gr[q] * kEcM + xr[cg] * A_PAD + yj
+                             : ty[q] == ST_I ? s_ecI + gr[q] * A_PAD + yj
+                                             : s_ecD + r * Pr.G + gr[q];
+              if (pe != 0.0) atomicAdd(bin_, pe);
+            } else if (ty[q] == ST_M) {
+              if (i >= 1 && j >= 1) mbuf[((size_t)(pd.T - 1 - t) * R + r) * Pr.nM + Pr.mrank[k]] = pe;
             } else if (pe != 0.0) {
               if (ty[q] == ST_I) atomicAdd(s_ecI + gr[q] * A_PAD + yj, pe);
               else atomicAdd(s_ecD + r * Pr.G + gr[q], pe);
@@ -1163,12 +1167,15 @@ __global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(2 * band_o
             atomicAdd(ecnt + (size_t)gr[q] * TAB_SZ + emit_entry(ty[q], xr[cg], yj), pv);
           }
         }
-        // push q: M-type to (i-1, j-1), I-type to (i, j-1), D-type to (i-1, j)
-        // (one store, its address chosen by selects)
+        // push q: M-type to (i-1, j-1), I-type to (i, j-1), D-type to (i-1, j),
+        // a padding slot's to the lane's sink (one store, its address chosen
+        // by selects)
         const int tq = ty[q];
         const bool last = tq != ST_I && rr + 1 >= nr;
-        double* dst = last ? &s_last[k] : &s_pre[tq == ST_M ? sl : sl1][tq == ST_I ? rr : rr + 1][k];
-        if (tq != 0) *dst = qv;
+        double* dst = tq == 0 ? &L.s_sink[lane]
+                      : last  ? &s_last[k]
+                              : &s_pre[tq == ST_M ? sl : sl1][tq == ST_I ? rr : rr + 1][k];
+        *dst = qv;
       }
       if (kl == 0 && in) {
         s_eI[sl1][rr] = EB;
