@@ -1396,6 +1396,14 @@ double cus_per_queue() {
   return std::max(1.0, cus / (double)kQueues);
 }
 
+// Pair groups per queue hold pairs until they can feed this many times the
+// XCD's workgroup slots (TKFMI_BAND_FEED overrides the default for tuning).
+double band_feed() {
+  const char* e = getenv("TKFMI_BAND_FEED");
+  const double v = e ? atof(e) : 0.0;
+  return v > 0.0 ? v : 3.0;
+}
+
 BPlan plan(const tkfmi_problem* p, uint32_t flags) {
   BPlan pl;
   if (!p || p->S < 2 || !p->state_types || !p->state_group) { pl.why = "bad problem"; return pl; }
@@ -1527,8 +1535,10 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
   // Within a queue, the pairs are taken in groups just large enough to keep
   // the XCD's compute units busy: a band can start about lag = R + CQ steps
   // after the band it reads from, so one pair can feed T / lag compute units
-  // (T: steps per band), and a group holds pairs until the sum reaches 1.5
-  // times the XCD's compute units.  A group's bands are listed layer by
+  // (T: steps per band), and a group holds pairs until the sum reaches 3
+  // times the XCD's workgroup slots (swept on 1024-pair batches at 32, 52,
+  // 82 and 162 states: 3 was fastest or within 0.5 % of it everywhere;
+  // 1.5 cost 1-3 %, 20 up to 13 %).  A group's bands are listed layer by
   // layer (band 0 of each pair, then band 1, ...; the backward from the
   // last band), so that a band's ticket comes about when its input is
   // ready, and a few pairs at a time share the XCD's L2.
@@ -1542,7 +1552,7 @@ BPlan plan(const tkfmi_problem* p, uint32_t flags) {
     queue[x].push_back(b);
     load[x] += pb[b].cost;
   }
-  const double lag = R + CQ, feed = 1.5 * cus_per_queue() * band_occ(pl.NT);
+  const double lag = R + CQ, feed = band_feed() * cus_per_queue() * band_occ(pl.NT);
   pl.qoff_f.assign(kQueues + 1, 0);
   pl.qoff_b.assign(kQueues + 1, 0);
   std::vector<BDesc> fwd_all, bwd_all;  // per pair, bands 0..G-1
