@@ -148,6 +148,20 @@ typedef struct tkfmi_outputs {
  * the reference (its remat is always on in the VJP). */
 int tkfmi_checkpoint_budget(size_t bytes);
 
+/* Antidiagonals per recomputed segment (the K of CHECKPOINT_K) when the
+ * one-workgroup-per-pair antidiagonal kernel checkpoints a pair: the
+ * forward keeps the restart state of every K-th antidiagonal and the
+ * backward recomputes K antidiagonals at a time, so lattice memory is about
+ * (2 (Lx + Ly) / K + K + 3) (Lx + 1) (NBP + 1) doubles.  k = 0 (default):
+ * K = max(4, ceil(sqrt(2 (Lx + Ly + 1)))) per pair, which minimises it;
+ * k >= 4: K = max(4, min(k, Lx + Ly + 1)) for every pair (k in 1..3:
+ * TKFMI_EINVAL).  Results do not depend on K.  The banded kernel's
+ * recomputation is per band and takes no K.  Process-wide; query workspace
+ * sizes after changing it.  The reference's counterpart is the segment
+ * length of its rematerialisation, python/tkfmixdom/jax/train/vjp.py:627
+ * (jax.checkpoint over the whole scan). */
+int tkfmi_checkpoint_segment(int k);
+
 /* Version of this header's ABI (TKFMI_ABI_VERSION). */
 int tkfmi_abi_version(void);
 

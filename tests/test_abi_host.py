@@ -195,6 +195,34 @@ def test_checkpoint_budget_applies_per_pair():
         _abi.set_checkpoint_budget(_abi.CHECKPOINT_BUDGET_DEFAULT)
 
 
+def test_checkpoint_segment_sets_the_antidiagonal_kernels_k():
+    """tkfmi_checkpoint_segment fixes K for the antidiagonal kernel's
+    checkpoints: the workspace of a checkpointed 3000 x 2500 pair at 162
+    states changes by exactly the restart states and segment rows that K
+    sets ((Lx + Ly) / K + 1 checkpoints of two rows, K segment rows; a row is
+    (Lx + 1)(NBP + 1) doubles); k = 0 restores the sqrt rule (K = 105 here);
+    k in 1..3 is rejected with a message."""
+    from tkfmi import _abi, mixdom
+    L = _abi.lib()
+    st = list(mixdom.state_types(8, 4))
+    p, keep = _problem(st, [0] * len(st), [3000], [2500])
+    flags = _abi.CHECKPOINT | _abi.CLASS_KERNEL
+    row = 3001 * (160 + 1) * 8
+    rows = lambda K: 2 * (5500 // K + 1) + K
+    try:
+        auto = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), flags)
+        for k in (50, 400):
+            _abi.set_checkpoint_segment(k)
+            got = L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), flags)
+            assert abs((got - auto) - (rows(k) - rows(105)) * row) < 512, k
+        _abi.set_checkpoint_segment(0)
+        assert L.tkfmi_fb2d_workspace_bytes(ctypes.byref(p), flags) == auto
+        assert L.tkfmi_checkpoint_segment(3) == -1  # TKFMI_EINVAL
+        assert "at least 4" in L.tkfmi_last_error().decode()
+    finally:
+        _abi.set_checkpoint_segment(0)
+
+
 def test_invalid_problems_rejected_with_message():
     from tkfmi import _abi
     L = _abi.lib()

@@ -117,6 +117,32 @@ def test_wide_checkpoint_matches_full_lattice():
         assert _close(res["emit_counts"][b].cpu().numpy(), ec_o)[0], b
 
 
+def test_wide_checkpoint_segment_length_does_not_change_results():
+    """tkfmi_checkpoint_segment: with K fixed at 4 (the smallest), 7 and 50
+    antidiagonals per segment instead of the per-pair sqrt rule, the
+    checkpointed antidiagonal kernel still equals its full-lattice run bit
+    for bit (emission counts to the FP64 atomics' summation order)."""
+    from tkfmi import _abi, hmm, synth
+    w = synth.make_mixdom2(seed=23, B=4, K=8, F=4, C=4, lengths=[70, 5, 120, 33])
+    kw = dict(want_posteriors=True, want_emit_counts=True, kernel="class")
+    full = hmm.fb2d_batch(w.log_trans, w.state_types, w.xs, w.ys, w.log_tab, w.state_group, **kw)
+    try:
+        for k in (4, 7, 50):
+            print(f"checkpoint segment K={k}", flush=True)
+            _abi.set_checkpoint_segment(k)
+            ck = hmm.fb2d_batch(w.log_trans, w.state_types, w.xs, w.ys, w.log_tab, w.state_group,
+                                checkpoint=True, **kw)
+            assert np.array_equal(full["log_p"].cpu().numpy(), ck["log_p"].cpu().numpy()), k
+            assert np.array_equal(full["n"].cpu().numpy(), ck["n"].cpu().numpy()), k
+            for b in range(4):
+                assert np.array_equal(full["posteriors"][b].cpu().numpy(),
+                                      ck["posteriors"][b].cpu().numpy()), (k, b)
+            ok, worst = _close(ck["emit_counts"].cpu().numpy(), full["emit_counts"].cpu().numpy(), rtol=1e-12)
+            assert ok, (k, worst)
+    finally:
+        _abi.set_checkpoint_segment(0)
+
+
 def test_budget_checkpointing_is_bit_identical():
     """Pairs over the checkpoint budget are checkpointed automatically (here
     a 2 MB budget catches the three longer pairs of the batch but not the

@@ -726,14 +726,24 @@ int tkfmi_stage_upload(void* dst, const void* src, size_t bytes, hipStream_t str
 
 namespace {
 std::atomic<size_t> g_ckpt_budget{(size_t)256 << 20};
+std::atomic<int> g_ckpt_segment{0};  // antidiagonals per checkpoint segment; 0 = sqrt rule
 }
 
 size_t tkfmi_checkpoint_budget_bytes() { return g_ckpt_budget.load(); }
+int tkfmi_checkpoint_segment_k() { return g_ckpt_segment.load(); }
 
 extern "C" {
 
 int tkfmi_checkpoint_budget(size_t bytes) {
   g_ckpt_budget.store(bytes);
+  return TKFMI_OK;
+}
+
+int tkfmi_checkpoint_segment(int k) {
+  if (k != 0 && k < 4) {
+    return fail(TKFMI_EINVAL, "tkfmi_checkpoint_segment: k must be 0 (automatic) or at least 4");
+  }
+  g_ckpt_segment.store(k);
   return TKFMI_OK;
 }
 

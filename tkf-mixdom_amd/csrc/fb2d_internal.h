@@ -56,6 +56,9 @@ int tkfmi_band_run(const tkfmi_problem* p, const tkfmi_outputs* o, void* ws, siz
 // Lattice bytes of one pair above which the 16..160-state kernel keeps only
 // checkpoints (tkfmi_checkpoint_budget; 0 = only with TKFMI_CHECKPOINT).
 size_t tkfmi_checkpoint_budget_bytes();
+// Antidiagonals per checkpoint segment of the antidiagonal kernel
+// (tkfmi_checkpoint_segment; 0 = K ~ sqrt(2 (Lx + Ly + 1)) per pair).
+int tkfmi_checkpoint_segment_k();
 
 // Records `msg` as this thread's tkfmi_last_error() and returns `code`.
 int tkfmi_fail(int code, const std::string& msg);

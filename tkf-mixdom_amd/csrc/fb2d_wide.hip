@@ -730,8 +730,11 @@ bool plan(const tkfmi_problem* p, uint32_t flags, WPlan& pl, std::string& why) {
       const int64_t full_bytes = (cells * pl.NBP + cells) * 8;
       const size_t budget = tkfmi_checkpoint_budget_bytes();
       if ((flags & TKFMI_CHECKPOINT) || (budget > 0 && (size_t)full_bytes > budget)) {
-        // K ~ sqrt(2 (Lx + Ly + 1)) minimises checkpoints (2 rows each) + segment
-        const int K = std::max(4, (int)std::ceil(std::sqrt(2.0 * (lx + ly + 1))));
+        // K ~ sqrt(2 (Lx + Ly + 1)) minimises checkpoints (2 rows each) + segment,
+        // unless the caller fixed K (at most the pair's antidiagonals, at least 4)
+        const int kset = tkfmi_checkpoint_segment_k();
+        const int K = kset > 0 ? std::max(4, std::min(kset, lx + ly + 1))
+                               : std::max(4, (int)std::ceil(std::sqrt(2.0 * (lx + ly + 1))));
         const int64_t nck = (lx + ly) / K + 1, row = (int64_t)(lx + 1) * (pl.NBP + 1);
         d.ck = K;
         off += nck * 2 * row + (int64_t)K * row + 3 * row;

@@ -184,7 +184,7 @@ EXPORTED = ("tkfmi_abi_version", "tkfmi_last_error",
             "tkfmi_build_workspace_bytes", "tkfmi_build_params_batched",
             "tkfmi_fb2d_batched_noalloc", "tkfmi_pairs_decode_workspace_bytes",
             "tkfmi_pairs_decode", "tkfmi_autotune_workspace_bytes", "tkfmi_autotune",
-            "tkfmi_checkpoint_budget", "tkfmi_tkf_trans_jac", "tkfmi_chi_workspace_bytes",
+            "tkfmi_checkpoint_budget", "tkfmi_checkpoint_segment", "tkfmi_tkf_trans_jac", "tkfmi_chi_workspace_bytes",
             "tkfmi_chi_forward", "tkfmi_chi_backward", "tkfmi_expm_batched", "tkfmi_expm_vjp_batched")
 EXPM_MAX_N = 40  # TKFMI_EXPM_MAX_N
 NOALLOC_SCRATCH_BYTES = 576 << 20  # TKFMI_NOALLOC_SCRATCH_BYTES
@@ -216,6 +216,8 @@ def lib():
                                                      _c_p, ctypes.c_uint32]
             L.tkfmi_checkpoint_budget.restype = ctypes.c_int
             L.tkfmi_checkpoint_budget.argtypes = [ctypes.c_size_t]
+            L.tkfmi_checkpoint_segment.restype = ctypes.c_int
+            L.tkfmi_checkpoint_segment.argtypes = [ctypes.c_int]
             L.tkfmi_autotune_workspace_bytes.restype = ctypes.c_size_t
             L.tkfmi_autotune_workspace_bytes.argtypes = [ctypes.POINTER(Problem), ctypes.c_uint32]
             L.tkfmi_autotune.restype = ctypes.c_int
@@ -402,6 +404,12 @@ def set_checkpoint_budget(nbytes):
     """Lattice bytes of one pair above which the 16..160-state kernel
     checkpoints it automatically (0: only with checkpoint=True)."""
     check(lib().tkfmi_checkpoint_budget(int(nbytes)))
+
+
+def set_checkpoint_segment(k):
+    """Antidiagonals per recomputed segment of the antidiagonal kernel's
+    checkpoints (0: the per-pair sqrt rule; otherwise at least 4)."""
+    check(lib().tkfmi_checkpoint_segment(int(k)))
 
 
 def check_status(status):
